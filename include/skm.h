@@ -1,0 +1,232 @@
+/*
+ * skm: the per-frame Gaussian-splat render path on one AMD Instinct MI355X (gfx950), behind a C ABI
+ * that mirrors the splatkit-engine renderer boundary:
+ *
+ *   class SplatRenderer        packages/splatkit-engine/include/splatkit/rendering/SplatRenderer.h:36-160
+ *   struct Frame, Range        SplatRenderer.h:100-131
+ *   struct GpuWorldInfo        SplatRenderer.h:25-28
+ *   struct RenderPolicy et al. packages/splatkit-engine/include/splatkit/rendering/RenderPolicy.h:22-77
+ *   captureNextFrame           packages/splatkit-ios/include/SplatKit/metal/MetalRenderer.h:29
+ *
+ * One entry point per virtual of SplatRenderer, plain pointers and sizes only. Every call returns a
+ * status (0 = ok) or a value; failures set a per-renderer message (skm_last_error) and, like the
+ * reference, leave the previous world, policy and buffers in place. Nothing throws across the ABI.
+ * A renderer is used from one thread at a time (SplatRenderer.h:32), except the stat getters.
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct skm_renderer skm_renderer;
+
+typedef enum skm_status {
+  SKM_OK = 0,
+  SKM_ERR_INVALID = -1,     /* bad argument: the call did nothing */
+  SKM_ERR_DEVICE = -2,      /* HIP runtime or kernel failure; the renderer latches gpu_failed */
+  SKM_ERR_MEMORY = -3,      /* device allocation failed; previous buffers kept */
+  SKM_ERR_STATE = -4,       /* no world / no extent / wrong order source */
+  SKM_ERR_UNSUPPORTED = -5, /* a feature this backend does not implement */
+  SKM_ERR_CAPACITY = -6     /* the frame's tile pairs exceeded capacity; it was dropped */
+} skm_status;
+
+typedef struct skm_config {
+  int32_t device;          /* HIP device ordinal */
+  uint32_t tile_size;      /* raster tile edge in pixels; 16 */
+  uint32_t pair_factor;    /* initial tile-pair capacity per splat slot (grows on overflow); 0 = 8 */
+  uint32_t reserved;
+} skm_config;
+
+/* SplatRenderer::Range (SplatRenderer.h:100-103). */
+typedef struct skm_range {
+  uint32_t offset;
+  uint32_t count;
+} skm_range;
+
+/* RevealUniform (packages/splatkit-engine/include/splatkit/reveal/Reveal.h:25-35); level = +inf is idle. */
+typedef struct skm_reveal {
+  float level;
+  float band;
+  float tint_amount;
+  float brightness_boost;
+  float tint[4];
+} skm_reveal;
+
+typedef enum skm_order_source { SKM_ORDER_CPU = 0, SKM_ORDER_GPU = 1 } skm_order_source;
+
+/* SplatRenderer::Frame (SplatRenderer.h:111-131). view/proj are column-major Mat4 (Mat4.h:10-14). */
+typedef struct skm_frame {
+  int32_t order_source;     /* skm_order_source; this backend sorts on the GPU only */
+  const uint32_t* order;    /* CPU order: unsupported, must be NULL */
+  uint32_t order_count;
+  uint32_t draw_count;
+  const skm_range* ranges;  /* GPU order: the ranges to draw this frame */
+  uint32_t range_count;
+  int32_t sh_degree;        /* capped by the world's degree */
+  float view[16];
+  float proj[16];
+  float camera_position[3];
+  const void* label_styles; /* part labels: unsupported, must be NULL */
+  skm_reveal reveal;
+} skm_frame;
+
+/* GpuWorldInfo (SplatRenderer.h:25-28). */
+typedef struct skm_world_info {
+  uint32_t count;
+  int32_t sh_degree;
+} skm_world_info;
+
+/* A SplatCloud (packages/splat-core/include/splat/formats/SplatCloud.h:18-39) as borrowed arrays:
+ * positions xyz, covariance xx xy xz yy yz zz, colour rgb in [0,1], opacity (LOD nodes up to 1000),
+ * and 3*((sh_degree+1)^2-1) SH floats per splat, coefficient-major, channel fastest. */
+typedef struct skm_cloud_view {
+  uint32_t count;
+  int32_t sh_degree;
+  const float* positions;
+  const float* covariances;
+  const float* colors;
+  const float* alphas;
+  const float* sh;
+} skm_cloud_view;
+
+/* RenderPolicy (RenderPolicy.h:22-40), same field order; bools as bytes. */
+typedef struct skm_render_policy {
+  float lod_error_pixels;
+  uint32_t lod_splat_limit;
+  float alpha_threshold;
+  float subpixel_threshold;
+  uint8_t enable_frustum_culling;
+  uint8_t enable_hiz_occlusion;
+  uint8_t enable_early_termination;
+  uint8_t reserved0;
+  uint32_t sort_depth; /* 16 or 32 */
+  double background_color[3];
+} skm_render_policy;
+
+/* RenderPolicySupport (RenderPolicy.h:45-60). */
+typedef struct skm_policy_support {
+  uint8_t lod_error_pixels;
+  uint8_t lod_splat_limit;
+  uint8_t alpha_threshold;
+  uint8_t subpixel_threshold;
+  uint8_t enable_frustum_culling;
+  uint8_t enable_hiz_occlusion;
+  uint8_t enable_early_termination;
+  uint8_t sort_depth;
+  uint8_t background_color;
+  uint8_t reserved[3];
+  float min_lod_error_pixels;
+  float max_lod_error_pixels;
+  float min_subpixel_threshold;
+  float max_subpixel_threshold;
+  skm_render_policy fallback;
+} skm_policy_support;
+
+/* DeviceCapabilities + SplatLimits (RenderPolicy.h:64-77). */
+typedef struct skm_device_capabilities {
+  uint32_t max_lod_capacity_splats;
+  uint32_t min_residency_capacity_splats;
+  uint32_t max_residency_capacity_splats;
+  uint8_t supports_hiz_occlusion;
+  uint8_t supports_subgroups;
+  uint8_t reserved[2];
+  uint32_t max_texture_dimension;
+  skm_policy_support policy;
+} skm_device_capabilities;
+
+typedef enum skm_pixel_format {
+  SKM_PIXELS_BGRA8 = 0,   /* MetalFrameCapture's BGRA, top-down rows (MetalFrameCapture.h:18) */
+  SKM_PIXELS_RGBA32F = 1  /* the composite as computed, top-down rows */
+} skm_pixel_format;
+
+/* ---- lifecycle ---- */
+skm_status skm_renderer_create(const skm_config* config, skm_renderer** out);
+void skm_renderer_destroy(skm_renderer* r);
+const char* skm_last_error(const skm_renderer* r);
+
+/* ---- presentation controls (SplatRenderer.h:42-52) ---- */
+void skm_set_render_scale(skm_renderer* r, float scale);
+float skm_render_scale(const skm_renderer* r);
+void skm_set_linear_blending(skm_renderer* r, int linear);
+int skm_linear_blending(const skm_renderer* r);
+void skm_set_vsync(skm_renderer* r, int vsync);
+/* Replaces the platform surface attach/resize: the size frames are drawn at. */
+skm_status skm_set_draw_extent(skm_renderer* r, uint32_t width, uint32_t height);
+
+/* ---- capabilities and policy (SplatRenderer.h:56-70) ---- */
+skm_status skm_get_device_capabilities(const skm_renderer* r, skm_device_capabilities* out);
+skm_status skm_apply_render_policy(skm_renderer* r, const skm_render_policy* policy, char* reason,
+                                   size_t reason_size);
+/* Returns 1 and sets *bytes when known (device memory still free), else 0. */
+int skm_available_memory(const skm_renderer* r, uint64_t* bytes);
+
+/* ---- state (SplatRenderer.h:73-83) ---- */
+int skm_ready(const skm_renderer* r);
+int skm_gpu_failed(skm_renderer* r);
+int skm_has_completed_world_frame(skm_renderer* r);
+void skm_draw_extent(const skm_renderer* r, uint32_t* width, uint32_t* height);
+uint32_t skm_generation(const skm_renderer* r);
+int skm_surface_rebuild_pending(const skm_renderer* r);
+
+/* ---- worlds (SplatRenderer.h:87-107) ---- */
+skm_status skm_upload_world(skm_renderer* r, const skm_cloud_view* cloud, int max_sh_degree);
+int skm_selects_lod_on_gpu(const skm_renderer* r);
+skm_status skm_create_slab(skm_renderer* r, uint32_t capacity, int sh_degree);
+skm_status skm_upload_tile(skm_renderer* r, uint32_t offset, const skm_cloud_view* cloud);
+/* Returns 1 and fills *out when a world is resident. */
+int skm_world(const skm_renderer* r, skm_world_info* out);
+int skm_sorts_on_gpu(const skm_renderer* r);
+
+/* ---- frames (SplatRenderer.h:134-159) ---- */
+skm_status skm_draw(skm_renderer* r, const skm_frame* frame);
+/* Waits until every submitted frame completed (the host side of a completion handler). */
+skm_status skm_finish(skm_renderer* r);
+uint64_t skm_submitted_frames(const skm_renderer* r);
+uint64_t skm_completed_frames(skm_renderer* r);
+double skm_last_gpu_millis(const skm_renderer* r);
+double skm_last_sort_millis(const skm_renderer* r);
+double skm_last_cull_millis(const skm_renderer* r);
+uint32_t skm_last_draw_count(const skm_renderer* r);
+uint32_t skm_last_selected_count(const skm_renderer* r);
+double skm_last_select_millis(const skm_renderer* r);
+const char* skm_device_description(const skm_renderer* r);
+/* captureNextFrame (MetalRenderer.h:29): the last completed frame's pixels. */
+skm_status skm_read_frame(skm_renderer* r, int format, void* dst, size_t dst_bytes);
+
+/* ---- inspection of the last completed frame (parity tests and tools) ---- */
+/* Survivor count, depth keys and 32-byte ProjectedSplat records (GpuLayout.h:60-70), compaction order. */
+skm_status skm_read_visible(skm_renderer* r, uint32_t* count, uint32_t* keys, uint32_t* projected,
+                            uint32_t capacity);
+/* The depth sort's permutation of the survivors (MetalRadixSort::order, MetalRadixSort.h:27). */
+skm_status skm_read_order(skm_renderer* r, uint32_t* order, uint32_t capacity);
+/* Tile pairs in (tile | depth) order and [start, end) per tile. */
+skm_status skm_read_tiles(skm_renderer* r, uint64_t* pairs, uint32_t* pair_tiles,
+                          uint32_t* pair_values, uint64_t capacity, uint32_t* ranges,
+                          uint32_t tile_capacity);
+/* GPU milliseconds of the last completed frame: cull, depth sort, bin+tile sort+ranges, raster, total. */
+skm_status skm_read_stage_millis(skm_renderer* r, double* ms5);
+
+/* Sums over the frames completed since the last reset: GPU milliseconds per stage (as above),
+ * survivors and tile pairs. For throughput reports over many frames. */
+typedef struct skm_frame_totals {
+  uint64_t frames;
+  double stage_ms[5];
+  uint64_t visible;
+  uint64_t pairs;
+} skm_frame_totals;
+skm_status skm_read_frame_totals(skm_renderer* r, skm_frame_totals* out, int reset);
+
+/* ---- the sort on its own (MetalRadixSort, MetalRadixSort.h:13-19) ---- */
+/* Stable ascending argsort of host keys; key_bits 16 (2 passes, keys < 2^16) or 32 (4 passes). */
+skm_status skm_argsort_u32(skm_renderer* r, const uint32_t* keys, uint32_t n, int key_bits,
+                           uint32_t* order);
+/* Same on device memory already resident; *ms gets the GPU time of the sort. */
+skm_status skm_argsort_u32_device(skm_renderer* r, void* keys_dev, uint32_t n, int key_bits,
+                                  void* order_dev, double* ms);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,116 @@
+// C entry points over the reference's own CPU sources, compiled where they lie under
+// /root/reference into oracle/_ref/libref.so by oracle/Makefile. Test infrastructure only: the
+// tests use it to pin the oracle's restatement (oracle.cpp) against the real reference code.
+// Nothing here re-implements the reference; each function only adapts types and calls it.
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "splat/lod/LodTree.h"
+#include "splat/math/Covariance.h"
+#include "splat/math/Half.h"
+#include "splat/math/Mat4.h"
+#include "splatkit/rendering/GpuLayout.h"
+#include "splatkit/rendering/LodThreshold.h"
+
+namespace {
+splat::SplatCloud make_cloud(uint32_t n, int sh_degree, const float* positions,
+                             const float* covariances, const float* colors, const float* alphas,
+                             const float* sh) {
+  splat::SplatCloud cloud;
+  cloud.positions.assign(positions, positions + size_t{n} * 3);
+  cloud.covariances.assign(covariances, covariances + size_t{n} * 6);
+  cloud.colors.assign(colors, colors + size_t{n} * 3);
+  cloud.alphas.assign(alphas, alphas + n);
+  cloud.shDegree = sh_degree;
+  if (sh_degree > 0 && sh != nullptr) {
+    const size_t k = static_cast<size_t>((sh_degree + 1) * (sh_degree + 1) - 1) * 3;
+    cloud.sh.assign(sh, sh + size_t{n} * k);
+  }
+  return cloud;
+}
+}  // namespace
+
+extern "C" {
+
+uint16_t ref_to_half(float v) { return splat::toHalf(v); }
+float ref_from_half(uint16_t h) { return splat::fromHalf(h); }
+
+// packSplatRange (GpuLayout.cpp:122-137): 8 words per splat.
+void ref_pack_splats(uint32_t n, const float* positions, const float* covariances,
+                     const float* colors, const float* alphas, uint32_t* out) {
+  const auto cloud = make_cloud(n, 0, positions, covariances, colors, alphas, nullptr);
+  std::vector<splatkit::GpuSplat> packed(n);
+  splatkit::packSplatRange(cloud, 0, n, packed.data());
+  std::memcpy(out, packed.data(), size_t{n} * sizeof(splatkit::GpuSplat));
+}
+
+// packShRange, float16 storage (GpuLayout.cpp:92-113).
+uint32_t ref_sh_stride(int degree) { return static_cast<uint32_t>(splatkit::shStride(degree)); }
+void ref_pack_sh(uint32_t n, int src_degree, const float* positions, const float* sh, int degree,
+                 uint32_t* out) {
+  std::vector<float> zeros(size_t{n} * 6, 0.0f), ones(n, 1.0f);
+  const auto cloud = make_cloud(n, src_degree, positions, zeros.data(), zeros.data(), ones.data(), sh);
+  splatkit::packShRange(cloud, degree, 0, n, out);
+}
+
+// covarianceFromQuaternionScale (Covariance.cpp:7-17).
+void ref_covariance(const float* quaternion, const float* scale, float* out6) {
+  const auto c = splat::covarianceFromQuaternionScale(quaternion, scale);
+  std::memcpy(out6, c.data(), 6 * sizeof(float));
+}
+
+// Mat4::perspective (Mat4.h:75-86), column-major.
+void ref_perspective(float fovy, float aspect, float near_plane, float far_plane, float* out16) {
+  const auto m = splat::Mat4::perspective(fovy, aspect, near_plane, far_plane);
+  std::memcpy(out16, m.m.data(), 64);
+}
+
+// LodThreshold::update (LodThreshold.h:27-36) over a sequence of readbacks.
+void ref_lod_threshold(float policy, uint32_t steps, const uint32_t* selected,
+                       const uint32_t* limited, const uint32_t* limit, float* out) {
+  splatkit::LodThreshold t(policy);
+  for (uint32_t i = 0; i < steps; ++i) out[i] = t.update(selected[i], limited[i], limit[i]);
+}
+
+// buildLodTree (LodTree.cpp:323-342) + buildLodSelectionData (LodSelection.cpp:47-133).
+// Returns an opaque tree; read it back with the accessors below, then free it.
+void* ref_lod_build(uint32_t n, int sh_degree, const float* positions, const float* covariances,
+                    const float* colors, const float* alphas, const float* sh, uint32_t depth,
+                    uint32_t cluster_leaves) {
+  auto cloud = make_cloud(n, sh_degree, positions, covariances, colors, alphas, sh);
+  splat::LodBuildOptions options;
+  options.octreeDepth = depth;
+  options.clusterLeaves = cluster_leaves;
+  auto* tree = new splat::LodTree(splat::buildLodTree(std::move(cloud), options));
+  tree->selection = splat::buildLodSelectionData(*tree);
+  return tree;
+}
+void ref_lod_free(void* t) { delete static_cast<splat::LodTree*>(t); }
+// counts[0..3] = nodes, leaves, clusters, leaf refs.
+void ref_lod_counts(const void* t, uint64_t* counts) {
+  const auto* tree = static_cast<const splat::LodTree*>(t);
+  counts[0] = tree->nodeCount();
+  counts[1] = tree->leafCount;
+  counts[2] = tree->selection.clusters.size();
+  counts[3] = tree->selection.leaves.size();
+}
+// Node attributes (SplatCloud SoA), layout (6 words per node), clusters (16 words), leaves.
+void ref_lod_read(const void* t, float* positions, float* covariances, float* colors,
+                  float* alphas, float* sh, uint32_t* layout, uint32_t* clusters,
+                  uint32_t* leaves) {
+  const auto* tree = static_cast<const splat::LodTree*>(t);
+  const auto& c = tree->nodes;
+  if (positions) std::memcpy(positions, c.positions.data(), c.positions.size() * 4);
+  if (covariances) std::memcpy(covariances, c.covariances.data(), c.covariances.size() * 4);
+  if (colors) std::memcpy(colors, c.colors.data(), c.colors.size() * 4);
+  if (alphas) std::memcpy(alphas, c.alphas.data(), c.alphas.size() * 4);
+  if (sh && !c.sh.empty()) std::memcpy(sh, c.sh.data(), c.sh.size() * 4);
+  if (layout) std::memcpy(layout, tree->layout.data(), tree->layout.size() * sizeof(splat::LodNode));
+  if (clusters)
+    std::memcpy(clusters, tree->selection.clusters.data(),
+                tree->selection.clusters.size() * sizeof(splat::LodCluster));
+  if (leaves) std::memcpy(leaves, tree->selection.leaves.data(), tree->selection.leaves.size() * 4);
+}
+
+}  // extern "C"

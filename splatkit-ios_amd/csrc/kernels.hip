@@ -1,0 +1,861 @@
+// HIP kernels of the per-frame splat render path, written for gfx950 (wave64, 160 KB LDS per CU).
+//
+//   k_pack      SplatCloud floats -> resident SoA records (GpuLayout.cpp:92-137)
+//   k_prologue  per-frame control block: epoch, counters, histograms, tile ranges
+//   k_cull      visibility, projection, SH colour, depth key (SplatVisibility.metal:39-106,
+//               SplatProjection.metalh:78-141), compacted in lane order by a decoupled look-back
+//   k_hist      onesweep global digit histograms of a key array, all passes in one read
+//   k_onesweep  one LSD radix pass, stable (MetalRadixSort.h:13-19): wave64 ballot ranking,
+//               LDS-staged digit histograms and scatter, per-digit decoupled look-back
+//   k_bin       tile rectangles of the depth-ordered survivors, one scan + emission pass
+//   k_ranges    [start, end) per tile from the tile-sorted pairs
+//   k_raster    16x16-tile front-to-back composite, LDS-staged batches, early termination
+#include "skm_internal.h"
+#include "skm_math.h"
+
+namespace skm {
+namespace {
+
+// ---------------------------------------------------------------------------------------------
+// Look-back status words: tag (30 bits) | flag (2 bits) | value (32 bits). The tag is derived from
+// the frame epoch and the pass, so words left by an earlier frame or pass never read as ready.
+constexpr uint32_t kFlagAggregate = 1;
+constexpr uint32_t kFlagPrefix = 2;
+
+__device__ __forceinline__ uint32_t make_tag(uint32_t epoch, uint32_t pass) {
+  return (epoch * 16u + pass) & 0x3fffffffu;
+}
+__device__ __forceinline__ uint64_t status_word(uint32_t tag, uint32_t flag, uint32_t value) {
+  return (static_cast<uint64_t>(tag) << 34) | (static_cast<uint64_t>(flag) << 32) | value;
+}
+__device__ __forceinline__ void status_store(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t status_load(uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
+  const uint32_t s = a + b;
+  return s < a ? 0xffffffffu : s;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+// Number of set bits of `mask` below this lane.
+__device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block; `scratch` holds 4 words.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* scratch, uint32_t* total) {
+  const uint32_t incl = wave_inclusive_scan(v);
+  const uint32_t wave = threadIdx.x >> 6;
+  if (lane_id() == 63) scratch[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < 4; ++w) {
+    const uint32_t c = scratch[w];
+    before += (w < wave) ? c : 0u;
+    all += c;
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
+// Decoupled look-back over single-value status words by one full wave: the 64 lanes read 64
+// predecessors at once, nearest first, until one of them holds an inclusive prefix. Every
+// predecessor took its ticket earlier and publishes without waiting on later blocks, so the wait
+// ends. Returns the exclusive prefix (saturating) in every lane.
+__device__ uint32_t wave_lookback(uint64_t* status, uint32_t block, uint32_t tag) {
+  const uint32_t lane = lane_id();
+  uint64_t excl = 0;
+  int32_t k = static_cast<int32_t>(block) - 1;
+  while (k >= 0) {
+    const int32_t idx = k - static_cast<int32_t>(lane);
+    uint32_t flag = kFlagPrefix, value = 0;
+    bool ready = true;
+    if (idx >= 0) {
+      const uint64_t s = status_load(status + idx);
+      flag = static_cast<uint32_t>(s >> 32) & 3u;
+      value = static_cast<uint32_t>(s);
+      ready = (static_cast<uint32_t>(s >> 34) == tag) && flag != 0u;
+    }
+    if (__ballot(!ready) != 0ull) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    const uint64_t prefixes = __ballot(flag == kFlagPrefix);
+    if (prefixes != 0ull) {
+      const uint32_t first = static_cast<uint32_t>(__builtin_ctzll(prefixes));
+      excl += wave_sum64(lane <= first ? value : 0u);
+      break;
+    }
+    excl += wave_sum64(value);
+    k -= 64;
+  }
+  return excl > 0xffffffffull ? 0xffffffffu : static_cast<uint32_t>(excl);
+}
+
+__device__ __forceinline__ void mat_vec(const float* m, float x, float y, float z, float w, float out[4]) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) out[r] = ((m[r] * x + m[4 + r] * y) + m[8 + r] * z) + m[12 + r] * w;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Projection of one splat (SplatProjection.metalh:78-141), the same specification as the CPU
+// oracle, written for registers: SH words arrive preloaded.
+template <int DEG>
+__device__ __forceinline__ float sh_channel(const uint32_t* words, int ch, float x, float y, float z) {
+  auto coef = [&](uint32_t k) {
+    const uint32_t h = k * 3u + static_cast<uint32_t>(ch);
+    const uint32_t w = words[h / 2u];
+    return (h & 1u) == 0u ? lo_half(w) : hi_half(w);
+  };
+  const float C1 = 0.4886025119f;
+  float c = ((-C1 * y) * coef(0) + (C1 * z) * coef(1)) - (C1 * x) * coef(2);
+  if constexpr (DEG >= 2) {
+    const float C2_0 = 1.0925484306f, C2_1 = -1.0925484306f, C2_2 = 0.3153915653f,
+                C2_3 = -1.0925484306f, C2_4 = 0.5462742153f;
+    const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+    const float t0 = (C2_0 * xy) * coef(3);
+    const float t1 = (C2_1 * yz) * coef(4);
+    const float t2 = (C2_2 * ((2.0f * zz - xx) - yy)) * coef(5);
+    const float t3 = (C2_3 * xz) * coef(6);
+    const float t4 = (C2_4 * (xx - yy)) * coef(7);
+    c = c + ((((t0 + t1) + t2) + t3) + t4);
+    if constexpr (DEG >= 3) {
+      const float C3_0 = -0.5900435899f, C3_1 = 2.8906114426f, C3_2 = -0.4570457995f,
+                  C3_3 = 0.3731763326f, C3_4 = -0.4570457995f, C3_5 = 1.4453057213f,
+                  C3_6 = -0.5900435899f;
+      const float u0 = ((C3_0 * y) * (3.0f * xx - yy)) * coef(8);
+      const float u1 = ((C3_1 * xy) * z) * coef(9);
+      const float u2 = ((C3_2 * y) * ((4.0f * zz - xx) - yy)) * coef(10);
+      const float u3 = ((C3_3 * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy)) * coef(11);
+      const float u4 = ((C3_4 * x) * ((4.0f * zz - xx) - yy)) * coef(12);
+      const float u5 = ((C3_5 * z) * (xx - yy)) * coef(13);
+      const float u6 = ((C3_6 * x) * (xx - 3.0f * yy)) * coef(14);
+      c = c + ((((((u0 + u1) + u2) + u3) + u4) + u5) + u6);
+    }
+  }
+  return c;
+}
+
+// revealGlow (SplatProjection.metalh:177-189).
+__device__ __forceinline__ void reveal_glow(const Camera& cam, float height, uint32_t rec[8]) {
+  if (!(cam.reveal[1] > 0.0f)) return;
+  float glow = mn(mx(1.0f - (cam.reveal[0] - height) / cam.reveal[1], 0.0f), 1.0f);
+  glow = glow * glow;
+  if (!(glow > 0.0f)) return;
+  float rgb[3] = {lo_half(rec[4]), hi_half(rec[4]), lo_half(rec[5])};
+  const float a = hi_half(rec[5]);
+  const float t = cam.reveal[2] * glow;
+  const float boost = 1.0f + cam.reveal[3] * glow;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float v = rgb[c];
+    if (cam.output_linear == 1u) v = pow_f(v, 1.0f / kDisplayGamma);
+    v = (v + (cam.reveal_tint[c] - v) * t) * boost;
+    if (cam.output_linear == 1u) v = pow_f(v, kDisplayGamma);
+    rgb[c] = v;
+  }
+  rec[4] = pack_rne(rgb[0], rgb[1]);
+  rec[5] = pack_rne(rgb[2], a);
+}
+
+template <int DEG, bool KEY16>
+__device__ __forceinline__ bool cull_lane(const Camera& cam, const World& w, uint32_t index,
+                                          uint32_t& key, uint32_t rec[8]) {
+  const float px = w.px[index], py = w.py[index], pz = w.pz[index];
+  const uint32_t rgba = w.rgba8[index];
+  const uint32_t lodw = w.lod_alpha[index];
+  float v[4];
+  mat_vec(cam.view, px, py, pz, 1.0f, v);
+  const float depth = -v[2];
+  if constexpr (KEY16) {
+    // quantizedDepthKey (SplatVisibility.metal:21-28).
+    const float zNear = cam.proj[14] / cam.proj[10];
+    const uint32_t steps = bits(mx(depth, zNear)) - bits(zNear);
+    key = min(steps >> 11, 65535u);
+  } else {
+    key = bits(depth);
+  }
+  if (!(cam.reveal[0] - py >= 0.0f)) return false;
+  const bool lod = lodw != 0u;
+  const float ca = static_cast<float>(rgba >> 24) / 255.0f;
+  const float alpha = lod ? f32(lodw) : ca;
+  if (alpha < kQuadMinAlpha) return false;
+  if (v[2] >= 0.0f) return false;
+  float clip[4];
+  mat_vec(cam.proj, v[0], v[1], v[2], v[3], clip);
+  if (clip[2] < 0.0f || clip[2] > clip[3]) return false;
+  if (clip[3] <= cam.proj[14] / cam.proj[10]) return false;
+
+  const uint32_t c0 = w.cov0[index], c1 = w.cov1[index], c2 = w.cov2[index];
+  const float xx = lo_half(c0), xy = hi_half(c0), xz = lo_half(c1);
+  const float yy = hi_half(c1), yz = lo_half(c2), zz = hi_half(c2);
+  const float invZ = 1.0f / v[2];
+  const float invZ2 = invZ * invZ;
+  const float limx = kProjectionFovClamp * cam.tan_half_fov[0];
+  const float limy = kProjectionFovClamp * cam.tan_half_fov[1];
+  const float tx = mn(mx(v[0] * invZ, -limx), limx) * v[2];
+  const float ty = mn(mx(v[1] * invZ, -limy), limy) * v[2];
+  const float j00 = cam.focal[0] * invZ;
+  const float j11 = cam.focal[1] * invZ;
+  const float j02 = ((-cam.focal[0]) * tx) * invZ2;
+  const float j12 = ((-cam.focal[1]) * ty) * invZ2;
+  const float* V = cam.view;
+  float T0[3], T1[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    T0[c] = j00 * V[c * 4 + 0] + j02 * V[c * 4 + 2];
+    T1[c] = j11 * V[c * 4 + 1] + j12 * V[c * 4 + 2];
+  }
+  const float S[3][3] = {{xx, xy, xz}, {xy, yy, yz}, {xz, yz, zz}};
+  float M0[3], M1[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    M0[c] = (T0[0] * S[0][c] + T0[1] * S[1][c]) + T0[2] * S[2][c];
+    M1[c] = (T1[0] * S[0][c] + T1[1] * S[1][c]) + T1[2] * S[2][c];
+  }
+  const float c00 = (M0[0] * T0[0] + M0[1] * T0[1]) + M0[2] * T0[2];
+  const float c10 = (M1[0] * T0[0] + M1[1] * T0[1]) + M1[2] * T0[2];
+  const float c11 = (M1[0] * T1[0] + M1[1] * T1[1]) + M1[2] * T1[2];
+  const float a = c00 + kLowPassVariance;
+  const float b = c10;
+  const float d = c11 + kLowPassVariance;
+
+  const float det = a * d - b * b;
+  const float mean = 0.5f * (a + d);
+  const float dist = mx(0.1f, sqrtf(mx(mean * mean - det, 0.0f)));
+  const float l1 = mean + dist;
+  const float l2 = mean - dist;
+  float e1x, e1y;
+  if (b == 0.0f) {
+    e1x = a > d ? 1.0f : 0.0f;
+    e1y = a > d ? 0.0f : 1.0f;
+  } else {
+    const float nx = b, ny = d - l2;
+    const float inv = 1.0f / sqrtf(nx * nx + ny * ny);
+    e1x = nx * inv;
+    e1y = ny * inv;
+  }
+  const float e2x = e1y, e2y = -e1x;
+  const float s1 = sqrtf(l1);
+  const float s2 = sqrtf(mx(l2, 0.0f));
+  const float a1x = e1x * s1, a1y = e1y * s1;
+  const float a2x = e2x * s2, a2y = e2y * s2;
+
+  const float ratio = mx(alpha / kQuadMinAlpha, 1.0f);
+  const float lg = static_cast<float>(log_d(static_cast<double>(ratio)));
+  float radius = mn(lod ? kLodBoundsRadius : kBoundsRadius, sqrtf(2.0f * lg));
+  const float sigma = sqrtf(mx(a1x * a1x + a1y * a1y, a2x * a2x + a2y * a2y));
+  radius = mn(radius, kMaxPixelRadius / mx(sigma, 1e-6f));
+
+  const float extx = radius * (fabsf(round_half(a1x)) + fabsf(round_half(a2x)));
+  const float exty = radius * (fabsf(round_half(a1y)) + fabsf(round_half(a2y)));
+  const float marg_x = (((extx + 0.5f) * 2.0f) * clip[3]) / cam.screen_size[0];
+  const float marg_y = (((exty + 0.5f) * 2.0f) * clip[3]) / cam.screen_size[1];
+  if (fabsf(clip[0]) > clip[3] + marg_x || fabsf(clip[1]) > clip[3] + marg_y) return false;
+
+  const float hd = 0.5f * (a - d);
+  const float sv = (0.5f * (a + d) + sqrtf(mx(hd * hd + b * b, 0.0f))) - kLowPassVariance;
+  if (!(sqrtf(mx(sv, 0.0f)) * radius >= cam.min_pixel_radius)) return false;
+
+  float rgb[3] = {static_cast<float>(rgba & 0xffu) / 255.0f,
+                  static_cast<float>((rgba >> 8) & 0xffu) / 255.0f,
+                  static_cast<float>((rgba >> 16) & 0xffu) / 255.0f};
+  if constexpr (DEG >= 1) {
+    constexpr uint32_t kStride = ((DEG + 1) * (DEG + 1) - 1) * 3 / 2 + (((DEG + 1) * (DEG + 1) - 1) * 3) % 2;
+    uint32_t words[kStride];
+#pragma unroll
+    for (uint32_t k = 0; k < kStride; ++k) words[k] = w.sh[static_cast<size_t>(k) * w.capacity + index];
+    const float dx = px - cam.camera_position[0];
+    const float dy = py - cam.camera_position[1];
+    const float dz = pz - cam.camera_position[2];
+    const float inv = 1.0f / sqrtf((dx * dx + dy * dy) + dz * dz);
+    const float nx = dx * inv, ny = dy * inv, nz = dz * inv;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rgb[c] = mx(rgb[c] + sh_channel<DEG>(words, c, nx, ny, nz), 0.0f);
+  }
+  if (cam.output_linear == 1u) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rgb[c] = pow_f(rgb[c], kDisplayGamma);
+  }
+  rec[0] = bits(clip[0] / clip[3]);
+  rec[1] = bits(clip[1] / clip[3]);
+  rec[2] = pack_rne(a1x, a1y);
+  rec[3] = pack_rne(a2x, a2y);
+  rec[4] = pack_rne(rgb[0], rgb[1]);
+  rec[5] = pack_rne(rgb[2], alpha);
+  rec[6] = bits(radius);
+  rec[7] = index;
+  reveal_glow(cam, py, rec);
+  return isfinite(depth);
+}
+
+// findRange (SplatVisibility.metal:30-37).
+__device__ __forceinline__ uint32_t find_range(const uint32_t* starts, uint32_t count, uint32_t t) {
+  uint32_t lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (starts[mid] <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pack(WorldOut w, uint32_t offset, uint32_t count,
+                                              const float* __restrict__ pos,
+                                              const float* __restrict__ cov,
+                                              const float* __restrict__ col,
+                                              const float* __restrict__ alpha,
+                                              const float* __restrict__ sh, int src_degree) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t o = offset + i;
+  w.px[o] = pos[3 * static_cast<size_t>(i)];
+  w.py[o] = pos[3 * static_cast<size_t>(i) + 1];
+  w.pz[o] = pos[3 * static_cast<size_t>(i) + 2];
+  // packRgba8 (GpuLayout.cpp:22-27): lround(clamp(v, 0, 1) * 255) per channel.
+  auto q = [](float v) {
+    const float c = v < 0.0f ? 0.0f : (1.0f < v ? 1.0f : v);
+    return static_cast<uint32_t>(roundf(c * 255.0f));
+  };
+  const float a = alpha[i];
+  const float* c = col + 3 * static_cast<size_t>(i);
+  w.rgba8[o] = q(c[0]) | (q(c[1]) << 8) | (q(c[2]) << 16) | (q(a) << 24);
+  const float* v = cov + 6 * static_cast<size_t>(i);
+  w.cov0[o] = half_sat(v[0]) | (half_sat(v[1]) << 16);
+  w.cov1[o] = half_sat(v[2]) | (half_sat(v[3]) << 16);
+  w.cov2[o] = half_sat(v[4]) | (half_sat(v[5]) << 16);
+  w.lod_alpha[o] = a > 1.0f ? bits(a) : 0u;
+  if (w.sh_degree > 0) {
+    const uint32_t halves = static_cast<uint32_t>((w.sh_degree + 1) * (w.sh_degree + 1) - 1) * 3u;
+    const uint32_t stride = (halves + 1u) / 2u;
+    const size_t src = static_cast<size_t>((src_degree + 1) * (src_degree + 1) - 1) * 3u;
+    const float* s = sh + src * i;
+    for (uint32_t k = 0; k < stride; ++k) {
+      const uint32_t h = 2u * k;
+      const uint32_t lo = (h < halves && src_degree > 0) ? half_sat(s[h]) : 0u;
+      const uint32_t hi = (h + 1u < halves && src_degree > 0) ? half_sat(s[h + 1]) : 0u;
+      w.sh[static_cast<size_t>(k) * w.capacity + o] = lo | (hi << 16);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_prologue(Ctrl* ctrl, uint32_t* hist, uint32_t hist_words,
+                                                  uint32_t* ranges, uint32_t range_words) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid == 0) {
+    ctrl->epoch = ctrl->epoch + 1u;
+    ctrl->visible = 0;
+    ctrl->pairs = 0;
+    ctrl->overflow = 0;
+    ctrl->sort_pairs = 0;
+    for (int k = 0; k < 16; ++k) ctrl->ticket[k] = 0;
+    for (int k = 0; k < 16; ++k) ctrl->lod[k] = 0;
+  }
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = gid; i < hist_words; i += stride) hist[i] = 0;
+  for (uint32_t i = gid; i < range_words; i += stride) ranges[i] = 0;
+}
+
+template <int DEG, bool INDEXED, bool KEY16>
+__global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
+  __shared__ uint32_t s_ticket;
+  __shared__ uint32_t s_wave[4];
+  __shared__ uint32_t s_prefix;
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) s_ticket = atomicAdd(&a.ctrl->ticket[kTicketCull], 1u);
+  __syncthreads();
+  const uint32_t b = s_ticket;
+  const uint32_t total = INDEXED ? min(*a.active_count, a.capacity) : a.total;
+  const uint32_t blocks = (total + 255u) / 256u;
+  if (b >= blocks) return;
+  const Camera& cam = *a.cam;
+  const uint32_t t = b * 256u + tid;
+  bool visible = false;
+  uint32_t key = 0;
+  uint32_t rec[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (t < total) {
+    uint32_t index;
+    if constexpr (INDEXED) {
+      index = a.active[t];
+    } else {
+      const uint32_t r = find_range(a.range_starts, a.range_count, t);
+      index = a.ranges[2 * r] + (t - a.range_starts[r]);
+    }
+    visible = cull_lane<DEG, KEY16>(cam, a.world, index, key, rec);
+  }
+  const uint64_t ballot = __ballot(visible);
+  const uint32_t wave = tid >> 6;
+  if (lane_id() == 0) s_wave[wave] = static_cast<uint32_t>(__popcll(ballot));
+  __syncthreads();
+  uint32_t before = 0, aggregate = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < 4; ++w) {
+    before += (w < wave) ? s_wave[w] : 0u;
+    aggregate += s_wave[w];
+  }
+  if (wave == 0) {
+    const uint32_t tag = make_tag(a.ctrl->epoch, 0);
+    uint32_t prefix = 0;
+    if (b == 0) {
+      if (lane_id() == 0) status_store(a.scan_status, status_word(tag, kFlagPrefix, aggregate));
+    } else {
+      if (lane_id() == 0) status_store(a.scan_status + b, status_word(tag, kFlagAggregate, aggregate));
+      prefix = wave_lookback(a.scan_status, b, tag);
+      if (lane_id() == 0)
+        status_store(a.scan_status + b, status_word(tag, kFlagPrefix, sat_add(prefix, aggregate)));
+    }
+    if (lane_id() == 0) s_prefix = prefix;
+  }
+  __syncthreads();
+  if (visible) {
+    const uint32_t off = s_prefix + before + rank_below(ballot);
+    a.cull_keys[off] = key;
+    uint4* dst = reinterpret_cast<uint4*>(a.projected + static_cast<size_t>(off) * 8u);
+    dst[0] = make_uint4(rec[0], rec[1], rec[2], rec[3]);
+    dst[1] = make_uint4(rec[4], rec[5], rec[6], rec[7]);
+  }
+  if (b == blocks - 1u && tid == 0) a.ctrl->visible = s_prefix + aggregate;
+}
+
+// Global digit histograms of every pass in one read of the keys (onesweep's upfront histogram).
+__global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
+                                              uint32_t* hist, int passes) {
+  __shared__ uint32_t s_h[4][4][256];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6;
+  for (uint32_t i = tid; i < 4 * 4 * 256; i += 256) (&s_h[0][0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t n = *n_ptr;
+  for (uint32_t i = blockIdx.x * 256u + tid; i < n; i += gridDim.x * 256u) {
+    const uint32_t k = keys[i];
+    for (int p = 0; p < passes; ++p) atomicAdd(&s_h[wave][p][(k >> (8 * p)) & 255u], 1u);
+  }
+  __syncthreads();
+  for (int p = 0; p < passes; ++p) {
+    const uint32_t v = s_h[0][p][tid] + s_h[1][p][tid] + s_h[2][p][tid] + s_h[3][p][tid];
+    if (v != 0u) atomicAdd(&hist[p * 256 + tid], v);
+  }
+}
+
+// One stable LSD pass over 4096-key tiles. Keys of tile b sit at b*4096 + wave*1024 + j*64 + lane:
+// each wave ranks its 16 rows in order with 8 ballots per row (lanes with equal digits form a
+// peer mask), so the in-tile order is position order and the pass is stable.
+__global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ keys_in,
+                                                  const uint32_t* __restrict__ vals_in,
+                                                  uint32_t* __restrict__ keys_out,
+                                                  uint32_t* __restrict__ vals_out, const uint32_t* n_ptr,
+                                                  const uint32_t* __restrict__ hist, uint32_t shift,
+                                                  uint64_t* status, Ctrl* ctrl, uint32_t ticket,
+                                                  uint32_t tag_pass) {
+  __shared__ uint32_t s_ticket;
+  __shared__ uint32_t s_wave_count[4][kRadix];
+  __shared__ uint32_t s_digit_start[kRadix];
+  __shared__ uint32_t s_base[kRadix];
+  __shared__ uint32_t s_scan[4];
+  __shared__ uint32_t s_keys[kSortTile];
+  __shared__ uint32_t s_vals[kSortTile];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
+  if (tid == 0) s_ticket = atomicAdd(&ctrl->ticket[ticket], 1u);
+#pragma unroll
+  for (uint32_t w = 0; w < 4; ++w) s_wave_count[w][tid] = 0;
+  __syncthreads();
+  const uint32_t b = s_ticket;
+  const uint32_t n = *n_ptr;
+  const uint32_t blocks = (n + kSortTile - 1u) / kSortTile;
+  if (b >= blocks) return;
+  const uint32_t base = b * kSortTile;
+  const uint32_t tag = make_tag(ctrl->epoch, tag_pass);
+
+  uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+#pragma unroll
+  for (uint32_t j = 0; j < kSortItems; ++j) {
+    const uint32_t pos = base + wave * 1024u + j * 64u + lane;
+    const bool valid = pos < n;
+    key[j] = valid ? keys_in[pos] : 0u;
+    val[j] = valid ? (vals_in != nullptr ? vals_in[pos] : pos) : 0u;
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kSortItems; ++j) {
+    const uint32_t pos = base + wave * 1024u + j * 64u + lane;
+    const bool valid = pos < n;
+    const uint32_t d = (key[j] >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (uint32_t bit = 0; bit < 8; ++bit) {
+      const bool set = (d >> bit) & 1u;
+      const uint64_t vote = __ballot(set);
+      peers &= set ? vote : ~vote;
+    }
+    const uint32_t before = s_wave_count[wave][d];
+    rank[j] = before + rank_below(peers);
+    const uint32_t leader = peers != 0ull ? static_cast<uint32_t>(__builtin_ctzll(peers)) : 64u;
+    if (valid && lane == leader) s_wave_count[wave][d] = before + static_cast<uint32_t>(__popcll(peers));
+  }
+  __syncthreads();
+  // Per digit (thread = digit): offsets of each wave, the tile's total, and its start in the tile.
+  const uint32_t c0 = s_wave_count[0][tid], c1 = s_wave_count[1][tid];
+  const uint32_t c2 = s_wave_count[2][tid], c3 = s_wave_count[3][tid];
+  const uint32_t count = c0 + c1 + c2 + c3;
+  uint32_t tile_total;
+  const uint32_t start = block_exclusive_scan(count, s_scan, &tile_total);
+  // Global start of each digit: exclusive scan of the pass histogram.
+  uint32_t all;
+  const uint32_t gstart = block_exclusive_scan(hist[tid], s_scan, &all);
+  s_wave_count[0][tid] = 0;
+  s_wave_count[1][tid] = c0;
+  s_wave_count[2][tid] = c0 + c1;
+  s_wave_count[3][tid] = c0 + c1 + c2;
+  s_digit_start[tid] = start;
+  // Per-digit look-back across tiles.
+  uint64_t* st = status + static_cast<size_t>(b) * kRadix;
+  uint32_t excl = 0;
+  if (b == 0) {
+    status_store(st + tid, status_word(tag, kFlagPrefix, count));
+  } else {
+    status_store(st + tid, status_word(tag, kFlagAggregate, count));
+    int32_t k = static_cast<int32_t>(b) - 1;
+    while (k >= 0) {
+      const uint64_t s = status_load(status + static_cast<size_t>(k) * kRadix + tid);
+      const uint32_t flag = static_cast<uint32_t>(s >> 32) & 3u;
+      if (static_cast<uint32_t>(s >> 34) != tag || flag == 0u) {
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      excl += static_cast<uint32_t>(s);
+      if (flag == kFlagPrefix) break;
+      --k;
+    }
+    status_store(st + tid, status_word(tag, kFlagPrefix, excl + count));
+  }
+  s_base[tid] = gstart + excl - start;
+  __syncthreads();
+  // Stage the tile in digit order in LDS.
+#pragma unroll
+  for (uint32_t j = 0; j < kSortItems; ++j) {
+    const uint32_t pos = base + wave * 1024u + j * 64u + lane;
+    if (pos < n) {
+      const uint32_t d = (key[j] >> shift) & 255u;
+      const uint32_t local = s_digit_start[d] + s_wave_count[wave][d] + rank[j];
+      s_keys[local] = key[j];
+      s_vals[local] = val[j];
+    }
+  }
+  __syncthreads();
+  const uint32_t in_tile = min(kSortTile, n - base);
+  for (uint32_t i = tid; i < in_tile; i += 256u) {
+    const uint32_t k = s_keys[i];
+    const uint32_t dst = s_base[(k >> shift) & 255u] + i;
+    if (keys_out != nullptr) keys_out[dst] = k;
+    vals_out[dst] = s_vals[i];
+  }
+}
+
+// Tile rectangle of a projected record (the CPU oracle's orc_tile_rect, same specification).
+struct Rect {
+  int32_t tx0, ty0, tx1, ty1;
+};
+__device__ __forceinline__ uint32_t tile_rect(const uint32_t* p, uint32_t width, uint32_t height,
+                                              uint32_t tile, Rect& r) {
+  const float cx = (f32(p[0]) * 0.5f + 0.5f) * static_cast<float>(width);
+  const float cy = (0.5f - f32(p[1]) * 0.5f) * static_cast<float>(height);
+  const float rad = f32(p[6]);
+  const float ex = rad * (fabsf(lo_half(p[2])) + fabsf(lo_half(p[3])));
+  const float ey = rad * (fabsf(hi_half(p[2])) + fabsf(hi_half(p[3])));
+  float x0 = ((cx - ex) - 0.5f) - kBinPad, x1 = ((cx + ex) - 0.5f) + kBinPad;
+  float y0 = ((cy - ey) - 0.5f) - kBinPad, y1 = ((cy + ey) - 0.5f) + kBinPad;
+  r.tx0 = r.ty0 = 0;
+  r.tx1 = r.ty1 = -1;
+  if (!(x0 <= x1) || !(y0 <= y1)) return 0;
+  x0 = mx(x0, -1.0f);
+  x1 = mn(x1, static_cast<float>(width));
+  y0 = mx(y0, -1.0f);
+  y1 = mn(y1, static_cast<float>(height));
+  int32_t px0 = static_cast<int32_t>(ceilf(x0)), px1 = static_cast<int32_t>(floorf(x1));
+  int32_t py0 = static_cast<int32_t>(ceilf(y0)), py1 = static_cast<int32_t>(floorf(y1));
+  px0 = max(px0, 0);
+  py0 = max(py0, 0);
+  px1 = min(px1, static_cast<int32_t>(width) - 1);
+  py1 = min(py1, static_cast<int32_t>(height) - 1);
+  if (px0 > px1 || py0 > py1) return 0;
+  r.tx0 = px0 / static_cast<int32_t>(tile);
+  r.ty0 = py0 / static_cast<int32_t>(tile);
+  r.tx1 = px1 / static_cast<int32_t>(tile);
+  r.ty1 = py1 / static_cast<int32_t>(tile);
+  return static_cast<uint32_t>((r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1));
+}
+
+// Tile pairs of the depth-ordered survivors: counts, a block scan plus look-back for the global
+// offset, then the block emits its pairs cooperatively (pair e of the block belongs to the last
+// survivor whose exclusive offset is <= e), so writes are contiguous whatever the footprints.
+__global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __restrict__ order) {
+  __shared__ uint32_t s_ticket, s_prefix;
+  __shared__ uint32_t s_scan[4];
+  __shared__ uint32_t s_excl[256], s_val[256], s_origin[256], s_width[256];
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) s_ticket = atomicAdd(&a.ctrl->ticket[kTicketBin], 1u);
+  __syncthreads();
+  const uint32_t b = s_ticket;
+  const uint32_t n = a.ctrl->visible;
+  const uint32_t blocks = (n + 255u) / 256u;
+  if (b >= blocks) return;
+  const uint32_t i = b * 256u + tid;
+  uint32_t count = 0, value = 0;
+  Rect r{0, 0, -1, -1};
+  if (i < n) {
+    value = order[i];
+    const uint4* src = reinterpret_cast<const uint4*>(a.projected + static_cast<size_t>(value) * 8u);
+    const uint4 w0 = src[0], w1 = src[1];
+    const uint32_t p[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+    count = tile_rect(p, a.width, a.height, a.tile, r);
+  }
+  uint32_t aggregate;
+  const uint32_t excl = block_exclusive_scan(count, s_scan, &aggregate);
+  if ((tid >> 6) == 0) {
+    const uint32_t tag = make_tag(a.ctrl->epoch, 1);
+    uint32_t prefix = 0;
+    if (b == 0) {
+      if (lane_id() == 0) status_store(a.scan_status, status_word(tag, kFlagPrefix, aggregate));
+    } else {
+      if (lane_id() == 0) status_store(a.scan_status + b, status_word(tag, kFlagAggregate, aggregate));
+      prefix = wave_lookback(a.scan_status, b, tag);
+      if (lane_id() == 0)
+        status_store(a.scan_status + b, status_word(tag, kFlagPrefix, sat_add(prefix, aggregate)));
+    }
+    if (lane_id() == 0) s_prefix = prefix;
+  }
+  s_excl[tid] = excl;
+  s_val[tid] = value;
+  s_origin[tid] = static_cast<uint32_t>(r.tx0) | (static_cast<uint32_t>(r.ty0) << 16);
+  s_width[tid] = static_cast<uint32_t>(r.tx1 - r.tx0 + 1);
+  __syncthreads();
+  const uint64_t prefix = s_prefix;
+  for (uint32_t e = tid; e < aggregate; e += 256u) {
+    uint32_t lo = 0, hi = 255;
+    while (lo < hi) {  // last j with s_excl[j] <= e
+      const uint32_t mid = (lo + hi + 1u) / 2u;
+      if (s_excl[mid] <= e) lo = mid; else hi = mid - 1u;
+    }
+    const uint32_t k = e - s_excl[lo];
+    const uint32_t wdt = s_width[lo];
+    const uint32_t tx = (s_origin[lo] & 0xffffu) + k % wdt;
+    const uint32_t ty = (s_origin[lo] >> 16) + k / wdt;
+    const uint64_t idx = prefix + e;
+    if (idx < a.pair_capacity) {
+      a.pair_keys[0][idx] = ty * a.tiles_x + tx;
+      a.pair_vals[0][idx] = s_val[lo];
+    }
+  }
+  if (b == blocks - 1u && tid == 0) {
+    const uint32_t total = sat_add(s_prefix, aggregate);
+    a.ctrl->pairs = total;
+    a.ctrl->overflow = total > a.pair_capacity ? 1u : 0u;
+  }
+}
+
+// Sort count for the tile pairs: zero when they overflowed the capacity (the frame is dropped).
+__global__ void k_pair_count(Ctrl* ctrl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctrl->sort_pairs = ctrl->overflow ? 0u : ctrl->pairs;
+}
+
+__global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
+                                                uint32_t* ranges) {
+  const uint32_t n = *n_ptr;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    const uint32_t t = keys[i];
+    if (i == 0 || keys[i - 1] != t) ranges[2 * t] = i;
+    if (i + 1 == n || keys[i + 1] != t) ranges[2 * t + 1] = i + 1;
+  }
+}
+
+// 16x16 tile, one pixel per thread (4 waves). Each batch of up to 256 pairs is prepared once
+// (centre in pixels, inverse axis matrix, radius, log2 peak, clamped colour) into LDS, then
+// every pixel walks it front to back. A pixel stops when its transmittance falls below
+// kTransmittanceStop; the tile stops when all of its pixels have.
+__global__ __launch_bounds__(256) void k_raster(FrameArgs a, const uint32_t* __restrict__ pair_vals,
+                                                const uint32_t* __restrict__ ranges) {
+  __shared__ float4 s_a[256], s_b[256], s_c[256];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t tile_id = blockIdx.y * a.tiles_x + blockIdx.x;
+  const uint32_t px = blockIdx.x * 16u + (tid & 15u);
+  const uint32_t py = blockIdx.y * 16u + (tid >> 4);
+  const bool in_image = px < a.width && py < a.height;
+  const uint32_t start = ranges[2 * tile_id], end = ranges[2 * tile_id + 1];
+  const float fw = static_cast<float>(a.width), fh = static_cast<float>(a.height);
+  const float fx = static_cast<float>(px) + 0.5f, fy = static_cast<float>(py) + 0.5f;
+  float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
+  bool done = !in_image;
+  for (uint32_t s0 = start; s0 < end; s0 += 256u) {
+    const uint32_t j = s0 + tid;
+    if (j < end) {
+      const uint32_t idx = pair_vals[j];
+      const uint4* src = reinterpret_cast<const uint4*>(a.projected + static_cast<size_t>(idx) * 8u);
+      const uint4 w0 = src[0], w1 = src[1];
+      const float cx = (f32(w0.x) * 0.5f + 0.5f) * fw;
+      const float cy = (0.5f - f32(w0.y) * 0.5f) * fh;
+      const float a1x = lo_half(w0.z), a1y = hi_half(w0.z);
+      const float a2x = lo_half(w0.w), a2y = hi_half(w0.w);
+      const float det = a1x * a2y - a2x * a1y;
+      const float inv = 1.0f / det;
+      const float a0 = hi_half(w1.y);
+      const float l2 = static_cast<float>(log_d(static_cast<double>(mx(a0, kHalfMin))) / kLn2);
+      const float log2a0 = mn(mx(l2, -kMaxLog2Alpha), kMaxLog2Alpha);
+      s_a[tid] = make_float4(cx, cy, a2y * inv, -(a2x * inv));
+      s_b[tid] = make_float4(-(a1y * inv), a1x * inv, f32(w1.z), log2a0);
+      s_c[tid] = make_float4(mn(mx(lo_half(w1.x), 0.0f), kMaxColorScale),
+                             mn(mx(hi_half(w1.x), 0.0f), kMaxColorScale),
+                             mn(mx(lo_half(w1.y), 0.0f), kMaxColorScale), 0.0f);
+    }
+    __syncthreads();
+    const uint32_t batch = min(256u, end - s0);
+    if (!done) {
+      for (uint32_t k = 0; k < batch; ++k) {
+        const float4 A = s_a[k], B = s_b[k];
+        const float dx = fx - A.x;
+        const float dy = A.y - fy;
+        const float u = A.z * dx + A.w * dy;
+        const float v = B.x * dx + B.y * dy;
+        if (!(fabsf(u) <= B.z && fabsf(v) <= B.z)) continue;
+        const float gu = kGaussianScale * u;
+        const float gv = kGaussianScale * v;
+        const float power = B.w - (gu * gu + gv * gv);
+        if (!(power >= kLog2MinAlpha)) continue;
+        const float alpha = exp2f(mn(power, 0.0f));
+        const float4 C = s_c[k];
+        const float w = T * alpha;
+        C0 = C0 + w * C.x;
+        C1 = C1 + w * C.y;
+        C2 = C2 + w * C.z;
+        T = T * (1.0f - alpha);
+        if (T < kTransmittanceStop) {
+          done = true;
+          break;
+        }
+      }
+    }
+    if (__syncthreads_count(done ? 0 : 1) == 0) break;
+  }
+  if (in_image) {
+    float4* out = reinterpret_cast<float4*>(a.image) + static_cast<size_t>(py) * a.width + px;
+    *out = make_float4(C0 + T * a.background[0], C1 + T * a.background[1], C2 + T * a.background[2], 1.0f);
+  }
+}
+
+template <int DEG, bool INDEXED>
+void launch_cull_key(const FrameArgs& a, uint32_t blocks, hipStream_t s) {
+  if (a.key16) hipLaunchKernelGGL((k_cull<DEG, INDEXED, true>), dim3(blocks), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_cull<DEG, INDEXED, false>), dim3(blocks), dim3(256), 0, s, a);
+}
+template <int DEG>
+void launch_cull_deg(const FrameArgs& a, uint32_t blocks, hipStream_t s) {
+  if (a.active != nullptr) launch_cull_key<DEG, true>(a, blocks, s);
+  else launch_cull_key<DEG, false>(a, blocks, s);
+}
+
+uint32_t grid_for(uint32_t items, uint32_t per_block) {
+  const uint32_t b = (items + per_block - 1u) / per_block;
+  return b == 0u ? 1u : b;
+}
+
+}  // namespace
+
+hipError_t launch_pack(const WorldOut& out, uint32_t offset, uint32_t count, const float* positions,
+                       const float* covariances, const float* colors, const float* alphas,
+                       const float* sh, int src_degree, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pack, dim3(grid_for(count, 256)), dim3(256), 0, stream, out, offset, count,
+                     positions, covariances, colors, alphas, sh, src_degree);
+  return hipGetLastError();
+}
+
+hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev,
+                          uint32_t capacity, int passes, uint32_t* hist, uint64_t* status,
+                          hipStream_t stream) {
+  // keys[0] holds the input; passes alternate keys[0] -> keys[1] -> keys[0] ...; the order ends in
+  // vals[(passes - 1) & 1].
+  hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, stream, ctrl, hist, 4u * 256u, nullptr, 0u);
+  hipLaunchKernelGGL(k_hist, dim3(min(grid_for(capacity, 256), 1024u)), dim3(256), 0, stream, keys[0],
+                     n_dev, hist, passes);
+  for (int p = 0; p < passes; ++p) {
+    const uint32_t* kin = keys[p & 1];
+    uint32_t* kout = (p + 1 < passes) ? keys[(p + 1) & 1] : nullptr;
+    const uint32_t* vin = p == 0 ? nullptr : vals[(p - 1) & 1];
+    hipLaunchKernelGGL(k_onesweep, dim3(grid_for(capacity, kSortTile)), dim3(256), 0, stream, kin, vin,
+                       kout, vals[p & 1], n_dev, hist + p * 256, 8u * p, status, ctrl,
+                       kTicketDepthSort + p, 2u + p);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_frame(const FrameArgs& a, int depth_passes, int tile_passes, hipEvent_t* marks,
+                        hipStream_t s) {
+  const uint32_t tiles = a.tiles_x * a.tiles_y;
+  if (marks) (void)hipEventRecord(marks[0], s);
+  hipLaunchKernelGGL(k_prologue, dim3(min(grid_for(tiles * 2u, 256), 256u)), dim3(256), 0, s, a.ctrl,
+                     a.hist, 2u * 4u * 256u, a.tile_ranges, tiles * 2u);
+  const uint32_t lanes = a.active != nullptr ? a.capacity : a.total;
+  const uint32_t cull_blocks = grid_for(lanes, 256);
+  switch (a.world.sh_degree) {
+    case 0: launch_cull_deg<0>(a, cull_blocks, s); break;
+    case 1: launch_cull_deg<1>(a, cull_blocks, s); break;
+    case 2: launch_cull_deg<2>(a, cull_blocks, s); break;
+    default: launch_cull_deg<3>(a, cull_blocks, s); break;
+  }
+  if (marks) (void)hipEventRecord(marks[1], s);
+  // Depth sort of the survivors: cull keys -> keys[0] -> keys[1] -> keys[0] ...
+  const uint32_t* visible = &a.ctrl->visible;
+  hipLaunchKernelGGL(k_hist, dim3(min(grid_for(a.capacity, 256), 1024u)), dim3(256), 0, s, a.cull_keys,
+                     visible, a.hist, depth_passes);
+  for (int p = 0; p < depth_passes; ++p) {
+    const uint32_t* kin = p == 0 ? a.cull_keys : a.keys[(p - 1) & 1];
+    uint32_t* kout = (p + 1 < depth_passes) ? a.keys[p & 1] : nullptr;
+    const uint32_t* vin = p == 0 ? nullptr : a.order[(p - 1) & 1];
+    hipLaunchKernelGGL(k_onesweep, dim3(grid_for(a.capacity, kSortTile)), dim3(256), 0, s, kin, vin, kout,
+                       a.order[p & 1], visible, a.hist + p * 256, 8u * p, a.sort_status, a.ctrl,
+                       kTicketDepthSort + p, 2u + p);
+  }
+  const uint32_t* order = a.order[(depth_passes - 1) & 1];
+  if (marks) (void)hipEventRecord(marks[2], s);
+  hipLaunchKernelGGL(k_bin, dim3(grid_for(a.capacity, 256)), dim3(256), 0, s, a, order);
+  const uint32_t* sort_n = &a.ctrl->sort_pairs;
+  hipLaunchKernelGGL(k_pair_count, dim3(1), dim3(64), 0, s, a.ctrl);
+  uint32_t* tile_hist = a.hist + 4 * 256;
+  hipLaunchKernelGGL(k_hist, dim3(min(grid_for(a.pair_capacity, 256), 2048u)), dim3(256), 0, s,
+                     a.pair_keys[0], sort_n, tile_hist, tile_passes);
+  for (int p = 0; p < tile_passes; ++p) {
+    hipLaunchKernelGGL(k_onesweep, dim3(grid_for(a.pair_capacity, kSortTile)), dim3(256), 0, s,
+                       a.pair_keys[p & 1], a.pair_vals[p & 1], a.pair_keys[(p + 1) & 1],
+                       a.pair_vals[(p + 1) & 1], sort_n, tile_hist + p * 256, 8u * p, a.sort_status,
+                       a.ctrl, kTicketTileSort + p, 6u + p);
+  }
+  const int fin = tile_passes & 1;
+  hipLaunchKernelGGL(k_ranges, dim3(min(grid_for(a.pair_capacity, 256), 2048u)), dim3(256), 0, s,
+                     a.pair_keys[fin], sort_n, a.tile_ranges);
+  if (marks) (void)hipEventRecord(marks[3], s);
+  hipLaunchKernelGGL(k_raster, dim3(a.tiles_x, a.tiles_y), dim3(256), 0, s, a, a.pair_vals[fin],
+                     a.tile_ranges);
+  if (marks) (void)hipEventRecord(marks[4], s);
+  return hipGetLastError();
+}
+
+}  // namespace skm

@@ -1,0 +1,924 @@
+// The C-ABI renderer (include/skm.h): resident worlds in HBM, two frames in flight on one HIP
+// stream, the frame pipeline of kernels.hip, per-stage HIP-event timing and readback.
+//
+// Mirrors MetalSplatRenderer (packages/splatkit-ios/Sources/SplatKitCore/rendering/metal/
+// MetalSplatRenderer.mm): transactional uploads (:166-208), range validation before any GPU work
+// (MetalVisibilityPass.mm:135-168), a sticky GPU-failure latch (MetalFrameLoop.h:51-55) and
+// completion-time counters (MetalFrameCompute.mm:210-237).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "skm.h"
+#include "skm_internal.h"
+
+using skm::Camera;
+using skm::Ctrl;
+
+namespace {
+
+constexpr uint32_t kSlots = 2;             // frames in flight (MetalFrameSlots.h:8)
+constexpr uint32_t kMaxExtent = 16384;     // per axis
+constexpr uint32_t kUploadChunk = 1u << 20;  // splats per staging copy
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+uint32_t sh_stride(int degree) {
+  if (degree <= 0) return 0;
+  return (static_cast<uint32_t>((degree + 1) * (degree + 1) - 1) * 3u + 1u) / 2u;
+}
+
+// One device allocation; freed on destruction. Moves only.
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr, o.bytes = 0; }
+  DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
+    if (this != &o) {
+      reset();
+      ptr = o.ptr;
+      bytes = o.bytes;
+      o.ptr = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DeviceBuffer() { reset(); }
+  void reset() {
+    if (ptr != nullptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  hipError_t allocate(size_t n) {
+    reset();
+    if (n == 0) n = 256;
+    const hipError_t e = hipMalloc(&ptr, n);
+    if (e != hipSuccess) {
+      ptr = nullptr;
+      return e;
+    }
+    bytes = n;
+    return hipSuccess;
+  }
+  template <class T>
+  T* as(size_t byte_offset = 0) const {
+    return reinterpret_cast<T*>(static_cast<char*>(ptr) + byte_offset);
+  }
+};
+
+struct PinnedBuffer {
+  void* ptr = nullptr;
+  PinnedBuffer() = default;
+  PinnedBuffer(const PinnedBuffer&) = delete;
+  PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+  ~PinnedBuffer() {
+    if (ptr != nullptr) (void)hipHostFree(ptr);
+  }
+  hipError_t allocate(size_t n) {
+    if (ptr != nullptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    return hipHostMalloc(&ptr, n, hipHostMallocDefault);
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(ptr);
+  }
+};
+
+// The resident world: every GpuSplat field as its own array, SH words word-major.
+struct WorldBuffers {
+  DeviceBuffer mem;
+  uint32_t capacity = 0;
+  uint32_t count = 0;
+  int sh_degree = 0;
+  bool slab = false;
+  size_t off[9] = {};
+
+  hipError_t allocate(uint32_t cap, int degree) {
+    size_t o = 0;
+    for (int k = 0; k < 8; ++k) {
+      off[k] = o;
+      o = align_up(o + size_t{cap} * 4u, 256);
+    }
+    off[8] = o;
+    o += size_t{cap} * sh_stride(degree) * 4u + 256;
+    capacity = cap;
+    sh_degree = degree;
+    return mem.allocate(o);
+  }
+  skm::World view() const {
+    return {mem.as<float>(off[0]),    mem.as<float>(off[1]),    mem.as<float>(off[2]),
+            mem.as<uint32_t>(off[3]), mem.as<uint32_t>(off[4]), mem.as<uint32_t>(off[5]),
+            mem.as<uint32_t>(off[6]), mem.as<uint32_t>(off[7]), mem.as<uint32_t>(off[8]),
+            capacity,                 sh_degree};
+  }
+  skm::WorldOut out() const {
+    return {mem.as<float>(off[0]),    mem.as<float>(off[1]),    mem.as<float>(off[2]),
+            mem.as<uint32_t>(off[3]), mem.as<uint32_t>(off[4]), mem.as<uint32_t>(off[5]),
+            mem.as<uint32_t>(off[6]), mem.as<uint32_t>(off[7]), mem.as<uint32_t>(off[8]),
+            capacity,                 sh_degree};
+  }
+};
+
+struct Slot {
+  DeviceBuffer cam, ranges, starts, image;
+  PinnedBuffer cam_host, ranges_host, starts_host, ctrl_host;
+  hipEvent_t marks[5] = {};
+  hipEvent_t done = nullptr;
+  bool in_flight = false;
+  bool with_world = false;
+  uint32_t width = 0, height = 0;
+  int depth_passes = 4, tile_passes = 2;
+};
+
+struct Policy {
+  float lod_error_pixels = 1.0f;
+  uint32_t lod_splat_limit = 0;
+  float subpixel = 1.0f;
+  bool key16 = true;  // RenderPolicy.h:36 default low16
+  double background[3] = {0.05, 0.05, 0.08};
+};
+
+bool valid_cloud(const skm_cloud_view* c) {
+  return c != nullptr && (c->count == 0 || (c->positions && c->covariances && c->colors && c->alphas)) &&
+         c->sh_degree >= 0 && c->sh_degree <= 3;
+}
+
+}  // namespace
+
+struct skm_renderer {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string error;
+  std::string description;
+  uint32_t tile = 16;
+  uint32_t pair_factor = 8;
+
+  float render_scale = 1.0f;
+  bool linear = false;
+  bool vsync = true;
+  uint32_t surface_w = 0, surface_h = 0;
+  uint32_t generation = 0;
+  Policy policy;
+
+  WorldBuffers world;
+  bool has_world = false;
+
+  // Frame scratch, shared by the frames on the stream.
+  uint32_t capacity = 0;       // survivor slots
+  uint32_t pair_capacity = 0;  // tile pairs
+  uint32_t tile_capacity = 0;  // tiles
+  DeviceBuffer ctrl, cull_keys, keys[2], order[2], projected, scan_status, sort_status, hist;
+  DeviceBuffer pair_keys[2], pair_vals[2], tile_ranges;
+  size_t sort_status_blocks = 0;
+  uint64_t pipelines = 0;  // prologue launches since the status arrays were last cleared
+
+  // Standalone sort scratch.
+  DeviceBuffer sort_keys, sort_vals, sort_n;
+  uint32_t sort_capacity = 0;
+
+  Slot slots[kSlots];
+  uint64_t submitted = 0, completed = 0;
+  int last_slot = -1;
+  bool gpu_failed = false;
+  bool completed_world_frame = false;
+  bool grow_pairs = false;
+  uint64_t pairs_needed = 0;
+  bool last_dropped = false;
+  double stage_ms[5] = {};
+  uint32_t last_draw_count = 0;
+  skm_frame_totals totals{};
+
+  skm_status fail(skm_status s, const char* fmt, ...) {
+    char buf[512];
+    va_list args;
+    va_start(args, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, args);
+    va_end(args);
+    error = buf;
+    if (s == SKM_ERR_DEVICE) gpu_failed = true;
+    return s;
+  }
+  skm_status check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return SKM_OK;
+    return fail(e == hipErrorOutOfMemory ? SKM_ERR_MEMORY : SKM_ERR_DEVICE, "%s: %s", what,
+                hipGetErrorString(e));
+  }
+
+  uint32_t draw_w() const {
+    return surface_w == 0 ? 0 : std::max(1u, static_cast<uint32_t>(surface_w * render_scale + 0.5f));
+  }
+  uint32_t draw_h() const {
+    return surface_h == 0 ? 0 : std::max(1u, static_cast<uint32_t>(surface_h * render_scale + 0.5f));
+  }
+
+  // Waits for a slot's frame and takes its counters and timings.
+  skm_status harvest(Slot& s) {
+    if (!s.in_flight) return SKM_OK;
+    const hipError_t e = hipEventSynchronize(s.done);
+    s.in_flight = false;
+    if (e != hipSuccess) return check(e, "frame");
+    const Ctrl* c = s.ctrl_host.as<Ctrl>();
+    float ms[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k) (void)hipEventElapsedTime(&ms[k], s.marks[k], s.marks[k + 1]);
+    (void)hipEventElapsedTime(&ms[4], s.marks[0], s.marks[4]);
+    for (int k = 0; k < 5; ++k) stage_ms[k] = ms[k];
+    last_draw_count = c->visible;
+    last_dropped = c->overflow != 0;
+    totals.frames += 1;
+    for (int k = 0; k < 5; ++k) totals.stage_ms[k] += ms[k];
+    totals.visible += c->visible;
+    totals.pairs += c->pairs;
+    if (last_dropped) {
+      grow_pairs = true;
+      pairs_needed = std::max<uint64_t>(pairs_needed, c->pairs);
+    }
+    ++completed;
+    if (s.with_world) completed_world_frame = true;
+    return SKM_OK;
+  }
+
+  skm_status drain() {
+    skm_status st = SKM_OK;
+    for (Slot& s : slots) {
+      const skm_status k = harvest(s);
+      if (k != SKM_OK) st = k;
+    }
+    return st;
+  }
+
+  skm_status ensure_status(uint32_t items) {
+    const size_t blocks = (size_t{items} + skm::kSortTile - 1) / skm::kSortTile + 1;
+    if (blocks <= sort_status_blocks) return SKM_OK;
+    DeviceBuffer b;
+    if (auto st = check(b.allocate(blocks * skm::kRadix * 8u), "sort status"); st != SKM_OK) return st;
+    if (auto st = check(hipMemsetAsync(b.ptr, 0, b.bytes, stream), "clear"); st != SKM_OK) return st;
+    sort_status = std::move(b);
+    sort_status_blocks = blocks;
+    return SKM_OK;
+  }
+
+  // Survivor scratch for `cap` lanes. Transactional: nothing changes unless every buffer fits.
+  skm_status ensure_capacity(uint32_t cap) {
+    cap = std::max(cap, 1u);
+    if (cap <= capacity) return SKM_OK;
+    DeviceBuffer ck, k0, k1, o0, o1, pr, ss;
+    const size_t n = size_t{cap};
+    if (auto st = check(ck.allocate(n * 4), "keys"); st != SKM_OK) return st;
+    if (auto st = check(k0.allocate(n * 4), "keys"); st != SKM_OK) return st;
+    if (auto st = check(k1.allocate(n * 4), "keys"); st != SKM_OK) return st;
+    if (auto st = check(o0.allocate(n * 4), "order"); st != SKM_OK) return st;
+    if (auto st = check(o1.allocate(n * 4), "order"); st != SKM_OK) return st;
+    if (auto st = check(pr.allocate(n * 32), "projected"); st != SKM_OK) return st;
+    if (auto st = check(ss.allocate(((n + 255) / 256 + 1) * 8), "scan status"); st != SKM_OK) return st;
+    if (auto st = check(hipMemsetAsync(ss.ptr, 0, ss.bytes, stream), "clear"); st != SKM_OK) return st;
+    if (auto st = ensure_status(cap); st != SKM_OK) return st;
+    cull_keys = std::move(ck);
+    keys[0] = std::move(k0);
+    keys[1] = std::move(k1);
+    order[0] = std::move(o0);
+    order[1] = std::move(o1);
+    projected = std::move(pr);
+    scan_status = std::move(ss);
+    capacity = cap;
+    return SKM_OK;
+  }
+
+  skm_status ensure_pairs(uint64_t want) {
+    want = std::min<uint64_t>(std::max<uint64_t>(want, 4096), 0xfffff000ull);
+    if (want <= pair_capacity) return SKM_OK;
+    DeviceBuffer k0, k1, v0, v1;
+    const size_t n = static_cast<size_t>(want);
+    if (auto st = check(k0.allocate(n * 4), "pair keys"); st != SKM_OK) return st;
+    if (auto st = check(k1.allocate(n * 4), "pair keys"); st != SKM_OK) return st;
+    if (auto st = check(v0.allocate(n * 4), "pair values"); st != SKM_OK) return st;
+    if (auto st = check(v1.allocate(n * 4), "pair values"); st != SKM_OK) return st;
+    if (auto st = ensure_status(static_cast<uint32_t>(want)); st != SKM_OK) return st;
+    pair_keys[0] = std::move(k0);
+    pair_keys[1] = std::move(k1);
+    pair_vals[0] = std::move(v0);
+    pair_vals[1] = std::move(v1);
+    pair_capacity = static_cast<uint32_t>(want);
+    return SKM_OK;
+  }
+
+  skm_status ensure_tiles(uint32_t tiles) {
+    if (tiles <= tile_capacity) return SKM_OK;
+    if (auto st = drain(); st != SKM_OK) return st;  // a frame in flight may still read the ranges
+    DeviceBuffer t;
+    if (auto st = check(t.allocate(size_t{tiles} * 8u), "tile ranges"); st != SKM_OK) return st;
+    tile_ranges = std::move(t);
+    tile_capacity = tiles;
+    return SKM_OK;
+  }
+
+  // Clears look-back status words well before the 30-bit tags they carry could repeat.
+  skm_status maybe_clear_status() {
+    if (++pipelines < (1ull << 24)) return SKM_OK;
+    if (auto st = drain(); st != SKM_OK) return st;
+    pipelines = 1;
+    if (auto st = check(hipMemsetAsync(sort_status.ptr, 0, sort_status.bytes, stream), "clear");
+        st != SKM_OK)
+      return st;
+    return check(hipMemsetAsync(scan_status.ptr, 0, scan_status.bytes, stream), "clear");
+  }
+
+  skm_status upload_range(WorldBuffers& w, uint32_t offset, const skm_cloud_view* c, int src_degree) {
+    const uint32_t n = c->count;
+    const size_t shf = src_degree > 0 ? static_cast<size_t>((src_degree + 1) * (src_degree + 1) - 1) * 3u : 0;
+    const uint32_t chunk = std::min(n, kUploadChunk);
+    DeviceBuffer staging;
+    const size_t per = 12 + 24 + 12 + 4 + shf * 4;
+    if (auto st = check(staging.allocate(size_t{std::max(chunk, 1u)} * per + 1024), "upload staging");
+        st != SKM_OK)
+      return st;
+    float* d_pos = staging.as<float>(0);
+    float* d_cov = d_pos + size_t{chunk} * 3;
+    float* d_col = d_cov + size_t{chunk} * 6;
+    float* d_alpha = d_col + size_t{chunk} * 3;
+    float* d_sh = d_alpha + chunk;
+    for (uint32_t base = 0; base < n; base += chunk) {
+      const uint32_t m = std::min(chunk, n - base);
+      hipError_t e = hipMemcpyAsync(d_pos, c->positions + size_t{base} * 3, size_t{m} * 12, hipMemcpyHostToDevice, stream);
+      if (e == hipSuccess) e = hipMemcpyAsync(d_cov, c->covariances + size_t{base} * 6, size_t{m} * 24, hipMemcpyHostToDevice, stream);
+      if (e == hipSuccess) e = hipMemcpyAsync(d_col, c->colors + size_t{base} * 3, size_t{m} * 12, hipMemcpyHostToDevice, stream);
+      if (e == hipSuccess) e = hipMemcpyAsync(d_alpha, c->alphas + base, size_t{m} * 4, hipMemcpyHostToDevice, stream);
+      if (e == hipSuccess && shf > 0)
+        e = hipMemcpyAsync(d_sh, c->sh + size_t{base} * shf, size_t{m} * shf * 4, hipMemcpyHostToDevice, stream);
+      if (e == hipSuccess)
+        e = skm::launch_pack(w.out(), offset + base, m, d_pos, d_cov, d_col, d_alpha, shf > 0 ? d_sh : nullptr,
+                             shf > 0 ? src_degree : 0, stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(stream);
+      if (auto st = check(e, "upload"); st != SKM_OK) return st;
+    }
+    return SKM_OK;
+  }
+};
+
+namespace {
+uint32_t digit_passes(uint32_t max_key) {
+  uint32_t passes = 1;
+  while (passes < 4 && (max_key >> (8 * passes)) != 0) ++passes;
+  return passes;
+}
+}  // namespace
+
+extern "C" {
+
+skm_status skm_renderer_create(const skm_config* config, skm_renderer** out) {
+  if (out == nullptr) return SKM_ERR_INVALID;
+  *out = nullptr;
+  int device = config ? config->device : 0;
+  const uint32_t tile = config && config->tile_size ? config->tile_size : 16u;
+  if (tile != 16u) return SKM_ERR_UNSUPPORTED;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return SKM_ERR_DEVICE;
+  auto* r = new skm_renderer();
+  r->device = device;
+  r->tile = tile;
+  if (config && config->pair_factor) r->pair_factor = config->pair_factor;
+  hipDeviceProp_t prop;
+  if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+      hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete r;
+    return SKM_ERR_DEVICE;
+  }
+  int runtime = 0;
+  (void)hipRuntimeGetVersion(&runtime);
+  char desc[320];
+  std::snprintf(desc, sizeof(desc), "%s (%s, %d CUs), HIP %d.%d", prop.name, prop.gcnArchName,
+                prop.multiProcessorCount, runtime / 10000000, (runtime / 100000) % 100);
+  r->description = desc;
+  bool ok = r->ctrl.allocate(sizeof(Ctrl)) == hipSuccess &&
+            hipMemset(r->ctrl.ptr, 0, sizeof(Ctrl)) == hipSuccess &&
+            r->hist.allocate(2 * 4 * 256 * 4) == hipSuccess;
+  for (Slot& s : r->slots) {
+    ok = ok && s.cam.allocate(sizeof(Camera)) == hipSuccess &&
+         s.ranges.allocate(size_t{skm::kMaxRanges} * 8) == hipSuccess &&
+         s.starts.allocate(size_t{skm::kMaxRanges + 1} * 4) == hipSuccess &&
+         s.cam_host.allocate(sizeof(Camera)) == hipSuccess &&
+         s.ranges_host.allocate(size_t{skm::kMaxRanges} * 8) == hipSuccess &&
+         s.starts_host.allocate(size_t{skm::kMaxRanges + 1} * 4) == hipSuccess &&
+         s.ctrl_host.allocate(sizeof(Ctrl)) == hipSuccess &&
+         hipEventCreate(&s.done) == hipSuccess;
+    for (auto& m : s.marks) ok = ok && hipEventCreate(&m) == hipSuccess;
+  }
+  ok = ok && r->ensure_capacity(1) == SKM_OK && r->ensure_pairs(4096) == SKM_OK &&
+       r->ensure_tiles(1) == SKM_OK && hipStreamSynchronize(r->stream) == hipSuccess;
+  if (!ok) {
+    skm_renderer_destroy(r);
+    return SKM_ERR_DEVICE;
+  }
+  *out = r;
+  return SKM_OK;
+}
+
+void skm_renderer_destroy(skm_renderer* r) {
+  if (r == nullptr) return;
+  (void)hipSetDevice(r->device);
+  if (r->stream) (void)hipStreamSynchronize(r->stream);
+  for (Slot& s : r->slots) {
+    if (s.done) (void)hipEventDestroy(s.done);
+    for (auto& m : s.marks)
+      if (m) (void)hipEventDestroy(m);
+  }
+  if (r->stream) (void)hipStreamDestroy(r->stream);
+  delete r;
+}
+
+const char* skm_last_error(const skm_renderer* r) { return r ? r->error.c_str() : "null renderer"; }
+
+void skm_set_render_scale(skm_renderer* r, float scale) {
+  if (r == nullptr || !std::isfinite(scale)) return;
+  const float s = std::clamp(scale, 0.1f, 2.0f);
+  if (s != r->render_scale) {
+    r->render_scale = s;
+    ++r->generation;
+  }
+}
+float skm_render_scale(const skm_renderer* r) { return r ? r->render_scale : 1.0f; }
+void skm_set_linear_blending(skm_renderer* r, int linear) {
+  if (r) r->linear = linear != 0;
+}
+int skm_linear_blending(const skm_renderer* r) { return r && r->linear ? 1 : 0; }
+void skm_set_vsync(skm_renderer* r, int vsync) {
+  if (r) r->vsync = vsync != 0;
+}
+
+skm_status skm_set_draw_extent(skm_renderer* r, uint32_t width, uint32_t height) {
+  if (r == nullptr) return SKM_ERR_INVALID;
+  if (width == 0 || height == 0 || width > kMaxExtent || height > kMaxExtent)
+    return r->fail(SKM_ERR_INVALID, "extent %ux%u outside 1..%u", width, height, kMaxExtent);
+  if (width != r->surface_w || height != r->surface_h) {
+    r->surface_w = width;
+    r->surface_h = height;
+    ++r->generation;
+  }
+  return SKM_OK;
+}
+
+// MetalSplatRenderer::deviceCapabilities (MetalSplatRenderer.mm:119-150).
+skm_status skm_get_device_capabilities(const skm_renderer* r, skm_device_capabilities* out) {
+  if (r == nullptr || out == nullptr) return SKM_ERR_INVALID;
+  std::memset(out, 0, sizeof(*out));
+  out->max_lod_capacity_splats = 1u << 27;
+  out->min_residency_capacity_splats = 1;
+  out->max_residency_capacity_splats = 0x7fffffffu;
+  out->supports_hiz_occlusion = 0;
+  out->supports_subgroups = 1;
+  out->max_texture_dimension = kMaxExtent;
+  skm_policy_support& p = out->policy;
+  p.lod_error_pixels = 1;
+  p.lod_splat_limit = 1;
+  p.subpixel_threshold = 1;
+  p.sort_depth = 1;
+  p.background_color = 1;
+  p.min_lod_error_pixels = 0.1f;   // LodThreshold::kMinPolicyPixels
+  p.max_lod_error_pixels = 16.0f;  // LodThreshold::kMaxPolicyPixels
+  p.min_subpixel_threshold = 0.0f;
+  p.max_subpixel_threshold = 3.402823466e38f;
+  p.fallback.lod_error_pixels = 1.0f;
+  p.fallback.alpha_threshold = 2.0f / 255.0f;
+  p.fallback.subpixel_threshold = 1.0f;
+  p.fallback.enable_frustum_culling = 1;
+  p.fallback.enable_early_termination = 1;
+  p.fallback.sort_depth = r->policy.key16 ? 16u : 32u;
+  p.fallback.background_color[0] = 0.05;
+  p.fallback.background_color[1] = 0.05;
+  p.fallback.background_color[2] = 0.08;
+  return SKM_OK;
+}
+
+// validRenderPolicy (RenderPolicy.cpp:30-53) for the fields this backend applies; a rejected
+// policy leaves the previous one in place (SplatRenderer.h:58-61).
+skm_status skm_apply_render_policy(skm_renderer* r, const skm_render_policy* p, char* reason,
+                                   size_t reason_size) {
+  if (r == nullptr || p == nullptr) return SKM_ERR_INVALID;
+  const char* why = nullptr;
+  if (!std::isfinite(p->lod_error_pixels) || !(p->lod_error_pixels > 0.0f))
+    why = "lodErrorPixels must be finite and positive";
+  else if (!std::isfinite(p->alpha_threshold) || p->alpha_threshold < 0.0f || p->alpha_threshold > 1.0f)
+    why = "alphaThreshold must be finite and in [0, 1]";
+  else if (!std::isfinite(p->subpixel_threshold) || p->subpixel_threshold < 0.0f)
+    why = "subpixelThreshold must be finite and non-negative";
+  else if (p->sort_depth != 16 && p->sort_depth != 32)
+    why = "sortDepth must be 16 or 32";
+  else
+    for (double c : p->background_color)
+      if (!std::isfinite(c) || c < 0 || c > 1) why = "backgroundColor must be finite and in [0, 1]";
+  if (why != nullptr) {
+    if (reason && reason_size) std::snprintf(reason, reason_size, "%s", why);
+    return r->fail(SKM_ERR_INVALID, "%s", why);
+  }
+  r->policy.lod_error_pixels = std::clamp(p->lod_error_pixels, 0.1f, 16.0f);
+  r->policy.lod_splat_limit = p->lod_splat_limit;
+  r->policy.subpixel = p->subpixel_threshold;
+  r->policy.key16 = p->sort_depth == 16;
+  for (int c = 0; c < 3; ++c) r->policy.background[c] = p->background_color[c];
+  if (reason && reason_size) reason[0] = '\0';
+  return SKM_OK;
+}
+
+int skm_available_memory(const skm_renderer* r, uint64_t* bytes) {
+  if (r == nullptr || bytes == nullptr) return 0;
+  size_t free_b = 0, total_b = 0;
+  if (hipSetDevice(r->device) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 0;
+  *bytes = free_b;
+  return 1;
+}
+
+int skm_ready(const skm_renderer* r) { return r && r->surface_w > 0 && !r->gpu_failed ? 1 : 0; }
+int skm_gpu_failed(skm_renderer* r) { return r && r->gpu_failed ? 1 : 0; }
+int skm_has_completed_world_frame(skm_renderer* r) {
+  if (r == nullptr) return 0;
+  for (Slot& s : r->slots)
+    if (s.in_flight && hipEventQuery(s.done) == hipSuccess) (void)r->harvest(s);
+  return r->completed_world_frame ? 1 : 0;
+}
+void skm_draw_extent(const skm_renderer* r, uint32_t* width, uint32_t* height) {
+  if (width) *width = r ? r->draw_w() : 0;
+  if (height) *height = r ? r->draw_h() : 0;
+}
+uint32_t skm_generation(const skm_renderer* r) { return r ? r->generation : 0; }
+int skm_surface_rebuild_pending(const skm_renderer*) { return 0; }
+
+// MetalSplatRenderer::uploadWorld (:166-178) and MetalWorld::upload (MetalWorld.mm:24-57): the
+// drawn degree is min(cloud, requested) when the cloud carries it, else 0.
+skm_status skm_upload_world(skm_renderer* r, const skm_cloud_view* cloud, int max_sh_degree) {
+  if (r == nullptr) return SKM_ERR_INVALID;
+  if (!valid_cloud(cloud)) return r->fail(SKM_ERR_INVALID, "incomplete cloud");
+  if (r->gpu_failed) return r->fail(SKM_ERR_DEVICE, "GPU failed earlier");
+  if (auto st = r->check(hipSetDevice(r->device), "device"); st != SKM_OK) return st;
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  int degree = std::clamp(std::min(cloud->sh_degree, max_sh_degree), 0, 3);
+  if (degree > 0 && cloud->sh == nullptr) degree = 0;
+  WorldBuffers w;
+  if (auto st = r->check(w.allocate(std::max(cloud->count, 1u), degree), "world"); st != SKM_OK) return st;
+  w.count = cloud->count;
+  if (auto st = r->upload_range(w, 0, cloud, degree > 0 ? cloud->sh_degree : 0); st != SKM_OK) return st;
+  if (auto st = r->ensure_capacity(cloud->count); st != SKM_OK) return st;
+  if (auto st = r->ensure_pairs(uint64_t{cloud->count} * r->pair_factor); st != SKM_OK) return st;
+  r->world = std::move(w);
+  r->has_world = true;
+  r->completed_world_frame = false;
+  return r->check(hipStreamSynchronize(r->stream), "upload");
+}
+
+int skm_selects_lod_on_gpu(const skm_renderer*) { return 0; }
+
+// MetalWorld::slab (MetalWorld.mm:59-79).
+skm_status skm_create_slab(skm_renderer* r, uint32_t capacity, int sh_degree) {
+  if (r == nullptr) return SKM_ERR_INVALID;
+  if (capacity == 0) return r->fail(SKM_ERR_INVALID, "empty slab");
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  WorldBuffers w;
+  if (auto st = r->check(w.allocate(capacity, std::clamp(sh_degree, 0, 3)), "slab"); st != SKM_OK) return st;
+  if (auto st = r->check(hipMemsetAsync(w.mem.ptr, 0, w.mem.bytes, r->stream), "slab"); st != SKM_OK) return st;
+  w.count = capacity;
+  w.slab = true;
+  if (auto st = r->ensure_capacity(capacity); st != SKM_OK) return st;
+  if (auto st = r->ensure_pairs(uint64_t{capacity} * r->pair_factor); st != SKM_OK) return st;
+  r->world = std::move(w);
+  r->has_world = true;
+  r->completed_world_frame = false;
+  return r->check(hipStreamSynchronize(r->stream), "slab");
+}
+
+// MetalWorld::uploadTile (MetalWorld.mm:81-100).
+skm_status skm_upload_tile(skm_renderer* r, uint32_t offset, const skm_cloud_view* cloud) {
+  if (r == nullptr) return SKM_ERR_INVALID;
+  if (!r->has_world || !r->world.slab) return r->fail(SKM_ERR_STATE, "no slab");
+  if (!valid_cloud(cloud)) return r->fail(SKM_ERR_INVALID, "incomplete cloud");
+  if (offset > r->world.count || cloud->count > r->world.count - offset)
+    return r->fail(SKM_ERR_INVALID, "tile [%u, +%u) outside the slab of %u", offset, cloud->count, r->world.count);
+  if (cloud->count == 0) return SKM_OK;
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  const int deg = r->world.sh_degree;
+  const bool carries = deg > 0 && cloud->sh_degree >= deg && cloud->sh != nullptr;
+  skm_cloud_view c = *cloud;
+  if (!carries) c.sh = nullptr;
+  return r->upload_range(r->world, offset, &c, carries ? cloud->sh_degree : 0);
+}
+
+int skm_world(const skm_renderer* r, skm_world_info* out) {
+  if (r == nullptr || !r->has_world) return 0;
+  if (out) {
+    out->count = r->world.count;
+    out->sh_degree = r->world.sh_degree;
+  }
+  return 1;
+}
+
+int skm_sorts_on_gpu(const skm_renderer*) { return 1; }
+
+skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
+  if (r == nullptr || f == nullptr) return SKM_ERR_INVALID;
+  if (r->gpu_failed) return r->fail(SKM_ERR_DEVICE, "GPU failed earlier");
+  if (r->surface_w == 0) return r->fail(SKM_ERR_STATE, "no draw extent");
+  if (f->order_source != SKM_ORDER_GPU || f->order != nullptr)
+    return r->fail(SKM_ERR_UNSUPPORTED, "this renderer orders on the GPU only");
+  if (f->label_styles != nullptr) return r->fail(SKM_ERR_UNSUPPORTED, "part labels are not supported");
+  if (f->range_count > skm::kMaxRanges || (f->range_count > 0 && f->ranges == nullptr))
+    return r->fail(SKM_ERR_INVALID, "invalid ranges");
+  // Validate before writing any slot input (MetalVisibilityPass.mm:155-162).
+  const uint32_t source = r->has_world ? r->world.count : 0;
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < f->range_count; ++i) {
+    const skm_range& g = f->ranges[i];
+    if (g.offset > source || g.count > source - g.offset || total + g.count > r->capacity)
+      return r->fail(SKM_ERR_INVALID, "range %u [%u, +%u) invalid for %u splats", i, g.offset, g.count, source);
+    total += g.count;
+  }
+  if (auto st = r->check(hipSetDevice(r->device), "device"); st != SKM_OK) return st;
+  const uint32_t W = r->draw_w(), H = r->draw_h();
+  const uint32_t tiles_x = (W + r->tile - 1) / r->tile, tiles_y = (H + r->tile - 1) / r->tile;
+  const uint32_t slot_id = static_cast<uint32_t>(r->submitted % kSlots);
+  Slot& s = r->slots[slot_id];
+  if (auto st = r->harvest(s); st != SKM_OK) return st;
+  if (r->grow_pairs) {
+    if (auto st = r->drain(); st != SKM_OK) return st;
+    const uint64_t want = std::max<uint64_t>(r->pairs_needed + r->pairs_needed / 4, uint64_t{r->pair_capacity} * 2);
+    if (auto st = r->ensure_pairs(want); st != SKM_OK) return st;
+    r->grow_pairs = false;
+  }
+  if (auto st = r->ensure_tiles(tiles_x * tiles_y); st != SKM_OK) return st;
+  if (s.width != W || s.height != H) {
+    DeviceBuffer img;
+    if (auto st = r->check(img.allocate(size_t{W} * H * 16u), "image"); st != SKM_OK) return st;
+    s.image = std::move(img);
+    s.width = W;
+    s.height = H;
+  }
+  if (auto st = r->maybe_clear_status(); st != SKM_OK) return st;
+
+  // MetalFrameCompute::updateCamera (MetalFrameCompute.mm:141-164).
+  Camera& cam = *s.cam_host.as<Camera>();
+  std::memset(&cam, 0, sizeof(cam));
+  std::memcpy(cam.view, f->view, 64);
+  std::memcpy(cam.proj, f->proj, 64);
+  cam.screen_size[0] = static_cast<float>(W);
+  cam.screen_size[1] = static_cast<float>(H);
+  cam.focal[0] = cam.screen_size[0] * f->proj[0] / 2;
+  cam.focal[1] = cam.screen_size[1] * f->proj[5] / 2;
+  cam.tan_half_fov[0] = 1 / f->proj[0];
+  cam.tan_half_fov[1] = 1 / f->proj[5];
+  cam.output_linear = r->linear ? 1u : 0u;
+  cam.min_pixel_radius = r->policy.subpixel;
+  cam.camera_position[0] = f->camera_position[0];
+  cam.camera_position[1] = f->camera_position[1];
+  cam.camera_position[2] = f->camera_position[2];
+  cam.camera_position[3] = 1;
+  std::memcpy(cam.reveal, &f->reveal, 16);
+  std::memcpy(cam.reveal_tint, f->reveal.tint, 16);
+
+  uint32_t* rh = s.ranges_host.as<uint32_t>();
+  uint32_t* sh = s.starts_host.as<uint32_t>();
+  sh[0] = 0;
+  for (uint32_t i = 0; i < f->range_count; ++i) {
+    rh[2 * i] = f->ranges[i].offset;
+    rh[2 * i + 1] = f->ranges[i].count;
+    sh[i + 1] = sh[i] + f->ranges[i].count;
+  }
+  hipError_t e = hipMemcpyAsync(s.cam.ptr, &cam, sizeof(Camera), hipMemcpyHostToDevice, r->stream);
+  if (e == hipSuccess && f->range_count > 0) {
+    e = hipMemcpyAsync(s.ranges.ptr, rh, size_t{f->range_count} * 8, hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(s.starts.ptr, sh, size_t{f->range_count + 1} * 4, hipMemcpyHostToDevice, r->stream);
+  }
+  if (auto st = r->check(e, "frame inputs"); st != SKM_OK) return st;
+
+  const bool world = r->has_world;
+  skm::FrameArgs a{};
+  a.cam = s.cam.as<Camera>();
+  a.world = world ? r->world.view() : skm::World{};
+  a.ranges = s.ranges.as<uint32_t>();
+  a.range_starts = s.starts.as<uint32_t>();
+  a.range_count = world ? f->range_count : 0;
+  a.total = world ? static_cast<uint32_t>(total) : 0;
+  a.active = nullptr;
+  a.active_count = nullptr;
+  a.key16 = r->policy.key16 ? 1 : 0;
+  a.width = W;
+  a.height = H;
+  a.tile = r->tile;
+  a.tiles_x = tiles_x;
+  a.tiles_y = tiles_y;
+  for (int c = 0; c < 3; ++c) a.background[c] = static_cast<float>(r->policy.background[c]);
+  a.ctrl = r->ctrl.as<Ctrl>();
+  a.cull_keys = r->cull_keys.as<uint32_t>();
+  for (int k = 0; k < 2; ++k) {
+    a.keys[k] = r->keys[k].as<uint32_t>();
+    a.order[k] = r->order[k].as<uint32_t>();
+    a.pair_keys[k] = r->pair_keys[k].as<uint32_t>();
+    a.pair_vals[k] = r->pair_vals[k].as<uint32_t>();
+  }
+  a.projected = r->projected.as<uint32_t>();
+  a.scan_status = r->scan_status.as<uint64_t>();
+  a.sort_status = r->sort_status.as<uint64_t>();
+  a.hist = r->hist.as<uint32_t>();
+  a.pair_capacity = r->pair_capacity;
+  a.tile_ranges = r->tile_ranges.as<uint32_t>();
+  a.image = s.image.as<float>();
+  a.capacity = r->capacity;
+  s.depth_passes = r->policy.key16 ? 2 : 4;
+  s.tile_passes = static_cast<int>(digit_passes(tiles_x * tiles_y - 1));
+  e = skm::launch_frame(a, s.depth_passes, s.tile_passes, s.marks, r->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(s.ctrl_host.ptr, r->ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost, r->stream);
+  if (e == hipSuccess) e = hipEventRecord(s.done, r->stream);
+  if (auto st = r->check(e, "frame launch"); st != SKM_OK) return st;
+  s.in_flight = true;
+  s.with_world = world;
+  r->last_slot = static_cast<int>(slot_id);
+  ++r->submitted;
+  return SKM_OK;
+}
+
+skm_status skm_finish(skm_renderer* r) {
+  if (r == nullptr) return SKM_ERR_INVALID;
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  if (r->last_dropped)
+    return r->fail(SKM_ERR_CAPACITY, "the last frame needed %llu tile pairs; capacity grows on the next draw",
+                   static_cast<unsigned long long>(r->pairs_needed));
+  return SKM_OK;
+}
+
+uint64_t skm_submitted_frames(const skm_renderer* r) { return r ? r->submitted : 0; }
+uint64_t skm_completed_frames(skm_renderer* r) {
+  if (r == nullptr) return 0;
+  if (r->gpu_failed) return 0;
+  for (Slot& s : r->slots)
+    if (s.in_flight && hipEventQuery(s.done) == hipSuccess) (void)r->harvest(s);
+  return r->completed;
+}
+double skm_last_gpu_millis(const skm_renderer* r) { return r ? r->stage_ms[4] : 0; }
+double skm_last_sort_millis(const skm_renderer* r) { return r ? r->stage_ms[1] + r->stage_ms[2] : 0; }
+double skm_last_cull_millis(const skm_renderer* r) { return r ? r->stage_ms[0] : 0; }
+uint32_t skm_last_draw_count(const skm_renderer* r) { return r ? r->last_draw_count : 0; }
+uint32_t skm_last_selected_count(const skm_renderer*) { return 0; }
+double skm_last_select_millis(const skm_renderer*) { return 0; }
+const char* skm_device_description(const skm_renderer* r) { return r ? r->description.c_str() : ""; }
+
+skm_status skm_read_stage_millis(skm_renderer* r, double* ms5) {
+  if (r == nullptr || ms5 == nullptr) return SKM_ERR_INVALID;
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  for (int k = 0; k < 5; ++k) ms5[k] = r->stage_ms[k];
+  return SKM_OK;
+}
+
+skm_status skm_read_frame_totals(skm_renderer* r, skm_frame_totals* out, int reset) {
+  if (r == nullptr || out == nullptr) return SKM_ERR_INVALID;
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  *out = r->totals;
+  if (reset) r->totals = skm_frame_totals{};
+  return SKM_OK;
+}
+
+skm_status skm_read_frame(skm_renderer* r, int format, void* dst, size_t dst_bytes) {
+  if (r == nullptr || dst == nullptr) return SKM_ERR_INVALID;
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  if (r->last_slot < 0) return r->fail(SKM_ERR_STATE, "no frame drawn");
+  Slot& s = r->slots[r->last_slot];
+  const size_t px = size_t{s.width} * s.height;
+  if (format == SKM_PIXELS_RGBA32F) {
+    if (dst_bytes < px * 16) return r->fail(SKM_ERR_INVALID, "destination too small");
+    return r->check(hipMemcpy(dst, s.image.ptr, px * 16, hipMemcpyDeviceToHost), "read frame");
+  }
+  if (format != SKM_PIXELS_BGRA8) return r->fail(SKM_ERR_INVALID, "unknown pixel format");
+  if (dst_bytes < px * 4) return r->fail(SKM_ERR_INVALID, "destination too small");
+  std::vector<float> rgba(px * 4);
+  if (auto st = r->check(hipMemcpy(rgba.data(), s.image.ptr, px * 16, hipMemcpyDeviceToHost), "read frame");
+      st != SKM_OK)
+    return st;
+  auto* out = static_cast<uint8_t*>(dst);
+  auto q = [](float v) { return static_cast<uint8_t>(std::lround(std::clamp(v, 0.0f, 1.0f) * 255.0f)); };
+  for (size_t i = 0; i < px; ++i) {
+    out[4 * i + 0] = q(rgba[4 * i + 2]);
+    out[4 * i + 1] = q(rgba[4 * i + 1]);
+    out[4 * i + 2] = q(rgba[4 * i + 0]);
+    out[4 * i + 3] = q(rgba[4 * i + 3]);
+  }
+  return SKM_OK;
+}
+
+skm_status skm_read_visible(skm_renderer* r, uint32_t* count, uint32_t* keys, uint32_t* projected,
+                            uint32_t capacity) {
+  if (r == nullptr || count == nullptr) return SKM_ERR_INVALID;
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  Ctrl c;
+  if (auto st = r->check(hipMemcpy(&c, r->ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost), "read"); st != SKM_OK)
+    return st;
+  *count = c.visible;
+  const uint32_t n = std::min(c.visible, capacity);
+  if (keys && n)
+    if (auto st = r->check(hipMemcpy(keys, r->cull_keys.ptr, size_t{n} * 4, hipMemcpyDeviceToHost), "read");
+        st != SKM_OK)
+      return st;
+  if (projected && n)
+    return r->check(hipMemcpy(projected, r->projected.ptr, size_t{n} * 32, hipMemcpyDeviceToHost), "read");
+  return SKM_OK;
+}
+
+skm_status skm_read_order(skm_renderer* r, uint32_t* order, uint32_t capacity) {
+  if (r == nullptr || order == nullptr) return SKM_ERR_INVALID;
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  if (r->last_slot < 0) return r->fail(SKM_ERR_STATE, "no frame drawn");
+  Ctrl c;
+  if (auto st = r->check(hipMemcpy(&c, r->ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost), "read"); st != SKM_OK)
+    return st;
+  const int passes = r->slots[r->last_slot].depth_passes;
+  const uint32_t n = std::min(c.visible, capacity);
+  if (n == 0) return SKM_OK;
+  return r->check(hipMemcpy(order, r->order[(passes - 1) & 1].ptr, size_t{n} * 4, hipMemcpyDeviceToHost), "read");
+}
+
+skm_status skm_read_tiles(skm_renderer* r, uint64_t* pairs, uint32_t* pair_tiles, uint32_t* pair_values,
+                          uint64_t capacity, uint32_t* ranges, uint32_t tile_capacity) {
+  if (r == nullptr || pairs == nullptr) return SKM_ERR_INVALID;
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  if (r->last_slot < 0) return r->fail(SKM_ERR_STATE, "no frame drawn");
+  Ctrl c;
+  if (auto st = r->check(hipMemcpy(&c, r->ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost), "read"); st != SKM_OK)
+    return st;
+  *pairs = c.pairs;
+  const Slot& s = r->slots[r->last_slot];
+  const int fin = s.tile_passes & 1;
+  const uint64_t n = std::min<uint64_t>(c.sort_pairs, capacity);
+  if (n > 0 && pair_tiles)
+    if (auto st = r->check(hipMemcpy(pair_tiles, r->pair_keys[fin].ptr, n * 4, hipMemcpyDeviceToHost), "read");
+        st != SKM_OK)
+      return st;
+  if (n > 0 && pair_values)
+    if (auto st = r->check(hipMemcpy(pair_values, r->pair_vals[fin].ptr, n * 4, hipMemcpyDeviceToHost), "read");
+        st != SKM_OK)
+      return st;
+  const uint32_t tiles = ((s.width + r->tile - 1) / r->tile) * ((s.height + r->tile - 1) / r->tile);
+  if (ranges && tile_capacity)
+    return r->check(hipMemcpy(ranges, r->tile_ranges.ptr, size_t{std::min(tiles, tile_capacity)} * 8,
+                              hipMemcpyDeviceToHost),
+                    "read");
+  return SKM_OK;
+}
+
+skm_status skm_argsort_u32_device(skm_renderer* r, void* keys_dev, uint32_t n, int key_bits,
+                                  void* order_dev, double* ms) {
+  if (r == nullptr || (n > 0 && (keys_dev == nullptr || order_dev == nullptr))) return SKM_ERR_INVALID;
+  if (key_bits != 16 && key_bits != 32) return r->fail(SKM_ERR_INVALID, "key_bits must be 16 or 32");
+  if (auto st = r->check(hipSetDevice(r->device), "device"); st != SKM_OK) return st;
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  if (n > r->sort_capacity || r->sort_n.ptr == nullptr) {
+    DeviceBuffer k, v, c;
+    const size_t cap = std::max<size_t>(n, 1);
+    if (auto st = r->check(k.allocate(cap * 4), "sort keys"); st != SKM_OK) return st;
+    if (auto st = r->check(v.allocate(cap * 4), "sort values"); st != SKM_OK) return st;
+    if (auto st = r->check(c.allocate(256), "sort count"); st != SKM_OK) return st;
+    if (auto st = r->ensure_status(static_cast<uint32_t>(cap)); st != SKM_OK) return st;
+    r->sort_keys = std::move(k);
+    r->sort_vals = std::move(v);
+    r->sort_n = std::move(c);
+    r->sort_capacity = static_cast<uint32_t>(cap);
+  }
+  if (auto st = r->maybe_clear_status(); st != SKM_OK) return st;
+  const int passes = key_bits == 16 ? 2 : 4;
+  uint32_t* keys[2] = {static_cast<uint32_t*>(keys_dev), r->sort_keys.as<uint32_t>()};
+  // The order ends in vals[(passes - 1) & 1] = vals[1].
+  uint32_t* vals[2] = {r->sort_vals.as<uint32_t>(), static_cast<uint32_t*>(order_dev)};
+  hipEvent_t t0 = r->slots[0].marks[0], t1 = r->slots[0].marks[4];
+  hipError_t e = hipMemcpyAsync(r->sort_n.ptr, &n, 4, hipMemcpyHostToDevice, r->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  if (e == hipSuccess) e = hipEventRecord(t0, r->stream);
+  if (e == hipSuccess)
+    e = skm::launch_argsort(r->ctrl.as<Ctrl>(), keys, vals, r->sort_n.as<uint32_t>(), std::max(n, 1u), passes,
+                            r->hist.as<uint32_t>(), r->sort_status.as<uint64_t>(), r->stream);
+  if (e == hipSuccess) e = hipEventRecord(t1, r->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(t1);
+  if (auto st = r->check(e, "sort"); st != SKM_OK) return st;
+  float elapsed = 0;
+  (void)hipEventElapsedTime(&elapsed, t0, t1);
+  if (ms) *ms = elapsed;
+  return SKM_OK;
+}
+
+skm_status skm_argsort_u32(skm_renderer* r, const uint32_t* keys, uint32_t n, int key_bits, uint32_t* order) {
+  if (r == nullptr || (n > 0 && (keys == nullptr || order == nullptr))) return SKM_ERR_INVALID;
+  if (n == 0) return SKM_OK;
+  if (auto st = r->check(hipSetDevice(r->device), "device"); st != SKM_OK) return st;
+  DeviceBuffer k, o;
+  if (auto st = r->check(k.allocate(size_t{n} * 4), "keys"); st != SKM_OK) return st;
+  if (auto st = r->check(o.allocate(size_t{n} * 4), "order"); st != SKM_OK) return st;
+  if (auto st = r->check(hipMemcpy(k.ptr, keys, size_t{n} * 4, hipMemcpyHostToDevice), "keys"); st != SKM_OK)
+    return st;
+  if (auto st = skm_argsort_u32_device(r, k.ptr, n, key_bits, o.ptr, nullptr); st != SKM_OK) return st;
+  return r->check(hipMemcpy(order, o.ptr, size_t{n} * 4, hipMemcpyDeviceToHost), "order");
+}
+
+}  // extern "C"

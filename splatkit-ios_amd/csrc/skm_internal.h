@@ -1,0 +1,124 @@
+// Layouts shared by the HIP kernels (kernels.hip) and the host renderer (renderer.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace skm {
+
+// Camera uniform, byte-identical to CameraUniform (MetalShaderTypes.h:14-30) and the shader's
+// Camera (SplatTypes.metalh:7-22).
+struct Camera {
+  float view[16];
+  float proj[16];
+  float focal[2];
+  float tan_half_fov[2];
+  float screen_size[2];
+  uint32_t output_linear;
+  float min_pixel_radius;
+  float camera_position[4];
+  float reveal[4];
+  float reveal_tint[4];
+};
+static_assert(sizeof(Camera) == 208, "camera layout");
+
+// The resident world in HBM, structure of arrays: every field of the reference's 32-byte GpuSplat
+// record (GpuLayout.h:50-56) in its own array so a wave's 64 lanes read 256 contiguous bytes per
+// field, and the half-packed SH words (GpuLayout.cpp:92-113) word-major: word w of splat i at
+// sh[w * capacity + i].
+struct World {
+  const float* px;
+  const float* py;
+  const float* pz;
+  const uint32_t* rgba8;
+  const uint32_t* cov0;
+  const uint32_t* cov1;
+  const uint32_t* cov2;
+  const uint32_t* lod_alpha;
+  const uint32_t* sh;
+  uint32_t capacity;
+  int32_t sh_degree;
+};
+
+struct WorldOut {
+  float* px;
+  float* py;
+  float* pz;
+  uint32_t* rgba8;
+  uint32_t* cov0;
+  uint32_t* cov1;
+  uint32_t* cov2;
+  uint32_t* lod_alpha;
+  uint32_t* sh;
+  uint32_t capacity;
+  int32_t sh_degree;
+};
+
+// Device-side control block of one frame pipeline. The prologue kernel bumps `epoch` and clears
+// the rest; look-back status words carry a tag derived from the epoch, so the status arrays are
+// never cleared.
+struct Ctrl {
+  uint32_t epoch;
+  uint32_t visible;   // survivors of the cull
+  uint32_t pairs;     // tile pairs, saturated at 2^32 - 1
+  uint32_t overflow;  // nonzero when the pairs exceeded the pair capacity
+  uint32_t sort_pairs;  // pairs the tile sort orders: `pairs`, or 0 after an overflow
+  uint32_t reserved[3];
+  uint32_t ticket[16];
+  uint32_t lod[16];   // LOD selection counters
+};
+
+enum Ticket : uint32_t {
+  kTicketCull = 0,
+  kTicketBin = 1,
+  kTicketDepthSort = 2,  // + pass (4)
+  kTicketTileSort = 6,   // + pass (4)
+};
+
+constexpr uint32_t kThreads = 256;
+constexpr uint32_t kSortItems = 16;
+constexpr uint32_t kSortTile = kThreads * kSortItems;  // 4096 keys per onesweep block
+constexpr uint32_t kRadix = 256;
+constexpr uint32_t kMaxRanges = 65536;  // MetalVisibilityPass.h:72
+constexpr float kTransmittanceStop = 1.0f / 4096.0f;
+
+struct FrameArgs {
+  const Camera* cam;
+  World world;
+  const uint32_t* ranges;  // pairs (offset, count)
+  const uint32_t* range_starts;
+  uint32_t range_count;
+  uint32_t total;          // lanes for range mode
+  const uint32_t* active;  // LOD indices, or null
+  const uint32_t* active_count;
+  int key16;
+  uint32_t width, height, tile, tiles_x, tiles_y;
+  float background[3];
+  // scratch
+  Ctrl* ctrl;
+  uint32_t* cull_keys;  // depth keys in compaction order, as the cull wrote them
+  uint32_t* keys[2];    // sort ping-pong
+  uint32_t* order[2];
+  uint32_t* projected;  // 8 words per survivor
+  uint64_t* scan_status;
+  uint64_t* sort_status;
+  uint32_t* hist;       // [2][4][256]: depth digits, tile digits
+  uint32_t* pair_keys[2];
+  uint32_t* pair_vals[2];
+  uint32_t pair_capacity;
+  uint32_t* tile_ranges;  // 2 words per tile
+  float* image;           // RGBA float, top-down rows
+  uint32_t capacity;      // survivor capacity
+};
+
+// Launchers (kernels.hip).
+hipError_t launch_pack(const WorldOut& out, uint32_t offset, uint32_t count, const float* positions,
+                       const float* covariances, const float* colors, const float* alphas,
+                       const float* sh, int src_degree, hipStream_t stream);
+hipError_t launch_frame(const FrameArgs& a, int depth_passes, int tile_passes, hipEvent_t* marks,
+                        hipStream_t stream);
+hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev,
+                          uint32_t capacity, int passes, uint32_t* hist, uint64_t* status,
+                          hipStream_t stream);
+
+}  // namespace skm

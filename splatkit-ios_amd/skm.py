@@ -1,0 +1,455 @@
+"""Python binding of the render path's C ABI (include/skm.h) and of the scene library
+(include/skm_scene.h), over ctypes. The HIP library is required: loading fails loudly when it is
+missing, and there is no CPU fallback.
+
+The Renderer class follows the engine-side interface of the reference's SplatRenderer
+(packages/splatkit-engine/include/splatkit/rendering/SplatRenderer.h:36-160): upload_world,
+create_slab/upload_tile, draw(frame), finish, the stat getters and read_frame (captureNextFrame).
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libskm.so")
+SCENE_PATH = os.path.join(HERE, "libskm_scene.so")
+
+SKM_OK, SKM_ERR_INVALID, SKM_ERR_DEVICE, SKM_ERR_MEMORY = 0, -1, -2, -3
+SKM_ERR_STATE, SKM_ERR_UNSUPPORTED, SKM_ERR_CAPACITY = -4, -5, -6
+PIXELS_BGRA8, PIXELS_RGBA32F = 0, 1
+TILE = 16
+
+_fp = C.POINTER(C.c_float)
+_u32p = C.POINTER(C.c_uint32)
+
+
+class Config(C.Structure):
+    _fields_ = [("device", C.c_int32), ("tile_size", C.c_uint32), ("pair_factor", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class Range(C.Structure):
+    _fields_ = [("offset", C.c_uint32), ("count", C.c_uint32)]
+
+
+class Reveal(C.Structure):
+    _fields_ = [("level", C.c_float), ("band", C.c_float), ("tint_amount", C.c_float),
+                ("brightness_boost", C.c_float), ("tint", C.c_float * 4)]
+
+
+class Frame(C.Structure):
+    _fields_ = [("order_source", C.c_int32), ("order", _u32p), ("order_count", C.c_uint32),
+                ("draw_count", C.c_uint32), ("ranges", C.POINTER(Range)), ("range_count", C.c_uint32),
+                ("sh_degree", C.c_int32), ("view", C.c_float * 16), ("proj", C.c_float * 16),
+                ("camera_position", C.c_float * 3), ("label_styles", C.c_void_p), ("reveal", Reveal)]
+
+
+class WorldInfo(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("sh_degree", C.c_int32)]
+
+
+class CloudView(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("sh_degree", C.c_int32), ("positions", _fp),
+                ("covariances", _fp), ("colors", _fp), ("alphas", _fp), ("sh", _fp)]
+
+
+class RenderPolicy(C.Structure):
+    _fields_ = [("lod_error_pixels", C.c_float), ("lod_splat_limit", C.c_uint32),
+                ("alpha_threshold", C.c_float), ("subpixel_threshold", C.c_float),
+                ("enable_frustum_culling", C.c_uint8), ("enable_hiz_occlusion", C.c_uint8),
+                ("enable_early_termination", C.c_uint8), ("reserved0", C.c_uint8),
+                ("sort_depth", C.c_uint32), ("background_color", C.c_double * 3)]
+
+
+class PolicySupport(C.Structure):
+    _fields_ = [("lod_error_pixels", C.c_uint8), ("lod_splat_limit", C.c_uint8),
+                ("alpha_threshold", C.c_uint8), ("subpixel_threshold", C.c_uint8),
+                ("enable_frustum_culling", C.c_uint8), ("enable_hiz_occlusion", C.c_uint8),
+                ("enable_early_termination", C.c_uint8), ("sort_depth", C.c_uint8),
+                ("background_color", C.c_uint8), ("reserved", C.c_uint8 * 3),
+                ("min_lod_error_pixels", C.c_float), ("max_lod_error_pixels", C.c_float),
+                ("min_subpixel_threshold", C.c_float), ("max_subpixel_threshold", C.c_float),
+                ("fallback", RenderPolicy)]
+
+
+class DeviceCapabilities(C.Structure):
+    _fields_ = [("max_lod_capacity_splats", C.c_uint32), ("min_residency_capacity_splats", C.c_uint32),
+                ("max_residency_capacity_splats", C.c_uint32), ("supports_hiz_occlusion", C.c_uint8),
+                ("supports_subgroups", C.c_uint8), ("reserved", C.c_uint8 * 2),
+                ("max_texture_dimension", C.c_uint32), ("policy", PolicySupport)]
+
+
+class FrameTotals(C.Structure):
+    _fields_ = [("frames", C.c_uint64), ("stage_ms", C.c_double * 5), ("visible", C.c_uint64),
+                ("pairs", C.c_uint64)]
+
+
+class SceneSpec(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("sh_degree", C.c_int32), ("seed", C.c_uint64),
+                ("layout", C.c_int32), ("spread", C.c_float), ("area", C.c_float),
+                ("surface_share", C.c_float), ("surface_sigma", C.c_float),
+                ("log_scale_mean", C.c_float), ("log_scale_sigma", C.c_float),
+                ("opacity_sigma", C.c_float), ("dc_sigma", C.c_float), ("sh_sigma", C.c_float)]
+
+
+# name -> (restype, argtypes); every symbol include/skm.h declares.
+_R = C.c_void_p
+API = {
+    "skm_renderer_create": (C.c_int, [C.POINTER(Config), C.POINTER(_R)]),
+    "skm_renderer_destroy": (None, [_R]),
+    "skm_last_error": (C.c_char_p, [_R]),
+    "skm_set_render_scale": (None, [_R, C.c_float]),
+    "skm_render_scale": (C.c_float, [_R]),
+    "skm_set_linear_blending": (None, [_R, C.c_int]),
+    "skm_linear_blending": (C.c_int, [_R]),
+    "skm_set_vsync": (None, [_R, C.c_int]),
+    "skm_set_draw_extent": (C.c_int, [_R, C.c_uint32, C.c_uint32]),
+    "skm_get_device_capabilities": (C.c_int, [_R, C.POINTER(DeviceCapabilities)]),
+    "skm_apply_render_policy": (C.c_int, [_R, C.POINTER(RenderPolicy), C.c_char_p, C.c_size_t]),
+    "skm_available_memory": (C.c_int, [_R, C.POINTER(C.c_uint64)]),
+    "skm_ready": (C.c_int, [_R]),
+    "skm_gpu_failed": (C.c_int, [_R]),
+    "skm_has_completed_world_frame": (C.c_int, [_R]),
+    "skm_draw_extent": (None, [_R, _u32p, _u32p]),
+    "skm_generation": (C.c_uint32, [_R]),
+    "skm_surface_rebuild_pending": (C.c_int, [_R]),
+    "skm_upload_world": (C.c_int, [_R, C.POINTER(CloudView), C.c_int]),
+    "skm_selects_lod_on_gpu": (C.c_int, [_R]),
+    "skm_create_slab": (C.c_int, [_R, C.c_uint32, C.c_int]),
+    "skm_upload_tile": (C.c_int, [_R, C.c_uint32, C.POINTER(CloudView)]),
+    "skm_world": (C.c_int, [_R, C.POINTER(WorldInfo)]),
+    "skm_sorts_on_gpu": (C.c_int, [_R]),
+    "skm_draw": (C.c_int, [_R, C.POINTER(Frame)]),
+    "skm_finish": (C.c_int, [_R]),
+    "skm_submitted_frames": (C.c_uint64, [_R]),
+    "skm_completed_frames": (C.c_uint64, [_R]),
+    "skm_last_gpu_millis": (C.c_double, [_R]),
+    "skm_last_sort_millis": (C.c_double, [_R]),
+    "skm_last_cull_millis": (C.c_double, [_R]),
+    "skm_last_draw_count": (C.c_uint32, [_R]),
+    "skm_last_selected_count": (C.c_uint32, [_R]),
+    "skm_last_select_millis": (C.c_double, [_R]),
+    "skm_device_description": (C.c_char_p, [_R]),
+    "skm_read_frame": (C.c_int, [_R, C.c_int, C.c_void_p, C.c_size_t]),
+    "skm_read_visible": (C.c_int, [_R, _u32p, _u32p, _u32p, C.c_uint32]),
+    "skm_read_order": (C.c_int, [_R, _u32p, C.c_uint32]),
+    "skm_read_tiles": (C.c_int, [_R, C.POINTER(C.c_uint64), _u32p, _u32p, C.c_uint64, _u32p, C.c_uint32]),
+    "skm_read_stage_millis": (C.c_int, [_R, C.POINTER(C.c_double)]),
+    "skm_read_frame_totals": (C.c_int, [_R, C.POINTER(FrameTotals), C.c_int]),
+    "skm_argsort_u32": (C.c_int, [_R, _u32p, C.c_uint32, C.c_int, _u32p]),
+    "skm_argsort_u32_device": (C.c_int, [_R, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p,
+                                         C.POINTER(C.c_double)]),
+}
+
+_lib = None
+_scene = None
+
+
+def lib():
+    """The HIP library; raises when it is not built (no CPU fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        handle = C.CDLL(LIB_PATH)
+        for name, (res, args) in API.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def scene_lib():
+    global _scene
+    if _scene is None:
+        if not os.path.exists(SCENE_PATH):
+            raise RuntimeError(f"{SCENE_PATH} is missing: run __graft_entry__.build() first")
+        h = C.CDLL(SCENE_PATH)
+        h.skm_scene_fill.restype = C.c_int
+        h.skm_scene_fill.argtypes = [C.POINTER(SceneSpec), _fp, _fp, _fp, _fp, _fp, C.c_uint32]
+        h.skm_look_at.argtypes = [_fp, _fp, _fp, _fp]
+        h.skm_perspective.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, _fp]
+        _scene = h
+    return _scene
+
+
+def fptr(a):
+    return a.ctypes.data_as(_fp) if a is not None else None
+
+
+def uptr(a):
+    return a.ctypes.data_as(_u32p) if a is not None else None
+
+
+# ---------------------------------------------------------------------------------------------
+# Scenes and cameras.
+
+def sh_floats(degree):
+    return 3 * ((degree + 1) ** 2 - 1)
+
+
+def make_cloud(count, sh_degree=0, seed=0x5EED0000, layout=0, spread=2.0, area=10.0, surface_share=0.7,
+               surface_sigma=0.05, log_scale_mean=-4.5, log_scale_sigma=0.5, opacity_sigma=1.5,
+               dc_sigma=0.5, sh_sigma=0.05, threads=8):
+    """A seeded synthetic SplatCloud as a dict of float32 arrays (see include/skm_scene.h)."""
+    spec = SceneSpec(count, sh_degree, seed, layout, spread, area, surface_share, surface_sigma,
+                     log_scale_mean, log_scale_sigma, opacity_sigma, dc_sigma, sh_sigma)
+    cloud = {
+        "positions": np.empty((count, 3), np.float32),
+        "covariances": np.empty((count, 6), np.float32),
+        "colors": np.empty((count, 3), np.float32),
+        "alphas": np.empty(count, np.float32),
+        "sh": np.empty((count, sh_floats(sh_degree)), np.float32) if sh_degree > 0 else None,
+        "sh_degree": sh_degree,
+    }
+    rc = scene_lib().skm_scene_fill(C.byref(spec), fptr(cloud["positions"]), fptr(cloud["covariances"]),
+                                    fptr(cloud["colors"]), fptr(cloud["alphas"]), fptr(cloud["sh"]), threads)
+    if rc != 0:
+        raise ValueError("invalid scene spec")
+    return cloud
+
+
+def empty_cloud(count, sh_degree=0):
+    return {
+        "positions": np.zeros((count, 3), np.float32), "covariances": np.zeros((count, 6), np.float32),
+        "colors": np.zeros((count, 3), np.float32), "alphas": np.zeros(count, np.float32),
+        "sh": np.zeros((count, sh_floats(sh_degree)), np.float32) if sh_degree > 0 else None,
+        "sh_degree": sh_degree,
+    }
+
+
+def look_at(eye, target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0)):
+    out = np.zeros(16, np.float32)
+    e, t, u = (np.asarray(v, np.float32) for v in (eye, target, up))
+    scene_lib().skm_look_at(fptr(e), fptr(t), fptr(u), fptr(out))
+    return out
+
+
+def perspective(fovy, aspect, near, far):
+    out = np.zeros(16, np.float32)
+    scene_lib().skm_perspective(fovy, aspect, near, far, fptr(out))
+    return out
+
+
+def orbit_camera(frame, width, height, radius=8.0, height_m=1.5, step_deg=6.0, fovy=math.pi / 3,
+                 near=0.2, far=100.0):
+    a = math.radians(step_deg * frame)
+    eye = (radius * math.sin(a), height_m, radius * math.cos(a))
+    return look_at(eye), perspective(fovy, width / height, near, far), np.asarray(eye, np.float32)
+
+
+# The workloads (seeded; scene k uses seed 0x5EED0000 + k).
+def scene_c0(count=100_000):
+    return make_cloud(count, 0, seed=0x5EED0000, layout=0, spread=2.0)
+
+
+def scene_c1(count=3_000_000, sh_degree=3):
+    return make_cloud(count, sh_degree, seed=0x5EED0001, layout=1, spread=5.0, area=10.0)
+
+
+def camera_c0(width=1280, height=720):
+    return look_at((0.0, 0.0, 6.0)), perspective(math.pi / 3, width / height, 0.2, 100.0), \
+        np.array([0.0, 0.0, 6.0], np.float32)
+
+
+# ---------------------------------------------------------------------------------------------
+
+class SkmError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"skm status {status}: {message}")
+        self.status = status
+
+
+class Renderer:
+    def __init__(self, device=0, pair_factor=0):
+        self._lib = lib()
+        self._h = _R()
+        cfg = Config(device, TILE, pair_factor, 0)
+        rc = self._lib.skm_renderer_create(C.byref(cfg), C.byref(self._h))
+        if rc != SKM_OK:
+            raise SkmError(rc, "renderer creation failed (no HIP device?)")
+        self.world_count = 0
+
+    def close(self):
+        if self._h:
+            self._lib.skm_renderer_destroy(self._h)
+            self._h = _R()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != SKM_OK:
+            raise SkmError(rc, self.error())
+        return rc
+
+    def error(self):
+        return (self._lib.skm_last_error(self._h) or b"").decode()
+
+    # -- configuration
+    def set_extent(self, width, height):
+        self._check(self._lib.skm_set_draw_extent(self._h, width, height))
+
+    def extent(self):
+        w, h = C.c_uint32(), C.c_uint32()
+        self._lib.skm_draw_extent(self._h, C.byref(w), C.byref(h))
+        return w.value, h.value
+
+    def apply_policy(self, sort_depth=32, subpixel=1.0, background=(0.05, 0.05, 0.08), lod_error=1.0,
+                     lod_limit=0):
+        p = RenderPolicy(lod_error, lod_limit, 1.0 / 255.0, subpixel, 1, 0, 1, 0, sort_depth,
+                         (C.c_double * 3)(*background))
+        reason = C.create_string_buffer(256)
+        self._check(self._lib.skm_apply_render_policy(self._h, C.byref(p), reason, 256))
+
+    def capabilities(self):
+        caps = DeviceCapabilities()
+        self._check(self._lib.skm_get_device_capabilities(self._h, C.byref(caps)))
+        return caps
+
+    def description(self):
+        return self._lib.skm_device_description(self._h).decode()
+
+    # -- worlds
+    @staticmethod
+    def _view(cloud):
+        n = len(cloud["alphas"])
+        keep = {k: np.ascontiguousarray(cloud[k], np.float32) for k in ("positions", "covariances", "colors", "alphas")}
+        sh = cloud.get("sh")
+        keep["sh"] = np.ascontiguousarray(sh, np.float32) if sh is not None else None
+        view = CloudView(n, cloud.get("sh_degree", 0) if sh is not None else 0, fptr(keep["positions"]),
+                         fptr(keep["covariances"]), fptr(keep["colors"]), fptr(keep["alphas"]), fptr(keep["sh"]))
+        return view, keep
+
+    def upload_world(self, cloud, max_sh_degree=3):
+        view, keep = self._view(cloud)
+        self._check(self._lib.skm_upload_world(self._h, C.byref(view), max_sh_degree))
+        self.world_count = view.count
+        del keep
+
+    def create_slab(self, capacity, sh_degree):
+        self._check(self._lib.skm_create_slab(self._h, capacity, sh_degree))
+        self.world_count = capacity
+
+    def upload_tile(self, offset, cloud):
+        view, keep = self._view(cloud)
+        self._check(self._lib.skm_upload_tile(self._h, offset, C.byref(view)))
+        del keep
+
+    def world(self):
+        info = WorldInfo()
+        return (info.count, info.sh_degree) if self._lib.skm_world(self._h, C.byref(info)) else None
+
+    # -- frames
+    def make_frame(self, view, proj, cam_pos, ranges=None, sh_degree=3, reveal=None):
+        if ranges is None:
+            ranges = [(0, self.world_count)]
+        rarr = (Range * max(1, len(ranges)))(*[Range(o, c) for o, c in ranges])
+        f = Frame()
+        f.order_source = 1
+        f.ranges = C.cast(rarr, C.POINTER(Range)) if ranges else None
+        f.range_count = len(ranges)
+        f.sh_degree = sh_degree
+        f.view[:] = [float(x) for x in np.asarray(view, np.float32)]
+        f.proj[:] = [float(x) for x in np.asarray(proj, np.float32)]
+        f.camera_position[:] = [float(x) for x in np.asarray(cam_pos, np.float32)[:3]]
+        if reveal is None:
+            f.reveal = Reveal(math.inf, 0.0, 0.0, 0.0, (C.c_float * 4)(1, 1, 1, 0))
+        else:
+            f.reveal = Reveal(*reveal[:4], (C.c_float * 4)(*reveal[4:8]))
+        f._ranges_keep = rarr
+        return f
+
+    def draw(self, frame):
+        return self._check(self._lib.skm_draw(self._h, C.byref(frame)))
+
+    def render(self, view, proj, cam_pos, **kw):
+        self.draw(self.make_frame(view, proj, cam_pos, **kw))
+        self.finish()
+
+    def finish(self):
+        return self._check(self._lib.skm_finish(self._h))
+
+    def stage_millis(self):
+        ms = (C.c_double * 5)()
+        self._check(self._lib.skm_read_stage_millis(self._h, ms))
+        return list(ms)
+
+    def frame_totals(self, reset=True):
+        t = FrameTotals()
+        self._check(self._lib.skm_read_frame_totals(self._h, C.byref(t), 1 if reset else 0))
+        return {"frames": t.frames, "stage_ms": list(t.stage_ms), "visible": t.visible, "pairs": t.pairs}
+
+    def read_frame(self, fmt=PIXELS_RGBA32F):
+        w, h = self.extent()
+        if fmt == PIXELS_RGBA32F:
+            out = np.empty((h, w, 4), np.float32)
+        else:
+            out = np.empty((h, w, 4), np.uint8)
+        self._check(self._lib.skm_read_frame(self._h, fmt, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def read_visible(self, capacity=None):
+        if capacity is None:
+            capacity = max(1, self.world_count)
+        count = C.c_uint32()
+        keys = np.empty(capacity, np.uint32)
+        proj = np.empty((capacity, 8), np.uint32)
+        self._check(self._lib.skm_read_visible(self._h, C.byref(count), uptr(keys), uptr(proj), capacity))
+        n = min(count.value, capacity)
+        return count.value, keys[:n].copy(), proj[:n].copy()
+
+    def read_order(self, n):
+        order = np.empty(max(1, n), np.uint32)
+        self._check(self._lib.skm_read_order(self._h, uptr(order), n))
+        return order[:n].copy()
+
+    def read_tiles(self, pair_capacity=None, tiles=None):
+        """Tile pairs and ranges of the last frame; capacities default to what the frame holds."""
+        pairs = C.c_uint64()
+        if tiles is None:
+            w, h = self.extent()
+            tiles = ((w + TILE - 1) // TILE) * ((h + TILE - 1) // TILE)
+        if pair_capacity is None:
+            self._check(self._lib.skm_read_tiles(self._h, C.byref(pairs), None, None, 0, None, 0))
+            pair_capacity = int(pairs.value)
+        pt = np.empty(max(1, pair_capacity), np.uint32)
+        pv = np.empty(max(1, pair_capacity), np.uint32)
+        ranges = np.empty((max(1, tiles), 2), np.uint32)
+        self._check(self._lib.skm_read_tiles(self._h, C.byref(pairs), uptr(pt), uptr(pv), pair_capacity,
+                                             uptr(ranges), tiles))
+        n = min(pairs.value, pair_capacity)
+        return pairs.value, pt[:n].copy(), pv[:n].copy(), ranges[:tiles].copy()
+
+    def argsort(self, keys, key_bits=32):
+        keys = np.ascontiguousarray(keys, np.uint32)
+        order = np.empty(max(1, len(keys)), np.uint32)
+        self._check(self._lib.skm_argsort_u32(self._h, uptr(keys), len(keys), key_bits, uptr(order)))
+        return order[:len(keys)]
+
+    def argsort_device(self, keys_ptr, n, key_bits, order_ptr):
+        ms = C.c_double()
+        self._check(self._lib.skm_argsort_u32_device(self._h, C.c_void_p(keys_ptr), n, key_bits,
+                                                     C.c_void_p(order_ptr), C.byref(ms)))
+        return ms.value
+
+    def stats(self):
+        L, h = self._lib, self._h
+        return {
+            "submitted": L.skm_submitted_frames(h), "completed": L.skm_completed_frames(h),
+            "gpu_ms": L.skm_last_gpu_millis(h), "sort_ms": L.skm_last_sort_millis(h),
+            "cull_ms": L.skm_last_cull_millis(h), "draw_count": L.skm_last_draw_count(h),
+            "gpu_failed": L.skm_gpu_failed(h),
+        }
