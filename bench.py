@@ -103,7 +103,7 @@ def main():
     P = tot["pairs"] / n_frames
     T = ((WIDTH + 15) // 16) * ((HEIGHT + 15) // 16)
     names = ["cull", "depth_sort", "bin_tile_sort_ranges", "raster"]
-    ms = {n: tot["stage_ms"][k] / n_frames for k, n in enumerate(names)}
+    ms = {n: tot["stage_ms"][k + 1] / n_frames for k, n in enumerate(names)}
     by = stage_bytes(N_SPLATS, V, P, T, 4, 2)
     stages = {}
     for n in names:
@@ -143,7 +143,7 @@ def main():
                    "tile": 16, "parallelism": "single GPU"},
         "msplats_per_s": round(N_SPLATS * fps / 1e6, 1),
         "visible_per_frame": int(V), "pairs_per_frame": int(P),
-        "gpu_ms_per_frame": round(tot["stage_ms"][4] / n_frames, 4),
+        "gpu_ms_per_frame": round(tot["stage_ms"][5] / n_frames, 4),
         "stages": stages, "roofline": roofline, "cpu_baseline": cpu,
         "device": r.description(), "scene_generation_s": round(gen_s, 2),
     }

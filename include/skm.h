@@ -22,6 +22,7 @@ extern "C" {
 #endif
 
 typedef struct skm_renderer skm_renderer;
+struct skm_lod_tree_view; /* include/skm_lod.h */
 
 typedef enum skm_status {
   SKM_OK = 0,
@@ -174,6 +175,11 @@ int skm_surface_rebuild_pending(const skm_renderer* r);
 /* ---- worlds (SplatRenderer.h:87-107) ---- */
 skm_status skm_upload_world(skm_renderer* r, const skm_cloud_view* cloud, int max_sh_degree);
 int skm_selects_lod_on_gpu(const skm_renderer* r);
+/* uploadLodWorld (SplatRenderer.h:90): the tree's nodes become the world; every frame selects a cut
+ * of at most `budget` nodes on the GPU (0 = the leaf count) at the policy's screen-space error,
+ * adapted like LodThreshold (LodThreshold.h:27-36), and draws only that cut. */
+skm_status skm_upload_lod_world(skm_renderer* r, const struct skm_lod_tree_view* tree, int max_sh_degree,
+                                uint32_t budget);
 skm_status skm_create_slab(skm_renderer* r, uint32_t capacity, int sh_degree);
 skm_status skm_upload_tile(skm_renderer* r, uint32_t offset, const skm_cloud_view* cloud);
 /* Returns 1 and fills *out when a world is resident. */
@@ -206,14 +212,19 @@ skm_status skm_read_order(skm_renderer* r, uint32_t* order, uint32_t capacity);
 skm_status skm_read_tiles(skm_renderer* r, uint64_t* pairs, uint32_t* pair_tiles,
                           uint32_t* pair_values, uint64_t capacity, uint32_t* ranges,
                           uint32_t tile_capacity);
-/* GPU milliseconds of the last completed frame: cull, depth sort, bin+tile sort+ranges, raster, total. */
-skm_status skm_read_stage_millis(skm_renderer* r, double* ms5);
+/* GPU milliseconds of the last completed frame: LOD cut, cull, depth sort, bin+tile sort+ranges,
+ * raster, total. */
+skm_status skm_read_stage_millis(skm_renderer* r, double* ms6);
+/* The last completed frame's LOD cut: node indices in emission order and
+ * stats6 = count, final frontier, packets, accepted extra, denied refinements, evaluated clusters. */
+skm_status skm_read_lod_cut(skm_renderer* r, uint32_t* count, uint32_t* indices, uint32_t capacity,
+                            uint32_t* stats6);
 
 /* Sums over the frames completed since the last reset: GPU milliseconds per stage (as above),
  * survivors and tile pairs. For throughput reports over many frames. */
 typedef struct skm_frame_totals {
   uint64_t frames;
-  double stage_ms[5];
+  double stage_ms[6];
   uint64_t visible;
   uint64_t pairs;
 } skm_frame_totals;
@@ -226,6 +237,14 @@ skm_status skm_argsort_u32(skm_renderer* r, const uint32_t* keys, uint32_t n, in
 /* Same on device memory already resident; *ms gets the GPU time of the sort. */
 skm_status skm_argsort_u32_device(skm_renderer* r, void* keys_dev, uint32_t n, int key_bits,
                                   void* order_dev, double* ms);
+
+/* ---- the LOD cut on its own (MetalLodSelection, MetalLodSelection.h:13-48) ---- */
+/* Uploads the tree's clusters and leaf references, selects one cut for this camera and reads it
+ * back: capacity is the most nodes the cut may hold, pixel_limit 0 refines every cluster. */
+skm_status skm_lod_select(skm_renderer* r, const struct skm_lod_tree_view* tree, const float* view16,
+                          const float* proj16, uint32_t width, uint32_t height, uint32_t capacity,
+                          float pixel_limit, float color_weight, int cull, uint32_t* indices,
+                          uint32_t* stats6);
 
 #ifdef __cplusplus
 }

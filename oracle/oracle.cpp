@@ -527,9 +527,133 @@ void raster_tiles(const uint32_t* projected, const uint32_t* pair_values, const 
   });
 }
 
+// A LodCluster (LodTree.h:28-35) as 16 words.
+struct Cluster {
+  float center[3], radius, extent[3], error, color_variance, opacity;
+  uint32_t node, child_start, child_count, leaf_start, leaf_count, subtree_leaves;
+};
+static_assert(sizeof(Cluster) == 64, "cluster layout");
+
+// lodOutside (SplatLOD.metal:26-32): sphere against the frustum's side planes, with a guard band
+// of two pixels at the cluster's depth.
+bool lod_outside(const orc_camera& cam, const Cluster& c) {
+  float v[4];
+  mat_vec(cam.view, c.center[0], c.center[1], c.center[2], 1.0f, v);
+  const float z = -v[2], r = c.radius;
+  const float tx = cam.tan_half_fov[0], ty = cam.tan_half_fov[1];
+  const float guard = (2.0f * mx(z, 0.0f)) / mx(mn(cam.focal[0], cam.focal[1]), 1.0f);
+  return z + r <= 0.0f || std::fabs(v[0]) - z * tx > r * std::sqrt(1.0f + tx * tx) + guard ||
+         std::fabs(v[1]) - z * ty > r * std::sqrt(1.0f + ty * ty) + guard;
+}
+
+// lodErrorPixels (SplatLOD.metal:33-43).
+float lod_error_pixels(const orc_camera& cam, const Cluster& c, float color_weight) {
+  float v[4];
+  mat_vec(cam.view, c.center[0], c.center[1], c.center[2], 1.0f, v);
+  const float depth = mx(-v[2] - c.radius, 1e-4f);
+  const float perspective = std::sqrt(1.0f + (v[0] * v[0] + v[1] * v[1]) / (depth * depth));
+  const float appearance = mn(std::sqrt(c.color_variance), 1.0f);
+  const float error = c.error + c.radius * appearance;
+  return ((((error * mx(cam.focal[0], cam.focal[1])) / depth) * perspective) * mx(c.opacity, 0.1f)) *
+         (1.0f + color_weight * std::sqrt(c.color_variance));
+}
+
 }  // namespace
 
 extern "C" {
+
+int orc_lod_cluster(const orc_camera* cam, const uint32_t* cluster16, float color_weight, float* error_out) {
+  Cluster c;
+  std::memcpy(&c, cluster16, 64);
+  if (error_out != nullptr) *error_out = lod_error_pixels(*cam, c, color_weight);
+  return lod_outside(*cam, c) ? 1 : 0;
+}
+
+// Rounds of evaluate, budget, compact, commit and scatter (SplatLOD.metal:54-208) over the frontier
+// in order, then the packet emission (:210-220). The budget keeps every refinement of the longest
+// frontier prefix whose extra cost fits (:116-132) and denies the rest (:148-151).
+uint32_t orc_lod_select(const orc_camera* cam, const uint32_t* clusters16, uint32_t cluster_count,
+                        const uint32_t* leaves, uint32_t rounds, uint32_t capacity, float pixel_limit,
+                        float color_weight, int cull, uint32_t* indices, uint32_t* stats) {
+  constexpr uint32_t kSplit = 0x80000000u, kDrop = 0x40000000u;
+  const Cluster* nodes = reinterpret_cast<const Cluster*>(clusters16);
+  std::vector<uint32_t> frontier;
+  if (cluster_count > 0) frontier.push_back(0);
+  struct Packet {
+    uint32_t leaf_start, count, dest;
+  };
+  std::vector<Packet> packets;
+  uint32_t count = 0, limited = 0, evaluated = 0, accepted = 0;
+  for (uint32_t level = 0; level < rounds; ++level) {
+    const uint32_t active = static_cast<uint32_t>(frontier.size());
+    std::vector<uint32_t> flags(active, 0), extra(active, 0);
+    uint32_t drops = 0;
+    uint64_t total = 0;
+    for (uint32_t t = 0; t < active; ++t) {
+      const Cluster& node = nodes[frontier[t]];
+      if (cull && lod_outside(*cam, node)) {
+        flags[t] = kDrop;
+        ++drops;
+      } else if (node.child_count + node.leaf_count > 1 &&
+                 (pixel_limit == 0.0f || lod_error_pixels(*cam, node, color_weight) > pixel_limit)) {
+        flags[t] = kSplit;
+        extra[t] = node.child_count + node.leaf_count - 1;
+      } else if (node.child_count == 1 && node.leaf_count == 0) {
+        flags[t] = kSplit;
+      }
+      total += extra[t];
+    }
+    const uint32_t available = capacity - count - active + drops;
+    std::vector<uint64_t> end(active);
+    uint64_t run = 0;
+    for (uint32_t t = 0; t < active; ++t) end[t] = (run += extra[t]);
+    if (total <= available) {
+      accepted = static_cast<uint32_t>(total);
+    } else {
+      uint32_t low = 0;
+      while (low < active && end[low] <= available) ++low;
+      accepted = low == 0 ? 0u : static_cast<uint32_t>(end[low - 1]);
+    }
+    std::vector<uint32_t> next;
+    uint32_t emitted = 0;
+    for (uint32_t t = 0; t < active; ++t) {
+      uint32_t f = flags[t];
+      if ((f & kSplit) && extra[t] > 0 && end[t] > accepted) {
+        f = 0;
+        ++limited;
+      }
+      if (f & kDrop) continue;
+      const Cluster& node = nodes[frontier[t]];
+      const bool split = (f & kSplit) != 0;
+      if (split)
+        for (uint32_t k = 0; k < node.child_count; ++k) next.push_back(node.child_start + k);
+      const uint32_t splats = split ? node.leaf_count : 1u;
+      const uint32_t dest = count + emitted;
+      if (splats > 32) {
+        packets.push_back({node.leaf_start, splats, dest});
+      } else if (!split) {
+        indices[dest] = node.node;
+      } else {
+        for (uint32_t k = 0; k < splats; ++k) indices[dest + k] = leaves[node.leaf_start + k];
+      }
+      emitted += splats;
+    }
+    evaluated += active;
+    count += emitted;
+    frontier.swap(next);
+  }
+  for (const Packet& p : packets)
+    for (uint32_t k = 0; k < p.count; ++k) indices[p.dest + k] = leaves[p.leaf_start + k];
+  if (stats != nullptr) {
+    stats[0] = count;
+    stats[1] = static_cast<uint32_t>(frontier.size());
+    stats[2] = static_cast<uint32_t>(packets.size());
+    stats[3] = accepted;
+    stats[4] = limited;
+    stats[5] = evaluated;
+  }
+  return count;
+}
 
 double orc_log_d(double x) { return log_d(x); }
 double orc_exp_d(double x) { return exp_d(x); }

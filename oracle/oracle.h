@@ -97,6 +97,18 @@ int orc_render_frame(const orc_camera* cam, const uint32_t* splats, const uint32
                      uint32_t threads, float* out_rgba, double* stage_seconds,
                      uint32_t* visible_out, uint64_t* pairs_out);
 
+/* LOD tree cut (SplatLOD.metal:26-220, MetalLodSelection.mm:109-192) restated as sequential rounds.
+ * clusters: 16 words each (LodCluster, LodTree.h:28-35). Writes the selected node indices in the
+ * reference's emission order and returns their count; stats[6] = count, active, packets, accepted,
+ * limited (denied refinements), evaluated (interior clusters evaluated). */
+uint32_t orc_lod_select(const orc_camera* cam, const uint32_t* clusters, uint32_t cluster_count,
+                        const uint32_t* leaves, uint32_t rounds, uint32_t capacity, float pixel_limit,
+                        float color_weight, int cull, uint32_t* indices, uint32_t* stats);
+/* The fp32 decisions of one cluster: returns 1 when outside the view; error_out gets the
+ * screen-space error in pixels. */
+int orc_lod_cluster(const orc_camera* cam, const uint32_t* cluster16, float color_weight,
+                    float* error_out);
+
 /* Transcendentals built from IEEE basic operations only (argument reduction plus series), so
  * any two IEEE machines compute identical bits. */
 double orc_log_d(double x);

@@ -51,6 +51,9 @@ _API = {
     "orc_decide": (C.c_int, [_u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _fp]),
     "orc_render_frame": (C.c_int, [C.POINTER(Camera), _u32p, _u32p, C.c_int, C.c_uint32, C.c_int, C.c_uint32,
                                    _fp, C.c_uint32, _fp, C.POINTER(C.c_double), _u32p, C.POINTER(C.c_uint64)]),
+    "orc_lod_select": (C.c_uint32, [C.POINTER(Camera), _u32p, C.c_uint32, _u32p, C.c_uint32, C.c_uint32, C.c_float,
+                                    C.c_float, C.c_int, _u32p, _u32p]),
+    "orc_lod_cluster": (C.c_int, [C.POINTER(Camera), _u32p, C.c_float, _fp]),
     "orc_log_d": (C.c_double, [C.c_double]),
     "orc_exp_d": (C.c_double, [C.c_double]),
 }
@@ -211,13 +214,13 @@ def decide(record, width, height, px, py):
 
 
 def frame(cloud, view, proj, cam_pos, width, height, key16=False, min_pixel_radius=1.0,
-          background=(0.05, 0.05, 0.08), ranges=None, reveal=None, output_linear=0, image=True):
+          background=(0.05, 0.05, 0.08), ranges=None, reveal=None, output_linear=0, image=True, active=None):
     """The whole oracle frame: packing, cull, depth order, tile pairs, ranges and the FP64 image."""
     degree = cloud.get("sh_degree", 0) if cloud.get("sh") is not None else 0
     splats = pack_splats(cloud)
     sh = pack_sh(cloud, degree)
     cam = make_camera(view, proj, width, height, cam_pos, output_linear, min_pixel_radius, reveal)
-    count, keys, projected = cull(cam, splats, sh, degree, ranges=ranges, key16=key16)
+    count, keys, projected = cull(cam, splats, sh, degree, ranges=ranges, active=active, key16=key16)
     order = argsort(keys)
     pairs, pt, pv, tranges = bin_tiles(projected, order, width, height)
     out = {"count": count, "keys": keys, "projected": projected, "order": order, "pairs": pairs,
@@ -241,3 +244,14 @@ def render_frame_timed(cam, splats, sh, degree, key16, background, threads):
                            up(np.ascontiguousarray(sh, np.uint32)), degree, n, 1 if key16 else 0, TILE, fp(bg),
                            threads, fp(out), secs, C.byref(vis), C.byref(pairs))
     return list(secs), vis.value, pairs.value, out
+
+
+def lod_select(cam, clusters, leaves, rounds, capacity, pixel_limit=1.0, color_weight=4.0, cull=True):
+    """The oracle's LOD cut: (node indices, stats6)."""
+    clusters = np.ascontiguousarray(clusters, np.uint32).reshape(-1, 16)
+    leaves = np.ascontiguousarray(leaves, np.uint32)
+    out = np.zeros(max(1, capacity), np.uint32)
+    stats = np.zeros(6, np.uint32)
+    n = lib().orc_lod_select(C.byref(cam), up(clusters), len(clusters), up(leaves if len(leaves) else np.zeros(1, np.uint32)),
+                             rounds, capacity, pixel_limit, color_weight, 1 if cull else 0, up(out), up(stats))
+    return out[:n].copy(), stats

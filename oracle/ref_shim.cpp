@@ -2,6 +2,7 @@
 // /root/reference into oracle/_ref/libref.so by oracle/Makefile. Test infrastructure only: the
 // tests use it to pin the oracle's restatement (oracle.cpp) against the real reference code.
 // Nothing here re-implements the reference; each function only adapts types and calls it.
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -79,6 +80,16 @@ void* ref_lod_build(uint32_t n, int sh_degree, const float* positions, const flo
                     const float* colors, const float* alphas, const float* sh, uint32_t depth,
                     uint32_t cluster_leaves) {
   auto cloud = make_cloud(n, sh_degree, positions, covariances, colors, alphas, sh);
+  // The decoder fills bounds with the positions' box (SpzDecoder); buildLodTree reads them.
+  for (int c = 0; c < 3; ++c) {
+    cloud.bounds.min[c] = n ? positions[c] : 0.0f;
+    cloud.bounds.max[c] = n ? positions[c] : 0.0f;
+  }
+  for (uint32_t i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c) {
+      cloud.bounds.min[c] = std::min(cloud.bounds.min[c], positions[size_t{i} * 3 + c]);
+      cloud.bounds.max[c] = std::max(cloud.bounds.max[c], positions[size_t{i} * 3 + c]);
+    }
   splat::LodBuildOptions options;
   options.octreeDepth = depth;
   options.clusterLeaves = cluster_leaves;

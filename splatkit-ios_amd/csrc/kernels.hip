@@ -11,6 +11,7 @@
 //   k_ranges    [start, end) per tile from the tile-sorted pairs
 //   k_raster    16x16-tile front-to-back composite, LDS-staged batches, early termination
 #include "skm_internal.h"
+#include "skm_lod_internal.h"
 #include "skm_math.h"
 
 namespace skm {
@@ -807,10 +808,15 @@ hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], cons
   return hipGetLastError();
 }
 
-hipError_t launch_frame(const FrameArgs& a, int depth_passes, int tile_passes, hipEvent_t* marks,
-                        hipStream_t s) {
+hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes, int tile_passes,
+                        hipEvent_t* marks, hipStream_t s) {
   const uint32_t tiles = a.tiles_x * a.tiles_y;
   if (marks) (void)hipEventRecord(marks[0], s);
+  if (lod != nullptr) {
+    const hipError_t e = launch_lod(*lod, s);
+    if (e != hipSuccess) return e;
+  }
+  if (marks) (void)hipEventRecord(marks[1], s);
   hipLaunchKernelGGL(k_prologue, dim3(min(grid_for(tiles * 2u, 256), 256u)), dim3(256), 0, s, a.ctrl,
                      a.hist, 2u * 4u * 256u, a.tile_ranges, tiles * 2u);
   const uint32_t lanes = a.active != nullptr ? a.capacity : a.total;
@@ -821,7 +827,7 @@ hipError_t launch_frame(const FrameArgs& a, int depth_passes, int tile_passes, h
     case 2: launch_cull_deg<2>(a, cull_blocks, s); break;
     default: launch_cull_deg<3>(a, cull_blocks, s); break;
   }
-  if (marks) (void)hipEventRecord(marks[1], s);
+  if (marks) (void)hipEventRecord(marks[2], s);
   // Depth sort of the survivors: cull keys -> keys[0] -> keys[1] -> keys[0] ...
   const uint32_t* visible = &a.ctrl->visible;
   hipLaunchKernelGGL(k_hist, dim3(min(grid_for(a.capacity, 256), 1024u)), dim3(256), 0, s, a.cull_keys,
@@ -835,7 +841,7 @@ hipError_t launch_frame(const FrameArgs& a, int depth_passes, int tile_passes, h
                        kTicketDepthSort + p, 2u + p);
   }
   const uint32_t* order = a.order[(depth_passes - 1) & 1];
-  if (marks) (void)hipEventRecord(marks[2], s);
+  if (marks) (void)hipEventRecord(marks[3], s);
   hipLaunchKernelGGL(k_bin, dim3(grid_for(a.capacity, 256)), dim3(256), 0, s, a, order);
   const uint32_t* sort_n = &a.ctrl->sort_pairs;
   hipLaunchKernelGGL(k_pair_count, dim3(1), dim3(64), 0, s, a.ctrl);
@@ -851,10 +857,10 @@ hipError_t launch_frame(const FrameArgs& a, int depth_passes, int tile_passes, h
   const int fin = tile_passes & 1;
   hipLaunchKernelGGL(k_ranges, dim3(min(grid_for(a.pair_capacity, 256), 2048u)), dim3(256), 0, s,
                      a.pair_keys[fin], sort_n, a.tile_ranges);
-  if (marks) (void)hipEventRecord(marks[3], s);
+  if (marks) (void)hipEventRecord(marks[4], s);
   hipLaunchKernelGGL(k_raster, dim3(a.tiles_x, a.tiles_y), dim3(256), 0, s, a, a.pair_vals[fin],
                      a.tile_ranges);
-  if (marks) (void)hipEventRecord(marks[4], s);
+  if (marks) (void)hipEventRecord(marks[5], s);
   return hipGetLastError();
 }
 

@@ -1,0 +1,330 @@
+// GPU level-of-detail tree cut (SplatLOD.metal:26-220 with the round structure of
+// MetalLodSelection.mm:109-192), for gfx950.
+//
+// The reference runs seven dispatches per round with indirect arguments (evaluate, scan, budget,
+// compact, scan, commit, scatter). HIP has no indirect dispatch, and a decoupled look-back turns
+// each scan into part of the kernel that produces its inputs, so a round here is three kernels:
+//   k_lod_eval     per frontier cluster: drop / split / keep and the extra cost of a split, with the
+//                  frontier-order prefix of the extra cost (look-back);
+//   k_lod_budget   the largest prefix of refinements that fits the capacity (one thread, binary
+//                  search over the prefix ends, :116-132);
+//   k_lod_compact  denials, then the next frontier, leaf emission and packet offsets by look-back
+//                  scans, scattered in the same kernel, and the next round's state.
+// k_lod_emit copies the leaf packets (> 32 leaves) cooperatively after the last round. Grids are
+// sized for the frontier capacity; blocks past the round's frontier exit at once. Per-round state
+// lives in its own record (state[r] read, state[r + 1] written), so no kernel overwrites what a
+// concurrent block still reads.
+#include "skm_lod_internal.h"
+#include "skm_math.h"
+
+namespace skm {
+namespace {
+
+constexpr uint32_t kSplit = 0x80000000u;
+constexpr uint32_t kDrop = 0x40000000u;
+constexpr uint32_t kCost = 0x3fffffffu;
+constexpr uint32_t kFlagAggregate = 1;
+constexpr uint32_t kFlagPrefix = 2;
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+__device__ __forceinline__ uint32_t make_tag(uint32_t epoch, uint32_t launch) {
+  return (epoch * 128u + launch) & 0x3fffffffu;
+}
+__device__ __forceinline__ uint64_t status_word(uint32_t tag, uint32_t flag, uint32_t value) {
+  return (static_cast<uint64_t>(tag) << 34) | (static_cast<uint64_t>(flag) << 32) | value;
+}
+__device__ __forceinline__ void status_store(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t status_load(uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
+  const uint32_t s = a + b;
+  return s < a ? 0xffffffffu : s;
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// Exclusive scan over the 256-thread block; scratch holds 4 words.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* scratch, uint32_t* total) {
+  const uint32_t incl = wave_inclusive_scan(v);
+  const uint32_t wave = threadIdx.x >> 6;
+  if (lane_id() == 63) scratch[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < 4; ++w) {
+    const uint32_t c = scratch[w];
+    before += (w < wave) ? c : 0u;
+    all += c;
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
+// Decoupled look-back by one wave (see kernels.hip): the exclusive prefix of block `block`.
+__device__ uint32_t wave_lookback(uint64_t* status, uint32_t block, uint32_t tag) {
+  const uint32_t lane = lane_id();
+  uint64_t excl = 0;
+  int32_t k = static_cast<int32_t>(block) - 1;
+  while (k >= 0) {
+    const int32_t idx = k - static_cast<int32_t>(lane);
+    uint32_t flag = kFlagPrefix, value = 0;
+    bool ready = true;
+    if (idx >= 0) {
+      const uint64_t s = status_load(status + idx);
+      flag = static_cast<uint32_t>(s >> 32) & 3u;
+      value = static_cast<uint32_t>(s);
+      ready = (static_cast<uint32_t>(s >> 34) == tag) && flag != 0u;
+    }
+    if (__ballot(!ready) != 0ull) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    const uint64_t prefixes = __ballot(flag == kFlagPrefix);
+    if (prefixes != 0ull) {
+      const uint32_t first = static_cast<uint32_t>(__builtin_ctzll(prefixes));
+      excl += wave_sum64(lane <= first ? value : 0u);
+      break;
+    }
+    excl += wave_sum64(value);
+    k -= 64;
+  }
+  return excl > 0xffffffffull ? 0xffffffffu : static_cast<uint32_t>(excl);
+}
+
+// Publishes this block's aggregate, looks back, publishes its inclusive prefix; returns the
+// exclusive prefix in every lane of wave 0 (callers use lane 0's).
+__device__ uint32_t scan_across_blocks(uint64_t* status, uint32_t block, uint32_t tag, uint32_t aggregate) {
+  uint32_t prefix = 0;
+  if (block == 0) {
+    if (lane_id() == 0) status_store(status, status_word(tag, kFlagPrefix, aggregate));
+  } else {
+    if (lane_id() == 0) status_store(status + block, status_word(tag, kFlagAggregate, aggregate));
+    prefix = wave_lookback(status, block, tag);
+    if (lane_id() == 0) status_store(status + block, status_word(tag, kFlagPrefix, sat_add(prefix, aggregate)));
+  }
+  return prefix;
+}
+
+__device__ __forceinline__ void view_point(const Camera& cam, const LodCluster& c, float v[3]) {
+  const float* m = cam.view;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+    v[r] = ((m[r] * c.center[0] + m[4 + r] * c.center[1]) + m[8 + r] * c.center[2]) + m[12 + r] * 1.0f;
+}
+
+// lodOutside (SplatLOD.metal:26-32).
+__device__ __forceinline__ bool lod_outside(const Camera& cam, const LodCluster& c) {
+  float v[3];
+  view_point(cam, c, v);
+  const float z = -v[2], r = c.radius;
+  const float tx = cam.tan_half_fov[0], ty = cam.tan_half_fov[1];
+  const float guard = (2.0f * mx(z, 0.0f)) / mx(mn(cam.focal[0], cam.focal[1]), 1.0f);
+  return z + r <= 0.0f || fabsf(v[0]) - z * tx > r * sqrtf(1.0f + tx * tx) + guard ||
+         fabsf(v[1]) - z * ty > r * sqrtf(1.0f + ty * ty) + guard;
+}
+
+// lodErrorPixels (SplatLOD.metal:33-43).
+__device__ __forceinline__ float lod_error_pixels(const Camera& cam, const LodCluster& c, float color_weight) {
+  float v[3];
+  view_point(cam, c, v);
+  const float depth = mx(-v[2] - c.radius, 1e-4f);
+  const float perspective = sqrtf(1.0f + (v[0] * v[0] + v[1] * v[1]) / (depth * depth));
+  const float appearance = mn(sqrtf(c.color_variance), 1.0f);
+  const float error = c.error + c.radius * appearance;
+  return ((((error * mx(cam.focal[0], cam.focal[1])) / depth) * perspective) * mx(c.opacity, 0.1f)) *
+         (1.0f + color_weight * sqrtf(c.color_variance));
+}
+
+__global__ void k_lod_prologue(LodArgs a) {
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) {
+    a.ctrl->epoch = a.ctrl->epoch + 1u;
+    a.ctrl->limited = 0;
+    LodState s0{};
+    s0.active = a.cluster_count > 0 ? 1u : 0u;
+    a.state[0] = s0;
+    a.frontier[0][0] = 0;
+  }
+  for (uint32_t k = tid; k < 2u * kLodMaxRounds; k += blockDim.x) a.ctrl->ticket[k] = 0;
+  for (uint32_t r = 1 + tid; r <= a.rounds; r += blockDim.x) a.state[r] = LodState{};
+}
+
+__global__ __launch_bounds__(256) void k_lod_eval(LodArgs a, uint32_t round) {
+  __shared__ uint32_t s_ticket, s_prefix[2], s_scan[4];
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) s_ticket = atomicAdd(&a.ctrl->ticket[2 * round], 1u);
+  __syncthreads();
+  const uint32_t b = s_ticket;
+  const uint32_t active = a.state[round].active;
+  const uint32_t blocks = (active + 255u) / 256u;
+  if (b >= blocks) return;
+  const uint32_t t = b * 256u + tid;
+  uint32_t extra = 0, flags = 0, dropped = 0;
+  if (t < active) {
+    const LodCluster node = a.clusters[a.frontier[round & 1u][t]];
+    const Camera& cam = *a.cam;
+    if (a.cull && lod_outside(cam, node)) {
+      flags = kDrop;
+      dropped = 1;
+    } else if (node.child_count + node.leaf_count > 1 &&
+               (a.pixel_limit == 0.0f || lod_error_pixels(cam, node, a.color_weight) > a.pixel_limit)) {
+      flags = kSplit;
+      extra = node.child_count + node.leaf_count - 1;
+    } else if (node.child_count == 1 && node.leaf_count == 0) {
+      flags = kSplit;
+    }
+  }
+  uint32_t agg_extra, agg_drop;
+  const uint32_t ex = block_exclusive_scan(extra, s_scan, &agg_extra);
+  block_exclusive_scan(dropped, s_scan, &agg_drop);
+  if ((tid >> 6) == 0) {
+    const uint32_t tag = make_tag(a.ctrl->epoch, 2 * round);
+    const uint32_t p0 = scan_across_blocks(a.status[0], b, tag, agg_extra);
+    const uint32_t p1 = scan_across_blocks(a.status[1], b, tag, agg_drop);
+    if (lane_id() == 0) {
+      s_prefix[0] = p0;
+      s_prefix[1] = p1;
+    }
+  }
+  __syncthreads();
+  if (t < active) {
+    a.cost[2 * t] = flags | extra;
+    a.cost[2 * t + 1] = sat_add(s_prefix[0], ex + extra);  // inclusive prefix end
+  }
+  if (b == blocks - 1u && tid == 0) {
+    a.state[round].total_extra = sat_add(s_prefix[0], agg_extra);
+    a.state[round].drops = s_prefix[1] + agg_drop;
+  }
+}
+
+// budgetSplatLOD (SplatLOD.metal:116-132).
+__global__ void k_lod_budget(LodArgs a, uint32_t round) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  LodState& s = a.state[round];
+  const uint32_t available = a.capacity - s.count - s.active + s.drops;
+  if (s.total_extra <= available) {
+    s.accepted = s.total_extra;
+    return;
+  }
+  uint32_t low = 0, high = s.active;
+  while (low < high) {
+    const uint32_t t = low + (high - low) / 2;
+    if (a.cost[2 * t + 1] <= available) low = t + 1; else high = t;
+  }
+  s.accepted = low == 0 ? 0u : a.cost[2 * (low - 1) + 1];
+}
+
+// compactSplatLOD + commitSplatLOD + scatterSplatLOD (SplatLOD.metal:133-208).
+__global__ __launch_bounds__(256) void k_lod_compact(LodArgs a, uint32_t round) {
+  __shared__ uint32_t s_ticket, s_prefix[3], s_scan[4];
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) s_ticket = atomicAdd(&a.ctrl->ticket[2 * round + 1], 1u);
+  __syncthreads();
+  const uint32_t b = s_ticket;
+  const LodState s = a.state[round];
+  const uint32_t blocks = (s.active + 255u) / 256u;
+  if (b >= blocks) return;
+  const uint32_t t = b * 256u + tid;
+  uint32_t kids = 0, splats = 0, packets = 0, denied = 0, flags = kDrop;
+  LodCluster node{};
+  if (t < s.active) {
+    flags = a.cost[2 * t] & ~kCost;
+    const uint32_t extra = a.cost[2 * t] & kCost;
+    if ((flags & kSplit) && extra > 0 && a.cost[2 * t + 1] > s.accepted) {
+      flags = 0;
+      denied = 1;
+    }
+    if (!(flags & kDrop)) {
+      node = a.clusters[a.frontier[round & 1u][t]];
+      kids = (flags & kSplit) ? node.child_count : 0u;
+      splats = (flags & kSplit) ? node.leaf_count : 1u;
+      packets = splats > 32u ? 1u : 0u;
+    }
+  }
+  uint32_t agg_kids, agg_splats, agg_packets, agg_denied;
+  const uint32_t ek = block_exclusive_scan(kids, s_scan, &agg_kids);
+  const uint32_t es = block_exclusive_scan(splats, s_scan, &agg_splats);
+  const uint32_t ep = block_exclusive_scan(packets, s_scan, &agg_packets);
+  block_exclusive_scan(denied, s_scan, &agg_denied);
+  if ((tid >> 6) == 0) {
+    const uint32_t tag = make_tag(a.ctrl->epoch, 2 * round + 1);
+    const uint32_t p0 = scan_across_blocks(a.status[0], b, tag, agg_kids);
+    const uint32_t p1 = scan_across_blocks(a.status[1], b, tag, agg_splats);
+    const uint32_t p2 = scan_across_blocks(a.status[2], b, tag, agg_packets);
+    if (lane_id() == 0) {
+      s_prefix[0] = p0;
+      s_prefix[1] = p1;
+      s_prefix[2] = p2;
+      if (agg_denied) atomicAdd(&a.ctrl->limited, agg_denied);
+    }
+  }
+  __syncthreads();
+  if (t < s.active && !(flags & kDrop)) {
+    const bool split = (flags & kSplit) != 0;
+    uint32_t* next = a.frontier[(round + 1) & 1u];
+    const uint32_t kid_base = s_prefix[0] + ek;
+    for (uint32_t k = 0; k < kids; ++k) next[kid_base + k] = node.child_start + k;
+    const uint32_t dest = s.count + s_prefix[1] + es;
+    if (splats > 32u) {
+      a.packets[s.packets + s_prefix[2] + ep] = make_uint4(node.leaf_start, splats, dest, 1u);
+    } else if (!split) {
+      a.indices[dest] = node.node;
+    } else {
+      for (uint32_t k = 0; k < splats; ++k) a.indices[dest + k] = a.leaves[node.leaf_start + k];
+    }
+  }
+  if (b == blocks - 1u && tid == 0) {
+    LodState n{};
+    n.count = s.count + s_prefix[1] + agg_splats;
+    n.active = s_prefix[0] + agg_kids;
+    n.packets = s.packets + s_prefix[2] + agg_packets;
+    n.accepted = s.accepted;
+    n.evaluated = s.evaluated + s.active;
+    a.state[round + 1] = n;
+  }
+}
+
+// emitSplatLOD (SplatLOD.metal:210-220): one packet per block iteration.
+__global__ __launch_bounds__(256) void k_lod_emit(LodArgs a) {
+  const LodState s = a.state[a.rounds];
+  for (uint32_t p = blockIdx.x; p < s.packets; p += gridDim.x) {
+    const uint4 packet = a.packets[p];
+    for (uint32_t k = threadIdx.x; k < packet.y; k += blockDim.x)
+      a.indices[packet.z + k] = packet.w ? a.leaves[packet.x + k] : packet.x;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl->final_limited = a.ctrl->limited;
+}
+
+}  // namespace
+
+hipError_t launch_lod(const LodArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_lod_prologue, dim3(1), dim3(256), 0, s, a);
+  const uint32_t blocks = (a.frontier_capacity + 255u) / 256u;
+  for (uint32_t r = 0; r < a.rounds; ++r) {
+    hipLaunchKernelGGL(k_lod_eval, dim3(blocks ? blocks : 1u), dim3(256), 0, s, a, r);
+    hipLaunchKernelGGL(k_lod_budget, dim3(1), dim3(64), 0, s, a, r);
+    hipLaunchKernelGGL(k_lod_compact, dim3(blocks ? blocks : 1u), dim3(256), 0, s, a, r);
+  }
+  hipLaunchKernelGGL(k_lod_emit, dim3(256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace skm

@@ -17,6 +17,8 @@
 
 #include "skm.h"
 #include "skm_internal.h"
+#include "skm_lod.h"
+#include "skm_lod_internal.h"
 
 using skm::Camera;
 using skm::Ctrl;
@@ -131,11 +133,13 @@ struct WorldBuffers {
 
 struct Slot {
   DeviceBuffer cam, ranges, starts, image;
-  PinnedBuffer cam_host, ranges_host, starts_host, ctrl_host;
-  hipEvent_t marks[5] = {};
+  PinnedBuffer cam_host, ranges_host, starts_host, ctrl_host, lod_host;
+  hipEvent_t marks[6] = {};
   hipEvent_t done = nullptr;
   bool in_flight = false;
   bool with_world = false;
+  bool with_lod = false;
+  uint32_t lod_limit = 0;
   uint32_t width = 0, height = 0;
   int depth_passes = 4, tile_passes = 2;
 };
@@ -147,6 +151,129 @@ struct Policy {
   bool key16 = true;  // RenderPolicy.h:36 default low16
   double background[3] = {0.05, 0.05, 0.08};
 };
+
+// The selection threshold of LodThreshold (LodThreshold.h:9-45): raised 10% while refinements are
+// denied (up to max(64 px, policy)), relaxed 3% while the cut stays under 90% of its limit.
+struct LodThreshold {
+  float policy = 1.0f, pixels = 1.0f;
+  void reset(float p) { policy = pixels = p; }
+  float update(uint32_t selected, uint32_t limited, uint32_t limit) {
+    if (limited > 0) {
+      pixels = std::min(pixels * 1.1f, std::max(64.0f, policy));
+    } else if (selected < limit / 10 * 9) {
+      pixels = std::max(pixels / 1.03f, policy);
+    }
+    return pixels;
+  }
+};
+
+// Device buffers of one tree's cut (MetalLodSelection::upload, MetalLodSelection.mm:60-108).
+struct LodBuffers {
+  DeviceBuffer clusters, leaves, state, ctrl, frontier[2], cost, packets, indices, status[3];
+  uint32_t cluster_count = 0, rounds = 0, capacity = 0, frontier_capacity = 0;
+
+  hipError_t upload(const skm_lod_tree_view* t, uint32_t cap, uint32_t n_rounds, hipStream_t stream) {
+    cluster_count = static_cast<uint32_t>(t->cluster_count);
+    rounds = n_rounds;
+    capacity = cap;
+    frontier_capacity = std::max(1u, std::min(cap, cluster_count));
+    const size_t fc = frontier_capacity;
+    const size_t blocks = (fc + 255) / 256 + 1;
+    hipError_t e = clusters.allocate(std::max<size_t>(t->cluster_count, 1) * 64);
+    if (e == hipSuccess) e = leaves.allocate(std::max<size_t>(t->leaf_ref_count, 1) * 4);
+    if (e == hipSuccess) e = state.allocate((size_t{rounds} + 1) * sizeof(skm::LodState));
+    if (e == hipSuccess) e = ctrl.allocate(sizeof(skm::LodCtrl));
+    if (e == hipSuccess) e = frontier[0].allocate(fc * 4);
+    if (e == hipSuccess) e = frontier[1].allocate(fc * 4);
+    if (e == hipSuccess) e = cost.allocate(fc * 8);
+    if (e == hipSuccess) e = packets.allocate(fc * 16);
+    if (e == hipSuccess) e = indices.allocate(std::max<size_t>(cap, 1) * 4);
+    for (auto& st : status)
+      if (e == hipSuccess) e = st.allocate(blocks * 8);
+    if (e == hipSuccess && t->cluster_count)
+      e = hipMemcpyAsync(clusters.ptr, t->clusters, t->cluster_count * 64, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess && t->leaf_ref_count)
+      e = hipMemcpyAsync(leaves.ptr, t->leaves, t->leaf_ref_count * 4, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemsetAsync(ctrl.ptr, 0, sizeof(skm::LodCtrl), stream);
+    for (auto& st : status)
+      if (e == hipSuccess) e = hipMemsetAsync(st.ptr, 0, st.bytes, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    return e;
+  }
+
+  skm::LodArgs args(const Camera* cam, uint32_t limit, float pixel_limit, float color_weight, int cull) const {
+    skm::LodArgs a{};
+    a.cam = cam;
+    a.clusters = clusters.as<skm::LodCluster>();
+    a.leaves = leaves.as<uint32_t>();
+    a.cluster_count = cluster_count;
+    a.rounds = rounds;
+    a.capacity = std::max(1u, std::min(limit == 0 ? capacity : limit, capacity));
+    a.frontier_capacity = frontier_capacity;
+    a.pixel_limit = pixel_limit;
+    a.color_weight = color_weight;
+    a.cull = cull;
+    a.ctrl = ctrl.as<skm::LodCtrl>();
+    a.state = state.as<skm::LodState>();
+    a.frontier[0] = frontier[0].as<uint32_t>();
+    a.frontier[1] = frontier[1].as<uint32_t>();
+    a.cost = cost.as<uint32_t>();
+    a.packets = packets.as<uint4>();
+    a.indices = indices.as<uint32_t>();
+    for (int k = 0; k < 3; ++k) a.status[k] = status[k].as<uint64_t>();
+    return a;
+  }
+};
+
+// The tree's indices must stay inside its arrays before any of it reaches the device: clusters
+// breadth-first (children after their parent), leaf references and nodes in range.
+const char* validate_tree(const skm_lod_tree_view* t, uint32_t* rounds) {
+  if (t == nullptr || t->layout == nullptr && t->nodes.count > 0) return "null tree";
+  if (t->nodes.count == 0 || t->cluster_count == 0 || t->clusters == nullptr) return "empty tree";
+  if (t->leaf_ref_count > 0 && t->leaves == nullptr) return "missing leaf references";
+  const uint64_t n = t->nodes.count;
+  for (uint64_t k = 0; k < t->cluster_count; ++k) {
+    const skm_lod_cluster& c = t->clusters[k];
+    if (c.node >= n) return "cluster node out of range";
+    if (c.child_count > 0 && (c.child_start <= k || uint64_t{c.child_start} + c.child_count > t->cluster_count))
+      return "cluster children out of order or range";
+    if (uint64_t{c.leaf_start} + c.leaf_count > t->leaf_ref_count) return "cluster leaves out of range";
+  }
+  for (uint64_t i = 0; i < t->leaf_ref_count; ++i)
+    if (t->leaves[i] >= n) return "leaf reference out of range";
+  *rounds = skm_lod_rounds(t);
+  if (*rounds == 0 || *rounds > skm::kLodMaxRounds) return "tree deeper than the selection supports";
+  return nullptr;
+}
+
+void fill_camera(Camera& cam, const float* view, const float* proj, uint32_t W, uint32_t H, bool linear,
+                 float min_px, const float* eye, const skm_reveal* reveal) {
+  // MetalFrameCompute::updateCamera (MetalFrameCompute.mm:141-164).
+  std::memset(&cam, 0, sizeof(cam));
+  std::memcpy(cam.view, view, 64);
+  std::memcpy(cam.proj, proj, 64);
+  cam.screen_size[0] = static_cast<float>(W);
+  cam.screen_size[1] = static_cast<float>(H);
+  cam.focal[0] = cam.screen_size[0] * proj[0] / 2;
+  cam.focal[1] = cam.screen_size[1] * proj[5] / 2;
+  cam.tan_half_fov[0] = 1 / proj[0];
+  cam.tan_half_fov[1] = 1 / proj[5];
+  cam.output_linear = linear ? 1u : 0u;
+  cam.min_pixel_radius = min_px;
+  if (eye != nullptr) {
+    cam.camera_position[0] = eye[0];
+    cam.camera_position[1] = eye[1];
+    cam.camera_position[2] = eye[2];
+  }
+  cam.camera_position[3] = 1;
+  if (reveal != nullptr) {
+    std::memcpy(cam.reveal, reveal, 16);
+    std::memcpy(cam.reveal_tint, reveal->tint, 16);
+  } else {
+    cam.reveal[0] = HUGE_VALF;
+    cam.reveal_tint[0] = cam.reveal_tint[1] = cam.reveal_tint[2] = 1.0f;
+  }
+}
 
 bool valid_cloud(const skm_cloud_view* c) {
   return c != nullptr && (c->count == 0 || (c->positions && c->covariances && c->colors && c->alphas)) &&
@@ -172,6 +299,11 @@ struct skm_renderer {
 
   WorldBuffers world;
   bool has_world = false;
+  bool lod_world = false;
+  LodBuffers lod;
+  LodThreshold threshold;
+  uint32_t last_selected = 0;
+  uint32_t lod_stats[6] = {};
 
   // Frame scratch, shared by the frames on the stream.
   uint32_t capacity = 0;       // survivor slots
@@ -194,7 +326,7 @@ struct skm_renderer {
   bool grow_pairs = false;
   uint64_t pairs_needed = 0;
   bool last_dropped = false;
-  double stage_ms[5] = {};
+  double stage_ms[6] = {};
   uint32_t last_draw_count = 0;
   skm_frame_totals totals{};
 
@@ -228,14 +360,27 @@ struct skm_renderer {
     s.in_flight = false;
     if (e != hipSuccess) return check(e, "frame");
     const Ctrl* c = s.ctrl_host.as<Ctrl>();
-    float ms[5] = {0, 0, 0, 0, 0};
-    for (int k = 0; k < 4; ++k) (void)hipEventElapsedTime(&ms[k], s.marks[k], s.marks[k + 1]);
-    (void)hipEventElapsedTime(&ms[4], s.marks[0], s.marks[4]);
-    for (int k = 0; k < 5; ++k) stage_ms[k] = ms[k];
+    float ms[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 5; ++k) (void)hipEventElapsedTime(&ms[k], s.marks[k], s.marks[k + 1]);
+    (void)hipEventElapsedTime(&ms[5], s.marks[0], s.marks[5]);
+    for (int k = 0; k < 6; ++k) stage_ms[k] = ms[k];
+    if (s.with_lod) {
+      const auto* st = s.lod_host.as<skm::LodState>();
+      const uint32_t limited = *reinterpret_cast<const uint32_t*>(s.lod_host.as<char>() + sizeof(skm::LodState));
+      last_selected = st->count;
+      lod_stats[0] = st->count;
+      lod_stats[1] = st->active;
+      lod_stats[2] = st->packets;
+      lod_stats[3] = st->accepted;
+      lod_stats[4] = limited;
+      lod_stats[5] = st->evaluated;
+      // adaptLodThreshold (MetalFrameCompute.mm:166-172).
+      threshold.update(st->count, limited, s.lod_limit);
+    }
     last_draw_count = c->visible;
     last_dropped = c->overflow != 0;
     totals.frames += 1;
-    for (int k = 0; k < 5; ++k) totals.stage_ms[k] += ms[k];
+    for (int k = 0; k < 6; ++k) totals.stage_ms[k] += ms[k];
     totals.visible += c->visible;
     totals.pairs += c->pairs;
     if (last_dropped) {
@@ -395,7 +540,7 @@ skm_status skm_renderer_create(const skm_config* config, skm_renderer** out) {
   int runtime = 0;
   (void)hipRuntimeGetVersion(&runtime);
   char desc[320];
-  std::snprintf(desc, sizeof(desc), "%s (%s, %d CUs), HIP %d.%d", prop.name, prop.gcnArchName,
+  std::snprintf(desc, sizeof(desc), "%s (%s, %d CUs), HIP %d.%d", prop.name[0] ? prop.name : "AMD GPU", prop.gcnArchName,
                 prop.multiProcessorCount, runtime / 10000000, (runtime / 100000) % 100);
   r->description = desc;
   bool ok = r->ctrl.allocate(sizeof(Ctrl)) == hipSuccess &&
@@ -409,6 +554,7 @@ skm_status skm_renderer_create(const skm_config* config, skm_renderer** out) {
          s.ranges_host.allocate(size_t{skm::kMaxRanges} * 8) == hipSuccess &&
          s.starts_host.allocate(size_t{skm::kMaxRanges + 1} * 4) == hipSuccess &&
          s.ctrl_host.allocate(sizeof(Ctrl)) == hipSuccess &&
+         s.lod_host.allocate(sizeof(skm::LodState) + 16) == hipSuccess &&
          hipEventCreate(&s.done) == hipSuccess;
     for (auto& m : s.marks) ok = ok && hipEventCreate(&m) == hipSuccess;
   }
@@ -524,6 +670,8 @@ skm_status skm_apply_render_policy(skm_renderer* r, const skm_render_policy* p, 
   r->policy.subpixel = p->subpixel_threshold;
   r->policy.key16 = p->sort_depth == 16;
   for (int c = 0; c < 3; ++c) r->policy.background[c] = p->background_color[c];
+  // A policy change restarts the LOD threshold at the policy's value (MetalFrameCompute.mm:71-103).
+  r->threshold.reset(r->policy.lod_error_pixels);
   if (reason && reason_size) reason[0] = '\0';
   return SKM_OK;
 }
@@ -569,11 +717,45 @@ skm_status skm_upload_world(skm_renderer* r, const skm_cloud_view* cloud, int ma
   if (auto st = r->ensure_pairs(uint64_t{cloud->count} * r->pair_factor); st != SKM_OK) return st;
   r->world = std::move(w);
   r->has_world = true;
+  r->lod_world = false;
   r->completed_world_frame = false;
   return r->check(hipStreamSynchronize(r->stream), "upload");
 }
 
-int skm_selects_lod_on_gpu(const skm_renderer*) { return 0; }
+int skm_selects_lod_on_gpu(const skm_renderer*) { return 1; }
+
+// MetalSplatRenderer::uploadLodWorld (MetalSplatRenderer.mm:180-194) with
+// MetalLodSelection::upload (MetalLodSelection.mm:60-108).
+skm_status skm_upload_lod_world(skm_renderer* r, const skm_lod_tree_view* tree, int max_sh_degree, uint32_t budget) {
+  if (r == nullptr) return SKM_ERR_INVALID;
+  uint32_t rounds = 0;
+  if (const char* why = validate_tree(tree, &rounds)) return r->fail(SKM_ERR_INVALID, "invalid LOD tree: %s", why);
+  if (!valid_cloud(&tree->nodes)) return r->fail(SKM_ERR_INVALID, "incomplete node cloud");
+  if (r->gpu_failed) return r->fail(SKM_ERR_DEVICE, "GPU failed earlier");
+  if (auto st = r->check(hipSetDevice(r->device), "device"); st != SKM_OK) return st;
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  const uint64_t leaves = std::max<uint64_t>(tree->leaf_count, 1);
+  uint32_t cap = static_cast<uint32_t>(std::min<uint64_t>({budget == 0 ? leaves : budget, leaves, 1ull << 27}));
+  cap = std::max(cap, 1u);
+  const skm_cloud_view* cloud = &tree->nodes;
+  int degree = std::clamp(std::min(cloud->sh_degree, max_sh_degree), 0, 3);
+  if (degree > 0 && cloud->sh == nullptr) degree = 0;
+  WorldBuffers w;
+  if (auto st = r->check(w.allocate(cloud->count, degree), "world"); st != SKM_OK) return st;
+  w.count = cloud->count;
+  if (auto st = r->upload_range(w, 0, cloud, degree > 0 ? cloud->sh_degree : 0); st != SKM_OK) return st;
+  LodBuffers lod;
+  if (auto st = r->check(lod.upload(tree, cap, rounds, r->stream), "LOD upload"); st != SKM_OK) return st;
+  if (auto st = r->ensure_capacity(cap); st != SKM_OK) return st;
+  if (auto st = r->ensure_pairs(uint64_t{cap} * r->pair_factor); st != SKM_OK) return st;
+  r->world = std::move(w);
+  r->lod = std::move(lod);
+  r->has_world = true;
+  r->lod_world = true;
+  r->threshold.reset(r->policy.lod_error_pixels);
+  r->completed_world_frame = false;
+  return r->check(hipStreamSynchronize(r->stream), "upload");
+}
 
 // MetalWorld::slab (MetalWorld.mm:59-79).
 skm_status skm_create_slab(skm_renderer* r, uint32_t capacity, int sh_degree) {
@@ -589,6 +771,7 @@ skm_status skm_create_slab(skm_renderer* r, uint32_t capacity, int sh_degree) {
   if (auto st = r->ensure_pairs(uint64_t{capacity} * r->pair_factor); st != SKM_OK) return st;
   r->world = std::move(w);
   r->has_world = true;
+  r->lod_world = false;
   r->completed_world_frame = false;
   return r->check(hipStreamSynchronize(r->stream), "slab");
 }
@@ -660,25 +843,8 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   }
   if (auto st = r->maybe_clear_status(); st != SKM_OK) return st;
 
-  // MetalFrameCompute::updateCamera (MetalFrameCompute.mm:141-164).
   Camera& cam = *s.cam_host.as<Camera>();
-  std::memset(&cam, 0, sizeof(cam));
-  std::memcpy(cam.view, f->view, 64);
-  std::memcpy(cam.proj, f->proj, 64);
-  cam.screen_size[0] = static_cast<float>(W);
-  cam.screen_size[1] = static_cast<float>(H);
-  cam.focal[0] = cam.screen_size[0] * f->proj[0] / 2;
-  cam.focal[1] = cam.screen_size[1] * f->proj[5] / 2;
-  cam.tan_half_fov[0] = 1 / f->proj[0];
-  cam.tan_half_fov[1] = 1 / f->proj[5];
-  cam.output_linear = r->linear ? 1u : 0u;
-  cam.min_pixel_radius = r->policy.subpixel;
-  cam.camera_position[0] = f->camera_position[0];
-  cam.camera_position[1] = f->camera_position[1];
-  cam.camera_position[2] = f->camera_position[2];
-  cam.camera_position[3] = 1;
-  std::memcpy(cam.reveal, &f->reveal, 16);
-  std::memcpy(cam.reveal_tint, f->reveal.tint, 16);
+  fill_camera(cam, f->view, f->proj, W, H, r->linear, r->policy.subpixel, f->camera_position, &f->reveal);
 
   uint32_t* rh = s.ranges_host.as<uint32_t>();
   uint32_t* sh = s.starts_host.as<uint32_t>();
@@ -697,15 +863,21 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   if (auto st = r->check(e, "frame inputs"); st != SKM_OK) return st;
 
   const bool world = r->has_world;
+  const bool lod_frame = world && r->lod_world;
+  skm::LodArgs lod{};
+  if (lod_frame) {
+    s.lod_limit = r->policy.lod_splat_limit == 0 ? r->lod.capacity : std::min(r->policy.lod_splat_limit, r->lod.capacity);
+    lod = r->lod.args(s.cam.as<Camera>(), s.lod_limit, r->threshold.pixels, 4.0f, 1);
+  }
   skm::FrameArgs a{};
   a.cam = s.cam.as<Camera>();
   a.world = world ? r->world.view() : skm::World{};
   a.ranges = s.ranges.as<uint32_t>();
   a.range_starts = s.starts.as<uint32_t>();
-  a.range_count = world ? f->range_count : 0;
-  a.total = world ? static_cast<uint32_t>(total) : 0;
-  a.active = nullptr;
-  a.active_count = nullptr;
+  a.range_count = world && !lod_frame ? f->range_count : 0;
+  a.total = world && !lod_frame ? static_cast<uint32_t>(total) : 0;
+  a.active = lod_frame ? lod.indices : nullptr;
+  a.active_count = lod_frame ? &lod.state[lod.rounds].count : nullptr;
   a.key16 = r->policy.key16 ? 1 : 0;
   a.width = W;
   a.height = H;
@@ -731,13 +903,20 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   a.capacity = r->capacity;
   s.depth_passes = r->policy.key16 ? 2 : 4;
   s.tile_passes = static_cast<int>(digit_passes(tiles_x * tiles_y - 1));
-  e = skm::launch_frame(a, s.depth_passes, s.tile_passes, s.marks, r->stream);
+  e = skm::launch_frame(a, lod_frame ? &lod : nullptr, s.depth_passes, s.tile_passes, s.marks, r->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(s.ctrl_host.ptr, r->ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost, r->stream);
+  if (e == hipSuccess && lod_frame) {
+    e = hipMemcpyAsync(s.lod_host.ptr, lod.state + lod.rounds, sizeof(skm::LodState), hipMemcpyDeviceToHost, r->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(s.lod_host.as<char>() + sizeof(skm::LodState), &lod.ctrl->final_limited, 4,
+                         hipMemcpyDeviceToHost, r->stream);
+  }
   if (e == hipSuccess) e = hipEventRecord(s.done, r->stream);
   if (auto st = r->check(e, "frame launch"); st != SKM_OK) return st;
   s.in_flight = true;
   s.with_world = world;
+  s.with_lod = lod_frame;
   r->last_slot = static_cast<int>(slot_id);
   ++r->submitted;
   return SKM_OK;
@@ -760,18 +939,66 @@ uint64_t skm_completed_frames(skm_renderer* r) {
     if (s.in_flight && hipEventQuery(s.done) == hipSuccess) (void)r->harvest(s);
   return r->completed;
 }
-double skm_last_gpu_millis(const skm_renderer* r) { return r ? r->stage_ms[4] : 0; }
-double skm_last_sort_millis(const skm_renderer* r) { return r ? r->stage_ms[1] + r->stage_ms[2] : 0; }
-double skm_last_cull_millis(const skm_renderer* r) { return r ? r->stage_ms[0] : 0; }
+double skm_last_gpu_millis(const skm_renderer* r) { return r ? r->stage_ms[5] : 0; }
+double skm_last_sort_millis(const skm_renderer* r) { return r ? r->stage_ms[2] + r->stage_ms[3] : 0; }
+double skm_last_cull_millis(const skm_renderer* r) { return r ? r->stage_ms[1] : 0; }
 uint32_t skm_last_draw_count(const skm_renderer* r) { return r ? r->last_draw_count : 0; }
-uint32_t skm_last_selected_count(const skm_renderer*) { return 0; }
-double skm_last_select_millis(const skm_renderer*) { return 0; }
+uint32_t skm_last_selected_count(const skm_renderer* r) { return r ? r->last_selected : 0; }
+double skm_last_select_millis(const skm_renderer* r) { return r ? r->stage_ms[0] : 0; }
 const char* skm_device_description(const skm_renderer* r) { return r ? r->description.c_str() : ""; }
 
-skm_status skm_read_stage_millis(skm_renderer* r, double* ms5) {
-  if (r == nullptr || ms5 == nullptr) return SKM_ERR_INVALID;
+skm_status skm_read_stage_millis(skm_renderer* r, double* ms6) {
+  if (r == nullptr || ms6 == nullptr) return SKM_ERR_INVALID;
   if (auto st = r->drain(); st != SKM_OK) return st;
-  for (int k = 0; k < 5; ++k) ms5[k] = r->stage_ms[k];
+  for (int k = 0; k < 6; ++k) ms6[k] = r->stage_ms[k];
+  return SKM_OK;
+}
+
+skm_status skm_read_lod_cut(skm_renderer* r, uint32_t* count, uint32_t* indices, uint32_t capacity, uint32_t* stats6) {
+  if (r == nullptr || count == nullptr) return SKM_ERR_INVALID;
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  if (!r->lod_world) return r->fail(SKM_ERR_STATE, "no LOD world");
+  *count = r->lod_stats[0];
+  if (stats6) std::memcpy(stats6, r->lod_stats, sizeof(r->lod_stats));
+  const uint32_t n = std::min(r->lod_stats[0], capacity);
+  if (indices && n) return r->check(hipMemcpy(indices, r->lod.indices.ptr, size_t{n} * 4, hipMemcpyDeviceToHost), "read");
+  return SKM_OK;
+}
+
+skm_status skm_lod_select(skm_renderer* r, const skm_lod_tree_view* tree, const float* view16, const float* proj16,
+                          uint32_t width, uint32_t height, uint32_t capacity, float pixel_limit, float color_weight,
+                          int cull, uint32_t* indices, uint32_t* stats6) {
+  if (r == nullptr || view16 == nullptr || proj16 == nullptr || width == 0 || height == 0) return SKM_ERR_INVALID;
+  uint32_t rounds = 0;
+  if (const char* why = validate_tree(tree, &rounds)) return r->fail(SKM_ERR_INVALID, "invalid LOD tree: %s", why);
+  if (!std::isfinite(pixel_limit) || pixel_limit < 0 || !std::isfinite(color_weight) || color_weight < 0)
+    return r->fail(SKM_ERR_INVALID, "invalid selection limits");
+  if (auto st = r->check(hipSetDevice(r->device), "device"); st != SKM_OK) return st;
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  const uint64_t leaves = std::max<uint64_t>(tree->leaf_count, 1);
+  const uint32_t cap = std::max(1u, static_cast<uint32_t>(std::min<uint64_t>({capacity == 0 ? leaves : capacity, leaves, 1ull << 27})));
+  LodBuffers lod;
+  if (auto st = r->check(lod.upload(tree, cap, rounds, r->stream), "LOD upload"); st != SKM_OK) return st;
+  DeviceBuffer cam_dev;
+  Camera cam;
+  fill_camera(cam, view16, proj16, width, height, false, 0.0f, nullptr, nullptr);
+  if (auto st = r->check(cam_dev.allocate(sizeof(Camera)), "camera"); st != SKM_OK) return st;
+  const skm::LodArgs a = lod.args(cam_dev.as<Camera>(), cap, pixel_limit, color_weight, cull);
+  skm::LodState final_state{};
+  uint32_t limited = 0;
+  hipError_t e = hipMemcpyAsync(cam_dev.ptr, &cam, sizeof(Camera), hipMemcpyHostToDevice, r->stream);
+  if (e == hipSuccess) e = skm::launch_lod(a, r->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&final_state, a.state + rounds, sizeof(final_state), hipMemcpyDeviceToHost, r->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&limited, &a.ctrl->final_limited, 4, hipMemcpyDeviceToHost, r->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  if (auto st = r->check(e, "LOD select"); st != SKM_OK) return st;
+  if (stats6) {
+    const uint32_t v[6] = {final_state.count, final_state.active, final_state.packets, final_state.accepted, limited,
+                           final_state.evaluated};
+    std::memcpy(stats6, v, sizeof(v));
+  }
+  const uint32_t n = std::min(final_state.count, cap);
+  if (indices && n) return r->check(hipMemcpy(indices, lod.indices.ptr, size_t{n} * 4, hipMemcpyDeviceToHost), "read");
   return SKM_OK;
 }
 

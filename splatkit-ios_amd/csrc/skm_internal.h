@@ -115,8 +115,11 @@ struct FrameArgs {
 hipError_t launch_pack(const WorldOut& out, uint32_t offset, uint32_t count, const float* positions,
                        const float* covariances, const float* colors, const float* alphas,
                        const float* sh, int src_degree, hipStream_t stream);
-hipError_t launch_frame(const FrameArgs& a, int depth_passes, int tile_passes, hipEvent_t* marks,
-                        hipStream_t stream);
+struct LodArgs;
+// marks: 6 events recorded at the start, after the LOD cut, the cull, the depth sort, the tile
+// binning/sort/ranges and the raster.
+hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes, int tile_passes,
+                        hipEvent_t* marks, hipStream_t stream);
 hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev,
                           uint32_t capacity, int passes, uint32_t* hist, uint64_t* status,
                           hipStream_t stream);

@@ -27,12 +27,31 @@ __device__ __forceinline__ float half_to_float(uint32_t h16) {
 }
 __device__ __forceinline__ float lo_half(uint32_t w) { return half_to_float(w & 0xffffu); }
 __device__ __forceinline__ float hi_half(uint32_t w) { return half_to_float(w >> 16); }
-// IEEE binary16, round to nearest even, overflow to infinity: the shader's half(float).
-__device__ __forceinline__ uint32_t half_rne(float f) {
-  const _Float16 h = static_cast<_Float16>(f);
-  uint16_t v;
-  __builtin_memcpy(&v, &h, 2);
-  return v;
+// IEEE binary16, round to nearest even, overflow to infinity: the shader's half(float). Written
+// with integer operations: a float conversion instruction would let the compiler fuse a preceding
+// multiply into v_fma_mix (one rounding from the exact product instead of two), which changes the
+// result at fp32 values that lie exactly between two halves.
+__device__ __forceinline__ uint32_t half_rne(float value) {
+  const uint32_t b = bits(value);
+  const uint32_t sign = (b >> 16) & 0x8000u;
+  const uint32_t biased = (b >> 23) & 0xffu;
+  uint32_t mant = b & 0x7fffffu;
+  if (biased == 0xffu) return sign | (mant ? 0x7e00u : 0x7c00u);
+  const int32_t e = static_cast<int32_t>(biased) - 112;
+  if (e >= 31) return sign | 0x7c00u;
+  if (e <= 0) {
+    if (e < -10) return sign;
+    mant |= 0x800000u;
+    const uint32_t shift = static_cast<uint32_t>(14 - e);
+    const uint32_t q = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1u);
+    const uint32_t mid = 1u << (shift - 1u);
+    return sign | (q + ((rem > mid || (rem == mid && (q & 1u))) ? 1u : 0u));
+  }
+  uint32_t h = (static_cast<uint32_t>(e) << 10) | (mant >> 13);
+  const uint32_t rem = mant & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;  // may carry into infinity: IEEE
+  return sign | h;
 }
 __device__ __forceinline__ uint32_t pack_rne(float a, float b) { return half_rne(a) | (half_rne(b) << 16); }
 __device__ __forceinline__ float round_half(float f) { return half_to_float(half_rne(f)); }

@@ -82,7 +82,7 @@ class DeviceCapabilities(C.Structure):
 
 
 class FrameTotals(C.Structure):
-    _fields_ = [("frames", C.c_uint64), ("stage_ms", C.c_double * 5), ("visible", C.c_uint64),
+    _fields_ = [("frames", C.c_uint64), ("stage_ms", C.c_double * 6), ("visible", C.c_uint64),
                 ("pairs", C.c_uint64)]
 
 
@@ -92,6 +92,24 @@ class SceneSpec(C.Structure):
                 ("surface_share", C.c_float), ("surface_sigma", C.c_float),
                 ("log_scale_mean", C.c_float), ("log_scale_sigma", C.c_float),
                 ("opacity_sigma", C.c_float), ("dc_sigma", C.c_float), ("sh_sigma", C.c_float)]
+
+
+class LodNode(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("size", C.c_float), ("child_start", C.c_uint32),
+                ("child_count", C.c_uint32)]
+
+
+class LodCluster(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("radius", C.c_float), ("extent", C.c_float * 3), ("error", C.c_float),
+                ("color_variance", C.c_float), ("opacity", C.c_float), ("node", C.c_uint32),
+                ("child_start", C.c_uint32), ("child_count", C.c_uint32), ("leaf_start", C.c_uint32),
+                ("leaf_count", C.c_uint32), ("subtree_leaves", C.c_uint32)]
+
+
+class LodTreeView(C.Structure):
+    _fields_ = [("nodes", CloudView), ("layout", C.POINTER(LodNode)), ("leaf_count", C.c_uint64),
+                ("clusters", C.POINTER(LodCluster)), ("cluster_count", C.c_uint64),
+                ("leaves", _u32p), ("leaf_ref_count", C.c_uint64)]
 
 
 # name -> (restype, argtypes); every symbol include/skm.h declares.
@@ -117,6 +135,7 @@ API = {
     "skm_surface_rebuild_pending": (C.c_int, [_R]),
     "skm_upload_world": (C.c_int, [_R, C.POINTER(CloudView), C.c_int]),
     "skm_selects_lod_on_gpu": (C.c_int, [_R]),
+    "skm_upload_lod_world": (C.c_int, [_R, C.POINTER(LodTreeView), C.c_int, C.c_uint32]),
     "skm_create_slab": (C.c_int, [_R, C.c_uint32, C.c_int]),
     "skm_upload_tile": (C.c_int, [_R, C.c_uint32, C.POINTER(CloudView)]),
     "skm_world": (C.c_int, [_R, C.POINTER(WorldInfo)]),
@@ -138,9 +157,21 @@ API = {
     "skm_read_tiles": (C.c_int, [_R, C.POINTER(C.c_uint64), _u32p, _u32p, C.c_uint64, _u32p, C.c_uint32]),
     "skm_read_stage_millis": (C.c_int, [_R, C.POINTER(C.c_double)]),
     "skm_read_frame_totals": (C.c_int, [_R, C.POINTER(FrameTotals), C.c_int]),
+    "skm_read_lod_cut": (C.c_int, [_R, _u32p, _u32p, C.c_uint32, _u32p]),
+    "skm_lod_select": (C.c_int, [_R, C.POINTER(LodTreeView), _fp, _fp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
+                                 C.c_float, C.c_int, _u32p, _u32p]),
     "skm_argsort_u32": (C.c_int, [_R, _u32p, C.c_uint32, C.c_int, _u32p]),
     "skm_argsort_u32_device": (C.c_int, [_R, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p,
                                          C.POINTER(C.c_double)]),
+}
+
+# include/skm_lod.h
+LOD_API = {
+    "skm_lod_build": (C.c_void_p, [C.POINTER(CloudView), C.c_uint32, C.c_uint32, C.c_uint32]),
+    "skm_lod_from_layout": (C.c_void_p, [C.POINTER(CloudView), C.POINTER(LodNode), C.c_uint64]),
+    "skm_lod_free": (None, [C.c_void_p]),
+    "skm_lod_view": (None, [C.c_void_p, C.POINTER(LodTreeView)]),
+    "skm_lod_rounds": (C.c_uint32, [C.POINTER(LodTreeView)]),
 }
 
 _lib = None
@@ -154,7 +185,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
         handle = C.CDLL(LIB_PATH)
-        for name, (res, args) in API.items():
+        for name, (res, args) in {**API, **LOD_API}.items():
             fn = getattr(handle, name)
             fn.restype = res
             fn.argtypes = args
@@ -348,6 +379,30 @@ class Renderer:
         self._check(self._lib.skm_upload_tile(self._h, offset, C.byref(view)))
         del keep
 
+    def upload_lod_world(self, tree, max_sh_degree=3, budget=0):
+        self._check(self._lib.skm_upload_lod_world(self._h, C.byref(tree.view), max_sh_degree, budget))
+        self.world_count = tree.node_count
+        self._lod_tree = tree  # the view borrows the tree's arrays only during the call; keep it alive anyway
+
+    def read_lod_cut(self):
+        count = C.c_uint32()
+        stats = np.zeros(6, np.uint32)
+        self._check(self._lib.skm_read_lod_cut(self._h, C.byref(count), None, 0, uptr(stats)))
+        idx = np.zeros(max(1, count.value), np.uint32)
+        self._check(self._lib.skm_read_lod_cut(self._h, C.byref(count), uptr(idx), count.value, uptr(stats)))
+        return idx[:count.value].copy(), stats
+
+    def lod_select(self, tree, view, proj, width, height, capacity, pixel_limit=1.0, color_weight=4.0, cull=True):
+        """One cut on its own (MetalLodSelection): node indices and stats6."""
+        cap = max(1, min(capacity or tree.leaf_count, tree.leaf_count))
+        idx = np.zeros(cap, np.uint32)
+        stats = np.zeros(6, np.uint32)
+        v = np.ascontiguousarray(view, np.float32)
+        p = np.ascontiguousarray(proj, np.float32)
+        self._check(self._lib.skm_lod_select(self._h, C.byref(tree.view), fptr(v), fptr(p), width, height, cap,
+                                             pixel_limit, color_weight, 1 if cull else 0, uptr(idx), uptr(stats)))
+        return idx[:stats[0]].copy(), stats
+
     def world(self):
         info = WorldInfo()
         return (info.count, info.sh_degree) if self._lib.skm_world(self._h, C.byref(info)) else None
@@ -383,7 +438,8 @@ class Renderer:
         return self._check(self._lib.skm_finish(self._h))
 
     def stage_millis(self):
-        ms = (C.c_double * 5)()
+        """GPU ms of the last frame: LOD cut, cull, depth sort, bin + tile sort + ranges, raster, total."""
+        ms = (C.c_double * 6)()
         self._check(self._lib.skm_read_stage_millis(self._h, ms))
         return list(ms)
 
@@ -451,5 +507,59 @@ class Renderer:
             "submitted": L.skm_submitted_frames(h), "completed": L.skm_completed_frames(h),
             "gpu_ms": L.skm_last_gpu_millis(h), "sort_ms": L.skm_last_sort_millis(h),
             "cull_ms": L.skm_last_cull_millis(h), "draw_count": L.skm_last_draw_count(h),
+            "selected": L.skm_last_selected_count(h), "select_ms": L.skm_last_select_millis(h),
             "gpu_failed": L.skm_gpu_failed(h),
         }
+
+
+class LodTree:
+    """A level-of-detail tree (include/skm_lod.h) built by the library; arrays are numpy copies."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise ValueError("invalid LOD input")
+        self._h = C.c_void_p(handle)
+        self.view = LodTreeView()
+        lib().skm_lod_view(self._h, C.byref(self.view))
+        v = self.view
+        n = v.nodes.count
+        self.node_count = n
+        self.leaf_count = v.leaf_count
+        self.sh_degree = v.nodes.sh_degree
+        arr = lambda p, shape, dt=np.float32: np.ctypeslib.as_array(p, shape=shape).astype(dt, copy=True) if n else np.zeros(shape, dt)
+        self.nodes = {"positions": arr(v.nodes.positions, (n, 3)), "covariances": arr(v.nodes.covariances, (n, 6)),
+                      "colors": arr(v.nodes.colors, (n, 3)), "alphas": arr(v.nodes.alphas, (n,)),
+                      "sh_degree": self.sh_degree,
+                      "sh": arr(v.nodes.sh, (n, sh_floats(self.sh_degree))) if self.sh_degree else None}
+        self.layout = np.ctypeslib.as_array(C.cast(v.layout, _u32p), shape=(n, 6)).copy() if n else np.zeros((0, 6), np.uint32)
+        k = v.cluster_count
+        self.clusters = np.ctypeslib.as_array(C.cast(v.clusters, _u32p), shape=(k, 16)).copy() if k else np.zeros((0, 16), np.uint32)
+        m = v.leaf_ref_count
+        self.leaves = np.ctypeslib.as_array(v.leaves, shape=(m,)).copy() if m else np.zeros(0, np.uint32)
+        self.rounds = lib().skm_lod_rounds(C.byref(self.view))
+
+    @classmethod
+    def build(cls, cloud, depth=21, cluster_leaves=32, threads=8):
+        view, keep = Renderer._view(cloud)
+        h = lib().skm_lod_build(C.byref(view), depth, cluster_leaves, threads)
+        del keep
+        return cls(h)
+
+    @classmethod
+    def from_layout(cls, nodes, layout, leaf_count):
+        view, keep = Renderer._view(nodes)
+        lay = np.ascontiguousarray(layout, np.uint32).reshape(-1, 6)
+        h = lib().skm_lod_from_layout(C.byref(view), lay.ctypes.data_as(C.POINTER(LodNode)), leaf_count)
+        del keep
+        return cls(h)
+
+    def close(self):
+        if self._h:
+            lib().skm_lod_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -110,3 +110,9 @@ def test_look_at_and_perspective(skm_mod):
     import cases
 
     assert np.array_equal(skm_mod.perspective(1.0, 1.5, 0.1, 100.0), cases.perspective(1.0, 1.5, 0.1, 100.0))
+
+
+def test_lod_header_is_exported(skm_mod):
+    names = declared("skm_lod.h")
+    assert names <= exported(skm_mod.LIB_PATH)
+    assert set(skm_mod.LOD_API) == names

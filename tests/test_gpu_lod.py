@@ -1,0 +1,93 @@
+"""The GPU LOD tree cut against the CPU oracle, bit-exact in node indices and counters, on the
+reference's selection-test trees and on built trees, then a whole LOD frame (cut, indexed cull,
+sort, tiles, raster) against the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+import lod_cases as lc
+from test_gpu_parity import assert_parity, gpu_frame
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def both(renderer, oracle, tree, view, proj, eye, w, h, capacity, pixel, cw=4.0, cull=True):
+    cap = min(capacity or tree.leaf_count, tree.leaf_count)
+    g_cut, g_stats = renderer.lod_select(tree, view, proj, w, h, cap, pixel, cw, cull)
+    cam = oracle.make_camera(view, proj, w, h, eye)
+    o_cut, o_stats = oracle.lod_select(cam, tree.clusters, tree.leaves, tree.rounds, cap, pixel, cw, cull)
+    assert np.array_equal(g_cut, o_cut), (len(g_cut), len(o_cut))
+    # count, packets, denied refinements and evaluated clusters (the final frontier and the last
+    # round's accepted extra are bookkeeping of the last round).
+    assert list(g_stats[[0, 2, 4, 5]]) == list(o_stats[[0, 2, 4, 5]]), (g_stats, o_stats)
+    return g_cut, g_stats
+
+
+def test_reference_selection_cases(renderer, skm_mod, oracle):
+    say("case: MetalLodSelectionTest trees")
+    tree = lc.hierarchy(skm_mod)
+    view, proj, eye = lc.camera(0.0)
+    for budget in (1, 7, 31, 32, 33, 64, 257, 400, 515):
+        cut, _ = both(renderer, oracle, tree, view, proj, eye, lc.W, lc.H, budget, 0.0)
+        assert 0 < len(cut) <= budget and lc.covers_every_leaf_once(tree, cut)
+    far = lc.camera(1_000_000)
+    assert len(both(renderer, oracle, tree, *far, lc.W, lc.H, 400, 1.0)[0]) == 1
+    behind = lc.camera(-100)
+    assert len(both(renderer, oracle, tree, *behind, lc.W, lc.H, 515, 0.0)[0]) == 0
+    cut, stats = both(renderer, oracle, lc.quality_tree(skm_mod), *lc.camera(100), lc.W, lc.H, 4, 1.0, cull=False)
+    assert set(cut.tolist()) == {1, 5, 6} and stats[4] == 0 and stats[5] == 3
+    cut, stats = both(renderer, oracle, lc.rounds_tree(skm_mod), view, proj, eye, lc.W, lc.H, 3, 0.0, cull=False)
+    assert list(cut) == [1, 3, 4] and stats[5] == 2
+    packet = lc.packet_tree(skm_mod)
+    cut, stats = both(renderer, oracle, packet, view, proj, eye, lc.W, lc.H, 513, 0.0)
+    assert list(cut) == list(range(1, 514)) and stats[4] == 0
+    cut, stats = both(renderer, oracle, packet, view, proj, eye, lc.W, lc.H, 512, 0.0)
+    assert list(cut) == [0] and stats[4] == 1
+
+
+@pytest.mark.parametrize("capacity", [20000, 32768])
+def test_binary_tree_crosses_blocks(renderer, skm_mod, oracle, capacity):
+    say(f"case: binary tree of 32768 leaves, capacity {capacity}")
+    tree = lc.binary_tree(skm_mod)
+    view, proj, eye = lc.camera(0.0)
+    cut, stats = both(renderer, oracle, tree, view, proj, eye, lc.W, lc.H, capacity, 0.0, cull=False)
+    assert len(cut) == capacity and lc.covers_every_leaf_once(tree, cut)
+
+
+def test_built_trees_at_many_cameras(renderer, skm_mod, oracle):
+    say("case: built trees, 200k splats, cameras x thresholds x budgets")
+    cloud = skm_mod.make_cloud(200_000, 1, seed=55, layout=1, spread=20.0, area=60.0)
+    tree = skm_mod.LodTree.build(cloud, 21, 64, threads=16)
+    proj = skm_mod.perspective(np.pi / 3, 16 / 9, 0.2, 1000.0)
+    for k, (eye, pixel, budget) in enumerate([((0, 3, 40), 1.5, 0), ((10, 1, 5), 1.5, 0), ((0, 50, 1), 0.5, 50_000),
+                                              ((-30, 2, -30), 4.0, 0), ((5, 2, 5), 1.0, 20_000)]):
+        view = skm_mod.look_at(eye)
+        both(renderer, oracle, tree, view, proj, np.asarray(eye, np.float32), 1920, 1080, budget, pixel)
+
+
+def test_lod_world_frame_matches_oracle(renderer, skm_mod, oracle):
+    say("case: LOD world frame (cut -> indexed cull -> sort -> tiles -> raster)")
+    cloud = skm_mod.make_cloud(150_000, 1, seed=66, layout=1, spread=10.0, area=30.0, log_scale_mean=-3.5)
+    tree = skm_mod.LodTree.build(cloud, 21, 256, threads=16)
+    renderer.upload_lod_world(tree, budget=60_000)
+    assert renderer.world() == (tree.node_count, 1)
+    eye = np.array([0, 2.0, 18.0], np.float32)
+    view = skm_mod.look_at(eye)
+    proj = skm_mod.perspective(np.pi / 3, 960 / 540, 0.2, 500.0)
+    renderer.set_extent(960, 540)
+    renderer.apply_policy(sort_depth=32, lod_error=1.5)
+    renderer.render(view, proj, eye)
+    cut, stats = renderer.read_lod_cut()
+    cam = oracle.make_camera(view, proj, 960, 540, eye)
+    o_cut, o_stats = oracle.lod_select(cam, tree.clusters, tree.leaves, tree.rounds, 60_000, 1.5, 4.0, True)
+    assert np.array_equal(cut, o_cut)
+    assert renderer.stats()["selected"] == len(o_cut)
+    g = gpu_frame(renderer, skm_mod, None, view, proj, eye, 960, 540, upload=False, lod_error=1.5)
+    o = oracle.frame(tree.nodes, view, proj, eye, 960, 540, active=o_cut)
+    assert_parity(g, o, label="LOD frame")

@@ -23,11 +23,11 @@ def say(msg):
 
 
 def gpu_frame(r, skm_mod, cloud, view, proj, eye, w, h, key16=False, min_px=1.0, ranges=None, upload=True,
-              background=cases.BACKGROUND, reveal=None, sh_degree=3):
+              background=cases.BACKGROUND, reveal=None, sh_degree=3, lod_error=1.0):
     if upload:
         r.upload_world(cloud)
     r.set_extent(w, h)
-    r.apply_policy(sort_depth=16 if key16 else 32, subpixel=min_px, background=background)
+    r.apply_policy(sort_depth=16 if key16 else 32, subpixel=min_px, background=background, lod_error=lod_error)
     r.render(view, proj, eye, ranges=ranges, reveal=reveal, sh_degree=sh_degree)
     count, keys, proj_rec = r.read_visible()
     order = r.read_order(count)
