@@ -379,43 +379,50 @@ __global__ __launch_bounds__(256) void k_prologue(Ctrl* ctrl, uint32_t* hist, ui
   for (uint32_t i = gid; i < range_words; i += stride) ranges[i] = 0;
 }
 
+// 256 threads, kCullItems lanes each: lane t = tile * 1024 + k * 256 + thread. The tile's
+// survivors are ranked in lane order (k, wave, lane), so the compaction keeps lane order. Blocks
+// past the tiles this frame needs exit before they take a ticket.
 template <int DEG, bool INDEXED, bool KEY16>
 __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
+  constexpr uint32_t kItems = kCullItems;
+  constexpr uint32_t kTile = 256u * kItems;
   __shared__ uint32_t s_ticket;
-  __shared__ uint32_t s_wave[4];
+  __shared__ uint32_t s_seg[kItems * 4];
   __shared__ uint32_t s_prefix;
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x, wave = tid >> 6;
+  const uint32_t total = INDEXED ? min(*a.active_count, a.capacity) : a.total;
+  const uint32_t blocks = (total + kTile - 1u) / kTile;
+  if (blockIdx.x >= blocks) return;
   if (tid == 0) s_ticket = atomicAdd(&a.ctrl->ticket[kTicketCull], 1u);
   __syncthreads();
   const uint32_t b = s_ticket;
-  const uint32_t total = INDEXED ? min(*a.active_count, a.capacity) : a.total;
-  const uint32_t blocks = (total + 255u) / 256u;
-  if (b >= blocks) return;
   const Camera& cam = *a.cam;
-  const uint32_t t = b * 256u + tid;
-  bool visible = false;
-  uint32_t key = 0;
-  uint32_t rec[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (t < total) {
-    uint32_t index;
-    if constexpr (INDEXED) {
-      index = a.active[t];
-    } else {
-      const uint32_t r = find_range(a.range_starts, a.range_count, t);
-      index = a.ranges[2 * r] + (t - a.range_starts[r]);
-    }
-    visible = cull_lane<DEG, KEY16>(cam, a.world, index, key, rec);
-  }
-  const uint64_t ballot = __ballot(visible);
-  const uint32_t wave = tid >> 6;
-  if (lane_id() == 0) s_wave[wave] = static_cast<uint32_t>(__popcll(ballot));
-  __syncthreads();
-  uint32_t before = 0, aggregate = 0;
+  uint32_t key[kItems], rec[kItems][8];
+  uint64_t ballot[kItems];
 #pragma unroll
-  for (uint32_t w = 0; w < 4; ++w) {
-    before += (w < wave) ? s_wave[w] : 0u;
-    aggregate += s_wave[w];
+  for (uint32_t k = 0; k < kItems; ++k) {
+    const uint32_t t = b * kTile + k * 256u + tid;
+    bool visible = false;
+    key[k] = 0;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) rec[k][w] = 0;
+    if (t < total) {
+      uint32_t index;
+      if constexpr (INDEXED) {
+        index = a.active[t];
+      } else {
+        const uint32_t r = find_range(a.range_starts, a.range_count, t);
+        index = a.ranges[2 * r] + (t - a.range_starts[r]);
+      }
+      visible = cull_lane<DEG, KEY16>(cam, a.world, index, key[k], rec[k]);
+    }
+    ballot[k] = __ballot(visible);
+    if (lane_id() == 0) s_seg[k * 4 + wave] = static_cast<uint32_t>(__popcll(ballot[k]));
   }
+  __syncthreads();
+  uint32_t aggregate = 0;
+#pragma unroll
+  for (uint32_t g = 0; g < kItems * 4; ++g) aggregate += s_seg[g];
   if (wave == 0) {
     const uint32_t tag = make_tag(a.ctrl->epoch, 0);
     uint32_t prefix = 0;
@@ -430,12 +437,21 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
     if (lane_id() == 0) s_prefix = prefix;
   }
   __syncthreads();
-  if (visible) {
-    const uint32_t off = s_prefix + before + rank_below(ballot);
-    a.cull_keys[off] = key;
-    uint4* dst = reinterpret_cast<uint4*>(a.projected + static_cast<size_t>(off) * 8u);
-    dst[0] = make_uint4(rec[0], rec[1], rec[2], rec[3]);
-    dst[1] = make_uint4(rec[4], rec[5], rec[6], rec[7]);
+  uint32_t before = s_prefix;
+#pragma unroll
+  for (uint32_t k = 0; k < kItems; ++k) {
+    uint32_t seg = before;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) seg += (w < wave) ? s_seg[k * 4 + w] : 0u;
+    if ((ballot[k] >> lane_id()) & 1ull) {
+      const uint32_t off = seg + rank_below(ballot[k]);
+      a.cull_keys[off] = key[k];
+      uint4* dst = reinterpret_cast<uint4*>(a.projected + static_cast<size_t>(off) * 8u);
+      dst[0] = make_uint4(rec[k][0], rec[k][1], rec[k][2], rec[k][3]);
+      dst[1] = make_uint4(rec[k][4], rec[k][5], rec[k][6], rec[k][7]);
+    }
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) before += s_seg[k * 4 + w];
   }
   if (b == blocks - 1u && tid == 0) a.ctrl->visible = s_prefix + aggregate;
 }
@@ -477,14 +493,14 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
   __shared__ uint32_t s_keys[kSortTile];
   __shared__ uint32_t s_vals[kSortTile];
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
+  const uint32_t n = *n_ptr;
+  const uint32_t blocks = (n + kSortTile - 1u) / kSortTile;
+  if (blockIdx.x >= blocks) return;  // before the ticket: exactly `blocks` tickets are taken
   if (tid == 0) s_ticket = atomicAdd(&ctrl->ticket[ticket], 1u);
 #pragma unroll
   for (uint32_t w = 0; w < 4; ++w) s_wave_count[w][tid] = 0;
   __syncthreads();
   const uint32_t b = s_ticket;
-  const uint32_t n = *n_ptr;
-  const uint32_t blocks = (n + kSortTile - 1u) / kSortTile;
-  if (b >= blocks) return;
   const uint32_t base = b * kSortTile;
   const uint32_t tag = make_tag(ctrl->epoch, tag_pass);
 
@@ -535,17 +551,27 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
     status_store(st + tid, status_word(tag, kFlagPrefix, count));
   } else {
     status_store(st + tid, status_word(tag, kFlagAggregate, count));
+    // Four predecessors per round trip, nearest first, until one holds an inclusive prefix.
     int32_t k = static_cast<int32_t>(b) - 1;
     while (k >= 0) {
-      const uint64_t s = status_load(status + static_cast<size_t>(k) * kRadix + tid);
-      const uint32_t flag = static_cast<uint32_t>(s >> 32) & 3u;
-      if (static_cast<uint32_t>(s >> 34) != tag || flag == 0u) {
-        __builtin_amdgcn_s_sleep(1);
-        continue;
+      uint64_t s4[4];
+#pragma unroll
+      for (int32_t j = 0; j < 4; ++j)
+        s4[j] = (k - j >= 0) ? status_load(status + static_cast<size_t>(k - j) * kRadix + tid) : 0ull;
+      int32_t j = 0;
+      bool done = false;
+      for (; j < 4 && k - j >= 0; ++j) {
+        const uint32_t flag = static_cast<uint32_t>(s4[j] >> 32) & 3u;
+        if (static_cast<uint32_t>(s4[j] >> 34) != tag || flag == 0u) break;  // not published yet
+        excl += static_cast<uint32_t>(s4[j]);
+        if (flag == kFlagPrefix) {
+          done = true;
+          break;
+        }
       }
-      excl += static_cast<uint32_t>(s);
-      if (flag == kFlagPrefix) break;
-      --k;
+      if (done) break;
+      if (j == 0) __builtin_amdgcn_s_sleep(1);
+      k -= j;
     }
     status_store(st + tid, status_word(tag, kFlagPrefix, excl + count));
   }
@@ -609,29 +635,42 @@ __device__ __forceinline__ uint32_t tile_rect(const uint32_t* p, uint32_t width,
 // Tile pairs of the depth-ordered survivors: counts, a block scan plus look-back for the global
 // offset, then the block emits its pairs cooperatively (pair e of the block belongs to the last
 // survivor whose exclusive offset is <= e), so writes are contiguous whatever the footprints.
+// 1024 survivors per block: position i = tile * 1024 + k * 256 + thread.
 __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __restrict__ order) {
+  constexpr uint32_t kItems = kBinItems;
+  constexpr uint32_t kTile = 256u * kItems;
   __shared__ uint32_t s_ticket, s_prefix;
   __shared__ uint32_t s_scan[4];
-  __shared__ uint32_t s_excl[256], s_val[256], s_origin[256], s_width[256];
+  __shared__ uint32_t s_excl[kTile], s_val[kTile], s_origin[kTile], s_width[kTile];
   const uint32_t tid = threadIdx.x;
+  const uint32_t n = a.ctrl->visible;
+  const uint32_t blocks = (n + kTile - 1u) / kTile;
+  if (blockIdx.x >= blocks) return;
   if (tid == 0) s_ticket = atomicAdd(&a.ctrl->ticket[kTicketBin], 1u);
   __syncthreads();
   const uint32_t b = s_ticket;
-  const uint32_t n = a.ctrl->visible;
-  const uint32_t blocks = (n + 255u) / 256u;
-  if (b >= blocks) return;
-  const uint32_t i = b * 256u + tid;
-  uint32_t count = 0, value = 0;
-  Rect r{0, 0, -1, -1};
-  if (i < n) {
-    value = order[i];
-    const uint4* src = reinterpret_cast<const uint4*>(a.projected + static_cast<size_t>(value) * 8u);
-    const uint4 w0 = src[0], w1 = src[1];
-    const uint32_t p[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
-    count = tile_rect(p, a.width, a.height, a.tile, r);
+  uint32_t aggregate = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kItems; ++k) {
+    const uint32_t slot = k * 256u + tid;
+    const uint32_t i = b * kTile + slot;
+    uint32_t count = 0, value = 0;
+    Rect r{0, 0, -1, -1};
+    if (i < n) {
+      value = order[i];
+      const uint4* src = reinterpret_cast<const uint4*>(a.projected + static_cast<size_t>(value) * 8u);
+      const uint4 w0 = src[0], w1 = src[1];
+      const uint32_t p[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+      count = tile_rect(p, a.width, a.height, a.tile, r);
+    }
+    uint32_t row_total;
+    const uint32_t excl = block_exclusive_scan(count, s_scan, &row_total);
+    s_excl[slot] = aggregate + excl;
+    s_val[slot] = value;
+    s_origin[slot] = static_cast<uint32_t>(r.tx0) | (static_cast<uint32_t>(r.ty0) << 16);
+    s_width[slot] = static_cast<uint32_t>(r.tx1 - r.tx0 + 1);
+    aggregate += row_total;
   }
-  uint32_t aggregate;
-  const uint32_t excl = block_exclusive_scan(count, s_scan, &aggregate);
   if ((tid >> 6) == 0) {
     const uint32_t tag = make_tag(a.ctrl->epoch, 1);
     uint32_t prefix = 0;
@@ -645,15 +684,11 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
     }
     if (lane_id() == 0) s_prefix = prefix;
   }
-  s_excl[tid] = excl;
-  s_val[tid] = value;
-  s_origin[tid] = static_cast<uint32_t>(r.tx0) | (static_cast<uint32_t>(r.ty0) << 16);
-  s_width[tid] = static_cast<uint32_t>(r.tx1 - r.tx0 + 1);
   __syncthreads();
   const uint64_t prefix = s_prefix;
   for (uint32_t e = tid; e < aggregate; e += 256u) {
-    uint32_t lo = 0, hi = 255;
-    while (lo < hi) {  // last j with s_excl[j] <= e
+    uint32_t lo = 0, hi = kTile - 1u;
+    while (lo < hi) {  // last slot with s_excl[slot] <= e
       const uint32_t mid = (lo + hi + 1u) / 2u;
       if (s_excl[mid] <= e) lo = mid; else hi = mid - 1u;
     }
@@ -820,7 +855,7 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes
   hipLaunchKernelGGL(k_prologue, dim3(min(grid_for(tiles * 2u, 256), 256u)), dim3(256), 0, s, a.ctrl,
                      a.hist, 2u * 4u * 256u, a.tile_ranges, tiles * 2u);
   const uint32_t lanes = a.active != nullptr ? a.capacity : a.total;
-  const uint32_t cull_blocks = grid_for(lanes, 256);
+  const uint32_t cull_blocks = grid_for(lanes, 256u * kCullItems);
   switch (a.world.sh_degree) {
     case 0: launch_cull_deg<0>(a, cull_blocks, s); break;
     case 1: launch_cull_deg<1>(a, cull_blocks, s); break;
@@ -842,7 +877,7 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes
   }
   const uint32_t* order = a.order[(depth_passes - 1) & 1];
   if (marks) (void)hipEventRecord(marks[3], s);
-  hipLaunchKernelGGL(k_bin, dim3(grid_for(a.capacity, 256)), dim3(256), 0, s, a, order);
+  hipLaunchKernelGGL(k_bin, dim3(grid_for(a.capacity, 256u * kBinItems)), dim3(256), 0, s, a, order);
   const uint32_t* sort_n = &a.ctrl->sort_pairs;
   hipLaunchKernelGGL(k_pair_count, dim3(1), dim3(64), 0, s, a.ctrl);
   uint32_t* tile_hist = a.hist + 4 * 256;

@@ -170,12 +170,14 @@ __global__ void k_lod_prologue(LodArgs a) {
 __global__ __launch_bounds__(256) void k_lod_eval(LodArgs a, uint32_t round) {
   __shared__ uint32_t s_ticket, s_prefix[2], s_scan[4];
   const uint32_t tid = threadIdx.x;
+  const uint32_t active = a.state[round].active;
+  // At least one block runs, even for an empty frontier: it writes the round's totals. Blocks past
+  // the frontier exit before they take a ticket.
+  const uint32_t blocks = max((active + 255u) / 256u, 1u);
+  if (blockIdx.x >= blocks) return;
   if (tid == 0) s_ticket = atomicAdd(&a.ctrl->ticket[2 * round], 1u);
   __syncthreads();
   const uint32_t b = s_ticket;
-  const uint32_t active = a.state[round].active;
-  const uint32_t blocks = (active + 255u) / 256u;
-  if (b >= blocks) return;
   const uint32_t t = b * 256u + tid;
   uint32_t extra = 0, flags = 0, dropped = 0;
   if (t < active) {
@@ -236,12 +238,13 @@ __global__ void k_lod_budget(LodArgs a, uint32_t round) {
 __global__ __launch_bounds__(256) void k_lod_compact(LodArgs a, uint32_t round) {
   __shared__ uint32_t s_ticket, s_prefix[3], s_scan[4];
   const uint32_t tid = threadIdx.x;
+  const LodState s = a.state[round];
+  // At least one block runs, even for an empty frontier: it carries the state to the next round.
+  const uint32_t blocks = max((s.active + 255u) / 256u, 1u);
+  if (blockIdx.x >= blocks) return;
   if (tid == 0) s_ticket = atomicAdd(&a.ctrl->ticket[2 * round + 1], 1u);
   __syncthreads();
   const uint32_t b = s_ticket;
-  const LodState s = a.state[round];
-  const uint32_t blocks = (s.active + 255u) / 256u;
-  if (b >= blocks) return;
   const uint32_t t = b * 256u + tid;
   uint32_t kids = 0, splats = 0, packets = 0, denied = 0, flags = kDrop;
   LodCluster node{};

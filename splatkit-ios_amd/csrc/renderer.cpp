@@ -439,7 +439,7 @@ struct skm_renderer {
   }
 
   skm_status ensure_pairs(uint64_t want) {
-    want = std::min<uint64_t>(std::max<uint64_t>(want, 4096), 0xfffff000ull);
+    want = std::min<uint64_t>(std::max<uint64_t>(want, uint64_t{1} << 22), 0xfffff000ull);
     if (want <= pair_capacity) return SKM_OK;
     DeviceBuffer k0, k1, v0, v1;
     const size_t n = static_cast<size_t>(want);
@@ -558,7 +558,7 @@ skm_status skm_renderer_create(const skm_config* config, skm_renderer** out) {
          hipEventCreate(&s.done) == hipSuccess;
     for (auto& m : s.marks) ok = ok && hipEventCreate(&m) == hipSuccess;
   }
-  ok = ok && r->ensure_capacity(1) == SKM_OK && r->ensure_pairs(4096) == SKM_OK &&
+  ok = ok && r->ensure_capacity(1) == SKM_OK && r->ensure_pairs(0) == SKM_OK &&
        r->ensure_tiles(1) == SKM_OK && hipStreamSynchronize(r->stream) == hipSuccess;
   if (!ok) {
     skm_renderer_destroy(r);
@@ -812,10 +812,11 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   if (f->label_styles != nullptr) return r->fail(SKM_ERR_UNSUPPORTED, "part labels are not supported");
   if (f->range_count > skm::kMaxRanges || (f->range_count > 0 && f->ranges == nullptr))
     return r->fail(SKM_ERR_INVALID, "invalid ranges");
-  // Validate before writing any slot input (MetalVisibilityPass.mm:155-162).
+  // Validate before writing any slot input (MetalVisibilityPass.mm:155-162). A tree world draws its
+  // cut and ignores the ranges (MetalFrameCompute.mm:213-215).
   const uint32_t source = r->has_world ? r->world.count : 0;
   uint64_t total = 0;
-  for (uint32_t i = 0; i < f->range_count; ++i) {
+  for (uint32_t i = 0; i < (r->lod_world ? 0u : f->range_count); ++i) {
     const skm_range& g = f->ranges[i];
     if (g.offset > source || g.count > source - g.offset || total + g.count > r->capacity)
       return r->fail(SKM_ERR_INVALID, "range %u [%u, +%u) invalid for %u splats", i, g.offset, g.count, source);

@@ -76,6 +76,8 @@ enum Ticket : uint32_t {
 };
 
 constexpr uint32_t kThreads = 256;
+constexpr uint32_t kCullItems = 4;  // lanes per thread of the cull (a 1024-lane tile per block)
+constexpr uint32_t kBinItems = 4;   // survivors per thread of the binning (1024 per block)
 constexpr uint32_t kSortItems = 16;
 constexpr uint32_t kSortTile = kThreads * kSortItems;  // 4096 keys per onesweep block
 constexpr uint32_t kRadix = 256;
@@ -100,7 +102,7 @@ struct FrameArgs {
   uint32_t* keys[2];    // sort ping-pong
   uint32_t* order[2];
   uint32_t* projected;  // 8 words per survivor
-  uint64_t* scan_status;
+  uint64_t* scan_status;  // one word per 1024-lane tile
   uint64_t* sort_status;
   uint32_t* hist;       // [2][4][256]: depth digits, tile digits
   uint32_t* pair_keys[2];

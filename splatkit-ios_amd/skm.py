@@ -431,8 +431,17 @@ class Renderer:
         return self._check(self._lib.skm_draw(self._h, C.byref(frame)))
 
     def render(self, view, proj, cam_pos, **kw):
-        self.draw(self.make_frame(view, proj, cam_pos, **kw))
-        self.finish()
+        """Draws one frame and waits for it. A frame whose tile pairs outgrew the capacity is
+        dropped and the capacity grows; it is then drawn once more."""
+        frame = self.make_frame(view, proj, cam_pos, **kw)
+        self.draw(frame)
+        try:
+            self.finish()
+        except SkmError as e:
+            if e.status != SKM_ERR_CAPACITY:
+                raise
+            self.draw(frame)
+            self.finish()
 
     def finish(self):
         return self._check(self._lib.skm_finish(self._h))
