@@ -52,6 +52,10 @@ constexpr uint32_t kDepthKeyMax = 65535;
 // Tile binning pads the quad's bounding box by this many pixels so the fp32 coverage rule can
 // never accept a pixel outside the listed tiles.
 constexpr float kBinPad = 1.0f / 16.0f;
+// 1 / kGaussianScale^2 = 2 ln 2: the footprint disc's radius^2 in axis units per unit of log2 alpha
+// above the cutoff, and the slack its bound carries over the fp32 rounding of the coverage rule.
+constexpr float kInvGaussianScale2 = 1.38629436f;
+constexpr float kFootprintSlack = 1.0009765625f;
 constexpr double kLn2 = 0.69314718055994530942;
 constexpr double kLn2Hi = 6.93147180369123816490e-01;
 constexpr double kLn2Lo = 1.90821492927058770002e-10;
@@ -399,15 +403,23 @@ bool visibility_lane(const orc_camera& cam, const uint32_t* splats, const uint32
 }
 
 // ---------------------------------------------------------------------------------------------
-// Raster rule. Per record: centre in pixels, inverse of the half-packed axis matrix, radius,
-// log2 of the half peak opacity (clamped like splatDepth, SplatProjection.metalh:169-171) and the
-// colour clamped to packColor's range [0, kMaxColorScale] (:146-149).
+// Raster rule. Per record: centre in pixels, the inverse of the half-packed axis matrix scaled by
+// kGaussianScale (so the quad coordinates come out in units where the Gaussian's log2 falloff is
+// their squared length), the radius in the same units, log2 of the half peak opacity (clamped like
+// splatDepth, SplatProjection.metalh:169-171) and the colour clamped to packColor's range
+// [0, kMaxColorScale] (:146-149).
 struct Prep {
-  float cx, cy, iux, iuy, ivx, ivy, r, log2a0;
+  float cx, cy, g00, g01, g10, g11, rk, log2a0;
   float c[3];
   // FP64 copies for the oracle's values.
   double cx64, cy64, a1x, a1y, a2x, a2y, a0;
 };
+
+float log2_peak(const uint32_t* p) {
+  const float a0 = hi_half(p[5]);
+  const float l2 = static_cast<float>(log_d(static_cast<double>(mx(a0, kHalfMin))) / kLn2);
+  return mn(mx(l2, -kMaxLog2Alpha), kMaxLog2Alpha);
+}
 
 Prep prep(const uint32_t* p, uint32_t width, uint32_t height) {
   Prep q;
@@ -417,15 +429,14 @@ Prep prep(const uint32_t* p, uint32_t width, uint32_t height) {
   const float a1x = lo_half(p[2]), a1y = hi_half(p[2]);
   const float a2x = lo_half(p[3]), a2y = hi_half(p[3]);
   const float det = a1x * a2y - a2x * a1y;
-  const float inv = 1.0f / det;
-  q.iux = a2y * inv;
-  q.iuy = -(a2x * inv);
-  q.ivx = -(a1y * inv);
-  q.ivy = a1x * inv;
-  q.r = f32(p[6]);
+  const float s = kGaussianScale / det;
+  q.g00 = a2y * s;
+  q.g01 = -(a2x * s);
+  q.g10 = -(a1y * s);
+  q.g11 = a1x * s;
+  q.rk = kGaussianScale * f32(p[6]);
+  q.log2a0 = log2_peak(p);
   const float a0 = hi_half(p[5]);
-  const float l2 = static_cast<float>(log_d(static_cast<double>(mx(a0, kHalfMin))) / kLn2);
-  q.log2a0 = mn(mx(l2, -kMaxLog2Alpha), kMaxLog2Alpha);
   const float col[3] = {lo_half(p[4]), hi_half(p[4]), lo_half(p[5])};
   for (int c = 0; c < 3; ++c) q.c[c] = mn(mx(col[c], 0.0f), kMaxColorScale);
   q.cx64 = (static_cast<double>(ncx) * 0.5 + 0.5) * width;
@@ -440,16 +451,16 @@ Prep prep(const uint32_t* p, uint32_t width, uint32_t height) {
 
 // The fp32 coverage rule: the pixel centre lies in the quad c + (u a1 + v a2), |u|,|v| <= r
 // (expandQuad, SplatProjection.metalh:155-165) and the Gaussian's log2 alpha is at or above
-// kLog2MinAlpha (splatFragmentUnder, SplatRaster.metal:40-44).
+// kLog2MinAlpha (splatFragmentUnder, SplatRaster.metal:40-44). In scaled units gu = K u,
+// gv = K v: |gu|,|gv| <= K r and log2 alpha = log2a0 - (gu^2 + gv^2), with two fused
+// multiply-adds per coordinate and one for the square sum.
 inline bool decide(const Prep& q, uint32_t px, uint32_t py, float* power) {
   const float dx = (static_cast<float>(px) + 0.5f) - q.cx;
   const float dy = q.cy - (static_cast<float>(py) + 0.5f);
-  const float u = q.iux * dx + q.iuy * dy;
-  const float v = q.ivx * dx + q.ivy * dy;
-  if (!(std::fabs(u) <= q.r && std::fabs(v) <= q.r)) return false;
-  const float gu = kGaussianScale * u;
-  const float gv = kGaussianScale * v;
-  *power = q.log2a0 - (gu * gu + gv * gv);
+  const float gu = std::fmaf(q.g00, dx, q.g01 * dy);
+  const float gv = std::fmaf(q.g10, dx, q.g11 * dy);
+  *power = q.log2a0 - std::fmaf(gu, gu, gv * gv);
+  if (!(std::fabs(gu) <= q.rk && std::fabs(gv) <= q.rk)) return false;
   return *power >= kLog2MinAlpha;
 }
 
@@ -514,8 +525,8 @@ void raster_tiles(const uint32_t* projected, const uint32_t* pair_values, const 
               if (!decide(q, px, py, &power)) continue;
               const float a = std::exp2(mn(power, 0.0f));
               const float w = T * a;
-              for (int c = 0; c < 3; ++c) C[c] = C[c] + w * q.c[c];
-              T = T * (1.0f - a);
+              for (int c = 0; c < 3; ++c) C[c] = std::fmaf(w, q.c[c], C[c]);
+              T = T - w;
               if (T < t_stop) break;
             }
             for (int c = 0; c < 3; ++c) o[c] = C[c] + T * bg[c];
@@ -778,20 +789,40 @@ void orc_argsort(const uint32_t* keys, uint32_t n, uint32_t* order) {
   std::stable_sort(order, order + n, [keys](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
 }
 
-// Tile rectangle of the raster quad: the bounding box c +- r(|a1| + |a2|) of expandQuad's
-// parallelogram on the half-packed axes (SplatProjection.metalh:112-116,155-165), in pixel
-// indices whose centres it may contain, padded by kBinPad, in 16x16 (or `tile`) tiles.
+// Footprint half-extents in pixels: every pixel centre the coverage rule accepts lies within
+// (ex, ey) of the centre. The quad bounds the offset by r(|a1| + |a2|) per axis (expandQuad,
+// SplatProjection.metalh:155-165); the cutoff bounds gu^2 + gv^2 by L = log2a0 - kLog2MinAlpha,
+// so u^2 + v^2 <= L / K^2 and |u a1x + v a2x| <= sqrt(L / K^2) sqrt(a1x^2 + a2x^2) (Cauchy-Schwarz).
+// Each axis takes the smaller bound, the disc's with slack over the rule's fp32 rounding. False when
+// L < 0: no pixel passes the cutoff.
+static bool footprint(const uint32_t* p, float* ex, float* ey) {
+  const float cut = log2_peak(p) - kLog2MinAlpha;
+  if (!(cut >= 0.0f)) return false;
+  const float rho = std::sqrt(cut * kInvGaussianScale2) * kFootprintSlack;
+  const float a1x = lo_half(p[2]), a1y = hi_half(p[2]);
+  const float a2x = lo_half(p[3]), a2y = hi_half(p[3]);
+  const float r = f32(p[6]);
+  const float qx = r * (std::fabs(a1x) + std::fabs(a2x));
+  const float qy = r * (std::fabs(a1y) + std::fabs(a2y));
+  const float sx = rho * std::sqrt(a1x * a1x + a2x * a2x);
+  const float sy = rho * std::sqrt(a1y * a1y + a2y * a2y);
+  *ex = mn(qx, sx);
+  *ey = mn(qy, sy);
+  return true;
+}
+
+// Tile rectangle of a record: the footprint box around the centre, in pixel indices whose centres
+// it may contain, padded by kBinPad, in 16x16 (or `tile`) tiles.
 uint32_t orc_tile_rect(const uint32_t* p, uint32_t width, uint32_t height, uint32_t tile,
                        int32_t* rect) {
   const float cx = (f32(p[0]) * 0.5f + 0.5f) * static_cast<float>(width);
   const float cy = (0.5f - f32(p[1]) * 0.5f) * static_cast<float>(height);
-  const float r = f32(p[6]);
-  const float ex = r * (std::fabs(lo_half(p[2])) + std::fabs(lo_half(p[3])));
-  const float ey = r * (std::fabs(hi_half(p[2])) + std::fabs(hi_half(p[3])));
-  float x0 = ((cx - ex) - 0.5f) - kBinPad, x1 = ((cx + ex) - 0.5f) + kBinPad;
-  float y0 = ((cy - ey) - 0.5f) - kBinPad, y1 = ((cy + ey) - 0.5f) + kBinPad;
   rect[0] = rect[1] = 0;
   rect[2] = rect[3] = -1;
+  float ex = 0.0f, ey = 0.0f;
+  if (!footprint(p, &ex, &ey)) return 0;
+  float x0 = ((cx - ex) - 0.5f) - kBinPad, x1 = ((cx + ex) - 0.5f) + kBinPad;
+  float y0 = ((cy - ey) - 0.5f) - kBinPad, y1 = ((cy + ey) - 0.5f) + kBinPad;
   if (!(x0 <= x1) || !(y0 <= y1)) return 0;
   x0 = mx(x0, -1.0f);
   x1 = mn(x1, static_cast<float>(width));
@@ -871,6 +902,37 @@ void orc_raster32(const uint32_t* projected, const uint32_t* pair_values, const 
                   float t_stop, uint32_t threads, float* out) {
   raster_tiles<false>(projected, pair_values, ranges, width, height, tile, background3, t_stop,
                       std::max(1u, threads), out);
+}
+
+uint64_t orc_missed_coverage(const uint32_t* projected, uint32_t count, uint32_t width,
+                             uint32_t height, uint32_t tile, const uint32_t* pair_values,
+                             const uint32_t* ranges, uint64_t* accepted) {
+  const uint32_t tiles_x = (width + tile - 1) / tile;
+  uint64_t missed = 0, taken = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const uint32_t* p = projected + size_t{i} * 8u;
+    const Prep q = prep(p, width, height);
+    const double r = f32(p[6]);
+    const double ex = r * (std::fabs(q.a1x) + std::fabs(q.a2x)) + 2.0;
+    const double ey = r * (std::fabs(q.a1y) + std::fabs(q.a2y)) + 2.0;
+    if (!(ex == ex) || !(ey == ey)) continue;
+    const int64_t x0 = std::max<int64_t>(0, static_cast<int64_t>(std::floor(q.cx - ex)));
+    const int64_t x1 = std::min<int64_t>(width - 1, static_cast<int64_t>(std::ceil(q.cx + ex)));
+    const int64_t y0 = std::max<int64_t>(0, static_cast<int64_t>(std::floor(q.cy - ey)));
+    const int64_t y1 = std::min<int64_t>(height - 1, static_cast<int64_t>(std::ceil(q.cy + ey)));
+    for (int64_t py = y0; py <= y1; ++py)
+      for (int64_t px = x0; px <= x1; ++px) {
+        float power;
+        if (!decide(q, static_cast<uint32_t>(px), static_cast<uint32_t>(py), &power)) continue;
+        ++taken;
+        const uint32_t t = static_cast<uint32_t>(py / tile) * tiles_x + static_cast<uint32_t>(px / tile);
+        const uint32_t* b = pair_values + ranges[2 * t];
+        const uint32_t* e = pair_values + ranges[2 * t + 1];
+        if (std::find(b, e, i) == e) ++missed;
+      }
+  }
+  if (accepted != nullptr) *accepted = taken;
+  return missed;
 }
 
 int orc_decide(const uint32_t* projected8, uint32_t width, uint32_t height, uint32_t px,

@@ -77,6 +77,13 @@ uint64_t orc_bin(const uint32_t* projected, const uint32_t* order, uint32_t n, u
 uint32_t orc_tile_rect(const uint32_t* projected8, uint32_t width, uint32_t height,
                        uint32_t tile, int32_t* rect);
 
+/* Brute-force check of the binning: counts (pixel, record) pairs the coverage rule accepts whose
+ * record is missing from the pixel's tile list. Scans every pixel of the quad's loose bounding box
+ * c +- r(|a1| + |a2|) plus two pixels per record. Test infrastructure. */
+uint64_t orc_missed_coverage(const uint32_t* projected, uint32_t count, uint32_t width,
+                             uint32_t height, uint32_t tile, const uint32_t* pair_values,
+                             const uint32_t* ranges, uint64_t* accepted);
+
 /* FP64 composite over the tile lists (no early termination). out: RGBA float, top-down rows. */
 void orc_raster64(const uint32_t* projected, const uint32_t* pair_values, const uint32_t* ranges,
                   uint32_t width, uint32_t height, uint32_t tile, const float* background3,

@@ -54,6 +54,8 @@ _API = {
     "orc_lod_select": (C.c_uint32, [C.POINTER(Camera), _u32p, C.c_uint32, _u32p, C.c_uint32, C.c_uint32, C.c_float,
                                     C.c_float, C.c_int, _u32p, _u32p]),
     "orc_lod_cluster": (C.c_int, [C.POINTER(Camera), _u32p, C.c_float, _fp]),
+    "orc_missed_coverage": (C.c_uint64, [_u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _u32p, _u32p,
+                                         C.POINTER(C.c_uint64)]),
     "orc_log_d": (C.c_double, [C.c_double]),
     "orc_exp_d": (C.c_double, [C.c_double]),
 }
@@ -205,6 +207,18 @@ def raster32(projected, pair_values, ranges, width, height, background=(0.05, 0.
                        up(np.ascontiguousarray(ranges, np.uint32)), width, height, tile, fp(bg), T_STOP, threads,
                        fp(out))
     return out
+
+
+def missed_coverage(projected, count, pair_values, ranges, width, height, tile=TILE):
+    """(missed, accepted): accepted (pixel, record) pairs of the coverage rule, and how many of
+    them are missing from their pixel's tile list."""
+    acc = C.c_uint64()
+    pv = np.ascontiguousarray(pair_values, np.uint32)
+    if pv.size == 0:
+        pv = np.zeros(1, np.uint32)
+    missed = lib().orc_missed_coverage(up(np.ascontiguousarray(projected, np.uint32)), count, width, height, tile,
+                                       up(pv), up(np.ascontiguousarray(ranges, np.uint32)), C.byref(acc))
+    return int(missed), int(acc.value)
 
 
 def decide(record, width, height, px, py):

@@ -598,6 +598,32 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
   }
 }
 
+// log2 of a record's half peak opacity, clamped like splatDepth (SplatProjection.metalh:169-171).
+__device__ __forceinline__ float log2_peak(uint32_t w5) {
+  const float a0 = hi_half(w5);
+  const float l2 = static_cast<float>(log_d(static_cast<double>(mx(a0, kHalfMin))) / kLn2);
+  return mn(mx(l2, -kMaxLog2Alpha), kMaxLog2Alpha);
+}
+
+// Footprint half-extents in pixels (the oracle's footprint, same specification): the smaller of the
+// quad's box r(|a1| + |a2|) and the cutoff disc's box sqrt(L / K^2) sqrt(a1^2 + a2^2) per axis, the
+// latter with slack. False when no pixel can pass the cutoff.
+__device__ __forceinline__ bool footprint(const uint32_t* p, float log2a0, float& ex, float& ey) {
+  const float cut = log2a0 - kLog2MinAlpha;
+  if (!(cut >= 0.0f)) return false;
+  const float rho = sqrtf(cut * kInvGaussianScale2) * kFootprintSlack;
+  const float a1x = lo_half(p[2]), a1y = hi_half(p[2]);
+  const float a2x = lo_half(p[3]), a2y = hi_half(p[3]);
+  const float r = f32(p[6]);
+  const float qx = r * (fabsf(a1x) + fabsf(a2x));
+  const float qy = r * (fabsf(a1y) + fabsf(a2y));
+  const float sx = rho * sqrtf(a1x * a1x + a2x * a2x);
+  const float sy = rho * sqrtf(a1y * a1y + a2y * a2y);
+  ex = mn(qx, sx);
+  ey = mn(qy, sy);
+  return true;
+}
+
 // Tile rectangle of a projected record (the CPU oracle's orc_tile_rect, same specification).
 struct Rect {
   int32_t tx0, ty0, tx1, ty1;
@@ -606,13 +632,12 @@ __device__ __forceinline__ uint32_t tile_rect(const uint32_t* p, uint32_t width,
                                               uint32_t tile, Rect& r) {
   const float cx = (f32(p[0]) * 0.5f + 0.5f) * static_cast<float>(width);
   const float cy = (0.5f - f32(p[1]) * 0.5f) * static_cast<float>(height);
-  const float rad = f32(p[6]);
-  const float ex = rad * (fabsf(lo_half(p[2])) + fabsf(lo_half(p[3])));
-  const float ey = rad * (fabsf(hi_half(p[2])) + fabsf(hi_half(p[3])));
-  float x0 = ((cx - ex) - 0.5f) - kBinPad, x1 = ((cx + ex) - 0.5f) + kBinPad;
-  float y0 = ((cy - ey) - 0.5f) - kBinPad, y1 = ((cy + ey) - 0.5f) + kBinPad;
   r.tx0 = r.ty0 = 0;
   r.tx1 = r.ty1 = -1;
+  float ex = 0.0f, ey = 0.0f;
+  if (!footprint(p, log2_peak(p[5]), ex, ey)) return 0;
+  float x0 = ((cx - ex) - 0.5f) - kBinPad, x1 = ((cx + ex) - 0.5f) + kBinPad;
+  float y0 = ((cy - ey) - 0.5f) - kBinPad, y1 = ((cy + ey) - 0.5f) + kBinPad;
   if (!(x0 <= x1) || !(y0 <= y1)) return 0;
   x0 = mx(x0, -1.0f);
   x1 = mn(x1, static_cast<float>(width));
@@ -724,72 +749,115 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ key
   }
 }
 
-// 16x16 tile, one pixel per thread (4 waves). Each batch of up to 256 pairs is prepared once
-// (centre in pixels, inverse axis matrix, radius, log2 peak, clamped colour) into LDS, then
-// every pixel walks it front to back. A pixel stops when its transmittance falls below
-// kTransmittanceStop; the tile stops when all of its pixels have.
+// One coverage-and-blend step of a pixel against a prepared record (the oracle's decide and fp32
+// composite): scaled quad coordinates from the pixel centre's offset, the quad and cutoff tests
+// select alpha = 2^min(log2 alpha, 0) or 0, then w = T alpha, C += w c, T -= w. No branch: a wave
+// runs the same instructions whatever its lanes decide.
+__device__ __forceinline__ void blend_step(const float4 A, const float4 B, const float4 Cc, float fx,
+                                           float fy, float& T, float& C0, float& C1, float& C2) {
+  const float dx = fx - A.x;
+  const float dy = A.y - fy;
+  const float gu = fmaf(A.z, dx, A.w * dy);
+  const float gv = fmaf(B.x, dx, B.y * dy);
+  const float e = B.w - fmaf(gu, gu, gv * gv);
+  const bool take = static_cast<int>(fabsf(gu) <= B.z) & static_cast<int>(fabsf(gv) <= B.z) &
+                    static_cast<int>(e >= kLog2MinAlpha);
+  // e is a number when take holds, so fminf agrees with the oracle's min; 2^-inf = 0.
+  const float alpha = __builtin_amdgcn_exp2f(take ? fminf(e, 0.0f) : -__builtin_inff());
+  const float w = T * alpha;
+  C0 = fmaf(w, Cc.x, C0);
+  C1 = fmaf(w, Cc.y, C1);
+  C2 = fmaf(w, Cc.z, C2);
+  T = T - w;
+}
+
+// 16x16 tile per 256-thread block; wave q owns the 8x8 quadrant (q & 1, q >> 1), one pixel per
+// lane. Each batch of up to 256 pairs is prepared once into LDS (centre in pixels, the inverse axis
+// matrix scaled by kGaussianScale, the scaled radius, log2 peak, clamped colour), and every
+// quadrant gets the list, in batch order, of the records whose footprint box reaches one of its
+// pixel centres. A wave walks its list front to back. Lanes keep compositing after their
+// transmittance fell below kTransmittanceStop (that only adds exact terms); a wave leaves its list
+// once all of its lanes are below it, and the tile stops when every lane is.
 __global__ __launch_bounds__(256) void k_raster(FrameArgs a, const uint32_t* __restrict__ pair_vals,
                                                 const uint32_t* __restrict__ ranges) {
   __shared__ float4 s_a[256], s_b[256], s_c[256];
-  const uint32_t tid = threadIdx.x;
+  __shared__ __align__(16) uint8_t s_list[4][256];
+  __shared__ uint32_t s_count[4][4];  // [quadrant][wave]
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
   const uint32_t tile_id = blockIdx.y * a.tiles_x + blockIdx.x;
-  const uint32_t px = blockIdx.x * 16u + (tid & 15u);
-  const uint32_t py = blockIdx.y * 16u + (tid >> 4);
+  const uint32_t ox = blockIdx.x * 16u, oy = blockIdx.y * 16u;
+  const uint32_t px = ox + (wave & 1u) * 8u + (lane & 7u);
+  const uint32_t py = oy + (wave >> 1) * 8u + (lane >> 3);
   const bool in_image = px < a.width && py < a.height;
   const uint32_t start = ranges[2 * tile_id], end = ranges[2 * tile_id + 1];
   const float fw = static_cast<float>(a.width), fh = static_cast<float>(a.height);
   const float fx = static_cast<float>(px) + 0.5f, fy = static_cast<float>(py) + 0.5f;
-  float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
-  bool done = !in_image;
+  const float tx = static_cast<float>(ox) + 0.5f, ty = static_cast<float>(oy) + 0.5f;
+  float T = in_image ? 1.0f : 0.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
   for (uint32_t s0 = start; s0 < end; s0 += 256u) {
     const uint32_t j = s0 + tid;
+    uint32_t quads = 0;
     if (j < end) {
       const uint32_t idx = pair_vals[j];
       const uint4* src = reinterpret_cast<const uint4*>(a.projected + static_cast<size_t>(idx) * 8u);
       const uint4 w0 = src[0], w1 = src[1];
+      const uint32_t p[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
       const float cx = (f32(w0.x) * 0.5f + 0.5f) * fw;
       const float cy = (0.5f - f32(w0.y) * 0.5f) * fh;
       const float a1x = lo_half(w0.z), a1y = hi_half(w0.z);
       const float a2x = lo_half(w0.w), a2y = hi_half(w0.w);
       const float det = a1x * a2y - a2x * a1y;
-      const float inv = 1.0f / det;
-      const float a0 = hi_half(w1.y);
-      const float l2 = static_cast<float>(log_d(static_cast<double>(mx(a0, kHalfMin))) / kLn2);
-      const float log2a0 = mn(mx(l2, -kMaxLog2Alpha), kMaxLog2Alpha);
-      s_a[tid] = make_float4(cx, cy, a2y * inv, -(a2x * inv));
-      s_b[tid] = make_float4(-(a1y * inv), a1x * inv, f32(w1.z), log2a0);
+      const float sc = kGaussianScale / det;
+      const float log2a0 = log2_peak(w1.y);
+      float ex = 0.0f, ey = 0.0f;
+      if (footprint(p, log2a0, ex, ey)) {
+        const float x0 = (cx - ex) - kBinPad, x1 = (cx + ex) + kBinPad;
+        const float y0 = (cy - ey) - kBinPad, y1 = (cy + ey) + kBinPad;
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+          const float qx = tx + static_cast<float>((q & 1u) * 8u), qy = ty + static_cast<float>((q >> 1) * 8u);
+          if (x1 >= qx && x0 <= qx + 7.0f && y1 >= qy && y0 <= qy + 7.0f) quads |= 1u << q;
+        }
+      }
+      s_a[tid] = make_float4(cx, cy, a2y * sc, -(a2x * sc));
+      s_b[tid] = make_float4(-(a1y * sc), a1x * sc, kGaussianScale * f32(w1.z), log2a0);
       s_c[tid] = make_float4(mn(mx(lo_half(w1.x), 0.0f), kMaxColorScale),
                              mn(mx(hi_half(w1.x), 0.0f), kMaxColorScale),
                              mn(mx(lo_half(w1.y), 0.0f), kMaxColorScale), 0.0f);
     }
+    uint64_t ball[4];
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+      ball[q] = __ballot((quads >> q) & 1u);
+      if (lane == 0) s_count[q][wave] = static_cast<uint32_t>(__popcll(ball[q]));
+    }
     __syncthreads();
-    const uint32_t batch = min(256u, end - s0);
-    if (!done) {
-      for (uint32_t k = 0; k < batch; ++k) {
-        const float4 A = s_a[k], B = s_b[k];
-        const float dx = fx - A.x;
-        const float dy = A.y - fy;
-        const float u = A.z * dx + A.w * dy;
-        const float v = B.x * dx + B.y * dy;
-        if (!(fabsf(u) <= B.z && fabsf(v) <= B.z)) continue;
-        const float gu = kGaussianScale * u;
-        const float gv = kGaussianScale * v;
-        const float power = B.w - (gu * gu + gv * gv);
-        if (!(power >= kLog2MinAlpha)) continue;
-        const float alpha = exp2f(mn(power, 0.0f));
-        const float4 C = s_c[k];
-        const float w = T * alpha;
-        C0 = C0 + w * C.x;
-        C1 = C1 + w * C.y;
-        C2 = C2 + w * C.z;
-        T = T * (1.0f - alpha);
-        if (T < kTransmittanceStop) {
-          done = true;
-          break;
-        }
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+      if ((quads >> q) & 1u) {
+        const uint32_t off = rank_below(ball[q]) + (wave > 0u ? s_count[q][0] : 0u) +
+                             (wave > 1u ? s_count[q][1] : 0u) + (wave > 2u ? s_count[q][2] : 0u);
+        s_list[q][off] = static_cast<uint8_t>(tid);
       }
     }
-    if (__syncthreads_count(done ? 0 : 1) == 0) break;
+    const uint32_t n = __builtin_amdgcn_readfirstlane(s_count[wave][0] + s_count[wave][1] +
+                                                      s_count[wave][2] + s_count[wave][3]);
+    __syncthreads();
+    if (__ballot(T >= kTransmittanceStop) != 0) {
+      const uint32_t* list = reinterpret_cast<const uint32_t*>(s_list[wave]);
+      for (uint32_t i = 0; i < n; i += 4u) {
+        const uint32_t four = list[i >> 2];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+          if (i + u < n) {
+            const uint32_t k = (four >> (8u * u)) & 255u;
+            blend_step(s_a[k], s_b[k], s_c[k], fx, fy, T, C0, C1, C2);
+          }
+        }
+        if (__ballot(T >= kTransmittanceStop) == 0) break;
+      }
+    }
+    if (__syncthreads_count(T >= kTransmittanceStop ? 1 : 0) == 0) break;
   }
   if (in_image) {
     float4* out = reinterpret_cast<float4*>(a.image) + static_cast<size_t>(py) * a.width + px;

@@ -228,7 +228,7 @@ struct LodBuffers {
 // The tree's indices must stay inside its arrays before any of it reaches the device: clusters
 // breadth-first (children after their parent), leaf references and nodes in range.
 const char* validate_tree(const skm_lod_tree_view* t, uint32_t* rounds) {
-  if (t == nullptr || t->layout == nullptr && t->nodes.count > 0) return "null tree";
+  if (t == nullptr || (t->layout == nullptr && t->nodes.count > 0)) return "null tree";
   if (t->nodes.count == 0 || t->cluster_count == 0 || t->clusters == nullptr) return "empty tree";
   if (t->leaf_ref_count > 0 && t->leaves == nullptr) return "missing leaf references";
   const uint64_t n = t->nodes.count;

@@ -143,5 +143,9 @@ constexpr float kMaxLog2Alpha = 16.0f;
 constexpr float kMaxColorScale = 4.0f;
 constexpr float kHalfMin = 6.103515625e-05f;
 constexpr float kBinPad = 1.0f / 16.0f;
+// 1 / kGaussianScale^2 = 2 ln 2, and the slack of the footprint disc over the coverage rule's fp32
+// rounding.
+constexpr float kInvGaussianScale2 = 1.38629436f;
+constexpr float kFootprintSlack = 1.0009765625f;
 
 }  // namespace skm

@@ -330,6 +330,38 @@ def test_tile_lists_cover_every_accepted_pixel(oracle, skm_mod):
                     assert not oracle.decide(o["projected"][j], w, h, px, py)[0]
 
 
+@pytest.mark.parametrize("w,h,n,log_scale,lod_alpha,seed", [
+    (96, 80, 400, -3.0, False, 3), (320, 180, 3000, -2.5, False, 11), (256, 200, 2000, -2.0, True, 12),
+    (160, 90, 5000, -4.0, True, 13)])
+def test_binning_misses_no_accepted_pixel(oracle, skm_mod, w, h, n, log_scale, lod_alpha, seed):
+    """Every pixel of every record's quad is tested with the coverage rule (in C): each accepted
+    (pixel, record) has the record in the pixel's tile list, with the tile rectangles cut to the
+    footprint box (quad box and cutoff disc box) rather than the quad's box."""
+    cloud = skm_mod.make_cloud(n, 1, seed=seed, layout=0, spread=0.8, log_scale_mean=log_scale)
+    if lod_alpha:
+        cloud["alphas"][::3] = np.float32(1.5) + np.arange(len(cloud["alphas"][::3]), dtype=np.float32) % 40
+    eye = np.array([0.2, 0.3, 3.5], np.float32)
+    view = skm_mod.look_at(eye)
+    proj = skm_mod.perspective(math.pi / 3, w / h, 0.2, 100.0)
+    o = oracle.frame(cloud, view, proj, eye, w, h, image=False)
+    missed, accepted = oracle.missed_coverage(o["projected"], o["count"], o["pair_values"], o["ranges"], w, h)
+    assert accepted > 0 and missed == 0
+    loose = 0
+    for j in range(o["count"]):
+        p = o["projected"][j]
+        f16 = lambda v: float(np.uint16(v).view(np.float16))
+        r = float(p[6].view(np.float32))
+        ex = r * (abs(f16(p[2] & 0xFFFF)) + abs(f16(p[3] & 0xFFFF)))
+        ey = r * (abs(f16(p[2] >> 16)) + abs(f16(p[3] >> 16)))
+        cx = (float(p[0].view(np.float32)) * 0.5 + 0.5) * w
+        cy = (0.5 - float(p[1].view(np.float32)) * 0.5) * h
+        x0, x1 = max(0, math.ceil(cx - ex - 0.5 - 1 / 16)), min(w - 1, math.floor(cx + ex - 0.5 + 1 / 16))
+        y0, y1 = max(0, math.ceil(cy - ey - 0.5 - 1 / 16)), min(h - 1, math.floor(cy + ey - 0.5 + 1 / 16))
+        if x0 <= x1 and y0 <= y1:
+            loose += (x1 // 16 - x0 // 16 + 1) * (y1 // 16 - y0 // 16 + 1)
+    assert o["pairs"] <= loose
+
+
 def test_raster_against_numpy_brute_force(oracle, skm_mod):
     """An independent FP64 composite in numpy over all survivors in depth order (exact 2x2 solve of
     the half axes, alpha = min(1, a0 exp(-q/2)), cutoff 1/255) matches the oracle within 1/255."""
