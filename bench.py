@@ -42,6 +42,40 @@ def stage_bytes(N, V, P, T, depth_passes, tile_passes):
     }
 
 
+def batched_vs_loop(r, skm, batches, loops):
+    """The 64-view batch of the same scene (720p/1080p/1440p views on spheres of radius 4-12 m): views/s
+    of one graph launch per batch against the plain path drawing the same views one per launch."""
+    views = skm.views_c2(64)
+    arr = r.make_views(views)
+    for _ in range(2):
+        r.render_views(arr)
+    t0 = time.perf_counter()
+    for _ in range(batches):
+        r.draw_views(arr)
+    r.finish()
+    batch_s = (time.perf_counter() - t0) / batches
+    st = r.batch_stats()
+    frames = [(r.make_frame(v, p, e), w, h) for v, p, e, w, h in views]
+    for f, w, h in frames[:4]:
+        r.set_extent(w, h)
+        r.draw(f)
+    r.finish()
+    t0 = time.perf_counter()
+    for _ in range(loops):
+        for f, w, h in frames:
+            r.set_extent(w, h)
+            r.draw(f)
+    r.finish()
+    loop_s = (time.perf_counter() - t0) / loops
+    return {"workload": "64 views of the 3M SH3 scene, resolutions uniform over 720p/1080p/1440p, eyes on "
+                        "spheres of radius U(4,12) m", "views": 64,
+            "batched_frames_per_s": round(64 / batch_s, 1), "batched_ms_per_batch": round(batch_s * 1000, 3),
+            "batched_gpu_ms_per_batch": round(st["gpu_ms"], 3), "graph": st["graph"],
+            "per_view_loop_frames_per_s": round(64 / loop_s, 1), "per_view_loop_ms": round(loop_s * 1000, 3),
+            "speedup": round(loop_s / batch_s, 3), "visible_per_batch": st["visible"], "pairs_per_batch": st["pairs"],
+            "batches": batches, "loops": loops}
+
+
 def cpu_baseline(frames, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -69,6 +103,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=2)
+    ap.add_argument("--batches", type=int, default=5, help="timed 64-view batches (0 skips the batch leg)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the render path on a single MI355X "
@@ -124,6 +159,8 @@ def main():
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                     "frac": round(gbs * 1e9 / HBM_PEAK, 4), "traffic": None}
 
+    batched = batched_vs_loop(r, skm, args.batches, 2) if args.batches > 0 else None
+
     cpu = None
     if not args.no_cpu:
         cpu_fps, cpu_stages = cpu_baseline(args.cpu_frames, 1)
@@ -144,7 +181,7 @@ def main():
         "msplats_per_s": round(N_SPLATS * fps / 1e6, 1),
         "visible_per_frame": int(V), "pairs_per_frame": int(P),
         "gpu_ms_per_frame": round(tot["stage_ms"][5] / n_frames, 4),
-        "stages": stages, "roofline": roofline, "cpu_baseline": cpu,
+        "stages": stages, "roofline": roofline, "cpu_baseline": cpu, "batched": batched,
         "device": r.description(), "scene_generation_s": round(gen_s, 2),
     }
     r.close()

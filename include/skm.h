@@ -246,6 +246,51 @@ skm_status skm_lod_select(skm_renderer* r, const struct skm_lod_tree_view* tree,
                           float pixel_limit, float color_weight, int cull, uint32_t* indices,
                           uint32_t* stats6);
 
+/* ---- view batches: many cameras over the resident world in one graph launch ----
+ * No SplatRenderer counterpart: this extends draw() (SplatRenderer.h:134) to many views per launch,
+ * beside the per-slot cameras and counts of the frame slots (MetalVisibilityPass.h:89-92). Every
+ * view draws the same ranges with the renderer's policy; each view's survivors, depth order, tile
+ * pairs and pixels are those of a single draw() of that view at that extent, bit for bit. */
+typedef struct skm_view {
+  float view[16];
+  float proj[16];
+  float camera_position[3];
+  uint32_t width;
+  uint32_t height;
+  uint32_t reserved;
+} skm_view;
+
+typedef struct skm_batch_tuning {
+  uint32_t tile_size;       /* raster tile edge in pixels (0 = 16) */
+  uint32_t tile_radix_bits; /* digit width of the tile sort (0 = 8) */
+  uint32_t use_graph;       /* 1: capture the batch's kernels in a hipGraph and replay it */
+  uint32_t reserved;
+} skm_batch_tuning;
+
+typedef struct skm_batch_stats {
+  uint32_t views;
+  uint32_t graph;     /* the batch ran as a graph replay */
+  double gpu_ms;      /* GPU time of the whole batch */
+  uint64_t visible;   /* survivors over all views */
+  uint64_t pairs;     /* tile pairs over all views */
+} skm_batch_stats;
+
+/* Enqueues the batch (waits for the previous batch first). ranges NULL draws the whole world.
+ * tuning NULL uses the renderer's current batch tuning (skm_autotune_views). A batch whose tile
+ * pairs overflowed is dropped: skm_finish then returns SKM_ERR_CAPACITY and the next draw fits. */
+skm_status skm_draw_views(skm_renderer* r, const skm_view* views, uint32_t count, const skm_range* ranges,
+                          uint32_t range_count, const skm_batch_tuning* tuning);
+skm_status skm_read_batch_stats(skm_renderer* r, skm_batch_stats* out);
+/* Read-backs of one view of the last completed batch, in skm_read_frame / skm_read_visible /
+ * skm_read_order / skm_read_tiles terms (indices local to the view). */
+skm_status skm_read_view_frame(skm_renderer* r, uint32_t view, int format, void* dst, size_t dst_bytes);
+skm_status skm_read_view_visible(skm_renderer* r, uint32_t view, uint32_t* count, uint32_t* keys,
+                                 uint32_t* projected, uint32_t capacity);
+skm_status skm_read_view_order(skm_renderer* r, uint32_t view, uint32_t* order, uint32_t capacity);
+skm_status skm_read_view_tiles(skm_renderer* r, uint32_t view, uint64_t* pairs, uint32_t* pair_tiles,
+                               uint32_t* pair_values, uint64_t capacity, uint32_t* ranges,
+                               uint32_t tile_capacity);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,385 @@
+// View batches (skm_batch.h): scratch sizing, camera and view uploads, hipGraph capture and
+// replay of the frame pipeline over all views, and per-view read-backs in single-frame terms.
+#include "skm_batch.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace skm {
+
+namespace {
+
+constexpr uint32_t kTile = 16;
+constexpr uint32_t kMaxViews = 4096;
+constexpr uint64_t kLaneLimit = 0xffff0000ull;  // survivor and pair indices are 32-bit
+
+hipError_t grow(DeviceBuffer& b, size_t bytes) {
+  if (b.bytes >= bytes && b.ptr != nullptr) return hipSuccess;
+  return b.allocate(bytes);
+}
+
+}  // namespace
+
+Batch::~Batch() {
+  if (in_flight_) (void)hipEventSynchronize(end_);
+  release_graph();
+  if (start_) (void)hipEventDestroy(start_);
+  if (end_) (void)hipEventDestroy(end_);
+}
+
+void Batch::release_graph() {
+  if (exec_ != nullptr) (void)hipGraphExecDestroy(exec_);
+  exec_ = nullptr;
+  graph_key_.clear();
+}
+
+hipError_t Batch::ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_t tiles, uint64_t pixels,
+                         uint32_t ranges) {
+  hipError_t e = hipSuccess;
+  bool moved = false;
+  auto need = [&](DeviceBuffer& b, size_t bytes) {
+    if (e != hipSuccess || (b.ptr != nullptr && b.bytes >= bytes)) return;
+    e = grow(b, bytes);
+    moved = true;
+  };
+  if (ctrl_.ptr == nullptr) {
+    need(ctrl_, sizeof(Ctrl));
+    need(hist_, 2 * 4 * 256 * 4);
+    if (e == hipSuccess) e = hipMemset(ctrl_.ptr, 0, sizeof(Ctrl));
+  }
+  if (views > view_capacity_) {
+    need(cams_, size_t{views} * sizeof(Camera));
+    need(views_, size_t{views} * sizeof(ViewInfo));
+    need(view_start_, (size_t{views} + 1) * 4);
+    if (e == hipSuccess) view_capacity_ = views;
+  }
+  if (lanes > lane_capacity_) {
+    const size_t n = static_cast<size_t>(lanes);
+    need(cull_keys_, n * 4);
+    need(keys_[0], n * 4);
+    need(keys_[1], n * 4);
+    need(order_[0], n * 4);
+    need(order_[1], n * 4);
+    need(projected_, n * 32);
+    need(survivor_view_, n * 2);
+    if (e == hipSuccess) lane_capacity_ = lanes;
+  }
+  if (pairs > pair_capacity_) {
+    const size_t n = static_cast<size_t>(pairs);
+    need(pair_keys_[0], n * 4);
+    need(pair_keys_[1], n * 4);
+    need(pair_vals_[0], n * 4);
+    need(pair_vals_[1], n * 4);
+    if (e == hipSuccess) pair_capacity_ = pairs;
+  }
+  if (tiles > tile_capacity_) {
+    need(tile_ranges_, size_t{tiles} * 8);
+    need(tile_view_, size_t{tiles} * 2);
+    if (e == hipSuccess) tile_capacity_ = tiles;
+  }
+  if (pixels > pixel_capacity_) {
+    need(image_, static_cast<size_t>(pixels) * 16);
+    if (e == hipSuccess) pixel_capacity_ = pixels;
+  }
+  if (ranges > range_capacity_ || ranges_.ptr == nullptr) {
+    need(ranges_, std::max<size_t>(ranges, 1) * 8);
+    need(starts_, (std::max<size_t>(ranges, 1) + 1) * 4);
+    if (e == hipSuccess) range_capacity_ = std::max(ranges, 1u);
+  }
+  // Look-back status words: one per 1024-lane tile (cull, binning); 256 per 4096-key sort tile.
+  const size_t scan = static_cast<size_t>(lane_capacity_ / 1024 + 2);
+  if (e == hipSuccess && scan > scan_words_) {
+    need(scan_status_, scan * 8);
+    if (e == hipSuccess) e = hipMemset(scan_status_.ptr, 0, scan_status_.bytes);
+    if (e == hipSuccess) scan_words_ = scan;
+  }
+  const size_t sort = static_cast<size_t>(std::max(lane_capacity_, pair_capacity_) / kSortTile + 2) * kRadix;
+  if (e == hipSuccess && sort > sort_words_) {
+    need(sort_status_, sort * 8);
+    if (e == hipSuccess) e = hipMemset(sort_status_.ptr, 0, sort_status_.bytes);
+    if (e == hipSuccess) sort_words_ = sort;
+  }
+  const size_t stage = size_t{views} * (sizeof(Camera) + sizeof(ViewInfo)) + size_t{ranges} * 8 +
+                       (size_t{ranges} + 1) * 4 + 64;
+  if (e == hipSuccess && stage > staging_bytes_) {
+    e = staging_.allocate(stage * 2);
+    if (e == hipSuccess) staging_bytes_ = stage * 2;
+  }
+  const size_t host = sizeof(Ctrl) + (size_t{views} + 1) * 4;
+  if (e == hipSuccess && host > host_bytes_) {
+    e = host_.allocate(host * 2);
+    if (e == hipSuccess) host_bytes_ = host * 2;
+  }
+  if (e == hipSuccess && start_ == nullptr) e = hipEventCreate(&start_);
+  if (e == hipSuccess && end_ == nullptr) e = hipEventCreate(&end_);
+  if (moved) release_graph();  // the captured kernels hold the old addresses
+  return e;
+}
+
+hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree, const skm_view* views,
+                         uint32_t count, const skm_range* ranges, uint32_t range_count,
+                         const BatchPolicy& policy, const skm_batch_tuning& tuning, hipStream_t stream,
+                         std::string* error) {
+  if (in_flight_) {
+    bool dropped = false;
+    if (const hipError_t e = wait(&dropped); e != hipSuccess) return e;
+  }
+  if (count == 0 || count > kMaxViews || (tuning.tile_size != 0 && tuning.tile_size != kTile) ||
+      (tuning.tile_radix_bits != 0 && tuning.tile_radix_bits != 8)) {
+    *error = "batch: 1 to 4096 views, 16-pixel tiles and 8-bit tile digits";
+    return hipErrorInvalidValue;
+  }
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < range_count; ++i) total += ranges[i].count;
+  const uint64_t view_lanes = (total + 1023) / 1024 * 1024;
+  const uint64_t lanes = view_lanes * count;
+  info_.assign(count, ViewInfo{});
+  uint64_t tiles = 0, pixels = 0;
+  for (uint32_t v = 0; v < count; ++v) {
+    const uint32_t W = views[v].width, H = views[v].height;
+    if (W == 0 || H == 0 || W > 16384 || H > 16384) {
+      *error = "batch: view extent out of range";
+      return hipErrorInvalidValue;
+    }
+    ViewInfo& vi = info_[v];
+    vi.width = W;
+    vi.height = H;
+    vi.tiles_x = (W + kTile - 1) / kTile;
+    vi.tiles_y = (H + kTile - 1) / kTile;
+    vi.tile_base = static_cast<uint32_t>(tiles);
+    vi.pixel_base = pixels;
+    tiles += uint64_t{vi.tiles_x} * vi.tiles_y;
+    pixels += uint64_t{W} * H;
+  }
+  if (lanes > kLaneLimit || tiles >= (1ull << 24)) {
+    *error = "batch: too many lanes or tiles for 32-bit indices";
+    return hipErrorInvalidValue;
+  }
+  const uint64_t want_pairs = std::min<uint64_t>(
+      std::max<uint64_t>({pair_want_, lanes * 3, uint64_t{1} << 22}), kLaneLimit);
+  if (hipError_t e = ensure(count, lanes, want_pairs, static_cast<uint32_t>(tiles), pixels, range_count);
+      e != hipSuccess) {
+    *error = "batch: scratch allocation failed";
+    return e;
+  }
+  stream_ = stream;
+
+  // Inputs: cameras, view descriptors, ranges (one staged copy each).
+  char* st = staging_.as<char>();
+  auto* cams = reinterpret_cast<Camera*>(st);
+  for (uint32_t v = 0; v < count; ++v)
+    fill_camera(cams[v], views[v].view, views[v].proj, views[v].width, views[v].height, policy.linear,
+                policy.subpixel, views[v].camera_position, nullptr);
+  auto* vinfo = reinterpret_cast<ViewInfo*>(st + size_t{count} * sizeof(Camera));
+  std::memcpy(vinfo, info_.data(), size_t{count} * sizeof(ViewInfo));
+  auto* rh = reinterpret_cast<uint32_t*>(st + size_t{count} * (sizeof(Camera) + sizeof(ViewInfo)));
+  uint32_t* sh = rh + 2 * size_t{range_count};
+  sh[0] = 0;
+  for (uint32_t i = 0; i < range_count; ++i) {
+    rh[2 * i] = ranges[i].offset;
+    rh[2 * i + 1] = ranges[i].count;
+    sh[i + 1] = sh[i] + ranges[i].count;
+  }
+  hipError_t e = hipMemcpyAsync(cams_.ptr, cams, size_t{count} * sizeof(Camera), hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(views_.ptr, vinfo, size_t{count} * sizeof(ViewInfo), hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess && range_count > 0) {
+    e = hipMemcpyAsync(ranges_.ptr, rh, size_t{range_count} * 8, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(starts_.ptr, sh, (size_t{range_count} + 1) * 4, hipMemcpyHostToDevice, stream);
+  }
+  // The view of each global tile: rebuilt when the extents change.
+  std::vector<uint16_t> tv(static_cast<size_t>(tiles));
+  for (uint32_t v = 0; v < count; ++v)
+    std::fill(tv.begin() + info_[v].tile_base, tv.begin() + info_[v].tile_base + info_[v].tiles_x * info_[v].tiles_y,
+              static_cast<uint16_t>(v));
+  if (e == hipSuccess && tv != tile_view_host_) {
+    e = hipMemcpy(tile_view_.ptr, tv.data(), tv.size() * 2, hipMemcpyHostToDevice);
+    tile_view_host_ = std::move(tv);
+  }
+  if (e == hipSuccess && ++pipelines_ >= (1ull << 24)) {
+    // Look-back tags carry 26 bits of epoch: clear the status words long before they repeat.
+    pipelines_ = 1;
+    e = hipMemsetAsync(scan_status_.ptr, 0, scan_status_.bytes, stream);
+    if (e == hipSuccess) e = hipMemsetAsync(sort_status_.ptr, 0, sort_status_.bytes, stream);
+  }
+  if (e != hipSuccess) {
+    *error = "batch: input upload failed";
+    return e;
+  }
+
+  FrameArgs a{};
+  a.cam = cams_.as<Camera>();
+  a.world = world;
+  a.ranges = ranges_.as<uint32_t>();
+  a.range_starts = starts_.as<uint32_t>();
+  a.range_count = range_count;
+  a.total = static_cast<uint32_t>(total);
+  a.key16 = policy.key16 ? 1 : 0;
+  a.width = info_[0].width;
+  a.height = info_[0].height;
+  a.tile = kTile;
+  a.tiles_x = info_[0].tiles_x;
+  a.tiles_y = info_[0].tiles_y;
+  for (int c = 0; c < 3; ++c) a.background[c] = policy.background[c];
+  a.ctrl = ctrl_.as<Ctrl>();
+  a.cull_keys = cull_keys_.as<uint32_t>();
+  for (int k = 0; k < 2; ++k) {
+    a.keys[k] = keys_[k].as<uint32_t>();
+    a.order[k] = order_[k].as<uint32_t>();
+    a.pair_keys[k] = pair_keys_[k].as<uint32_t>();
+    a.pair_vals[k] = pair_vals_[k].as<uint32_t>();
+  }
+  a.projected = projected_.as<uint32_t>();
+  a.scan_status = scan_status_.as<uint64_t>();
+  a.sort_status = sort_status_.as<uint64_t>();
+  a.hist = hist_.as<uint32_t>();
+  a.pair_capacity = static_cast<uint32_t>(pair_capacity_);
+  a.tile_ranges = tile_ranges_.as<uint32_t>();
+  a.image = image_.as<float>();
+  a.capacity = static_cast<uint32_t>(lane_capacity_);
+  a.view_count = count;
+  a.views = views_.as<ViewInfo>();
+  a.survivor_view = survivor_view_.as<uint16_t>();
+  a.view_start = view_start_.as<uint32_t>();
+  a.tile_view = tile_view_.as<uint16_t>();
+  a.total_tiles = static_cast<uint32_t>(tiles);
+  depth_passes_ = policy.key16 ? 2 : 4;
+  tile_passes_ = static_cast<int>(digit_passes(static_cast<uint32_t>(tiles - 1)));
+  (void)world_count;
+  (void)sh_degree;
+
+  graph_used_ = tuning.use_graph != 0;
+  if (graph_used_) {
+    // What the captured kernels bake in: sizes, passes, addresses (reset on reallocation), the
+    // background and the world. Cameras, views and ranges are read from device memory.
+    std::vector<uint64_t> key = {count, total, range_count, uint64_t(world.sh_degree), uint64_t(policy.key16),
+                                 tiles, uint64_t(depth_passes_), uint64_t(tile_passes_),
+                                 reinterpret_cast<uint64_t>(world.px), world.capacity, pair_capacity_,
+                                 lane_capacity_};
+    for (int c = 0; c < 3; ++c) {
+      uint32_t b;
+      std::memcpy(&b, &policy.background[c], 4);
+      key.push_back(b);
+    }
+    for (uint32_t v = 0; v < count; ++v) key.push_back((uint64_t{info_[v].width} << 32) | info_[v].height);
+    if (exec_ == nullptr || key != graph_key_) {
+      release_graph();
+      hipGraph_t g = nullptr;
+      e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        const hipError_t le = launch_frame(a, nullptr, depth_passes_, tile_passes_, nullptr, stream);
+        e = hipStreamEndCapture(stream, &g);
+        if (e == hipSuccess) e = le;
+      }
+      if (e == hipSuccess) e = hipGraphInstantiateWithFlags(&exec_, g, 0);
+      if (g != nullptr) (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        exec_ = nullptr;
+        *error = "batch: graph capture failed";
+        return e;
+      }
+      graph_key_ = std::move(key);
+    }
+    e = hipEventRecord(start_, stream);
+    if (e == hipSuccess) e = hipGraphLaunch(exec_, stream);
+  } else {
+    e = hipEventRecord(start_, stream);
+    if (e == hipSuccess) e = launch_frame(a, nullptr, depth_passes_, tile_passes_, nullptr, stream);
+  }
+  if (e == hipSuccess) e = hipEventRecord(end_, stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(host_.ptr, ctrl_.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(host_.as<char>() + sizeof(Ctrl), view_start_.ptr, (size_t{count} + 1) * 4,
+                       hipMemcpyDeviceToHost, stream);
+  if (e != hipSuccess) {
+    *error = "batch: launch failed";
+    return e;
+  }
+  in_flight_ = true;
+  return hipSuccess;
+}
+
+hipError_t Batch::wait(bool* dropped) {
+  *dropped = false;
+  if (!in_flight_) return hipSuccess;
+  in_flight_ = false;
+  const hipError_t e = hipStreamSynchronize(stream_);
+  if (e != hipSuccess) return e;
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, start_, end_);
+  gpu_ms_ = ms;
+  if (ctrl_host()->overflow != 0) {
+    *dropped = true;
+    pair_want_ = std::min<uint64_t>(uint64_t{ctrl_host()->pairs} + ctrl_host()->pairs / 4 + 4096, kLaneLimit);
+  }
+  return hipSuccess;
+}
+
+hipError_t Batch::read_image(uint32_t v, float* rgba) const {
+  const ViewInfo& vi = info_[v];
+  return hipMemcpy(rgba, image_.as<float>() + vi.pixel_base * 4, size_t{vi.width} * vi.height * 16,
+                   hipMemcpyDeviceToHost);
+}
+
+hipError_t Batch::read_visible(uint32_t v, uint32_t* keys, uint32_t* projected, uint32_t capacity) const {
+  const uint32_t first = view_first(v), n = std::min(view_visible(v), capacity);
+  hipError_t e = hipSuccess;
+  if (keys && n) e = hipMemcpy(keys, cull_keys_.as<uint32_t>() + first, size_t{n} * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && projected && n)
+    e = hipMemcpy(projected, projected_.as<uint32_t>() + size_t{first} * 8, size_t{n} * 32, hipMemcpyDeviceToHost);
+  return e;
+}
+
+hipError_t Batch::read_order(uint32_t v, uint32_t* order, uint32_t capacity) const {
+  const uint32_t total = ctrl_host()->visible, first = view_first(v), n = view_visible(v);
+  std::vector<uint32_t> all(total);
+  const uint32_t* src = order_[(depth_passes_ - 1) & 1].as<uint32_t>();
+  if (total) {
+    if (hipError_t e = hipMemcpy(all.data(), src, size_t{total} * 4, hipMemcpyDeviceToHost); e != hipSuccess)
+      return e;
+  }
+  uint32_t k = 0;
+  for (uint32_t i = 0; i < total && k < std::min(n, capacity); ++i)
+    if (all[i] >= first && all[i] - first < n) order[k++] = all[i] - first;
+  return hipSuccess;
+}
+
+hipError_t Batch::read_tiles(uint32_t v, uint64_t* pairs, uint32_t* pair_tiles, uint32_t* pair_values,
+                             uint64_t capacity, uint32_t* ranges, uint32_t tile_capacity) const {
+  const ViewInfo& vi = info_[v];
+  const uint32_t tiles = vi.tiles_x * vi.tiles_y;
+  std::vector<uint32_t> rg(size_t{tiles} * 2);
+  hipError_t e = hipMemcpy(rg.data(), tile_ranges_.as<uint32_t>() + size_t{vi.tile_base} * 2, rg.size() * 4,
+                           hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return e;
+  uint32_t p0 = 0, p1 = 0;
+  bool any = false;
+  for (uint32_t t = 0; t < tiles; ++t) {
+    if (rg[2 * t + 1] <= rg[2 * t]) continue;
+    if (!any) p0 = rg[2 * t];
+    p1 = rg[2 * t + 1];
+    any = true;
+  }
+  *pairs = p1 - p0;
+  if (ranges != nullptr)
+    for (uint32_t t = 0; t < std::min(tiles, tile_capacity); ++t) {
+      const bool full = rg[2 * t + 1] > rg[2 * t];
+      ranges[2 * t] = full ? rg[2 * t] - p0 : 0;
+      ranges[2 * t + 1] = full ? rg[2 * t + 1] - p0 : 0;
+    }
+  const uint64_t n = std::min<uint64_t>(p1 - p0, capacity);
+  const int fin = tile_passes_ & 1;
+  if (pair_tiles && n) {
+    e = hipMemcpy(pair_tiles, pair_keys_[fin].as<uint32_t>() + p0, n * 4, hipMemcpyDeviceToHost);
+    for (uint64_t i = 0; e == hipSuccess && i < n; ++i) pair_tiles[i] -= vi.tile_base;
+  }
+  if (e == hipSuccess && pair_values && n) {
+    e = hipMemcpy(pair_values, pair_vals_[fin].as<uint32_t>() + p0, n * 4, hipMemcpyDeviceToHost);
+    const uint32_t first = view_first(v);
+    for (uint64_t i = 0; e == hipSuccess && i < n; ++i) pair_values[i] -= first;
+  }
+  return e;
+}
+
+}  // namespace skm

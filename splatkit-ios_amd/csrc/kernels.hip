@@ -363,7 +363,8 @@ __global__ __launch_bounds__(256) void k_pack(WorldOut w, uint32_t offset, uint3
 }
 
 __global__ __launch_bounds__(256) void k_prologue(Ctrl* ctrl, uint32_t* hist, uint32_t hist_words,
-                                                  uint32_t* ranges, uint32_t range_words) {
+                                                  uint32_t* ranges, uint32_t range_words,
+                                                  uint32_t* extra, uint32_t extra_words) {
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid == 0) {
     ctrl->epoch = ctrl->epoch + 1u;
@@ -377,6 +378,7 @@ __global__ __launch_bounds__(256) void k_prologue(Ctrl* ctrl, uint32_t* hist, ui
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = gid; i < hist_words; i += stride) hist[i] = 0;
   for (uint32_t i = gid; i < range_words; i += stride) ranges[i] = 0;
+  for (uint32_t i = gid; i < extra_words; i += stride) extra[i] = 0;
 }
 
 // 256 threads, kCullItems lanes each: lane t = tile * 1024 + k * 256 + thread. The tile's
@@ -391,17 +393,20 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
   __shared__ uint32_t s_prefix;
   const uint32_t tid = threadIdx.x, wave = tid >> 6;
   const uint32_t total = INDEXED ? min(*a.active_count, a.capacity) : a.total;
-  const uint32_t blocks = (total + kTile - 1u) / kTile;
+  const uint32_t view_tiles = (total + kTile - 1u) / kTile;
+  const uint32_t blocks = view_tiles * a.view_count;
   if (blockIdx.x >= blocks) return;
   if (tid == 0) s_ticket = atomicAdd(&a.ctrl->ticket[kTicketCull], 1u);
   __syncthreads();
   const uint32_t b = s_ticket;
-  const Camera& cam = *a.cam;
+  const uint32_t view = a.view_count > 1u ? b / view_tiles : 0u;
+  const uint32_t lt = b - view * view_tiles;  // the tile within its view
+  const Camera& cam = a.cam[view];
   uint32_t key[kItems], rec[kItems][8];
   uint64_t ballot[kItems];
 #pragma unroll
   for (uint32_t k = 0; k < kItems; ++k) {
-    const uint32_t t = b * kTile + k * 256u + tid;
+    const uint32_t t = lt * kTile + k * 256u + tid;
     bool visible = false;
     key[k] = 0;
 #pragma unroll
@@ -435,6 +440,7 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
         status_store(a.scan_status + b, status_word(tag, kFlagPrefix, sat_add(prefix, aggregate)));
     }
     if (lane_id() == 0) s_prefix = prefix;
+    if (lane_id() == 0 && lt == 0u && a.view_start != nullptr) a.view_start[view] = prefix;
   }
   __syncthreads();
   uint32_t before = s_prefix;
@@ -449,11 +455,15 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
       uint4* dst = reinterpret_cast<uint4*>(a.projected + static_cast<size_t>(off) * 8u);
       dst[0] = make_uint4(rec[k][0], rec[k][1], rec[k][2], rec[k][3]);
       dst[1] = make_uint4(rec[k][4], rec[k][5], rec[k][6], rec[k][7]);
+      if (a.survivor_view != nullptr) a.survivor_view[off] = static_cast<uint16_t>(view);
     }
 #pragma unroll
     for (uint32_t w = 0; w < 4; ++w) before += s_seg[k * 4 + w];
   }
-  if (b == blocks - 1u && tid == 0) a.ctrl->visible = s_prefix + aggregate;
+  if (b == blocks - 1u && tid == 0) {
+    a.ctrl->visible = s_prefix + aggregate;
+    if (a.view_start != nullptr) a.view_start[a.view_count] = s_prefix + aggregate;
+  }
 }
 
 // Global digit histograms of every pass in one read of the keys (onesweep's upfront histogram).
@@ -666,7 +676,7 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
   constexpr uint32_t kTile = 256u * kItems;
   __shared__ uint32_t s_ticket, s_prefix;
   __shared__ uint32_t s_scan[4];
-  __shared__ uint32_t s_excl[kTile], s_val[kTile], s_origin[kTile], s_width[kTile];
+  __shared__ uint32_t s_excl[kTile], s_val[kTile], s_origin[kTile], s_width[kTile], s_base[kTile];
   const uint32_t tid = threadIdx.x;
   const uint32_t n = a.ctrl->visible;
   const uint32_t blocks = (n + kTile - 1u) / kTile;
@@ -679,21 +689,29 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
   for (uint32_t k = 0; k < kItems; ++k) {
     const uint32_t slot = k * 256u + tid;
     const uint32_t i = b * kTile + slot;
-    uint32_t count = 0, value = 0;
+    uint32_t count = 0, value = 0, base = 0, tiles_x = a.tiles_x;
     Rect r{0, 0, -1, -1};
     if (i < n) {
       value = order[i];
       const uint4* src = reinterpret_cast<const uint4*>(a.projected + static_cast<size_t>(value) * 8u);
       const uint4 w0 = src[0], w1 = src[1];
       const uint32_t p[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
-      count = tile_rect(p, a.width, a.height, a.tile, r);
+      if (a.survivor_view != nullptr) {
+        const ViewInfo& vi = a.views[a.survivor_view[value]];
+        base = vi.tile_base;
+        tiles_x = vi.tiles_x;
+        count = tile_rect(p, vi.width, vi.height, a.tile, r);
+      } else {
+        count = tile_rect(p, a.width, a.height, a.tile, r);
+      }
     }
     uint32_t row_total;
     const uint32_t excl = block_exclusive_scan(count, s_scan, &row_total);
     s_excl[slot] = aggregate + excl;
     s_val[slot] = value;
     s_origin[slot] = static_cast<uint32_t>(r.tx0) | (static_cast<uint32_t>(r.ty0) << 16);
-    s_width[slot] = static_cast<uint32_t>(r.tx1 - r.tx0 + 1);
+    s_width[slot] = static_cast<uint32_t>(r.tx1 - r.tx0 + 1) | (tiles_x << 16);
+    s_base[slot] = base;
     aggregate += row_total;
   }
   if ((tid >> 6) == 0) {
@@ -718,12 +736,12 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
       if (s_excl[mid] <= e) lo = mid; else hi = mid - 1u;
     }
     const uint32_t k = e - s_excl[lo];
-    const uint32_t wdt = s_width[lo];
+    const uint32_t wdt = s_width[lo] & 0xffffu, stride = s_width[lo] >> 16;
     const uint32_t tx = (s_origin[lo] & 0xffffu) + k % wdt;
     const uint32_t ty = (s_origin[lo] >> 16) + k / wdt;
     const uint64_t idx = prefix + e;
     if (idx < a.pair_capacity) {
-      a.pair_keys[0][idx] = ty * a.tiles_x + tx;
+      a.pair_keys[0][idx] = s_base[lo] + ty * stride + tx;
       a.pair_vals[0][idx] = s_val[lo];
     }
   }
@@ -784,13 +802,23 @@ __global__ __launch_bounds__(256) void k_raster(FrameArgs a, const uint32_t* __r
   __shared__ __align__(16) uint8_t s_list[4][256];
   __shared__ uint32_t s_count[4][4];  // [quadrant][wave]
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
-  const uint32_t tile_id = blockIdx.y * a.tiles_x + blockIdx.x;
-  const uint32_t ox = blockIdx.x * 16u, oy = blockIdx.y * 16u;
+  const uint32_t tile_id = blockIdx.x;  // global tile id
+  uint32_t width = a.width, height = a.height, tiles_x = a.tiles_x, local = tile_id;
+  uint64_t pixel_base = 0;
+  if (a.tile_view != nullptr) {
+    const ViewInfo& vi = a.views[a.tile_view[tile_id]];
+    width = vi.width;
+    height = vi.height;
+    tiles_x = vi.tiles_x;
+    local = tile_id - vi.tile_base;
+    pixel_base = vi.pixel_base;
+  }
+  const uint32_t ox = (local % tiles_x) * 16u, oy = (local / tiles_x) * 16u;
   const uint32_t px = ox + (wave & 1u) * 8u + (lane & 7u);
   const uint32_t py = oy + (wave >> 1) * 8u + (lane >> 3);
-  const bool in_image = px < a.width && py < a.height;
+  const bool in_image = px < width && py < height;
   const uint32_t start = ranges[2 * tile_id], end = ranges[2 * tile_id + 1];
-  const float fw = static_cast<float>(a.width), fh = static_cast<float>(a.height);
+  const float fw = static_cast<float>(width), fh = static_cast<float>(height);
   const float fx = static_cast<float>(px) + 0.5f, fy = static_cast<float>(py) + 0.5f;
   const float tx = static_cast<float>(ox) + 0.5f, ty = static_cast<float>(oy) + 0.5f;
   float T = in_image ? 1.0f : 0.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
@@ -860,7 +888,7 @@ __global__ __launch_bounds__(256) void k_raster(FrameArgs a, const uint32_t* __r
     if (__syncthreads_count(T >= kTransmittanceStop ? 1 : 0) == 0) break;
   }
   if (in_image) {
-    float4* out = reinterpret_cast<float4*>(a.image) + static_cast<size_t>(py) * a.width + px;
+    float4* out = reinterpret_cast<float4*>(a.image) + pixel_base + static_cast<size_t>(py) * width + px;
     *out = make_float4(C0 + T * a.background[0], C1 + T * a.background[1], C2 + T * a.background[2], 1.0f);
   }
 }
@@ -897,7 +925,8 @@ hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], cons
                           hipStream_t stream) {
   // keys[0] holds the input; passes alternate keys[0] -> keys[1] -> keys[0] ...; the order ends in
   // vals[(passes - 1) & 1].
-  hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, stream, ctrl, hist, 4u * 256u, nullptr, 0u);
+  hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, stream, ctrl, hist, 4u * 256u, nullptr, 0u, nullptr,
+                     0u);
   hipLaunchKernelGGL(k_hist, dim3(min(grid_for(capacity, 256), 1024u)), dim3(256), 0, stream, keys[0],
                      n_dev, hist, passes);
   for (int p = 0; p < passes; ++p) {
@@ -913,7 +942,7 @@ hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], cons
 
 hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes, int tile_passes,
                         hipEvent_t* marks, hipStream_t s) {
-  const uint32_t tiles = a.tiles_x * a.tiles_y;
+  const uint32_t tiles = a.total_tiles;
   if (marks) (void)hipEventRecord(marks[0], s);
   if (lod != nullptr) {
     const hipError_t e = launch_lod(*lod, s);
@@ -921,8 +950,10 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes
   }
   if (marks) (void)hipEventRecord(marks[1], s);
   hipLaunchKernelGGL(k_prologue, dim3(min(grid_for(tiles * 2u, 256), 256u)), dim3(256), 0, s, a.ctrl,
-                     a.hist, 2u * 4u * 256u, a.tile_ranges, tiles * 2u);
-  const uint32_t lanes = a.active != nullptr ? a.capacity : a.total;
+                     a.hist, 2u * 4u * 256u, a.tile_ranges, tiles * 2u, a.view_start,
+                     a.view_start != nullptr ? a.view_count + 1u : 0u);
+  const uint32_t view_lanes = grid_for(a.total, 256u * kCullItems) * 256u * kCullItems;
+  const uint32_t lanes = a.active != nullptr ? a.capacity : view_lanes * a.view_count;
   const uint32_t cull_blocks = grid_for(lanes, 256u * kCullItems);
   switch (a.world.sh_degree) {
     case 0: launch_cull_deg<0>(a, cull_blocks, s); break;
@@ -961,7 +992,7 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes
   hipLaunchKernelGGL(k_ranges, dim3(min(grid_for(a.pair_capacity, 256), 2048u)), dim3(256), 0, s,
                      a.pair_keys[fin], sort_n, a.tile_ranges);
   if (marks) (void)hipEventRecord(marks[4], s);
-  hipLaunchKernelGGL(k_raster, dim3(a.tiles_x, a.tiles_y), dim3(256), 0, s, a, a.pair_vals[fin],
+  hipLaunchKernelGGL(k_raster, dim3(tiles), dim3(256), 0, s, a, a.pair_vals[fin],
                      a.tile_ranges);
   if (marks) (void)hipEventRecord(marks[5], s);
   return hipGetLastError();

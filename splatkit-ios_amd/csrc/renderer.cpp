@@ -13,88 +13,33 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "skm.h"
+#include "skm_batch.h"
+#include "skm_host.h"
 #include "skm_internal.h"
 #include "skm_lod.h"
 #include "skm_lod_internal.h"
 
+using skm::align_up;
 using skm::Camera;
 using skm::Ctrl;
+using skm::DeviceBuffer;
+using skm::digit_passes;
+using skm::fill_camera;
+using skm::PinnedBuffer;
+using skm::sh_stride;
+
+static_assert(sizeof(skm_view) == 152 && sizeof(skm_batch_tuning) == 16 && sizeof(skm_batch_stats) == 32,
+              "batch ABI layouts");
 
 namespace {
 
 constexpr uint32_t kSlots = 2;             // frames in flight (MetalFrameSlots.h:8)
 constexpr uint32_t kMaxExtent = 16384;     // per axis
 constexpr uint32_t kUploadChunk = 1u << 20;  // splats per staging copy
-
-size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-uint32_t sh_stride(int degree) {
-  if (degree <= 0) return 0;
-  return (static_cast<uint32_t>((degree + 1) * (degree + 1) - 1) * 3u + 1u) / 2u;
-}
-
-// One device allocation; freed on destruction. Moves only.
-struct DeviceBuffer {
-  void* ptr = nullptr;
-  size_t bytes = 0;
-  DeviceBuffer() = default;
-  DeviceBuffer(const DeviceBuffer&) = delete;
-  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-  DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr, o.bytes = 0; }
-  DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
-    if (this != &o) {
-      reset();
-      ptr = o.ptr;
-      bytes = o.bytes;
-      o.ptr = nullptr;
-      o.bytes = 0;
-    }
-    return *this;
-  }
-  ~DeviceBuffer() { reset(); }
-  void reset() {
-    if (ptr != nullptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    bytes = 0;
-  }
-  hipError_t allocate(size_t n) {
-    reset();
-    if (n == 0) n = 256;
-    const hipError_t e = hipMalloc(&ptr, n);
-    if (e != hipSuccess) {
-      ptr = nullptr;
-      return e;
-    }
-    bytes = n;
-    return hipSuccess;
-  }
-  template <class T>
-  T* as(size_t byte_offset = 0) const {
-    return reinterpret_cast<T*>(static_cast<char*>(ptr) + byte_offset);
-  }
-};
-
-struct PinnedBuffer {
-  void* ptr = nullptr;
-  PinnedBuffer() = default;
-  PinnedBuffer(const PinnedBuffer&) = delete;
-  PinnedBuffer& operator=(const PinnedBuffer&) = delete;
-  ~PinnedBuffer() {
-    if (ptr != nullptr) (void)hipHostFree(ptr);
-  }
-  hipError_t allocate(size_t n) {
-    if (ptr != nullptr) (void)hipHostFree(ptr);
-    ptr = nullptr;
-    return hipHostMalloc(&ptr, n, hipHostMallocDefault);
-  }
-  template <class T>
-  T* as() const {
-    return static_cast<T*>(ptr);
-  }
-};
 
 // The resident world: every GpuSplat field as its own array, SH words word-major.
 struct WorldBuffers {
@@ -246,35 +191,6 @@ const char* validate_tree(const skm_lod_tree_view* t, uint32_t* rounds) {
   return nullptr;
 }
 
-void fill_camera(Camera& cam, const float* view, const float* proj, uint32_t W, uint32_t H, bool linear,
-                 float min_px, const float* eye, const skm_reveal* reveal) {
-  // MetalFrameCompute::updateCamera (MetalFrameCompute.mm:141-164).
-  std::memset(&cam, 0, sizeof(cam));
-  std::memcpy(cam.view, view, 64);
-  std::memcpy(cam.proj, proj, 64);
-  cam.screen_size[0] = static_cast<float>(W);
-  cam.screen_size[1] = static_cast<float>(H);
-  cam.focal[0] = cam.screen_size[0] * proj[0] / 2;
-  cam.focal[1] = cam.screen_size[1] * proj[5] / 2;
-  cam.tan_half_fov[0] = 1 / proj[0];
-  cam.tan_half_fov[1] = 1 / proj[5];
-  cam.output_linear = linear ? 1u : 0u;
-  cam.min_pixel_radius = min_px;
-  if (eye != nullptr) {
-    cam.camera_position[0] = eye[0];
-    cam.camera_position[1] = eye[1];
-    cam.camera_position[2] = eye[2];
-  }
-  cam.camera_position[3] = 1;
-  if (reveal != nullptr) {
-    std::memcpy(cam.reveal, reveal, 16);
-    std::memcpy(cam.reveal_tint, reveal->tint, 16);
-  } else {
-    cam.reveal[0] = HUGE_VALF;
-    cam.reveal_tint[0] = cam.reveal_tint[1] = cam.reveal_tint[2] = 1.0f;
-  }
-}
-
 bool valid_cloud(const skm_cloud_view* c) {
   return c != nullptr && (c->count == 0 || (c->positions && c->covariances && c->colors && c->alphas)) &&
          c->sh_degree >= 0 && c->sh_degree <= 3;
@@ -329,6 +245,12 @@ struct skm_renderer {
   double stage_ms[6] = {};
   uint32_t last_draw_count = 0;
   skm_frame_totals totals{};
+
+  // View batches (skm_draw_views).
+  std::unique_ptr<skm::Batch> batch;
+  skm_batch_tuning batch_tuning{16, 8, 1, 0};
+  bool batch_dropped = false;
+  bool batch_done = false;
 
   skm_status fail(skm_status s, const char* fmt, ...) {
     char buf[512];
@@ -397,6 +319,12 @@ struct skm_renderer {
     for (Slot& s : slots) {
       const skm_status k = harvest(s);
       if (k != SKM_OK) st = k;
+    }
+    if (batch) {
+      bool dropped = false;
+      const skm_status k = check(batch->wait(&dropped), "batch");
+      if (k != SKM_OK) st = k;
+      if (dropped) batch_dropped = true;
     }
     return st;
   }
@@ -510,11 +438,6 @@ struct skm_renderer {
 };
 
 namespace {
-uint32_t digit_passes(uint32_t max_key) {
-  uint32_t passes = 1;
-  while (passes < 4 && (max_key >> (8 * passes)) != 0) ++passes;
-  return passes;
-}
 }  // namespace
 
 extern "C" {
@@ -572,6 +495,7 @@ void skm_renderer_destroy(skm_renderer* r) {
   if (r == nullptr) return;
   (void)hipSetDevice(r->device);
   if (r->stream) (void)hipStreamSynchronize(r->stream);
+  r->batch.reset();
   for (Slot& s : r->slots) {
     if (s.done) (void)hipEventDestroy(s.done);
     for (auto& m : s.marks)
@@ -902,6 +826,8 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   a.tile_ranges = r->tile_ranges.as<uint32_t>();
   a.image = s.image.as<float>();
   a.capacity = r->capacity;
+  a.view_count = 1;
+  a.total_tiles = tiles_x * tiles_y;
   s.depth_passes = r->policy.key16 ? 2 : 4;
   s.tile_passes = static_cast<int>(digit_passes(tiles_x * tiles_y - 1));
   e = skm::launch_frame(a, lod_frame ? &lod : nullptr, s.depth_passes, s.tile_passes, s.marks, r->stream);
@@ -929,6 +855,8 @@ skm_status skm_finish(skm_renderer* r) {
   if (r->last_dropped)
     return r->fail(SKM_ERR_CAPACITY, "the last frame needed %llu tile pairs; capacity grows on the next draw",
                    static_cast<unsigned long long>(r->pairs_needed));
+  if (r->batch_dropped)
+    return r->fail(SKM_ERR_CAPACITY, "the last batch's tile pairs overflowed; capacity grows on the next batch");
   return SKM_OK;
 }
 
@@ -1147,6 +1075,108 @@ skm_status skm_argsort_u32(skm_renderer* r, const uint32_t* keys, uint32_t n, in
     return st;
   if (auto st = skm_argsort_u32_device(r, k.ptr, n, key_bits, o.ptr, nullptr); st != SKM_OK) return st;
   return r->check(hipMemcpy(order, o.ptr, size_t{n} * 4, hipMemcpyDeviceToHost), "order");
+}
+
+
+skm_status skm_draw_views(skm_renderer* r, const skm_view* views, uint32_t count, const skm_range* ranges,
+                          uint32_t range_count, const skm_batch_tuning* tuning) {
+  if (r == nullptr || views == nullptr || count == 0) return SKM_ERR_INVALID;
+  if (r->gpu_failed) return r->fail(SKM_ERR_DEVICE, "GPU failed earlier");
+  if (!r->has_world) return r->fail(SKM_ERR_STATE, "no world");
+  if (r->lod_world) return r->fail(SKM_ERR_UNSUPPORTED, "view batches draw flat worlds; a tree world draws per view");
+  if (range_count > skm::kMaxRanges || (range_count > 0 && ranges == nullptr))
+    return r->fail(SKM_ERR_INVALID, "invalid ranges");
+  const uint32_t source = r->world.count;
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < range_count; ++i) {
+    const skm_range& g = ranges[i];
+    if (g.offset > source || g.count > source - g.offset || total + g.count > source)
+      return r->fail(SKM_ERR_INVALID, "range %u [%u, +%u) invalid for %u splats", i, g.offset, g.count, source);
+    total += g.count;
+  }
+  const skm_range whole{0, source};
+  if (ranges == nullptr) {
+    ranges = &whole;
+    range_count = 1;
+  }
+  if (auto st = r->check(hipSetDevice(r->device), "device"); st != SKM_OK) return st;
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  r->batch_dropped = false;
+  r->batch_done = false;
+  if (!r->batch) r->batch = std::make_unique<skm::Batch>();
+  skm::BatchPolicy p;
+  p.key16 = r->policy.key16;
+  p.linear = r->linear;
+  p.subpixel = r->policy.subpixel;
+  for (int c = 0; c < 3; ++c) p.background[c] = static_cast<float>(r->policy.background[c]);
+  skm_batch_tuning t = tuning ? *tuning : r->batch_tuning;
+  std::string why;
+  const hipError_t e = r->batch->launch(r->world.view(), r->world.count, r->world.sh_degree, views, count, ranges,
+                                        range_count, p, t, r->stream, &why);
+  if (e == hipErrorInvalidValue) return r->fail(SKM_ERR_INVALID, "%s", why.c_str());
+  if (e != hipSuccess) return r->check(e, why.c_str());
+  r->batch_done = true;
+  return SKM_OK;
+}
+
+skm_status skm_read_batch_stats(skm_renderer* r, skm_batch_stats* out) {
+  if (r == nullptr || out == nullptr) return SKM_ERR_INVALID;
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  if (!r->batch_done) return r->fail(SKM_ERR_STATE, "no batch drawn");
+  out->views = r->batch->view_count();
+  out->graph = r->batch->graph_used() ? 1u : 0u;
+  out->gpu_ms = r->batch->gpu_ms();
+  out->visible = r->batch->visible();
+  out->pairs = r->batch->pairs();
+  return SKM_OK;
+}
+
+static skm_status batch_view(skm_renderer* r, uint32_t view) {
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  if (!r->batch_done) return r->fail(SKM_ERR_STATE, "no batch drawn");
+  if (r->batch_dropped) return r->fail(SKM_ERR_CAPACITY, "the last batch was dropped");
+  if (view >= r->batch->view_count()) return r->fail(SKM_ERR_INVALID, "view %u out of range", view);
+  return SKM_OK;
+}
+
+skm_status skm_read_view_frame(skm_renderer* r, uint32_t view, int format, void* dst, size_t dst_bytes) {
+  if (r == nullptr || dst == nullptr) return SKM_ERR_INVALID;
+  if (auto st = batch_view(r, view); st != SKM_OK) return st;
+  const skm::ViewInfo& vi = r->batch->info(view);
+  const size_t px = size_t{vi.width} * vi.height;
+  if (format == SKM_PIXELS_RGBA32F) {
+    if (dst_bytes < px * 16) return r->fail(SKM_ERR_INVALID, "destination too small");
+    return r->check(r->batch->read_image(view, static_cast<float*>(dst)), "read view");
+  }
+  if (format != SKM_PIXELS_BGRA8) return r->fail(SKM_ERR_INVALID, "unknown pixel format");
+  if (dst_bytes < px * 4) return r->fail(SKM_ERR_INVALID, "destination too small");
+  std::vector<float> rgba(px * 4);
+  if (auto st = r->check(r->batch->read_image(view, rgba.data()), "read view"); st != SKM_OK) return st;
+  skm::to_bgra8(rgba.data(), px, static_cast<uint8_t*>(dst));
+  return SKM_OK;
+}
+
+skm_status skm_read_view_visible(skm_renderer* r, uint32_t view, uint32_t* count, uint32_t* keys,
+                                 uint32_t* projected, uint32_t capacity) {
+  if (r == nullptr || count == nullptr) return SKM_ERR_INVALID;
+  if (auto st = batch_view(r, view); st != SKM_OK) return st;
+  *count = r->batch->view_visible(view);
+  return r->check(r->batch->read_visible(view, keys, projected, capacity), "read view");
+}
+
+skm_status skm_read_view_order(skm_renderer* r, uint32_t view, uint32_t* order, uint32_t capacity) {
+  if (r == nullptr || (order == nullptr && capacity > 0)) return SKM_ERR_INVALID;
+  if (auto st = batch_view(r, view); st != SKM_OK) return st;
+  return r->check(r->batch->read_order(view, order, capacity), "read view");
+}
+
+skm_status skm_read_view_tiles(skm_renderer* r, uint32_t view, uint64_t* pairs, uint32_t* pair_tiles,
+                               uint32_t* pair_values, uint64_t capacity, uint32_t* ranges,
+                               uint32_t tile_capacity) {
+  if (r == nullptr || pairs == nullptr) return SKM_ERR_INVALID;
+  if (auto st = batch_view(r, view); st != SKM_OK) return st;
+  return r->check(r->batch->read_tiles(view, pairs, pair_tiles, pair_values, capacity, ranges, tile_capacity),
+                  "read view");
 }
 
 }  // extern "C"

@@ -84,6 +84,16 @@ constexpr uint32_t kRadix = 256;
 constexpr uint32_t kMaxRanges = 65536;  // MetalVisibilityPass.h:72
 constexpr float kTransmittanceStop = 1.0f / 4096.0f;
 
+// One view of a batch: its extent, its tiles' place in the batch's global tile ids, and its first
+// pixel in the batch image. A single-view frame has none (its extent sits in FrameArgs).
+struct ViewInfo {
+  uint32_t width, height, tiles_x, tiles_y;
+  uint32_t tile_base;
+  uint32_t reserved;
+  uint64_t pixel_base;
+};
+static_assert(sizeof(ViewInfo) == 32, "view layout");
+
 struct FrameArgs {
   const Camera* cam;
   World world;
@@ -111,6 +121,15 @@ struct FrameArgs {
   uint32_t* tile_ranges;  // 2 words per tile
   float* image;           // RGBA float, top-down rows
   uint32_t capacity;      // survivor capacity
+  // Batches (view_count > 1): one camera per view at cam[v]; every view culls the same lanes
+  // [0, total); cull tiles never straddle views, so survivors come out view-major. Null for a
+  // single view.
+  uint32_t view_count;
+  const ViewInfo* views;
+  uint16_t* survivor_view;     // the view of each survivor
+  uint32_t* view_start;        // view_count + 1 survivor offsets
+  const uint16_t* tile_view;   // the view of each global tile
+  uint32_t total_tiles;        // tiles of all views
 };
 
 // Launchers (kernels.hip).

@@ -86,6 +86,21 @@ class FrameTotals(C.Structure):
                 ("pairs", C.c_uint64)]
 
 
+class View(C.Structure):
+    _fields_ = [("view", C.c_float * 16), ("proj", C.c_float * 16), ("camera_position", C.c_float * 3),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("tile_size", C.c_uint32), ("tile_radix_bits", C.c_uint32), ("use_graph", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("views", C.c_uint32), ("graph", C.c_uint32), ("gpu_ms", C.c_double), ("visible", C.c_uint64),
+                ("pairs", C.c_uint64)]
+
+
 class SceneSpec(C.Structure):
     _fields_ = [("count", C.c_uint32), ("sh_degree", C.c_int32), ("seed", C.c_uint64),
                 ("layout", C.c_int32), ("spread", C.c_float), ("area", C.c_float),
@@ -160,6 +175,14 @@ API = {
     "skm_read_lod_cut": (C.c_int, [_R, _u32p, _u32p, C.c_uint32, _u32p]),
     "skm_lod_select": (C.c_int, [_R, C.POINTER(LodTreeView), _fp, _fp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
                                  C.c_float, C.c_int, _u32p, _u32p]),
+    "skm_draw_views": (C.c_int, [_R, C.POINTER(View), C.c_uint32, C.POINTER(Range), C.c_uint32,
+                                 C.POINTER(BatchTuning)]),
+    "skm_read_batch_stats": (C.c_int, [_R, C.POINTER(BatchStats)]),
+    "skm_read_view_frame": (C.c_int, [_R, C.c_uint32, C.c_int, C.c_void_p, C.c_size_t]),
+    "skm_read_view_visible": (C.c_int, [_R, C.c_uint32, _u32p, _u32p, _u32p, C.c_uint32]),
+    "skm_read_view_order": (C.c_int, [_R, C.c_uint32, _u32p, C.c_uint32]),
+    "skm_read_view_tiles": (C.c_int, [_R, C.c_uint32, C.POINTER(C.c_uint64), _u32p, _u32p, C.c_uint64, _u32p,
+                                      C.c_uint32]),
     "skm_argsort_u32": (C.c_int, [_R, _u32p, C.c_uint32, C.c_int, _u32p]),
     "skm_argsort_u32_device": (C.c_int, [_R, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p,
                                          C.POINTER(C.c_double)]),
@@ -279,6 +302,27 @@ def scene_c0(count=100_000):
 
 def scene_c1(count=3_000_000, sh_degree=3):
     return make_cloud(count, sh_degree, seed=0x5EED0001, layout=1, spread=5.0, area=10.0)
+
+
+RESOLUTIONS_C2 = ((1280, 720), (1920, 1080), (2560, 1440))
+
+
+def views_c2(count=64, seed=0x5EED0002, radius=(4.0, 12.0), resolutions=RESOLUTIONS_C2, fovy=math.pi / 3,
+             near=0.2, far=100.0):
+    """Batch views of the 3M-splat scene: eyes uniform on spheres of radius U(4, 12) m looking at the
+    origin, each at a resolution drawn uniformly from 720p / 1080p / 1440p. Returns a list of
+    (view, proj, eye, width, height)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        eye = (d * rng.uniform(*radius)).astype(np.float32)
+        if abs(d[1]) > 0.999:
+            eye[0] += 1e-3
+        w, h = resolutions[int(rng.integers(len(resolutions)))]
+        out.append((look_at(eye), perspective(fovy, w / h, near, far), eye, w, h))
+    return out
 
 
 def camera_c0(width=1280, height=720):
@@ -445,6 +489,78 @@ class Renderer:
 
     def finish(self):
         return self._check(self._lib.skm_finish(self._h))
+
+    # -- view batches
+    @staticmethod
+    def make_views(views):
+        arr = (View * len(views))()
+        for k, (view, proj, eye, w, h) in enumerate(views):
+            arr[k].view[:] = [float(x) for x in np.asarray(view, np.float32)]
+            arr[k].proj[:] = [float(x) for x in np.asarray(proj, np.float32)]
+            arr[k].camera_position[:] = [float(x) for x in np.asarray(eye, np.float32)[:3]]
+            arr[k].width, arr[k].height = int(w), int(h)
+        return arr
+
+    def draw_views(self, views, ranges=None, tile_size=16, radix_bits=8, graph=True):
+        """Enqueues one batch; `views` is a list of (view, proj, eye, width, height) or a View array."""
+        arr = views if isinstance(views, C.Array) else self.make_views(views)
+        rarr = None
+        if ranges is not None:
+            rarr = (Range * max(1, len(ranges)))(*[Range(o, c) for o, c in ranges])
+        t = BatchTuning(tile_size, radix_bits, 1 if graph else 0, 0)
+        self._views = arr
+        return self._check(self._lib.skm_draw_views(self._h, arr, len(arr), rarr, 0 if ranges is None else len(ranges),
+                                                    C.byref(t)))
+
+    def render_views(self, views, **kw):
+        """Draws one batch and waits; a batch whose tile pairs outgrew the capacity is drawn again."""
+        arr = views if isinstance(views, C.Array) else self.make_views(views)
+        self.draw_views(arr, **kw)
+        try:
+            self.finish()
+        except SkmError as e:
+            if e.status != SKM_ERR_CAPACITY:
+                raise
+            self.draw_views(arr, **kw)
+            self.finish()
+
+    def batch_stats(self):
+        s = BatchStats()
+        self._check(self._lib.skm_read_batch_stats(self._h, C.byref(s)))
+        return {"views": s.views, "graph": bool(s.graph), "gpu_ms": s.gpu_ms, "visible": s.visible, "pairs": s.pairs}
+
+    def read_view_frame(self, v, fmt=PIXELS_RGBA32F):
+        w, h = self._views[v].width, self._views[v].height
+        out = np.empty((h, w, 4), np.float32 if fmt == PIXELS_RGBA32F else np.uint8)
+        self._check(self._lib.skm_read_view_frame(self._h, v, fmt, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def read_view_visible(self, v, capacity=None):
+        capacity = capacity or max(1, self.world_count)
+        count = C.c_uint32()
+        keys = np.empty(capacity, np.uint32)
+        proj = np.empty((capacity, 8), np.uint32)
+        self._check(self._lib.skm_read_view_visible(self._h, v, C.byref(count), uptr(keys), uptr(proj), capacity))
+        n = min(count.value, capacity)
+        return count.value, keys[:n].copy(), proj[:n].copy()
+
+    def read_view_order(self, v, n):
+        order = np.empty(max(1, n), np.uint32)
+        self._check(self._lib.skm_read_view_order(self._h, v, uptr(order), n))
+        return order[:n].copy()
+
+    def read_view_tiles(self, v):
+        w, h = self._views[v].width, self._views[v].height
+        tiles = ((w + TILE - 1) // TILE) * ((h + TILE - 1) // TILE)
+        pairs = C.c_uint64()
+        self._check(self._lib.skm_read_view_tiles(self._h, v, C.byref(pairs), None, None, 0, None, 0))
+        cap = int(pairs.value)
+        pt = np.empty(max(1, cap), np.uint32)
+        pv = np.empty(max(1, cap), np.uint32)
+        ranges = np.empty((max(1, tiles), 2), np.uint32)
+        self._check(self._lib.skm_read_view_tiles(self._h, v, C.byref(pairs), uptr(pt), uptr(pv), cap, uptr(ranges),
+                                                  tiles))
+        return pairs.value, pt[:cap].copy(), pv[:cap].copy(), ranges[:tiles].copy()
 
     def stage_millis(self):
         """GPU ms of the last frame: LOD cut, cull, depth sort, bin + tile sort + ranges, raster, total."""

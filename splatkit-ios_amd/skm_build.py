@@ -38,7 +38,7 @@ def _run(cmd):
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in ("kernels.hip", "renderer.cpp", "lod_build.cpp", "lod.hip", "batch.hip", "server.cpp"):
+    for src in ("kernels.hip", "renderer.cpp", "batch.cpp", "lod_build.cpp", "lod.hip", "server.cpp"):
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue

@@ -44,6 +44,9 @@ def test_struct_layouts_match_the_reference(skm_mod):
     assert C.sizeof(skm_mod.Reveal) == 32
     assert C.sizeof(skm_mod.Range) == 8
     assert skm_mod.RenderPolicy.background_color.offset == 24 and C.sizeof(skm_mod.RenderPolicy) == 48
+    # The batch extension's structs (static_assert-ed to the same sizes in renderer.cpp).
+    assert C.sizeof(skm_mod.View) == 152 and C.sizeof(skm_mod.BatchTuning) == 16
+    assert C.sizeof(skm_mod.BatchStats) == 32
 
 
 def test_renderer_creation_fails_cleanly_without_a_device(skm_mod):
