@@ -76,6 +76,59 @@ def batched_vs_loop(r, skm, batches, loops):
             "batches": batches, "loops": loops}
 
 
+SERVING_SIZES = (500_000, 1_000_000, 2_000_000, 3_000_000, 5_000_000, 6_000_000, 8_000_000, 10_000_000)
+SERVING_RESOLUTIONS = ((1280, 720), (1920, 1080), (2560, 1440), (3840, 2160))
+
+
+def serving(skm, rates, per_rate):
+    """8 resident scenes (0.5M-10M splats, SH0-3 cyclic, drawn like the 3M scene), requests with a
+    random scene, a random camera on a sphere of radius 4-12 m and a resolution from 720p to 2160p,
+    arriving as a Poisson stream at each rate. Continuous batching against one request per launch."""
+    import numpy as np
+
+    clouds = [skm.make_cloud(n, k % 4, seed=0x5EED0004 + k, layout=1, spread=5.0, area=10.0)
+              for k, n in enumerate(SERVING_SIZES)]
+    out = {"workload": "8 resident scenes of 0.5/1/2/3/5/6/8/10M splats, SH0-3 cyclic; requests uniform over "
+                       "scenes, cameras on spheres of radius U(4,12) m, resolutions uniform over "
+                       "720p/1080p/1440p/2160p; Poisson arrivals; BGRA8 pixels delivered to host memory",
+           "requests_per_rate": per_rate, "modes": {}}
+    for mode, batch, flight in (("continuous_batching", 64, 0), ("one_request_per_launch", 1, 1)):
+        srv = skm.Server(max_batch_views=batch, max_in_flight=flight)
+        for c in clouds:
+            srv.add_scene(c)
+        rows = []
+        for k, rate in enumerate(rates):
+            rng = np.random.default_rng(1000 + k)
+            views = skm.views_c2(per_rate, seed=2000 + k, resolutions=SERVING_RESOLUTIONS)
+            scenes = rng.integers(0, len(clouds), per_rate)
+            if k == 0:  # warm-up: scratch growth and graph capture outside the measured window
+                srv.wait_all([srv.submit(int(s), *v) for s, v in zip(scenes[:32], views[:32])], timeout_s=300)
+                srv.stats(reset=True)
+            arrivals = np.cumsum(rng.exponential(1.0 / rate, per_rate))
+            t0 = time.perf_counter()
+            tickets = []
+            for s, v, at in zip(scenes, views, arrivals):
+                while time.perf_counter() - t0 < at:
+                    pass
+                tickets.append(srv.submit(int(s), *v))
+            done = srv.wait_all(tickets, timeout_s=300)
+            wall = time.perf_counter() - t0
+            st = srv.stats(reset=True)
+            for t in tickets:
+                srv.release(t)
+            ok = sum(1 for c in done.values() if c["status"] == 0)
+            rows.append({"offered_per_s": rate, "frames_per_s": round(ok / wall, 1), "p50_ms": round(st["p50_ms"], 2),
+                         "p99_ms": round(st["p99_ms"], 2), "mean_ms": round(st["mean_ms"], 2),
+                         "mean_batch_views": round(st["mean_batch_views"], 2), "launches": st["launches"],
+                         "rejected": st["rejected"]})
+        out["modes"][mode] = rows
+        srv.close()
+    best = {m: max(r["frames_per_s"] for r in rows) for m, rows in out["modes"].items()}
+    out["max_frames_per_s"] = best
+    out["speedup_at_saturation"] = round(best["continuous_batching"] / best["one_request_per_launch"], 3)
+    return out
+
+
 def cpu_baseline(frames, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -104,6 +157,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=2)
     ap.add_argument("--batches", type=int, default=5, help="timed 64-view batches (0 skips the batch leg)")
+    ap.add_argument("--serving-requests", type=int, default=300,
+                    help="requests per offered rate in the serving leg (0 skips it)")
+    ap.add_argument("--serving-rates", default="150,300,600,1200,2400")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the render path on a single MI355X "
@@ -116,6 +172,7 @@ def main():
     cloud = skm.scene_c1(N_SPLATS, SH_DEGREE)
     gen_s = time.time() - t0
     r = skm.Renderer(0)
+    device = r.description()
     r.upload_world(cloud)
     r.set_extent(WIDTH, HEIGHT)
     r.apply_policy(sort_depth=32)
@@ -160,6 +217,10 @@ def main():
                     "frac": round(gbs * 1e9 / HBM_PEAK, 4), "traffic": None}
 
     batched = batched_vs_loop(r, skm, args.batches, 2) if args.batches > 0 else None
+    r.close()
+    served = None
+    if args.serving_requests > 0:
+        served = serving(skm, [float(x) for x in args.serving_rates.split(",")], args.serving_requests)
 
     cpu = None
     if not args.no_cpu:
@@ -181,10 +242,9 @@ def main():
         "msplats_per_s": round(N_SPLATS * fps / 1e6, 1),
         "visible_per_frame": int(V), "pairs_per_frame": int(P),
         "gpu_ms_per_frame": round(tot["stage_ms"][5] / n_frames, 4),
-        "stages": stages, "roofline": roofline, "cpu_baseline": cpu, "batched": batched,
-        "device": r.description(), "scene_generation_s": round(gen_s, 2),
+        "stages": stages, "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "serving": served,
+        "device": device, "scene_generation_s": round(gen_s, 2),
     }
-    r.close()
     print(json.dumps(out), flush=True)
 
 

@@ -36,8 +36,11 @@ typedef enum skm_status {
 
 typedef struct skm_config {
   int32_t device;          /* HIP device ordinal */
-  uint32_t tile_size;      /* raster tile edge in pixels; 16 */
+  uint32_t tile_size;      /* raster tile edge in pixels: 8, 16 or 32 (0 = 16) */
   uint32_t pair_factor;    /* initial tile-pair capacity per splat slot (grows on overflow); 0 = 8 */
+  uint32_t sort_kernel;    /* radix pass: 0 = reduce then scan of tile counts, 1 = onesweep look-back */
+  uint32_t raster_px;      /* raster pixels per lane side: 1 or 2 (0 = 1, or 2 at 32-pixel tiles);
+                            * supported (tile, px): (8, 1), (16, 1), (16, 2), (32, 2) */
   uint32_t reserved;
 } skm_config;
 
@@ -234,7 +237,8 @@ skm_status skm_read_frame_totals(skm_renderer* r, skm_frame_totals* out, int res
 /* Stable ascending argsort of host keys; key_bits 16 (2 passes, keys < 2^16) or 32 (4 passes). */
 skm_status skm_argsort_u32(skm_renderer* r, const uint32_t* keys, uint32_t n, int key_bits,
                            uint32_t* order);
-/* Same on device memory already resident; *ms gets the GPU time of the sort. */
+/* Same on device memory already resident; *ms gets the GPU time of the sort. The key buffer is
+ * used as ping-pong scratch: its contents are overwritten. */
 skm_status skm_argsort_u32_device(skm_renderer* r, void* keys_dev, uint32_t n, int key_bits,
                                   void* order_dev, double* ms);
 
@@ -261,9 +265,11 @@ typedef struct skm_view {
 } skm_view;
 
 typedef struct skm_batch_tuning {
-  uint32_t tile_size;       /* raster tile edge in pixels (0 = 16) */
+  uint32_t tile_size;       /* raster tile edge in pixels, as skm_config (0 = 16) */
   uint32_t tile_radix_bits; /* digit width of the tile sort (0 = 8) */
   uint32_t use_graph;       /* 1: capture the batch's kernels in a hipGraph and replay it */
+  uint32_t sort_kernel;     /* radix pass variant, as skm_config.sort_kernel */
+  uint32_t raster_px;       /* raster pixels per lane side, as skm_config.raster_px */
   uint32_t reserved;
 } skm_batch_tuning;
 
@@ -281,6 +287,19 @@ typedef struct skm_batch_stats {
 skm_status skm_draw_views(skm_renderer* r, const skm_view* views, uint32_t count, const skm_range* ranges,
                           uint32_t range_count, const skm_batch_tuning* tuning);
 skm_status skm_read_batch_stats(skm_renderer* r, skm_batch_stats* out);
+
+/* The batch autotuner: times every supported configuration (raster tile and block shape, radix pass
+ * variant) on this batch of views, `repeats` timed launches each after one untimed launch, and makes
+ * the fastest the renderer's batch tuning (used when skm_draw_views gets tuning NULL). results gets
+ * up to `capacity` candidates with their best GPU milliseconds; *best the choice. Every candidate
+ * gives the same pixels: the composite of a pixel does not depend on tiles or block shape. */
+typedef struct skm_tuning_result {
+  skm_batch_tuning tuning;
+  double gpu_ms;
+} skm_tuning_result;
+skm_status skm_autotune_views(skm_renderer* r, const skm_view* views, uint32_t count, const skm_range* ranges,
+                              uint32_t range_count, uint32_t repeats, skm_tuning_result* results,
+                              uint32_t capacity, uint32_t* result_count, skm_batch_tuning* best);
 /* Read-backs of one view of the last completed batch, in skm_read_frame / skm_read_visible /
  * skm_read_order / skm_read_tiles terms (indices local to the view). */
 skm_status skm_read_view_frame(skm_renderer* r, uint32_t view, int format, void* dst, size_t dst_bytes);

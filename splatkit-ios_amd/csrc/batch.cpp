@@ -9,7 +9,6 @@ namespace skm {
 
 namespace {
 
-constexpr uint32_t kTile = 16;
 constexpr uint32_t kMaxViews = 4096;
 constexpr uint64_t kLaneLimit = 0xffff0000ull;  // survivor and pair indices are 32-bit
 
@@ -96,6 +95,7 @@ hipError_t Batch::ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_
   const size_t sort = static_cast<size_t>(std::max(lane_capacity_, pair_capacity_) / kSortTile + 2) * kRadix;
   if (e == hipSuccess && sort > sort_words_) {
     need(sort_status_, sort * 8);
+    need(sort_counts_, sort * 4);
     if (e == hipSuccess) e = hipMemset(sort_status_.ptr, 0, sort_status_.bytes);
     if (e == hipSuccess) sort_words_ = sort;
   }
@@ -124,9 +124,11 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
     bool dropped = false;
     if (const hipError_t e = wait(&dropped); e != hipSuccess) return e;
   }
-  if (count == 0 || count > kMaxViews || (tuning.tile_size != 0 && tuning.tile_size != kTile) ||
-      (tuning.tile_radix_bits != 0 && tuning.tile_radix_bits != 8)) {
-    *error = "batch: 1 to 4096 views, 16-pixel tiles and 8-bit tile digits";
+  const uint32_t kTile = tuning.tile_size ? tuning.tile_size : 16u;
+  const uint32_t px = tuning.raster_px ? tuning.raster_px : (kTile == 32u ? 2u : 1u);
+  if (count == 0 || count > kMaxViews || !raster_shape_supported(kTile, px) ||
+      (tuning.tile_radix_bits != 0 && tuning.tile_radix_bits != 8) || tuning.sort_kernel > 1) {
+    *error = "batch: 1 to 4096 views, a supported raster shape and 8-bit tile digits";
     return hipErrorInvalidValue;
   }
   uint64_t total = 0;
@@ -244,6 +246,9 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
   a.view_start = view_start_.as<uint32_t>();
   a.tile_view = tile_view_.as<uint16_t>();
   a.total_tiles = static_cast<uint32_t>(tiles);
+  a.sort_variant = tuning.sort_kernel == 1 ? kSortOnesweep : kSortReduceScan;
+  a.raster_px = px;
+  a.sort_counts = sort_counts_.as<uint32_t>();
   depth_passes_ = policy.key16 ? 2 : 4;
   tile_passes_ = static_cast<int>(digit_passes(static_cast<uint32_t>(tiles - 1)));
   (void)world_count;
@@ -254,6 +259,7 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
     // What the captured kernels bake in: sizes, passes, addresses (reset on reallocation), the
     // background and the world. Cameras, views and ranges are read from device memory.
     std::vector<uint64_t> key = {count, total, range_count, uint64_t(world.sh_degree), uint64_t(policy.key16),
+                                 uint64_t(a.sort_variant), kTile, px,
                                  tiles, uint64_t(depth_passes_), uint64_t(tile_passes_),
                                  reinterpret_cast<uint64_t>(world.px), world.capacity, pair_capacity_,
                                  lane_capacity_};
@@ -314,6 +320,40 @@ hipError_t Batch::wait(bool* dropped) {
     pair_want_ = std::min<uint64_t>(uint64_t{ctrl_host()->pairs} + ctrl_host()->pairs / 4 + 4096, kLaneLimit);
   }
   return hipSuccess;
+}
+
+uint64_t Batch::device_bytes() const {
+  uint64_t b = 0;
+  for (const DeviceBuffer* d : {&ctrl_, &cams_, &views_, &tile_view_, &view_start_, &survivor_view_, &cull_keys_,
+                                &keys_[0], &keys_[1], &order_[0], &order_[1], &projected_, &scan_status_,
+                                &sort_status_, &sort_counts_, &hist_, &pair_keys_[0], &pair_keys_[1], &pair_vals_[0],
+                                &pair_vals_[1], &tile_ranges_, &image_, &ranges_, &starts_})
+    b += d->bytes;
+  return b;
+}
+
+uint64_t Batch::bytes_for(uint32_t views, uint64_t lanes, uint32_t tiles, uint64_t pixels) const {
+  lanes = std::max(lanes, lane_capacity_);
+  const uint64_t pairs = std::max<uint64_t>({pair_want_, pair_capacity_, lanes * 3, uint64_t{1} << 22});
+  tiles = std::max(tiles, tile_capacity_);
+  pixels = std::max(pixels, pixel_capacity_);
+  views = std::max(views, view_capacity_);
+  const uint64_t status = (lanes / 1024 + 2) * 8 + (std::max(lanes, pairs) / kSortTile + 2) * kRadix * 12;
+  return lanes * (5 * 4 + 32 + 2) + pairs * 16 + uint64_t{tiles} * 10 + pixels * 16 +
+         uint64_t{views} * (sizeof(Camera) + sizeof(ViewInfo) + 4) + status + (1u << 20);
+}
+
+void Batch::release() {
+  release_graph();
+  for (DeviceBuffer* d : {&ctrl_, &cams_, &views_, &tile_view_, &view_start_, &survivor_view_, &cull_keys_,
+                          &keys_[0], &keys_[1], &order_[0], &order_[1], &projected_, &scan_status_, &sort_status_,
+                          &sort_counts_, &hist_, &pair_keys_[0], &pair_keys_[1], &pair_vals_[0], &pair_vals_[1], &tile_ranges_,
+                          &image_, &ranges_, &starts_})
+    d->reset();
+  lane_capacity_ = pair_capacity_ = pixel_capacity_ = 0;
+  view_capacity_ = tile_capacity_ = range_capacity_ = 0;
+  scan_words_ = sort_words_ = 0;
+  tile_view_host_.clear();
 }
 
 hipError_t Batch::read_image(uint32_t v, float* rgba) const {

@@ -10,6 +10,8 @@
 //   k_bin       tile rectangles of the depth-ordered survivors, one scan + emission pass
 //   k_ranges    [start, end) per tile from the tile-sorted pairs
 //   k_raster    16x16-tile front-to-back composite, LDS-staged batches, early termination
+#include <algorithm>
+
 #include "skm_internal.h"
 #include "skm_lod_internal.h"
 #include "skm_math.h"
@@ -485,16 +487,25 @@ __global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ keys,
   }
 }
 
+constexpr uint32_t kLookback = 16;
+
 // One stable LSD pass over 4096-key tiles. Keys of tile b sit at b*4096 + wave*1024 + j*64 + lane:
 // each wave ranks its 16 rows in order with 8 ballots per row (lanes with equal digits form a
-// peer mask), so the in-tile order is position order and the pass is stable.
+// peer mask), so the in-tile order is position order and the pass is stable. The tile then learns
+// where each of its digits starts in the output:
+//   LOOKBACK (onesweep): tiles take tickets in order and look back across predecessors' per-digit
+//     status words (decoupled look-back), one kernel per pass;
+//   !LOOKBACK (reduce then scan): k_tile_counts and k_digit_scan have already written the start of
+//     digit d of tile t at offsets[d * stride + t].
+template <bool LOOKBACK>
 __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ keys_in,
                                                   const uint32_t* __restrict__ vals_in,
                                                   uint32_t* __restrict__ keys_out,
                                                   uint32_t* __restrict__ vals_out, const uint32_t* n_ptr,
                                                   const uint32_t* __restrict__ hist, uint32_t shift,
                                                   uint64_t* status, Ctrl* ctrl, uint32_t ticket,
-                                                  uint32_t tag_pass) {
+                                                  uint32_t tag_pass, const uint32_t* __restrict__ offsets,
+                                                  uint32_t stride) {
   __shared__ uint32_t s_ticket;
   __shared__ uint32_t s_wave_count[4][kRadix];
   __shared__ uint32_t s_digit_start[kRadix];
@@ -506,13 +517,12 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
   const uint32_t n = *n_ptr;
   const uint32_t blocks = (n + kSortTile - 1u) / kSortTile;
   if (blockIdx.x >= blocks) return;  // before the ticket: exactly `blocks` tickets are taken
-  if (tid == 0) s_ticket = atomicAdd(&ctrl->ticket[ticket], 1u);
+  if (LOOKBACK && tid == 0) s_ticket = atomicAdd(&ctrl->ticket[ticket], 1u);
 #pragma unroll
   for (uint32_t w = 0; w < 4; ++w) s_wave_count[w][tid] = 0;
   __syncthreads();
-  const uint32_t b = s_ticket;
+  const uint32_t b = LOOKBACK ? s_ticket : blockIdx.x;
   const uint32_t base = b * kSortTile;
-  const uint32_t tag = make_tag(ctrl->epoch, tag_pass);
 
   uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
 #pragma unroll
@@ -546,46 +556,51 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
   const uint32_t count = c0 + c1 + c2 + c3;
   uint32_t tile_total;
   const uint32_t start = block_exclusive_scan(count, s_scan, &tile_total);
-  // Global start of each digit: exclusive scan of the pass histogram.
-  uint32_t all;
-  const uint32_t gstart = block_exclusive_scan(hist[tid], s_scan, &all);
   s_wave_count[0][tid] = 0;
   s_wave_count[1][tid] = c0;
   s_wave_count[2][tid] = c0 + c1;
   s_wave_count[3][tid] = c0 + c1 + c2;
   s_digit_start[tid] = start;
-  // Per-digit look-back across tiles.
-  uint64_t* st = status + static_cast<size_t>(b) * kRadix;
-  uint32_t excl = 0;
-  if (b == 0) {
-    status_store(st + tid, status_word(tag, kFlagPrefix, count));
-  } else {
-    status_store(st + tid, status_word(tag, kFlagAggregate, count));
-    // Four predecessors per round trip, nearest first, until one holds an inclusive prefix.
-    int32_t k = static_cast<int32_t>(b) - 1;
-    while (k >= 0) {
-      uint64_t s4[4];
+  if constexpr (LOOKBACK) {
+    // Global start of each digit: exclusive scan of the pass histogram.
+    uint32_t all;
+    const uint32_t gstart = block_exclusive_scan(hist[tid], s_scan, &all);
+    const uint32_t tag = make_tag(ctrl->epoch, tag_pass);
+    // Per-digit look-back across tiles.
+    uint64_t* st = status + static_cast<size_t>(b) * kRadix;
+    uint32_t excl = 0;
+    if (b == 0) {
+      status_store(st + tid, status_word(tag, kFlagPrefix, count));
+    } else {
+      status_store(st + tid, status_word(tag, kFlagAggregate, count));
+      // kLookback predecessors per round trip (loads in flight together), nearest first, until one
+      // holds an inclusive prefix.
+      int32_t k = static_cast<int32_t>(b) - 1;
+      while (k >= 0) {
+        uint64_t sw[kLookback];
 #pragma unroll
-      for (int32_t j = 0; j < 4; ++j)
-        s4[j] = (k - j >= 0) ? status_load(status + static_cast<size_t>(k - j) * kRadix + tid) : 0ull;
-      int32_t j = 0;
-      bool done = false;
-      for (; j < 4 && k - j >= 0; ++j) {
-        const uint32_t flag = static_cast<uint32_t>(s4[j] >> 32) & 3u;
-        if (static_cast<uint32_t>(s4[j] >> 34) != tag || flag == 0u) break;  // not published yet
-        excl += static_cast<uint32_t>(s4[j]);
-        if (flag == kFlagPrefix) {
-          done = true;
-          break;
+        for (int32_t j = 0; j < static_cast<int32_t>(kLookback); ++j)
+          sw[j] = (k - j >= 0) ? status_load(status + static_cast<size_t>(k - j) * kRadix + tid) : 0ull;
+        int32_t j = 0;
+        bool done = false;
+#pragma unroll
+        for (int32_t i = 0; i < static_cast<int32_t>(kLookback); ++i) {
+          if (done || j != i || k - i < 0) continue;
+          const uint32_t flag = static_cast<uint32_t>(sw[i] >> 32) & 3u;
+          if (static_cast<uint32_t>(sw[i] >> 34) != tag || flag == 0u) continue;  // not published yet
+          excl += static_cast<uint32_t>(sw[i]);
+          j = i + 1;
+          if (flag == kFlagPrefix) done = true;
         }
+        if (done) break;
+        k -= j;
       }
-      if (done) break;
-      if (j == 0) __builtin_amdgcn_s_sleep(1);
-      k -= j;
+      status_store(st + tid, status_word(tag, kFlagPrefix, excl + count));
     }
-    status_store(st + tid, status_word(tag, kFlagPrefix, excl + count));
+    s_base[tid] = gstart + excl - start;
+  } else {
+    s_base[tid] = offsets[static_cast<size_t>(tid) * stride + b] - start;
   }
-  s_base[tid] = gstart + excl - start;
   __syncthreads();
   // Stage the tile in digit order in LDS.
 #pragma unroll
@@ -605,6 +620,64 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
     const uint32_t dst = s_base[(k >> shift) & 255u] + i;
     if (keys_out != nullptr) keys_out[dst] = k;
     vals_out[dst] = s_vals[i];
+  }
+}
+
+// Reduce then scan, step 1: the digit counts of each 4096-key tile, digit-major
+// (counts[d * stride + tile]), from per-wave LDS histograms.
+__global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
+                                                     uint32_t shift, uint32_t* __restrict__ counts,
+                                                     uint32_t stride) {
+  __shared__ uint32_t s_h[4][kRadix];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6;
+  const uint32_t n = *n_ptr;
+  if (blockIdx.x >= (n + kSortTile - 1u) / kSortTile) return;
+#pragma unroll
+  for (uint32_t w = 0; w < 4; ++w) s_h[w][tid] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kSortTile;
+#pragma unroll
+  for (uint32_t j = 0; j < kSortItems; ++j) {
+    const uint32_t pos = base + j * 256u + tid;
+    if (pos < n) atomicAdd(&s_h[wave][(keys[pos] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  counts[static_cast<size_t>(tid) * stride + blockIdx.x] = s_h[0][tid] + s_h[1][tid] + s_h[2][tid] + s_h[3][tid];
+}
+
+// Reduce then scan, step 2: one block per digit turns its row of tile counts into output starts:
+// the digit's global start (the pass histogram's exclusive scan) plus the exclusive scan over tiles.
+__global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ counts, const uint32_t* n_ptr,
+                                                    const uint32_t* __restrict__ hist, uint32_t stride) {
+  constexpr uint32_t kPer = 8;
+  __shared__ uint32_t s_scan[4];
+  const uint32_t tid = threadIdx.x, d = blockIdx.x;
+  const uint32_t n = *n_ptr;
+  const uint32_t tiles = (n + kSortTile - 1u) / kSortTile;
+  uint32_t total;
+  const uint32_t below = block_exclusive_scan(hist[tid], s_scan, &total);
+  __shared__ uint32_t s_start;
+  if (tid == d) s_start = below;
+  __syncthreads();
+  uint32_t carry = s_start;
+  uint32_t* row = counts + static_cast<size_t>(d) * stride;
+  for (uint32_t t0 = 0; t0 < tiles; t0 += 256u * kPer) {
+    uint32_t v[kPer], sum = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+      const uint32_t t = t0 + tid * kPer + i;
+      v[i] = t < tiles ? row[t] : 0u;
+      sum += v[i];
+    }
+    uint32_t chunk;
+    uint32_t run = carry + block_exclusive_scan(sum, s_scan, &chunk);
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+      const uint32_t t = t0 + tid * kPer + i;
+      if (t < tiles) row[t] = run;
+      run += v[i];
+    }
+    carry += chunk;
   }
 }
 
@@ -769,8 +842,10 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ key
 
 // One coverage-and-blend step of a pixel against a prepared record (the oracle's decide and fp32
 // composite): scaled quad coordinates from the pixel centre's offset, the quad and cutoff tests
-// select alpha = 2^min(log2 alpha, 0) or 0, then w = T alpha, C += w c, T -= w. No branch: a wave
-// runs the same instructions whatever its lanes decide.
+// select alpha = 2^min(log2 alpha, 0) or 0, then w = T alpha, C += w c, T -= w. A pixel whose
+// transmittance is already below kTransmittanceStop takes no more splats, so a pixel's value does not
+// depend on which other pixels share its wave. No branch: a wave runs the same instructions whatever
+// its lanes decide.
 __device__ __forceinline__ void blend_step(const float4 A, const float4 B, const float4 Cc, float fx,
                                            float fy, float& T, float& C0, float& C1, float& C2) {
   const float dx = fx - A.x;
@@ -779,7 +854,7 @@ __device__ __forceinline__ void blend_step(const float4 A, const float4 B, const
   const float gv = fmaf(B.x, dx, B.y * dy);
   const float e = B.w - fmaf(gu, gu, gv * gv);
   const bool take = static_cast<int>(fabsf(gu) <= B.z) & static_cast<int>(fabsf(gv) <= B.z) &
-                    static_cast<int>(e >= kLog2MinAlpha);
+                    static_cast<int>(e >= kLog2MinAlpha) & static_cast<int>(T >= kTransmittanceStop);
   // e is a number when take holds, so fminf agrees with the oracle's min; 2^-inf = 0.
   const float alpha = __builtin_amdgcn_exp2f(take ? fminf(e, 0.0f) : -__builtin_inff());
   const float w = T * alpha;
@@ -789,18 +864,33 @@ __device__ __forceinline__ void blend_step(const float4 A, const float4 B, const
   T = T - w;
 }
 
-// 16x16 tile per 256-thread block; wave q owns the 8x8 quadrant (q & 1, q >> 1), one pixel per
-// lane. Each batch of up to 256 pairs is prepared once into LDS (centre in pixels, the inverse axis
-// matrix scaled by kGaussianScale, the scaled radius, log2 peak, clamped colour), and every
-// quadrant gets the list, in batch order, of the records whose footprint box reaches one of its
-// pixel centres. A wave walks its list front to back. Lanes keep compositing after their
-// transmittance fell below kTransmittanceStop (that only adds exact terms); a wave leaves its list
-// once all of its lanes are below it, and the tile stops when every lane is.
-__global__ __launch_bounds__(256) void k_raster(FrameArgs a, const uint32_t* __restrict__ pair_vals,
-                                                const uint32_t* __restrict__ ranges) {
-  __shared__ float4 s_a[256], s_b[256], s_c[256];
-  __shared__ __align__(16) uint8_t s_list[4][256];
-  __shared__ uint32_t s_count[4][4];  // [quadrant][wave]
+// Raster block shapes: a TILE x TILE tile per block, cut into square regions of 8 * PX pixels, one
+// wave per region, PX x PX pixels per lane (lane (x, y) of the 8 x 8 lane grid takes the pixels
+// (x + 8 i, y + 8 j)).
+template <int TILE, int PX>
+struct RasterShape {
+  static constexpr uint32_t kRegion = 8u * PX;
+  static constexpr uint32_t kSide = TILE / kRegion;     // regions per tile side
+  static constexpr uint32_t kWaves = kSide * kSide;
+  static constexpr uint32_t kThreads = 64u * kWaves;   // also the records prepared per batch
+  static constexpr uint32_t kPixels = PX * PX;
+  static_assert(TILE % kRegion == 0 && kWaves <= 4, "shape");
+};
+
+// TILE x TILE tile per block. Each batch of up to kThreads pairs is prepared once into LDS (centre in
+// pixels, the inverse axis matrix scaled by kGaussianScale, the scaled radius, log2 peak, clamped
+// colour), and every region gets the list, in batch order, of the records whose footprint box
+// reaches one of its pixel centres. A wave walks its region's list front to back; it leaves the list
+// once all of its pixels are below kTransmittanceStop, and the tile stops when every pixel is.
+template <int TILE, int PX>
+__global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(FrameArgs a,
+                                                                           const uint32_t* __restrict__ pair_vals,
+                                                                           const uint32_t* __restrict__ ranges) {
+  using S = RasterShape<TILE, PX>;
+  constexpr uint32_t kB = S::kThreads, kW = S::kWaves, kP = S::kPixels;
+  __shared__ float4 s_a[kB], s_b[kB], s_c[kB];
+  __shared__ __align__(16) uint8_t s_list[kW][kB];
+  __shared__ uint32_t s_count[kW][kW];  // [region][wave]
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
   const uint32_t tile_id = blockIdx.x;  // global tile id
   uint32_t width = a.width, height = a.height, tiles_x = a.tiles_x, local = tile_id;
@@ -813,18 +903,30 @@ __global__ __launch_bounds__(256) void k_raster(FrameArgs a, const uint32_t* __r
     local = tile_id - vi.tile_base;
     pixel_base = vi.pixel_base;
   }
-  const uint32_t ox = (local % tiles_x) * 16u, oy = (local / tiles_x) * 16u;
-  const uint32_t px = ox + (wave & 1u) * 8u + (lane & 7u);
-  const uint32_t py = oy + (wave >> 1) * 8u + (lane >> 3);
-  const bool in_image = px < width && py < height;
+  const uint32_t ox = (local % tiles_x) * TILE, oy = (local / tiles_x) * TILE;
+  const uint32_t rx = ox + (wave % S::kSide) * S::kRegion, ry = oy + (wave / S::kSide) * S::kRegion;
   const uint32_t start = ranges[2 * tile_id], end = ranges[2 * tile_id + 1];
   const float fw = static_cast<float>(width), fh = static_cast<float>(height);
-  const float fx = static_cast<float>(px) + 0.5f, fy = static_cast<float>(py) + 0.5f;
-  const float tx = static_cast<float>(ox) + 0.5f, ty = static_cast<float>(oy) + 0.5f;
-  float T = in_image ? 1.0f : 0.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
-  for (uint32_t s0 = start; s0 < end; s0 += 256u) {
+  float fx[kP], fy[kP], T[kP], C0[kP], C1[kP], C2[kP];
+  bool in_image[kP];
+#pragma unroll
+  for (uint32_t q = 0; q < kP; ++q) {
+    const uint32_t px = rx + (lane & 7u) + 8u * (q % PX), py = ry + (lane >> 3) + 8u * (q / PX);
+    in_image[q] = px < width && py < height;
+    fx[q] = static_cast<float>(px) + 0.5f;
+    fy[q] = static_cast<float>(py) + 0.5f;
+    T[q] = in_image[q] ? 1.0f : 0.0f;
+    C0[q] = C1[q] = C2[q] = 0.0f;
+  }
+  auto live = [&]() {
+    bool any = false;
+#pragma unroll
+    for (uint32_t q = 0; q < kP; ++q) any |= T[q] >= kTransmittanceStop;
+    return any;
+  };
+  for (uint32_t s0 = start; s0 < end; s0 += kB) {
     const uint32_t j = s0 + tid;
-    uint32_t quads = 0;
+    uint32_t regions = 0;
     if (j < end) {
       const uint32_t idx = pair_vals[j];
       const uint4* src = reinterpret_cast<const uint4*>(a.projected + static_cast<size_t>(idx) * 8u);
@@ -842,9 +944,11 @@ __global__ __launch_bounds__(256) void k_raster(FrameArgs a, const uint32_t* __r
         const float x0 = (cx - ex) - kBinPad, x1 = (cx + ex) + kBinPad;
         const float y0 = (cy - ey) - kBinPad, y1 = (cy + ey) + kBinPad;
 #pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) {
-          const float qx = tx + static_cast<float>((q & 1u) * 8u), qy = ty + static_cast<float>((q >> 1) * 8u);
-          if (x1 >= qx && x0 <= qx + 7.0f && y1 >= qy && y0 <= qy + 7.0f) quads |= 1u << q;
+        for (uint32_t q = 0; q < kW; ++q) {
+          const float qx = static_cast<float>(ox + (q % S::kSide) * S::kRegion) + 0.5f;
+          const float qy = static_cast<float>(oy + (q / S::kSide) * S::kRegion) + 0.5f;
+          constexpr float kSpan = static_cast<float>(S::kRegion - 1u);
+          if (x1 >= qx && x0 <= qx + kSpan && y1 >= qy && y0 <= qy + kSpan) regions |= 1u << q;
         }
       }
       s_a[tid] = make_float4(cx, cy, a2y * sc, -(a2x * sc));
@@ -853,25 +957,28 @@ __global__ __launch_bounds__(256) void k_raster(FrameArgs a, const uint32_t* __r
                              mn(mx(hi_half(w1.x), 0.0f), kMaxColorScale),
                              mn(mx(lo_half(w1.y), 0.0f), kMaxColorScale), 0.0f);
     }
-    uint64_t ball[4];
+    uint64_t ball[kW];
 #pragma unroll
-    for (uint32_t q = 0; q < 4; ++q) {
-      ball[q] = __ballot((quads >> q) & 1u);
+    for (uint32_t q = 0; q < kW; ++q) {
+      ball[q] = __ballot((regions >> q) & 1u);
       if (lane == 0) s_count[q][wave] = static_cast<uint32_t>(__popcll(ball[q]));
     }
     __syncthreads();
 #pragma unroll
-    for (uint32_t q = 0; q < 4; ++q) {
-      if ((quads >> q) & 1u) {
-        const uint32_t off = rank_below(ball[q]) + (wave > 0u ? s_count[q][0] : 0u) +
-                             (wave > 1u ? s_count[q][1] : 0u) + (wave > 2u ? s_count[q][2] : 0u);
+    for (uint32_t q = 0; q < kW; ++q) {
+      if ((regions >> q) & 1u) {
+        uint32_t off = rank_below(ball[q]);
+#pragma unroll
+        for (uint32_t w = 0; w + 1 < kW; ++w) off += (w < wave) ? s_count[q][w] : 0u;
         s_list[q][off] = static_cast<uint8_t>(tid);
       }
     }
-    const uint32_t n = __builtin_amdgcn_readfirstlane(s_count[wave][0] + s_count[wave][1] +
-                                                      s_count[wave][2] + s_count[wave][3]);
+    uint32_t n = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kW; ++w) n += s_count[wave][w];
+    n = __builtin_amdgcn_readfirstlane(n);
     __syncthreads();
-    if (__ballot(T >= kTransmittanceStop) != 0) {
+    if (__ballot(live()) != 0) {
       const uint32_t* list = reinterpret_cast<const uint32_t*>(s_list[wave]);
       for (uint32_t i = 0; i < n; i += 4u) {
         const uint32_t four = list[i >> 2];
@@ -879,17 +986,60 @@ __global__ __launch_bounds__(256) void k_raster(FrameArgs a, const uint32_t* __r
         for (uint32_t u = 0; u < 4; ++u) {
           if (i + u < n) {
             const uint32_t k = (four >> (8u * u)) & 255u;
-            blend_step(s_a[k], s_b[k], s_c[k], fx, fy, T, C0, C1, C2);
+            const float4 A = s_a[k], B = s_b[k], Cc = s_c[k];
+#pragma unroll
+            for (uint32_t q = 0; q < kP; ++q) blend_step(A, B, Cc, fx[q], fy[q], T[q], C0[q], C1[q], C2[q]);
           }
         }
-        if (__ballot(T >= kTransmittanceStop) == 0) break;
+        if (__ballot(live()) == 0) break;
       }
     }
-    if (__syncthreads_count(T >= kTransmittanceStop ? 1 : 0) == 0) break;
+    if (__syncthreads_count(live() ? 1 : 0) == 0) break;
   }
-  if (in_image) {
+#pragma unroll
+  for (uint32_t q = 0; q < kP; ++q) {
+    if (!in_image[q]) continue;
+    const uint32_t px = rx + (lane & 7u) + 8u * (q % PX), py = ry + (lane >> 3) + 8u * (q / PX);
     float4* out = reinterpret_cast<float4*>(a.image) + pixel_base + static_cast<size_t>(py) * width + px;
-    *out = make_float4(C0 + T * a.background[0], C1 + T * a.background[1], C2 + T * a.background[2], 1.0f);
+    *out = make_float4(C0[q] + T[q] * a.background[0], C1[q] + T[q] * a.background[1],
+                       C2[q] + T[q] * a.background[2], 1.0f);
+  }
+}
+
+// The raster block shapes on offer: (tile, pixels per lane side).
+hipError_t launch_raster(const FrameArgs& a, const uint32_t* pair_vals, uint32_t tiles, hipStream_t s) {
+  const int shape = a.tile * 4 + a.raster_px;
+  switch (shape) {
+    case 8 * 4 + 1:
+      hipLaunchKernelGGL((k_raster<8, 1>), dim3(tiles), dim3(64), 0, s, a, pair_vals, a.tile_ranges);
+      break;
+    case 16 * 4 + 2:
+      hipLaunchKernelGGL((k_raster<16, 2>), dim3(tiles), dim3(64), 0, s, a, pair_vals, a.tile_ranges);
+      break;
+    case 32 * 4 + 2:
+      hipLaunchKernelGGL((k_raster<32, 2>), dim3(tiles), dim3(256), 0, s, a, pair_vals, a.tile_ranges);
+      break;
+    case 16 * 4 + 1:
+      hipLaunchKernelGGL((k_raster<16, 1>), dim3(tiles), dim3(256), 0, s, a, pair_vals, a.tile_ranges);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// MetalFrameCapture's BGRA8 (MetalFrameCapture.h:18) of the RGBA float composite: each channel
+// clamped to [0, 1], times 255, rounded half away from zero (the host conversion's lround).
+__global__ __launch_bounds__(256) void k_to_bgra8(const float4* __restrict__ rgba, uint32_t* __restrict__ out,
+                                                  uint64_t pixels) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < pixels; i += stride) {
+    const float4 c = rgba[i];
+    auto q = [](float v) {
+      const float x = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+      return static_cast<uint32_t>(roundf(x * 255.0f));
+    };
+    out[i] = q(c.z) | (q(c.y) << 8) | (q(c.x) << 16) | (q(c.w) << 24);
   }
 }
 
@@ -909,6 +1059,23 @@ uint32_t grid_for(uint32_t items, uint32_t per_block) {
   return b == 0u ? 1u : b;
 }
 
+// One LSD pass of 8 bits at `shift` in the chosen variant; `counts` holds 256 words per 4096-key
+// tile of `capacity` (reduce then scan only).
+void sort_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, const uint32_t* n_ptr,
+               uint32_t capacity, const uint32_t* hist, uint32_t shift, uint64_t* status, Ctrl* ctrl,
+               uint32_t ticket, uint32_t tag_pass, uint32_t* counts, int variant, hipStream_t s) {
+  const uint32_t tiles = grid_for(capacity, kSortTile);
+  if (variant == kSortOnesweep) {
+    hipLaunchKernelGGL(k_onesweep<true>, dim3(tiles), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist, shift,
+                       status, ctrl, ticket, tag_pass, nullptr, 0u);
+  } else {
+    hipLaunchKernelGGL(k_tile_counts, dim3(tiles), dim3(256), 0, s, kin, n_ptr, shift, counts, tiles);
+    hipLaunchKernelGGL(k_digit_scan, dim3(kRadix), dim3(256), 0, s, counts, n_ptr, hist, tiles);
+    hipLaunchKernelGGL(k_onesweep<false>, dim3(tiles), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist, shift,
+                       status, ctrl, ticket, tag_pass, counts, tiles);
+  }
+}
+
 }  // namespace
 
 hipError_t launch_pack(const WorldOut& out, uint32_t offset, uint32_t count, const float* positions,
@@ -920,9 +1087,17 @@ hipError_t launch_pack(const WorldOut& out, uint32_t offset, uint32_t count, con
   return hipGetLastError();
 }
 
+hipError_t launch_to_bgra8(const float* rgba, uint32_t* out, uint64_t pixels, hipStream_t stream) {
+  if (pixels == 0) return hipSuccess;
+  const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((pixels + 255) / 256, 4096));
+  hipLaunchKernelGGL(k_to_bgra8, dim3(blocks), dim3(256), 0, stream, reinterpret_cast<const float4*>(rgba), out,
+                     pixels);
+  return hipGetLastError();
+}
+
 hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev,
-                          uint32_t capacity, int passes, uint32_t* hist, uint64_t* status,
-                          hipStream_t stream) {
+                          uint32_t capacity, int passes, uint32_t* hist, uint64_t* status, uint32_t* counts,
+                          int variant, hipStream_t stream) {
   // keys[0] holds the input; passes alternate keys[0] -> keys[1] -> keys[0] ...; the order ends in
   // vals[(passes - 1) & 1].
   hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, stream, ctrl, hist, 4u * 256u, nullptr, 0u, nullptr,
@@ -933,9 +1108,8 @@ hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], cons
     const uint32_t* kin = keys[p & 1];
     uint32_t* kout = (p + 1 < passes) ? keys[(p + 1) & 1] : nullptr;
     const uint32_t* vin = p == 0 ? nullptr : vals[(p - 1) & 1];
-    hipLaunchKernelGGL(k_onesweep, dim3(grid_for(capacity, kSortTile)), dim3(256), 0, stream, kin, vin,
-                       kout, vals[p & 1], n_dev, hist + p * 256, 8u * p, status, ctrl,
-                       kTicketDepthSort + p, 2u + p);
+    sort_pass(kin, vin, kout, vals[p & 1], n_dev, capacity, hist + p * 256, 8u * p, status, ctrl,
+              kTicketDepthSort + p, 2u + p, counts, variant, stream);
   }
   return hipGetLastError();
 }
@@ -970,9 +1144,8 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes
     const uint32_t* kin = p == 0 ? a.cull_keys : a.keys[(p - 1) & 1];
     uint32_t* kout = (p + 1 < depth_passes) ? a.keys[p & 1] : nullptr;
     const uint32_t* vin = p == 0 ? nullptr : a.order[(p - 1) & 1];
-    hipLaunchKernelGGL(k_onesweep, dim3(grid_for(a.capacity, kSortTile)), dim3(256), 0, s, kin, vin, kout,
-                       a.order[p & 1], visible, a.hist + p * 256, 8u * p, a.sort_status, a.ctrl,
-                       kTicketDepthSort + p, 2u + p);
+    sort_pass(kin, vin, kout, a.order[p & 1], visible, a.capacity, a.hist + p * 256, 8u * p, a.sort_status,
+              a.ctrl, kTicketDepthSort + p, 2u + p, a.sort_counts, a.sort_variant, s);
   }
   const uint32_t* order = a.order[(depth_passes - 1) & 1];
   if (marks) (void)hipEventRecord(marks[3], s);
@@ -983,17 +1156,15 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes
   hipLaunchKernelGGL(k_hist, dim3(min(grid_for(a.pair_capacity, 256), 2048u)), dim3(256), 0, s,
                      a.pair_keys[0], sort_n, tile_hist, tile_passes);
   for (int p = 0; p < tile_passes; ++p) {
-    hipLaunchKernelGGL(k_onesweep, dim3(grid_for(a.pair_capacity, kSortTile)), dim3(256), 0, s,
-                       a.pair_keys[p & 1], a.pair_vals[p & 1], a.pair_keys[(p + 1) & 1],
-                       a.pair_vals[(p + 1) & 1], sort_n, tile_hist + p * 256, 8u * p, a.sort_status,
-                       a.ctrl, kTicketTileSort + p, 6u + p);
+    sort_pass(a.pair_keys[p & 1], a.pair_vals[p & 1], a.pair_keys[(p + 1) & 1], a.pair_vals[(p + 1) & 1],
+              sort_n, a.pair_capacity, tile_hist + p * 256, 8u * p, a.sort_status, a.ctrl, kTicketTileSort + p,
+              6u + p, a.sort_counts, a.sort_variant, s);
   }
   const int fin = tile_passes & 1;
   hipLaunchKernelGGL(k_ranges, dim3(min(grid_for(a.pair_capacity, 256), 2048u)), dim3(256), 0, s,
                      a.pair_keys[fin], sort_n, a.tile_ranges);
   if (marks) (void)hipEventRecord(marks[4], s);
-  hipLaunchKernelGGL(k_raster, dim3(tiles), dim3(256), 0, s, a, a.pair_vals[fin],
-                     a.tile_ranges);
+  if (const hipError_t e = launch_raster(a, a.pair_vals[fin], tiles, s); e != hipSuccess) return e;
   if (marks) (void)hipEventRecord(marks[5], s);
   return hipGetLastError();
 }

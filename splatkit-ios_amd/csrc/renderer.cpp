@@ -32,7 +32,8 @@ using skm::fill_camera;
 using skm::PinnedBuffer;
 using skm::sh_stride;
 
-static_assert(sizeof(skm_view) == 152 && sizeof(skm_batch_tuning) == 16 && sizeof(skm_batch_stats) == 32,
+static_assert(sizeof(skm_view) == 152 && sizeof(skm_batch_tuning) == 24 && sizeof(skm_batch_stats) == 32 &&
+                  sizeof(skm_config) == 24,
               "batch ABI layouts");
 
 namespace {
@@ -204,6 +205,7 @@ struct skm_renderer {
   std::string error;
   std::string description;
   uint32_t tile = 16;
+  uint32_t raster_px = 1;
   uint32_t pair_factor = 8;
 
   float render_scale = 1.0f;
@@ -225,7 +227,8 @@ struct skm_renderer {
   uint32_t capacity = 0;       // survivor slots
   uint32_t pair_capacity = 0;  // tile pairs
   uint32_t tile_capacity = 0;  // tiles
-  DeviceBuffer ctrl, cull_keys, keys[2], order[2], projected, scan_status, sort_status, hist;
+  DeviceBuffer ctrl, cull_keys, keys[2], order[2], projected, scan_status, sort_status, sort_counts, hist;
+  int sort_variant = skm::kSortReduceScan;
   DeviceBuffer pair_keys[2], pair_vals[2], tile_ranges;
   size_t sort_status_blocks = 0;
   uint64_t pipelines = 0;  // prologue launches since the status arrays were last cleared
@@ -248,7 +251,7 @@ struct skm_renderer {
 
   // View batches (skm_draw_views).
   std::unique_ptr<skm::Batch> batch;
-  skm_batch_tuning batch_tuning{16, 8, 1, 0};
+  skm_batch_tuning batch_tuning{16, 8, 1, 0, 1, 0};
   bool batch_dropped = false;
   bool batch_done = false;
 
@@ -332,10 +335,12 @@ struct skm_renderer {
   skm_status ensure_status(uint32_t items) {
     const size_t blocks = (size_t{items} + skm::kSortTile - 1) / skm::kSortTile + 1;
     if (blocks <= sort_status_blocks) return SKM_OK;
-    DeviceBuffer b;
+    DeviceBuffer b, c;
     if (auto st = check(b.allocate(blocks * skm::kRadix * 8u), "sort status"); st != SKM_OK) return st;
+    if (auto st = check(c.allocate(blocks * skm::kRadix * 4u), "sort counts"); st != SKM_OK) return st;
     if (auto st = check(hipMemsetAsync(b.ptr, 0, b.bytes, stream), "clear"); st != SKM_OK) return st;
     sort_status = std::move(b);
+    sort_counts = std::move(c);
     sort_status_blocks = blocks;
     return SKM_OK;
   }
@@ -440,6 +445,15 @@ struct skm_renderer {
 namespace {
 }  // namespace
 
+namespace skm {
+Batch* renderer_batch(skm_renderer* r) {
+  if (!r->batch) r->batch = std::make_unique<Batch>();
+  return r->batch.get();
+}
+hipStream_t renderer_stream(skm_renderer* r) { return r->stream; }
+uint64_t renderer_world_bytes(const skm_renderer* r) { return r->world.mem.bytes; }
+}  // namespace skm
+
 extern "C" {
 
 skm_status skm_renderer_create(const skm_config* config, skm_renderer** out) {
@@ -447,13 +461,17 @@ skm_status skm_renderer_create(const skm_config* config, skm_renderer** out) {
   *out = nullptr;
   int device = config ? config->device : 0;
   const uint32_t tile = config && config->tile_size ? config->tile_size : 16u;
-  if (tile != 16u) return SKM_ERR_UNSUPPORTED;
+  const uint32_t px = config && config->raster_px ? config->raster_px : (tile == 32u ? 2u : 1u);
+  if (!skm::raster_shape_supported(tile, px)) return SKM_ERR_UNSUPPORTED;
+  if (config && config->sort_kernel > 1) return SKM_ERR_INVALID;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return SKM_ERR_DEVICE;
   auto* r = new skm_renderer();
   r->device = device;
   r->tile = tile;
+  r->raster_px = px;
   if (config && config->pair_factor) r->pair_factor = config->pair_factor;
+  if (config) r->sort_variant = static_cast<int>(config->sort_kernel);
   hipDeviceProp_t prop;
   if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
       hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -828,6 +846,9 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   a.capacity = r->capacity;
   a.view_count = 1;
   a.total_tiles = tiles_x * tiles_y;
+  a.sort_variant = r->sort_variant;
+  a.sort_counts = r->sort_counts.as<uint32_t>();
+  a.raster_px = r->raster_px;
   s.depth_passes = r->policy.key16 ? 2 : 4;
   s.tile_passes = static_cast<int>(digit_passes(tiles_x * tiles_y - 1));
   e = skm::launch_frame(a, lod_frame ? &lod : nullptr, s.depth_passes, s.tile_passes, s.marks, r->stream);
@@ -1054,7 +1075,8 @@ skm_status skm_argsort_u32_device(skm_renderer* r, void* keys_dev, uint32_t n, i
   if (e == hipSuccess) e = hipEventRecord(t0, r->stream);
   if (e == hipSuccess)
     e = skm::launch_argsort(r->ctrl.as<Ctrl>(), keys, vals, r->sort_n.as<uint32_t>(), std::max(n, 1u), passes,
-                            r->hist.as<uint32_t>(), r->sort_status.as<uint64_t>(), r->stream);
+                            r->hist.as<uint32_t>(), r->sort_status.as<uint64_t>(),
+                            r->sort_counts.as<uint32_t>(), r->sort_variant, r->stream);
   if (e == hipSuccess) e = hipEventRecord(t1, r->stream);
   if (e == hipSuccess) e = hipEventSynchronize(t1);
   if (auto st = r->check(e, "sort"); st != SKM_OK) return st;
@@ -1128,6 +1150,44 @@ skm_status skm_read_batch_stats(skm_renderer* r, skm_batch_stats* out) {
   out->gpu_ms = r->batch->gpu_ms();
   out->visible = r->batch->visible();
   out->pairs = r->batch->pairs();
+  return SKM_OK;
+}
+
+skm_status skm_autotune_views(skm_renderer* r, const skm_view* views, uint32_t count, const skm_range* ranges,
+                              uint32_t range_count, uint32_t repeats, skm_tuning_result* results,
+                              uint32_t capacity, uint32_t* result_count, skm_batch_tuning* best) {
+  if (r == nullptr || views == nullptr || count == 0 || (capacity > 0 && results == nullptr)) return SKM_ERR_INVALID;
+  static const uint32_t kShapes[][2] = {{16, 1}, {16, 2}, {32, 2}, {8, 1}};
+  repeats = std::max(1u, repeats);
+  skm_batch_tuning chosen{};
+  double chosen_ms = 0.0;
+  uint32_t n = 0;
+  for (const auto& shape : kShapes) {
+    for (uint32_t sort_kernel = 0; sort_kernel < 2; ++sort_kernel) {
+      const skm_batch_tuning t{shape[0], 8, 1, sort_kernel, shape[1], 0};
+      double ms = 0.0;
+      for (uint32_t k = 0; k <= repeats; ++k) {
+        skm_status st = skm_draw_views(r, views, count, ranges, range_count, &t);
+        if (st == SKM_OK) st = skm_finish(r);
+        if (st == SKM_ERR_CAPACITY) {  // the pairs outgrew the scratch: it has grown, run again
+          st = skm_draw_views(r, views, count, ranges, range_count, &t);
+          if (st == SKM_OK) st = skm_finish(r);
+        }
+        if (st != SKM_OK) return st;
+        const double g = r->batch->gpu_ms();
+        if (k == 1 || (k > 1 && g < ms)) ms = g;  // launch 0 is untimed
+      }
+      if (n < capacity) results[n] = skm_tuning_result{t, ms};
+      ++n;
+      if (chosen_ms == 0.0 || ms < chosen_ms) {
+        chosen = t;
+        chosen_ms = ms;
+      }
+    }
+  }
+  r->batch_tuning = chosen;
+  if (result_count) *result_count = n;
+  if (best) *best = chosen;
   return SKM_OK;
 }
 

@@ -49,6 +49,13 @@ class Batch {
   uint64_t visible() const { return ctrl_host()->visible; }
   uint64_t pairs() const { return ctrl_host()->pairs; }
   double gpu_ms() const { return gpu_ms_; }
+  const float* image_device() const { return image_.as<float>(); }
+  uint64_t pixels() const { return info_.empty() ? 0 : info_.back().pixel_base + uint64_t{info_.back().width} * info_.back().height; }
+  // Device bytes held now, and what a batch of this shape would hold (scratch grows, never shrinks).
+  uint64_t device_bytes() const;
+  uint64_t bytes_for(uint32_t views, uint64_t lanes, uint32_t tiles, uint64_t pixels) const;
+  // Frees every device buffer (no batch may be in flight).
+  void release();
   bool graph_used() const { return graph_used_; }
 
   // Read-backs of the last completed batch, in the single-view frame's terms.
@@ -67,7 +74,7 @@ class Batch {
 
   // Device scratch.
   DeviceBuffer ctrl_, cams_, views_, tile_view_, view_start_, survivor_view_;
-  DeviceBuffer cull_keys_, keys_[2], order_[2], projected_, scan_status_, sort_status_, hist_;
+  DeviceBuffer cull_keys_, keys_[2], order_[2], projected_, scan_status_, sort_status_, sort_counts_, hist_;
   DeviceBuffer pair_keys_[2], pair_vals_[2], tile_ranges_, image_, ranges_, starts_;
   PinnedBuffer staging_, host_;
   size_t staging_bytes_ = 0, host_bytes_ = 0;
@@ -89,5 +96,10 @@ class Batch {
   std::vector<uint64_t> graph_key_;
   bool graph_used_ = false;
 };
+
+// The renderer's batch and stream, for the server (renderer.cpp).
+Batch* renderer_batch(skm_renderer* r);
+hipStream_t renderer_stream(skm_renderer* r);
+uint64_t renderer_world_bytes(const skm_renderer* r);
 
 }  // namespace skm

@@ -75,6 +75,10 @@ enum Ticket : uint32_t {
   kTicketTileSort = 6,   // + pass (4)
 };
 
+// How a radix pass finds where each tile's digits go: a reduce-then-scan of per-tile digit counts
+// (three kernels, no inter-block waiting), or onesweep's decoupled look-back (one kernel).
+enum SortVariant : int { kSortReduceScan = 0, kSortOnesweep = 1 };
+
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kCullItems = 4;  // lanes per thread of the cull (a 1024-lane tile per block)
 constexpr uint32_t kBinItems = 4;   // survivors per thread of the binning (1024 per block)
@@ -130,7 +134,19 @@ struct FrameArgs {
   uint32_t* view_start;        // view_count + 1 survivor offsets
   const uint16_t* tile_view;   // the view of each global tile
   uint32_t total_tiles;        // tiles of all views
+  // Radix sort variant and its tile-count scratch (256 words per 4096-key tile of the larger of the
+  // survivor and pair capacities).
+  int sort_variant;
+  uint32_t* sort_counts;
+  // Raster block shape: pixels per lane side (1 or 2) at tile size `tile` (8, 16 or 32); the
+  // supported pairs are (8, 1), (16, 1), (16, 2), (32, 2).
+  uint32_t raster_px;
 };
+
+// Whether the raster has a block shape for (tile, pixels per lane side).
+inline bool raster_shape_supported(uint32_t tile, uint32_t px) {
+  return (tile == 8 && px == 1) || (tile == 16 && (px == 1 || px == 2)) || (tile == 32 && px == 2);
+}
 
 // Launchers (kernels.hip).
 hipError_t launch_pack(const WorldOut& out, uint32_t offset, uint32_t count, const float* positions,
@@ -141,8 +157,10 @@ struct LodArgs;
 // binning/sort/ranges and the raster.
 hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes, int tile_passes,
                         hipEvent_t* marks, hipStream_t stream);
+// BGRA8 bytes (one u32 per pixel) of an RGBA float image.
+hipError_t launch_to_bgra8(const float* rgba, uint32_t* out, uint64_t pixels, hipStream_t stream);
 hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev,
-                          uint32_t capacity, int passes, uint32_t* hist, uint64_t* status,
-                          hipStream_t stream);
+                          uint32_t capacity, int passes, uint32_t* hist, uint64_t* status, uint32_t* counts,
+                          int variant, hipStream_t stream);
 
 }  // namespace skm

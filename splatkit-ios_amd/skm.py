@@ -8,6 +8,7 @@ create_slab/upload_tile, draw(frame), finish, the stat getters and read_frame (c
 """
 import ctypes as C
 import math
+import time
 import os
 
 import numpy as np
@@ -27,7 +28,7 @@ _u32p = C.POINTER(C.c_uint32)
 
 class Config(C.Structure):
     _fields_ = [("device", C.c_int32), ("tile_size", C.c_uint32), ("pair_factor", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("sort_kernel", C.c_uint32), ("raster_px", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class Range(C.Structure):
@@ -93,12 +94,37 @@ class View(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("tile_size", C.c_uint32), ("tile_radix_bits", C.c_uint32), ("use_graph", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("sort_kernel", C.c_uint32), ("raster_px", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class TuningResult(C.Structure):
+    _fields_ = [("tuning", BatchTuning), ("gpu_ms", C.c_double)]
 
 
 class BatchStats(C.Structure):
     _fields_ = [("views", C.c_uint32), ("graph", C.c_uint32), ("gpu_ms", C.c_double), ("visible", C.c_uint64),
                 ("pairs", C.c_uint64)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("max_batch_views", C.c_uint32), ("hbm_budget_bytes", C.c_uint64),
+                ("sort_depth", C.c_uint32), ("max_in_flight", C.c_uint32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("reserved", C.c_uint32), ("user_data", C.c_uint64), ("view", View)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("user_data", C.c_uint64), ("status", C.c_int32),
+                ("batch_views", C.c_uint32), ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("launches", C.c_uint64), ("resident_bytes", C.c_uint64), ("scratch_bytes", C.c_uint64),
+                ("budget_bytes", C.c_uint64), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+                ("mean_batch_views", C.c_double), ("frames_per_s", C.c_double)]
 
 
 class SceneSpec(C.Structure):
@@ -178,6 +204,8 @@ API = {
     "skm_draw_views": (C.c_int, [_R, C.POINTER(View), C.c_uint32, C.POINTER(Range), C.c_uint32,
                                  C.POINTER(BatchTuning)]),
     "skm_read_batch_stats": (C.c_int, [_R, C.POINTER(BatchStats)]),
+    "skm_autotune_views": (C.c_int, [_R, C.POINTER(View), C.c_uint32, C.POINTER(Range), C.c_uint32, C.c_uint32,
+                                     C.POINTER(TuningResult), C.c_uint32, _u32p, C.POINTER(BatchTuning)]),
     "skm_read_view_frame": (C.c_int, [_R, C.c_uint32, C.c_int, C.c_void_p, C.c_size_t]),
     "skm_read_view_visible": (C.c_int, [_R, C.c_uint32, _u32p, _u32p, _u32p, C.c_uint32]),
     "skm_read_view_order": (C.c_int, [_R, C.c_uint32, _u32p, C.c_uint32]),
@@ -186,6 +214,20 @@ API = {
     "skm_argsort_u32": (C.c_int, [_R, _u32p, C.c_uint32, C.c_int, _u32p]),
     "skm_argsort_u32_device": (C.c_int, [_R, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p,
                                          C.POINTER(C.c_double)]),
+}
+
+_S = C.c_void_p
+# include/skm_server.h
+SERVER_API = {
+    "skm_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(_S)]),
+    "skm_server_destroy": (None, [_S]),
+    "skm_server_last_error": (C.c_char_p, [_S]),
+    "skm_server_add_scene": (C.c_int, [_S, C.POINTER(CloudView), C.c_int, _u32p]),
+    "skm_server_submit": (C.c_int, [_S, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "skm_server_poll": (C.c_int, [_S, C.POINTER(Completion), C.c_uint32, _u32p, C.c_double]),
+    "skm_server_image": (C.c_int, [_S, C.c_uint64, C.POINTER(C.POINTER(C.c_uint8)), _u32p, _u32p]),
+    "skm_server_release": (C.c_int, [_S, C.c_uint64]),
+    "skm_server_stats_read": (C.c_int, [_S, C.POINTER(ServerStats), C.c_int]),
 }
 
 # include/skm_lod.h
@@ -208,7 +250,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
         handle = C.CDLL(LIB_PATH)
-        for name, (res, args) in {**API, **LOD_API}.items():
+        for name, (res, args) in {**API, **LOD_API, **SERVER_API}.items():
             fn = getattr(handle, name)
             fn.restype = res
             fn.argtypes = args
@@ -339,10 +381,11 @@ class SkmError(RuntimeError):
 
 
 class Renderer:
-    def __init__(self, device=0, pair_factor=0):
+    def __init__(self, device=0, pair_factor=0, sort_kernel=0, tile=TILE, raster_px=0):
         self._lib = lib()
         self._h = _R()
-        cfg = Config(device, TILE, pair_factor, 0)
+        self.tile = tile
+        cfg = Config(device, tile, pair_factor, sort_kernel, raster_px, 0)
         rc = self._lib.skm_renderer_create(C.byref(cfg), C.byref(self._h))
         if rc != SKM_OK:
             raise SkmError(rc, "renderer creation failed (no HIP device?)")
@@ -501,14 +544,15 @@ class Renderer:
             arr[k].width, arr[k].height = int(w), int(h)
         return arr
 
-    def draw_views(self, views, ranges=None, tile_size=16, radix_bits=8, graph=True):
+    def draw_views(self, views, ranges=None, tile_size=16, radix_bits=8, graph=True, sort_kernel=0, raster_px=0):
         """Enqueues one batch; `views` is a list of (view, proj, eye, width, height) or a View array."""
         arr = views if isinstance(views, C.Array) else self.make_views(views)
         rarr = None
         if ranges is not None:
             rarr = (Range * max(1, len(ranges)))(*[Range(o, c) for o, c in ranges])
-        t = BatchTuning(tile_size, radix_bits, 1 if graph else 0, 0)
+        t = BatchTuning(tile_size, radix_bits, 1 if graph else 0, sort_kernel, raster_px, 0)
         self._views = arr
+        self._view_tile = tile_size
         return self._check(self._lib.skm_draw_views(self._h, arr, len(arr), rarr, 0 if ranges is None else len(ranges),
                                                     C.byref(t)))
 
@@ -523,6 +567,22 @@ class Renderer:
                 raise
             self.draw_views(arr, **kw)
             self.finish()
+
+    def autotune_views(self, views, ranges=None, repeats=2):
+        """Times every batch configuration on these views and keeps the fastest; returns
+        (best, [(tile, raster_px, sort_kernel, gpu_ms), ...])."""
+        arr = views if isinstance(views, C.Array) else self.make_views(views)
+        rarr = None
+        if ranges is not None:
+            rarr = (Range * max(1, len(ranges)))(*[Range(o, c) for o, c in ranges])
+        res = (TuningResult * 16)()
+        n = C.c_uint32()
+        best = BatchTuning()
+        self._check(self._lib.skm_autotune_views(self._h, arr, len(arr), rarr, 0 if ranges is None else len(ranges),
+                                                 repeats, res, 16, C.byref(n), C.byref(best)))
+        table = [(r.tuning.tile_size, r.tuning.raster_px, r.tuning.sort_kernel, r.gpu_ms) for r in res[:n.value]]
+        self._views = arr
+        return {"tile_size": best.tile_size, "raster_px": best.raster_px, "sort_kernel": best.sort_kernel}, table
 
     def batch_stats(self):
         s = BatchStats()
@@ -551,7 +611,8 @@ class Renderer:
 
     def read_view_tiles(self, v):
         w, h = self._views[v].width, self._views[v].height
-        tiles = ((w + TILE - 1) // TILE) * ((h + TILE - 1) // TILE)
+        t = self._view_tile
+        tiles = ((w + t - 1) // t) * ((h + t - 1) // t)
         pairs = C.c_uint64()
         self._check(self._lib.skm_read_view_tiles(self._h, v, C.byref(pairs), None, None, 0, None, 0))
         cap = int(pairs.value)
@@ -602,7 +663,7 @@ class Renderer:
         pairs = C.c_uint64()
         if tiles is None:
             w, h = self.extent()
-            tiles = ((w + TILE - 1) // TILE) * ((h + TILE - 1) // TILE)
+            tiles = ((w + self.tile - 1) // self.tile) * ((h + self.tile - 1) // self.tile)
         if pair_capacity is None:
             self._check(self._lib.skm_read_tiles(self._h, C.byref(pairs), None, None, 0, None, 0))
             pair_capacity = int(pairs.value)
@@ -635,6 +696,84 @@ class Renderer:
             "selected": L.skm_last_selected_count(h), "select_ms": L.skm_last_select_millis(h),
             "gpu_failed": L.skm_gpu_failed(h),
         }
+
+
+class Server:
+    """The request loop of include/skm_server.h: resident scenes, continuous batching under an HBM
+    budget, BGRA8 outputs in host memory."""
+
+    def __init__(self, device=0, max_batch_views=64, budget_bytes=0, sort_depth=16, max_in_flight=0):
+        self._lib = lib()
+        self._h = _S()
+        cfg = ServerConfig(device, max_batch_views, budget_bytes, sort_depth, max_in_flight)
+        rc = self._lib.skm_server_create(C.byref(cfg), C.byref(self._h))
+        if rc != 0:
+            raise SkmError(rc, "server creation failed")
+
+    def close(self):
+        if self._h:
+            self._lib.skm_server_destroy(self._h)
+            self._h = _S()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise SkmError(rc, self._lib.skm_server_last_error(self._h).decode())
+        return rc
+
+    def add_scene(self, cloud, max_sh_degree=3):
+        view, keep = Renderer._view(cloud)
+        sid = C.c_uint32()
+        self._check(self._lib.skm_server_add_scene(self._h, C.byref(view), max_sh_degree, C.byref(sid)))
+        return sid.value
+
+    def submit(self, scene, view, proj, eye, width, height, user_data=0):
+        q = Request()
+        q.scene, q.user_data = scene, user_data
+        q.view.view[:] = [float(x) for x in np.asarray(view, np.float32)]
+        q.view.proj[:] = [float(x) for x in np.asarray(proj, np.float32)]
+        q.view.camera_position[:] = [float(x) for x in np.asarray(eye, np.float32)[:3]]
+        q.view.width, q.view.height = int(width), int(height)
+        t = C.c_uint64()
+        self._check(self._lib.skm_server_submit(self._h, C.byref(q), C.byref(t)))
+        return t.value
+
+    def poll(self, capacity=256, timeout_ms=0.0):
+        out = (Completion * capacity)()
+        n = C.c_uint32()
+        self._check(self._lib.skm_server_poll(self._h, out, capacity, C.byref(n), timeout_ms))
+        return [{"ticket": c.ticket, "user_data": c.user_data, "status": c.status, "batch_views": c.batch_views,
+                 "queue_ms": c.queue_ms, "latency_ms": c.latency_ms} for c in out[:n.value]]
+
+    def image(self, ticket):
+        ptr = C.POINTER(C.c_uint8)()
+        w, h = C.c_uint32(), C.c_uint32()
+        self._check(self._lib.skm_server_image(self._h, ticket, C.byref(ptr), C.byref(w), C.byref(h)))
+        return np.ctypeslib.as_array(ptr, shape=(h.value, w.value, 4)).copy()
+
+    def release(self, ticket):
+        self._check(self._lib.skm_server_release(self._h, ticket))
+
+    def stats(self, reset=False):
+        s = ServerStats()
+        self._check(self._lib.skm_server_stats_read(self._h, C.byref(s), 1 if reset else 0))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def wait_all(self, tickets, timeout_s=120.0):
+        """Polls until every ticket in `tickets` completed; returns their completions by ticket."""
+        want, got = set(tickets), {}
+        t0 = time.time()
+        while want - set(got):
+            for c in self.poll(timeout_ms=50.0):
+                got[c["ticket"]] = c
+            if time.time() - t0 > timeout_s:
+                raise TimeoutError(f"{len(want - set(got))} requests still pending")
+        return got
 
 
 class LodTree:

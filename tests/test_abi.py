@@ -33,6 +33,20 @@ def test_skm_exports_every_declared_function(skm_mod):
     assert set(skm_mod.API) == names
 
 
+def test_server_exports_every_declared_function(skm_mod):
+    names = declared("skm_server.h")
+    assert len(names) >= 9
+    assert names <= exported(skm_mod.LIB_PATH), sorted(names - exported(skm_mod.LIB_PATH))
+    assert set(skm_mod.SERVER_API) == names
+    assert ctypes.sizeof(skm_mod.Request) == 168 and ctypes.sizeof(skm_mod.Completion) == 40
+    assert ctypes.sizeof(skm_mod.ServerConfig) == 24 and ctypes.sizeof(skm_mod.ServerStats) == 96
+
+
+def test_every_header_is_covered():
+    """Each public header's functions are exported by one of the libraries."""
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["skm.h", "skm_lod.h", "skm_scene.h", "skm_server.h"]
+
+
 def test_scene_exports_every_declared_function(skm_mod):
     names = declared("skm_scene.h")
     assert names <= exported(skm_mod.SCENE_PATH)
@@ -45,7 +59,8 @@ def test_struct_layouts_match_the_reference(skm_mod):
     assert C.sizeof(skm_mod.Range) == 8
     assert skm_mod.RenderPolicy.background_color.offset == 24 and C.sizeof(skm_mod.RenderPolicy) == 48
     # The batch extension's structs (static_assert-ed to the same sizes in renderer.cpp).
-    assert C.sizeof(skm_mod.View) == 152 and C.sizeof(skm_mod.BatchTuning) == 16
+    assert C.sizeof(skm_mod.View) == 152 and C.sizeof(skm_mod.BatchTuning) == 24
+    assert C.sizeof(skm_mod.Config) == 24
     assert C.sizeof(skm_mod.BatchStats) == 32
 
 

@@ -1,0 +1,131 @@
+"""The serving loop on one MI355X (include/skm_server.h): resident scenes, continuous batching
+under an HBM budget. A request's BGRA8 pixels are the same alone and in a mix of scenes and
+resolutions, and the same as the renderer's single draw of that view."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def scenes(skm_mod):
+    return [skm_mod.make_cloud(100_000, 0, seed=61, layout=0, spread=2.0),
+            skm_mod.make_cloud(200_000, 1, seed=62, layout=1, spread=3.0, area=6.0),
+            skm_mod.make_cloud(300_000, 3, seed=63, layout=1, spread=4.0, area=8.0)]
+
+
+def requests(skm_mod, count, seed, n_scenes=3):
+    views = skm_mod.views_c2(count, seed=seed, radius=(3.0, 9.0),
+                             resolutions=((320, 180), (640, 360), (1280, 720), (333, 217)))
+    rng = np.random.default_rng(seed)
+    return [(int(rng.integers(n_scenes)),) + v for v in views]
+
+
+def single_bgra(r, skm_mod, cloud, view, proj, eye, w, h):
+    r.upload_world(cloud)
+    r.apply_policy(sort_depth=16)
+    r.set_extent(w, h)
+    r.render(view, proj, eye)
+    return r.read_frame(skm_mod.PIXELS_BGRA8)
+
+
+def test_request_alone_and_in_a_mix_give_the_same_pixels(skm_mod, scenes, renderer):
+    say("case: 3 scenes, one request alone, then inside a mix of 40")
+    srv = skm_mod.Server(max_batch_views=16)
+    try:
+        for c in scenes:
+            srv.add_scene(c)
+        probe = (1,) + skm_mod.views_c2(1, seed=71, radius=(3.0, 6.0), resolutions=((640, 360),))[0]
+        t = srv.submit(*probe)
+        done = srv.wait_all([t])
+        assert done[t]["status"] == 0 and done[t]["batch_views"] == 1
+        alone = srv.image(t)
+        srv.release(t)
+        mix = requests(skm_mod, 40, seed=72)
+        tickets = [srv.submit(*q, user_data=k) for k, q in enumerate(mix[:20])]
+        tp = srv.submit(*probe, user_data=999)
+        tickets += [srv.submit(*q, user_data=20 + k) for k, q in enumerate(mix[20:])]
+        done = srv.wait_all(tickets + [tp])
+        assert all(c["status"] == 0 for c in done.values())
+        say(f"probe batched with {done[tp]['batch_views']} views; stats {srv.stats()}")
+        assert np.array_equal(srv.image(tp), alone)
+        assert max(c["batch_views"] for c in done.values()) > 1  # requests were batched
+        # Every request equals the renderer's single draw of that view (a sample).
+        for k in (0, 7, 19, 33):
+            s, v, p, e, w, h = mix[k]
+            want = single_bgra(renderer, skm_mod, scenes[s], v, p, e, w, h)
+            tk = tickets[k] if k < 20 else tickets[k]
+            assert np.array_equal(srv.image(tk), want), k
+        assert np.array_equal(alone, single_bgra(renderer, skm_mod, scenes[1], *probe[1:]))
+    finally:
+        srv.close()
+
+
+def test_one_request_per_launch_mode(skm_mod, scenes):
+    say("case: batching off, one launch in flight")
+    srv = skm_mod.Server(max_batch_views=1, max_in_flight=1)
+    try:
+        for c in scenes:
+            srv.add_scene(c)
+        qs = requests(skm_mod, 12, seed=73)
+        tickets = [srv.submit(*q) for q in qs]
+        done = srv.wait_all(tickets)
+        st = srv.stats()
+        assert st["launches"] == 12 and all(c["batch_views"] == 1 for c in done.values())
+        assert st["completed"] == 12 and st["p99_ms"] >= st["p50_ms"] > 0
+    finally:
+        srv.close()
+
+
+def test_budget_admission_and_rejection(skm_mod, scenes):
+    say("case: a budget that holds the scenes and small launches only")
+    resident = sum(c["positions"].shape[0] * (32 + 4 * {0: 0, 1: 5, 2: 12, 3: 23}[c["sh_degree"]]) for c in scenes)
+    budget = resident + 200 * 1024 * 1024
+    srv = skm_mod.Server(max_batch_views=64, budget_bytes=budget)
+    try:
+        for c in scenes:
+            srv.add_scene(c)
+        big = skm_mod.views_c2(1, seed=74, resolutions=((7680, 4320),))[0]
+        tb = srv.submit(2, *big)
+        small = requests(skm_mod, 16, seed=75)
+        ts = [srv.submit(*q) for q in small]
+        done = srv.wait_all(ts + [tb])
+        st = srv.stats()
+        say(f"stats {st}")
+        assert done[tb]["status"] == -3  # SKM_ERR_MEMORY: cannot fit even alone
+        assert all(done[t]["status"] == 0 for t in ts)
+        assert st["resident_bytes"] + st["scratch_bytes"] <= budget
+        assert st["rejected"] == 1
+    finally:
+        srv.close()
+
+
+def test_poisson_stream_reports_latency(skm_mod, scenes):
+    say("case: a short Poisson stream at 200 requests/s")
+    srv = skm_mod.Server(max_batch_views=32)
+    try:
+        for c in scenes:
+            srv.add_scene(c)
+        qs = requests(skm_mod, 60, seed=76)
+        rng = np.random.default_rng(77)
+        t0 = time.perf_counter()
+        arrival = np.cumsum(rng.exponential(1 / 200.0, len(qs)))
+        tickets = []
+        for q, at in zip(qs, arrival):
+            while time.perf_counter() - t0 < at:
+                pass
+            tickets.append(srv.submit(*q))
+        srv.wait_all(tickets)
+        st = srv.stats()
+        say(f"stats {st}")
+        assert st["completed"] == 60 and 0 < st["p50_ms"] <= st["p99_ms"]
+    finally:
+        srv.close()
