@@ -31,6 +31,19 @@ SH_BYTES = {0: 0, 1: 20, 2: 48, 3: 92}
 RASTER_FLOPS_PER_EVAL = 14
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` at this workload, from the committed counter runs (fetched
+    and written bytes, each scaled by a 1 GiB copy's reading), or None when they are absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_3M_sh3_1080p.json")
+    if not os.path.exists(path):
+        return None
+    data = json.load(open(path))
+    for name, v in data["kernels"].items():
+        if name.startswith(kernel):
+            return v["fetch_bytes"] + v["write_bytes"]
+    return None
+
+
 def stage_bytes(N, V, P, T, depth_passes, tile_passes):
     """Algorithmic HBM bytes per frame of each stage, in the reference's packed-record units."""
     k = 4
@@ -129,6 +142,58 @@ def serving(skm, rates, per_rate):
     return out
 
 
+def fly_camera(frame, frames, width=1920, height=1080):
+    """A straight pass over the 200 m square at 4 m height, weaving 30 m either side, looking ahead
+    and slightly down."""
+    import numpy as np
+    import skm
+
+    t = frame / max(1, frames - 1)
+    eye = np.array([-90.0 + 180.0 * t, 4.0, 30.0 * math.sin(2 * math.pi * t)], np.float32)
+    ahead = eye + np.array([10.0, -1.0, 30.0 * 2 * math.pi / 180.0 * math.cos(2 * math.pi * t)], np.float32)
+    return skm.look_at(eye, ahead), skm.perspective(math.pi / 3, width / height, 0.2, 400.0), eye
+
+
+def lod_flythrough(skm, frames, leaves):
+    """The LOD world: `leaves` splats at SH1 drawn like the 3M scene over a 200 x 200 m square, an
+    octree with at most 256 leaves per cluster, a cut per frame at 1.5 px error (color weight 4,
+    culling on), 1920x1080, over a 120-frame fly-through, with 16-bit and 32-bit depth keys."""
+    t0 = time.time()
+    cloud = skm.make_cloud(leaves, 1, seed=0x5EED0003, layout=1, spread=100.0, area=200.0)
+    gen_s = time.time() - t0
+    t0 = time.time()
+    tree = skm.LodTree.build(cloud, 21, 256, threads=min(16, os.cpu_count() or 1))
+    build_s = time.time() - t0
+    del cloud
+    r = skm.Renderer(0)
+    r.upload_lod_world(tree, 1)
+    r.set_extent(WIDTH, HEIGHT)
+    cams = [fly_camera(f, frames) for f in range(frames)]
+    out = {"workload": f"{leaves} leaves SH1 over 200x200 m, octree (<=256 leaves per cluster), cut at 1.5 px, "
+                       f"1920x1080, {frames}-frame fly-through", "leaves": leaves, "nodes": tree.node_count,
+           "tree_build_s": round(build_s, 2), "scene_generation_s": round(gen_s, 2), "keys": {}}
+    names = ["lod_cut", "cull", "depth_sort", "bin_tile_sort_ranges", "raster"]
+    for bits in (16, 32):
+        r.apply_policy(sort_depth=bits, lod_error=1.5)
+        for f in range(5):
+            r.render(*cams[f])
+        r.frame_totals(reset=True)
+        t0 = time.perf_counter()
+        for f in range(frames):
+            r.draw(r.make_frame(*cams[f]))
+        r.finish()
+        elapsed = time.perf_counter() - t0
+        tot = r.frame_totals(reset=True)
+        n = max(1, tot["frames"])
+        out["keys"][str(bits)] = {"frames_per_s": round(frames / elapsed, 1),
+                                  "stage_ms": {k: round(tot["stage_ms"][i] / n, 4) for i, k in enumerate(names)},
+                                  "gpu_ms_per_frame": round(tot["stage_ms"][5] / n, 4),
+                                  "visible_per_frame": int(tot["visible"] / n), "pairs_per_frame": int(tot["pairs"] / n)}
+    r.close()
+    tree.close()
+    return out
+
+
 def cpu_baseline(frames, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -160,6 +225,11 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=300,
                     help="requests per offered rate in the serving leg (0 skips it)")
     ap.add_argument("--serving-rates", default="150,300,600,1200,2400")
+    ap.add_argument("--lod-leaves", type=int, default=20_000_000, help="LOD leg's leaves (0 skips it)")
+    ap.add_argument("--lod-frames", type=int, default=120)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first copy this many MiB device to device (a known-size kernel for the HBM "
+                         "byte counters of a profiler)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the render path on a single MI355X "
@@ -167,6 +237,16 @@ def main():
         sys.exit(2)
 
     import skm
+
+    if args.copy_calibration_mib > 0:
+        import torch
+
+        src = torch.ones(args.copy_calibration_mib * (1 << 18), dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     t0 = time.time()
     cloud = skm.scene_c1(N_SPLATS, SH_DEGREE)
@@ -207,10 +287,13 @@ def main():
     stages["raster"]["pair_evals"] = int(evals)
     stages["raster"]["coverage_TFLOP_s"] = round(raster_tflops, 2)
     dominant = max(names, key=lambda n: ms[n])
+    traffic = measured_traffic("k_raster")
     if dominant == "raster":
         roofline = {"bound": "valu", "kernel": "k_raster", "achieved": round(raster_tflops, 3),
                     "peak": VALU_PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(raster_tflops * 1e12 / VALU_PEAK_FP32, 4),
-                    "traffic": None, "note": "fp32 VALU work of the coverage test over 256 x pairs evaluations"}
+                    "traffic": traffic, "note": "fp32 VALU work of the coverage test over 256 x pairs evaluations; "
+                                                "traffic = HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE counter runs "
+                                                "in profiles/hbm_traffic_3M_sh3_1080p.md"}
     else:
         gbs = stages[dominant]["GB_s"]
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -218,6 +301,7 @@ def main():
 
     batched = batched_vs_loop(r, skm, args.batches, 2) if args.batches > 0 else None
     r.close()
+    lod = lod_flythrough(skm, args.lod_frames, args.lod_leaves) if args.lod_leaves > 0 else None
     served = None
     if args.serving_requests > 0:
         served = serving(skm, [float(x) for x in args.serving_rates.split(",")], args.serving_requests)
@@ -242,7 +326,7 @@ def main():
         "msplats_per_s": round(N_SPLATS * fps / 1e6, 1),
         "visible_per_frame": int(V), "pairs_per_frame": int(P),
         "gpu_ms_per_frame": round(tot["stage_ms"][5] / n_frames, 4),
-        "stages": stages, "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "serving": served,
+        "stages": stages, "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "serving": served, "lod": lod,
         "device": device, "scene_generation_s": round(gen_s, 2),
     }
     print(json.dumps(out), flush=True)

@@ -49,7 +49,7 @@ hipError_t Batch::ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_
   if (views > view_capacity_) {
     need(cams_, size_t{views} * sizeof(Camera));
     need(views_, size_t{views} * sizeof(ViewInfo));
-    need(view_start_, (size_t{views} + 1) * 4);
+    need(view_visible_, size_t{views} * 4);
     if (e == hipSuccess) view_capacity_ = views;
   }
   if (lanes > lane_capacity_) {
@@ -243,7 +243,7 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
   a.view_count = count;
   a.views = views_.as<ViewInfo>();
   a.survivor_view = survivor_view_.as<uint16_t>();
-  a.view_start = view_start_.as<uint32_t>();
+  a.view_visible = view_visible_.as<uint32_t>();
   a.tile_view = tile_view_.as<uint16_t>();
   a.total_tiles = static_cast<uint32_t>(tiles);
   a.sort_variant = tuning.sort_kernel == 1 ? kSortOnesweep : kSortReduceScan;
@@ -296,13 +296,14 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
   if (e == hipSuccess) e = hipEventRecord(end_, stream);
   if (e == hipSuccess) e = hipMemcpyAsync(host_.ptr, ctrl_.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(host_.as<char>() + sizeof(Ctrl), view_start_.ptr, (size_t{count} + 1) * 4,
+    e = hipMemcpyAsync(host_.as<char>() + sizeof(Ctrl), view_visible_.ptr, size_t{count} * 4,
                        hipMemcpyDeviceToHost, stream);
   if (e != hipSuccess) {
     *error = "batch: launch failed";
     return e;
   }
   in_flight_ = true;
+  sv_ready_ = false;
   return hipSuccess;
 }
 
@@ -324,7 +325,7 @@ hipError_t Batch::wait(bool* dropped) {
 
 uint64_t Batch::device_bytes() const {
   uint64_t b = 0;
-  for (const DeviceBuffer* d : {&ctrl_, &cams_, &views_, &tile_view_, &view_start_, &survivor_view_, &cull_keys_,
+  for (const DeviceBuffer* d : {&ctrl_, &cams_, &views_, &tile_view_, &view_visible_, &survivor_view_, &cull_keys_,
                                 &keys_[0], &keys_[1], &order_[0], &order_[1], &projected_, &scan_status_,
                                 &sort_status_, &sort_counts_, &hist_, &pair_keys_[0], &pair_keys_[1], &pair_vals_[0],
                                 &pair_vals_[1], &tile_ranges_, &image_, &ranges_, &starts_})
@@ -345,7 +346,7 @@ uint64_t Batch::bytes_for(uint32_t views, uint64_t lanes, uint32_t tiles, uint64
 
 void Batch::release() {
   release_graph();
-  for (DeviceBuffer* d : {&ctrl_, &cams_, &views_, &tile_view_, &view_start_, &survivor_view_, &cull_keys_,
+  for (DeviceBuffer* d : {&ctrl_, &cams_, &views_, &tile_view_, &view_visible_, &survivor_view_, &cull_keys_,
                           &keys_[0], &keys_[1], &order_[0], &order_[1], &projected_, &scan_status_, &sort_status_,
                           &sort_counts_, &hist_, &pair_keys_[0], &pair_keys_[1], &pair_vals_[0], &pair_vals_[1], &tile_ranges_,
                           &image_, &ranges_, &starts_})
@@ -362,17 +363,56 @@ hipError_t Batch::read_image(uint32_t v, float* rgba) const {
                    hipMemcpyDeviceToHost);
 }
 
+hipError_t Batch::survivor_views() const {
+  if (sv_ready_) return hipSuccess;
+  const uint32_t total = ctrl_host()->visible;
+  sv_view_.resize(total);
+  sv_local_.resize(total);
+  if (total) {
+    if (hipError_t e = hipMemcpy(sv_view_.data(), survivor_view_.ptr, size_t{total} * 2, hipMemcpyDeviceToHost);
+        e != hipSuccess)
+      return e;
+  }
+  std::vector<uint32_t> next(info_.size(), 0);
+  for (uint32_t i = 0; i < total; ++i) sv_local_[i] = next[sv_view_[i]]++;
+  sv_ready_ = true;
+  return hipSuccess;
+}
+
 hipError_t Batch::read_visible(uint32_t v, uint32_t* keys, uint32_t* projected, uint32_t capacity) const {
-  const uint32_t first = view_first(v), n = std::min(view_visible(v), capacity);
-  hipError_t e = hipSuccess;
-  if (keys && n) e = hipMemcpy(keys, cull_keys_.as<uint32_t>() + first, size_t{n} * 4, hipMemcpyDeviceToHost);
-  if (e == hipSuccess && projected && n)
-    e = hipMemcpy(projected, projected_.as<uint32_t>() + size_t{first} * 8, size_t{n} * 32, hipMemcpyDeviceToHost);
-  return e;
+  if (hipError_t e = survivor_views(); e != hipSuccess) return e;
+  const uint32_t total = ctrl_host()->visible;
+  constexpr uint32_t kChunk = 1u << 22;
+  std::vector<uint32_t> kc, pc;
+  uint32_t k = 0;
+  for (uint32_t c0 = 0; c0 < total && k < capacity; c0 += kChunk) {
+    const uint32_t n = std::min(kChunk, total - c0);
+    bool any = false;
+    for (uint32_t i = c0; i < c0 + n && !any; ++i) any = sv_view_[i] == v;
+    if (!any) continue;
+    hipError_t e = hipSuccess;
+    if (keys) {
+      kc.resize(n);
+      e = hipMemcpy(kc.data(), cull_keys_.as<uint32_t>() + c0, size_t{n} * 4, hipMemcpyDeviceToHost);
+    }
+    if (e == hipSuccess && projected) {
+      pc.resize(size_t{n} * 8);
+      e = hipMemcpy(pc.data(), projected_.as<uint32_t>() + size_t{c0} * 8, size_t{n} * 32, hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) return e;
+    for (uint32_t i = 0; i < n && k < capacity; ++i) {
+      if (sv_view_[c0 + i] != v) continue;
+      if (keys) keys[k] = kc[i];
+      if (projected) std::memcpy(projected + size_t{k} * 8, pc.data() + size_t{i} * 8, 32);
+      ++k;
+    }
+  }
+  return hipSuccess;
 }
 
 hipError_t Batch::read_order(uint32_t v, uint32_t* order, uint32_t capacity) const {
-  const uint32_t total = ctrl_host()->visible, first = view_first(v), n = view_visible(v);
+  if (hipError_t e = survivor_views(); e != hipSuccess) return e;
+  const uint32_t total = ctrl_host()->visible, n = view_visible(v);
   std::vector<uint32_t> all(total);
   const uint32_t* src = order_[(depth_passes_ - 1) & 1].as<uint32_t>();
   if (total) {
@@ -381,7 +421,7 @@ hipError_t Batch::read_order(uint32_t v, uint32_t* order, uint32_t capacity) con
   }
   uint32_t k = 0;
   for (uint32_t i = 0; i < total && k < std::min(n, capacity); ++i)
-    if (all[i] >= first && all[i] - first < n) order[k++] = all[i] - first;
+    if (sv_view_[all[i]] == v) order[k++] = sv_local_[all[i]];
   return hipSuccess;
 }
 
@@ -415,9 +455,9 @@ hipError_t Batch::read_tiles(uint32_t v, uint64_t* pairs, uint32_t* pair_tiles, 
     for (uint64_t i = 0; e == hipSuccess && i < n; ++i) pair_tiles[i] -= vi.tile_base;
   }
   if (e == hipSuccess && pair_values && n) {
+    if (e = survivor_views(); e != hipSuccess) return e;
     e = hipMemcpy(pair_values, pair_vals_[fin].as<uint32_t>() + p0, n * 4, hipMemcpyDeviceToHost);
-    const uint32_t first = view_first(v);
-    for (uint64_t i = 0; e == hipSuccess && i < n; ++i) pair_values[i] -= first;
+    for (uint64_t i = 0; e == hipSuccess && i < n; ++i) pair_values[i] = sv_local_[pair_values[i]];
   }
   return e;
 }

@@ -401,8 +401,8 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
   if (tid == 0) s_ticket = atomicAdd(&a.ctrl->ticket[kTicketCull], 1u);
   __syncthreads();
   const uint32_t b = s_ticket;
-  const uint32_t view = a.view_count > 1u ? b / view_tiles : 0u;
-  const uint32_t lt = b - view * view_tiles;  // the tile within its view
+  const uint32_t view = a.view_count > 1u ? b % a.view_count : 0u;
+  const uint32_t lt = a.view_count > 1u ? b / a.view_count : b;  // the lane tile
   const Camera& cam = a.cam[view];
   uint32_t key[kItems], rec[kItems][8];
   uint64_t ballot[kItems];
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
         status_store(a.scan_status + b, status_word(tag, kFlagPrefix, sat_add(prefix, aggregate)));
     }
     if (lane_id() == 0) s_prefix = prefix;
-    if (lane_id() == 0 && lt == 0u && a.view_start != nullptr) a.view_start[view] = prefix;
+    if (lane_id() == 0 && aggregate != 0u && a.view_visible != nullptr) atomicAdd(&a.view_visible[view], aggregate);
   }
   __syncthreads();
   uint32_t before = s_prefix;
@@ -462,10 +462,7 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
 #pragma unroll
     for (uint32_t w = 0; w < 4; ++w) before += s_seg[k * 4 + w];
   }
-  if (b == blocks - 1u && tid == 0) {
-    a.ctrl->visible = s_prefix + aggregate;
-    if (a.view_start != nullptr) a.view_start[a.view_count] = s_prefix + aggregate;
-  }
+  if (b == blocks - 1u && tid == 0) a.ctrl->visible = s_prefix + aggregate;
 }
 
 // Global digit histograms of every pass in one read of the keys (onesweep's upfront histogram).
@@ -1124,8 +1121,8 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes
   }
   if (marks) (void)hipEventRecord(marks[1], s);
   hipLaunchKernelGGL(k_prologue, dim3(min(grid_for(tiles * 2u, 256), 256u)), dim3(256), 0, s, a.ctrl,
-                     a.hist, 2u * 4u * 256u, a.tile_ranges, tiles * 2u, a.view_start,
-                     a.view_start != nullptr ? a.view_count + 1u : 0u);
+                     a.hist, 2u * 4u * 256u, a.tile_ranges, tiles * 2u, a.view_visible,
+                     a.view_visible != nullptr ? a.view_count : 0u);
   const uint32_t view_lanes = grid_for(a.total, 256u * kCullItems) * 256u * kCullItems;
   const uint32_t lanes = a.active != nullptr ? a.capacity : view_lanes * a.view_count;
   const uint32_t cull_blocks = grid_for(lanes, 256u * kCullItems);

@@ -1,10 +1,10 @@
 // Batches of views over one resident world: every stage of the frame pipeline runs once for all
-// views (cull lanes view-major, one depth sort over every view's survivors, tile pairs keyed by
-// batch-global tile ids, one raster grid over every view's tiles), captured into a hipGraph and
-// replayed while the batch's shape stays the same. Per view, the survivors, their depth order, the
-// (tile | depth) pairs and the pixels are the single-view frame's, bit for bit: survivor order is
-// lane order within a view, the sorts are stable, and global tile ids keep each view's tiles
-// contiguous and in order.
+// views (cull tiles tile-major over the views, one depth sort over every view's survivors, tile
+// pairs keyed by batch-global tile ids, one raster grid over every view's tiles), captured into a
+// hipGraph and replayed while the batch's shape stays the same. Per view, the survivors, their depth
+// order, the (tile | depth) pairs and the pixels are the single-view frame's, bit for bit: survivor
+// order is lane order within a view, the sorts are stable, and global tile ids keep each view's
+// tiles contiguous and in order.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -44,8 +44,7 @@ class Batch {
 
   uint32_t view_count() const { return static_cast<uint32_t>(info_.size()); }
   const ViewInfo& info(uint32_t v) const { return info_[v]; }
-  uint32_t view_visible(uint32_t v) const { return starts_host()[v + 1] - starts_host()[v]; }
-  uint32_t view_first(uint32_t v) const { return starts_host()[v]; }
+  uint32_t view_visible(uint32_t v) const { return counts_host()[v]; }
   uint64_t visible() const { return ctrl_host()->visible; }
   uint64_t pairs() const { return ctrl_host()->pairs; }
   double gpu_ms() const { return gpu_ms_; }
@@ -67,13 +66,18 @@ class Batch {
 
  private:
   const Ctrl* ctrl_host() const { return host_.as<Ctrl>(); }
-  const uint32_t* starts_host() const { return reinterpret_cast<const uint32_t*>(host_.as<char>() + sizeof(Ctrl)); }
+  const uint32_t* counts_host() const { return reinterpret_cast<const uint32_t*>(host_.as<char>() + sizeof(Ctrl)); }
+  // The view of every survivor and its index among its view's survivors (read back on demand).
+  hipError_t survivor_views() const;
+  mutable std::vector<uint16_t> sv_view_;
+  mutable std::vector<uint32_t> sv_local_;
+  mutable bool sv_ready_ = false;
   hipError_t ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_t tiles, uint64_t pixels,
                     uint32_t ranges);
   void release_graph();
 
   // Device scratch.
-  DeviceBuffer ctrl_, cams_, views_, tile_view_, view_start_, survivor_view_;
+  DeviceBuffer ctrl_, cams_, views_, tile_view_, view_visible_, survivor_view_;
   DeviceBuffer cull_keys_, keys_[2], order_[2], projected_, scan_status_, sort_status_, sort_counts_, hist_;
   DeviceBuffer pair_keys_[2], pair_vals_[2], tile_ranges_, image_, ranges_, starts_;
   PinnedBuffer staging_, host_;

@@ -126,12 +126,14 @@ struct FrameArgs {
   float* image;           // RGBA float, top-down rows
   uint32_t capacity;      // survivor capacity
   // Batches (view_count > 1): one camera per view at cam[v]; every view culls the same lanes
-  // [0, total); cull tiles never straddle views, so survivors come out view-major. Null for a
-  // single view.
+  // [0, total). Cull tiles go tile-major (the views of one 1024-lane tile take consecutive
+  // tickets, so its records come from HBM once and from cache for the other views); survivors are
+  // compacted in (tile, view, lane) order, which is lane order within each view. Null for a single
+  // view.
   uint32_t view_count;
   const ViewInfo* views;
   uint16_t* survivor_view;     // the view of each survivor
-  uint32_t* view_start;        // view_count + 1 survivor offsets
+  uint32_t* view_visible;      // survivors per view
   const uint16_t* tile_view;   // the view of each global tile
   uint32_t total_tiles;        // tiles of all views
   // Radix sort variant and its tile-count scratch (256 words per 4096-key tile of the larger of the
