@@ -39,7 +39,7 @@ def measured_traffic(kernel):
         return None
     data = json.load(open(path))
     for name, v in data["kernels"].items():
-        if name.startswith(kernel):
+        if name == kernel:
             return v["fetch_bytes"] + v["write_bytes"]
     return None
 
@@ -60,11 +60,12 @@ def batched_vs_loop(r, skm, batches, loops):
     of one graph launch per batch against the plain path drawing the same views one per launch."""
     views = skm.views_c2(64)
     arr = r.make_views(views)
+    best, table = r.autotune_views(arr, repeats=2)
     for _ in range(2):
-        r.render_views(arr)
+        r.render_views(arr, **best)
     t0 = time.perf_counter()
     for _ in range(batches):
-        r.draw_views(arr)
+        r.draw_views(arr, **best)
     r.finish()
     batch_s = (time.perf_counter() - t0) / batches
     st = r.batch_stats()
@@ -86,7 +87,8 @@ def batched_vs_loop(r, skm, batches, loops):
             "batched_gpu_ms_per_batch": round(st["gpu_ms"], 3), "graph": st["graph"],
             "per_view_loop_frames_per_s": round(64 / loop_s, 1), "per_view_loop_ms": round(loop_s * 1000, 3),
             "speedup": round(loop_s / batch_s, 3), "visible_per_batch": st["visible"], "pairs_per_batch": st["pairs"],
-            "batches": batches, "loops": loops}
+            "batches": batches, "loops": loops, "autotuned": best,
+            "autotune_ms": [{"tile": t, "raster_px": p, "sort_kernel": k, "gpu_ms": round(ms, 3)} for t, p, k, ms in table]}
 
 
 SERVING_SIZES = (500_000, 1_000_000, 2_000_000, 3_000_000, 5_000_000, 6_000_000, 8_000_000, 10_000_000)
@@ -257,6 +259,10 @@ def main():
     r.set_extent(WIDTH, HEIGHT)
     r.apply_policy(sort_depth=32)
     frames = [skm.orbit_camera(f, WIDTH, HEIGHT) for f in range(60)]
+    # The autotuner picks the tile size, raster block shape and radix pass variant on four views of
+    # the orbit; single draws use its choice.
+    plain_best, plain_table = r.autotune_views([frames[f] + (WIDTH, HEIGHT) for f in (0, 15, 30, 45)], repeats=2)
+    r.set_tuning(**plain_best)
     for i in range(args.warmup):
         r.draw(r.make_frame(*frames[i % 60]))
     r.finish()
@@ -273,25 +279,28 @@ def main():
     n_frames = max(1, tot["frames"])
     V = tot["visible"] / n_frames
     P = tot["pairs"] / n_frames
-    T = ((WIDTH + 15) // 16) * ((HEIGHT + 15) // 16)
+    T = ((WIDTH + plain_best["tile_size"] - 1) // plain_best["tile_size"]) * \
+        ((HEIGHT + plain_best["tile_size"] - 1) // plain_best["tile_size"])
     names = ["cull", "depth_sort", "bin_tile_sort_ranges", "raster"]
     ms = {n: tot["stage_ms"][k + 1] / n_frames for k, n in enumerate(names)}
-    by = stage_bytes(N_SPLATS, V, P, T, 4, 2)
+    by = stage_bytes(N_SPLATS, V, P, T, 4, 2 if T <= 65536 else 3)
     stages = {}
     for n in names:
         gbs = by[n] / (ms[n] * 1e-3) / 1e9 if ms[n] > 0 else 0.0
         stages[n] = {"ms": round(ms[n], 4), "bytes": int(by[n]), "GB_s": round(gbs, 1),
                      "hbm_frac": round(gbs * 1e9 / HBM_PEAK, 4)}
-    evals = 256.0 * P
+    tile = plain_best["tile_size"]
+    evals = float(tile * tile) * P
     raster_tflops = evals * RASTER_FLOPS_PER_EVAL / (ms["raster"] * 1e-3) / 1e12 if ms["raster"] > 0 else 0.0
     stages["raster"]["pair_evals"] = int(evals)
     stages["raster"]["coverage_TFLOP_s"] = round(raster_tflops, 2)
     dominant = max(names, key=lambda n: ms[n])
-    traffic = measured_traffic("k_raster")
+    traffic = measured_traffic(f"k_raster<{tile}, {plain_best['raster_px']}>")
     if dominant == "raster":
-        roofline = {"bound": "valu", "kernel": "k_raster", "achieved": round(raster_tflops, 3),
+        roofline = {"bound": "valu", "kernel": f"k_raster<{tile}, {plain_best['raster_px']}>",
+                    "achieved": round(raster_tflops, 3),
                     "peak": VALU_PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(raster_tflops * 1e12 / VALU_PEAK_FP32, 4),
-                    "traffic": traffic, "note": "fp32 VALU work of the coverage test over 256 x pairs evaluations; "
+                    "traffic": traffic, "note": "fp32 VALU work of the coverage test over tile^2 x pairs evaluations; "
                                                 "traffic = HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE counter runs "
                                                 "in profiles/hbm_traffic_3M_sh3_1080p.md"}
     else:
@@ -322,7 +331,11 @@ def main():
         "data": "synthetic (seeded 3M-Gaussian SH3 scene: 70% near a ground square and boxes, 30% iid N(0,5^2))",
         "config": {"workload": "plain single-view render, 3M Gaussians SH3, 1920x1080, 60-frame orbit, 32-bit keys",
                    "splats": N_SPLATS, "sh_degree": SH_DEGREE, "width": WIDTH, "height": HEIGHT,
-                   "tile": 16, "parallelism": "single GPU"},
+                   "tile": plain_best["tile_size"], "raster_px": plain_best["raster_px"],
+                   "sort_kernel": ["reduce_then_scan", "onesweep_lookback"][plain_best["sort_kernel"]],
+                   "parallelism": "single GPU"},
+        "autotune_ms": [{"tile": t, "raster_px": p, "sort_kernel": k, "gpu_ms_4_views": round(ms, 3)}
+                        for t, p, k, ms in plain_table],
         "msplats_per_s": round(N_SPLATS * fps / 1e6, 1),
         "visible_per_frame": int(V), "pairs_per_frame": int(P),
         "gpu_ms_per_frame": round(tot["stage_ms"][5] / n_frames, 4),

@@ -293,6 +293,10 @@ skm_status skm_read_batch_stats(skm_renderer* r, skm_batch_stats* out);
  * the fastest the renderer's batch tuning (used when skm_draw_views gets tuning NULL). results gets
  * up to `capacity` candidates with their best GPU milliseconds; *best the choice. Every candidate
  * gives the same pixels: the composite of a pixel does not depend on tiles or block shape. */
+/* Applies a tuning (tile size, raster pixels per lane, radix pass variant) to this renderer's
+ * single-view draws as well; the next frame uses it. */
+skm_status skm_set_tuning(skm_renderer* r, const skm_batch_tuning* tuning);
+
 typedef struct skm_tuning_result {
   skm_batch_tuning tuning;
   double gpu_ms;

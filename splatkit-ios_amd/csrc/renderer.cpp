@@ -86,7 +86,7 @@ struct Slot {
   bool with_world = false;
   bool with_lod = false;
   uint32_t lod_limit = 0;
-  uint32_t width = 0, height = 0;
+  uint32_t width = 0, height = 0, tile = 16;
   int depth_passes = 4, tile_passes = 2;
 };
 
@@ -849,6 +849,7 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   a.sort_variant = r->sort_variant;
   a.sort_counts = r->sort_counts.as<uint32_t>();
   a.raster_px = r->raster_px;
+  s.tile = r->tile;
   s.depth_passes = r->policy.key16 ? 2 : 4;
   s.tile_passes = static_cast<int>(digit_passes(tiles_x * tiles_y - 1));
   e = skm::launch_frame(a, lod_frame ? &lod : nullptr, s.depth_passes, s.tile_passes, s.marks, r->stream);
@@ -1038,7 +1039,7 @@ skm_status skm_read_tiles(skm_renderer* r, uint64_t* pairs, uint32_t* pair_tiles
     if (auto st = r->check(hipMemcpy(pair_values, r->pair_vals[fin].ptr, n * 4, hipMemcpyDeviceToHost), "read");
         st != SKM_OK)
       return st;
-  const uint32_t tiles = ((s.width + r->tile - 1) / r->tile) * ((s.height + r->tile - 1) / r->tile);
+  const uint32_t tiles = ((s.width + s.tile - 1) / s.tile) * ((s.height + s.tile - 1) / s.tile);
   if (ranges && tile_capacity)
     return r->check(hipMemcpy(ranges, r->tile_ranges.ptr, size_t{std::min(tiles, tile_capacity)} * 8,
                               hipMemcpyDeviceToHost),
@@ -1188,6 +1189,22 @@ skm_status skm_autotune_views(skm_renderer* r, const skm_view* views, uint32_t c
   r->batch_tuning = chosen;
   if (result_count) *result_count = n;
   if (best) *best = chosen;
+  return SKM_OK;
+}
+
+skm_status skm_set_tuning(skm_renderer* r, const skm_batch_tuning* t) {
+  if (r == nullptr || t == nullptr) return SKM_ERR_INVALID;
+  const uint32_t tile = t->tile_size ? t->tile_size : 16u;
+  const uint32_t px = t->raster_px ? t->raster_px : (tile == 32u ? 2u : 1u);
+  if (!skm::raster_shape_supported(tile, px) || t->sort_kernel > 1 || (t->tile_radix_bits != 0 && t->tile_radix_bits != 8))
+    return r->fail(SKM_ERR_INVALID, "unsupported tuning");
+  if (auto st = r->drain(); st != SKM_OK) return st;
+  r->tile = tile;
+  r->raster_px = px;
+  r->sort_variant = static_cast<int>(t->sort_kernel);
+  r->batch_tuning = *t;
+  r->batch_tuning.tile_size = tile;
+  r->batch_tuning.raster_px = px;
   return SKM_OK;
 }
 

@@ -204,6 +204,7 @@ API = {
     "skm_draw_views": (C.c_int, [_R, C.POINTER(View), C.c_uint32, C.POINTER(Range), C.c_uint32,
                                  C.POINTER(BatchTuning)]),
     "skm_read_batch_stats": (C.c_int, [_R, C.POINTER(BatchStats)]),
+    "skm_set_tuning": (C.c_int, [_R, C.POINTER(BatchTuning)]),
     "skm_autotune_views": (C.c_int, [_R, C.POINTER(View), C.c_uint32, C.POINTER(Range), C.c_uint32, C.c_uint32,
                                      C.POINTER(TuningResult), C.c_uint32, _u32p, C.POINTER(BatchTuning)]),
     "skm_read_view_frame": (C.c_int, [_R, C.c_uint32, C.c_int, C.c_void_p, C.c_size_t]),
@@ -582,7 +583,15 @@ class Renderer:
                                                  repeats, res, 16, C.byref(n), C.byref(best)))
         table = [(r.tuning.tile_size, r.tuning.raster_px, r.tuning.sort_kernel, r.gpu_ms) for r in res[:n.value]]
         self._views = arr
-        return {"tile_size": best.tile_size, "raster_px": best.raster_px, "sort_kernel": best.sort_kernel}, table
+        best = {"tile_size": best.tile_size, "raster_px": best.raster_px, "sort_kernel": best.sort_kernel}
+        self._view_tile = best["tile_size"]
+        return best, table
+
+    def set_tuning(self, tile_size=16, raster_px=0, sort_kernel=0):
+        """Tile size, raster pixels per lane and radix pass variant for single draws and batches."""
+        t = BatchTuning(tile_size, 8, 1, sort_kernel, raster_px, 0)
+        self._check(self._lib.skm_set_tuning(self._h, C.byref(t)))
+        self.tile = tile_size
 
     def batch_stats(self):
         s = BatchStats()
