@@ -100,6 +100,31 @@ double exp_d(double x) {
   return std::ldexp(p, static_cast<int>(k));
 }
 
+// Natural logarithm in fp32 with a fixed operation order (the kernels' log_f): x = m 2^e,
+// m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m-1)/(m+1), by the odd series of atanh to
+// s^11 (Abramowitz & Stegun 4.5.64), e ln 2 with a split constant. For x normal and positive.
+constexpr float kLn2HiF = 0.693145751953125f;
+constexpr float kLn2LoF = 1.428606765330187e-06f;
+constexpr float kInvLn2F = 1.44269504088896341f;
+float log_f(float x) {
+  int e = 0;
+  float m = std::frexp(x, &e);
+  if (m < 0.707106781f) {
+    m = m * 2.0f;
+    e = e - 1;
+  }
+  const float s = (m - 1.0f) / (m + 1.0f);
+  const float z = s * s;
+  float p = 0.0909090936f;
+  p = 0.111111112f + z * p;
+  p = 0.142857149f + z * p;
+  p = 0.200000003f + z * p;
+  p = 0.333333343f + z * p;
+  p = 1.0f + z * p;
+  const float fe = static_cast<float>(e);
+  return fe * kLn2HiF + (fe * kLn2LoF + (2.0f * s) * p);
+}
+
 float pow_f(float x, float y) {
   if (!(x > 0.0f)) return 0.0f;
   return static_cast<float>(exp_d(static_cast<double>(y) * log_d(static_cast<double>(x))));
@@ -341,7 +366,7 @@ bool project(const orc_camera& cam, const uint32_t* s, const uint32_t* shw, int 
 
   // Quad radius in sigma (:104-108).
   const float ratio = mx(alpha / kQuadMinAlpha, 1.0f);
-  const float lg = static_cast<float>(log_d(static_cast<double>(ratio)));
+  const float lg = log_f(ratio);
   float radius = mn(lod ? kLodBoundsRadius : kBoundsRadius, std::sqrt(2.0f * lg));
   const float sigma = std::sqrt(mx(a1x * a1x + a1y * a1y, a2x * a2x + a2y * a2y));
   radius = mn(radius, kMaxPixelRadius / mx(sigma, 1e-6f));
@@ -417,7 +442,7 @@ struct Prep {
 
 float log2_peak(const uint32_t* p) {
   const float a0 = hi_half(p[5]);
-  const float l2 = static_cast<float>(log_d(static_cast<double>(mx(a0, kHalfMin))) / kLn2);
+  const float l2 = log_f(mx(a0, kHalfMin)) * kInvLn2F;
   return mn(mx(l2, -kMaxLog2Alpha), kMaxLog2Alpha);
 }
 
@@ -667,6 +692,7 @@ uint32_t orc_lod_select(const orc_camera* cam, const uint32_t* clusters16, uint3
 }
 
 double orc_log_d(double x) { return log_d(x); }
+float orc_log_f(float x) { return log_f(x); }
 double orc_exp_d(double x) { return exp_d(x); }
 uint16_t orc_to_half(float v) { return to_half_sat(v); }
 uint16_t orc_half_rne(float v) { return half_rne(v); }
@@ -929,6 +955,57 @@ uint64_t orc_missed_coverage(const uint32_t* projected, uint32_t count, uint32_t
         const uint32_t* b = pair_values + ranges[2 * t];
         const uint32_t* e = pair_values + ranges[2 * t + 1];
         if (std::find(b, e, i) == e) ++missed;
+      }
+  }
+  if (accepted != nullptr) *accepted = taken;
+  return missed;
+}
+
+// The raster's region test (the kernel's per-region record lists), restated to check that it never
+// drops a record the coverage rule would take in the region: the footprint box against the region's
+// pixel centres [x0, x0 + span] x [y0, y0 + span], then the quad's slabs |gu|, |gv| <= K r widened by
+// gu's variation over the region.
+int orc_region_keep(const uint32_t* p, uint32_t width, uint32_t height, uint32_t x0, uint32_t y0,
+                    uint32_t span) {
+  const Prep q = prep(p, width, height);
+  float ex = 0.0f, ey = 0.0f;
+  if (!footprint(p, &ex, &ey)) return 0;
+  const float bx0 = (q.cx - ex) - kBinPad, bx1 = (q.cx + ex) + kBinPad;
+  const float by0 = (q.cy - ey) - kBinPad, by1 = (q.cy + ey) + kBinPad;
+  const float qx = static_cast<float>(x0) + 0.5f, qy = static_cast<float>(y0) + 0.5f;
+  const float fs = static_cast<float>(span);
+  const float half = 0.5f * fs + kBinPad;
+  const float hu = (std::fabs(q.g00) + std::fabs(q.g01)) * half, hv = (std::fabs(q.g10) + std::fabs(q.g11)) * half;
+  const float lim_u = (q.rk + hu) * kFootprintSlack, lim_v = (q.rk + hv) * kFootprintSlack;
+  const float dxc = (qx + 0.5f * fs) - q.cx, dyc = q.cy - (qy + 0.5f * fs);
+  const float uc = std::fmaf(q.g00, dxc, q.g01 * dyc), vc = std::fmaf(q.g10, dxc, q.g11 * dyc);
+  return bx1 >= qx && bx0 <= qx + fs && by1 >= qy && by0 <= qy + fs && std::fabs(uc) <= lim_u &&
+         std::fabs(vc) <= lim_v;
+}
+
+// Brute force over every pixel of every record's quad box: counts accepted (pixel, record) pairs
+// whose region (the aligned `region` x `region` square holding the pixel) the region test drops.
+uint64_t orc_region_misses(const uint32_t* projected, uint32_t count, uint32_t width, uint32_t height,
+                           uint32_t region, uint64_t* accepted) {
+  uint64_t missed = 0, taken = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const uint32_t* p = projected + size_t{i} * 8u;
+    const Prep q = prep(p, width, height);
+    const double r = f32(p[6]);
+    const double ex = r * (std::fabs(q.a1x) + std::fabs(q.a2x)) + 2.0;
+    const double ey = r * (std::fabs(q.a1y) + std::fabs(q.a2y)) + 2.0;
+    if (!(ex == ex) || !(ey == ey)) continue;
+    const int64_t x0 = std::max<int64_t>(0, static_cast<int64_t>(std::floor(q.cx - ex)));
+    const int64_t x1 = std::min<int64_t>(width - 1, static_cast<int64_t>(std::ceil(q.cx + ex)));
+    const int64_t y0 = std::max<int64_t>(0, static_cast<int64_t>(std::floor(q.cy - ey)));
+    const int64_t y1 = std::min<int64_t>(height - 1, static_cast<int64_t>(std::ceil(q.cy + ey)));
+    for (int64_t py = y0; py <= y1; ++py)
+      for (int64_t px = x0; px <= x1; ++px) {
+        float power;
+        if (!decide(q, static_cast<uint32_t>(px), static_cast<uint32_t>(py), &power)) continue;
+        ++taken;
+        const uint32_t rx = static_cast<uint32_t>(px) / region * region, ry = static_cast<uint32_t>(py) / region * region;
+        if (!orc_region_keep(p, width, height, rx, ry, region - 1)) ++missed;
       }
   }
   if (accepted != nullptr) *accepted = taken;

@@ -84,6 +84,14 @@ uint64_t orc_missed_coverage(const uint32_t* projected, uint32_t count, uint32_t
                              uint32_t height, uint32_t tile, const uint32_t* pair_values,
                              const uint32_t* ranges, uint64_t* accepted);
 
+/* The raster's per-region record test over pixel centres [x0, x0+span] x [y0, y0+span], and a brute
+ * force count of accepted (pixel, record) pairs whose `region`-aligned square it drops. Test
+ * infrastructure: checks that the kernel's region lists are conservative. */
+int orc_region_keep(const uint32_t* projected8, uint32_t width, uint32_t height, uint32_t x0, uint32_t y0,
+                    uint32_t span);
+uint64_t orc_region_misses(const uint32_t* projected, uint32_t count, uint32_t width, uint32_t height,
+                           uint32_t region, uint64_t* accepted);
+
 /* FP64 composite over the tile lists (no early termination). out: RGBA float, top-down rows. */
 void orc_raster64(const uint32_t* projected, const uint32_t* pair_values, const uint32_t* ranges,
                   uint32_t width, uint32_t height, uint32_t tile, const float* background3,
@@ -119,6 +127,7 @@ int orc_lod_cluster(const orc_camera* cam, const uint32_t* cluster16, float colo
 /* Transcendentals built from IEEE basic operations only (argument reduction plus series), so
  * any two IEEE machines compute identical bits. */
 double orc_log_d(double x);
+float orc_log_f(float x);
 double orc_exp_d(double x);
 
 #ifdef __cplusplus

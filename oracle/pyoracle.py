@@ -56,7 +56,11 @@ _API = {
     "orc_lod_cluster": (C.c_int, [C.POINTER(Camera), _u32p, C.c_float, _fp]),
     "orc_missed_coverage": (C.c_uint64, [_u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _u32p, _u32p,
                                          C.POINTER(C.c_uint64)]),
+    "orc_region_keep": (C.c_int, [_u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "orc_region_misses": (C.c_uint64, [_u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                       C.POINTER(C.c_uint64)]),
     "orc_log_d": (C.c_double, [C.c_double]),
+    "orc_log_f": (C.c_float, [C.c_float]),
     "orc_exp_d": (C.c_double, [C.c_double]),
 }
 
@@ -218,6 +222,13 @@ def missed_coverage(projected, count, pair_values, ranges, width, height, tile=T
         pv = np.zeros(1, np.uint32)
     missed = lib().orc_missed_coverage(up(np.ascontiguousarray(projected, np.uint32)), count, width, height, tile,
                                        up(pv), up(np.ascontiguousarray(ranges, np.uint32)), C.byref(acc))
+    return int(missed), int(acc.value)
+
+
+def region_misses(projected, count, width, height, region):
+    acc = C.c_uint64()
+    missed = lib().orc_region_misses(up(np.ascontiguousarray(projected, np.uint32)), count, width, height, region,
+                                     C.byref(acc))
     return int(missed), int(acc.value)
 
 

@@ -60,6 +60,7 @@ hipError_t Batch::ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_
     need(order_[0], n * 4);
     need(order_[1], n * 4);
     need(projected_, n * 32);
+    need(rects_, n * 8);
     need(survivor_view_, n * 2);
     if (e == hipSuccess) lane_capacity_ = lanes;
   }
@@ -233,6 +234,7 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
     a.pair_vals[k] = pair_vals_[k].as<uint32_t>();
   }
   a.projected = projected_.as<uint32_t>();
+  a.rects = rects_.as<uint64_t>();
   a.scan_status = scan_status_.as<uint64_t>();
   a.sort_status = sort_status_.as<uint64_t>();
   a.hist = hist_.as<uint32_t>();
@@ -326,7 +328,7 @@ hipError_t Batch::wait(bool* dropped) {
 uint64_t Batch::device_bytes() const {
   uint64_t b = 0;
   for (const DeviceBuffer* d : {&ctrl_, &cams_, &views_, &tile_view_, &view_visible_, &survivor_view_, &cull_keys_,
-                                &keys_[0], &keys_[1], &order_[0], &order_[1], &projected_, &scan_status_,
+                                &keys_[0], &keys_[1], &order_[0], &order_[1], &projected_, &rects_, &scan_status_,
                                 &sort_status_, &sort_counts_, &hist_, &pair_keys_[0], &pair_keys_[1], &pair_vals_[0],
                                 &pair_vals_[1], &tile_ranges_, &image_, &ranges_, &starts_})
     b += d->bytes;
@@ -340,14 +342,14 @@ uint64_t Batch::bytes_for(uint32_t views, uint64_t lanes, uint32_t tiles, uint64
   pixels = std::max(pixels, pixel_capacity_);
   views = std::max(views, view_capacity_);
   const uint64_t status = (lanes / 1024 + 2) * 8 + (std::max(lanes, pairs) / kSortTile + 2) * kRadix * 12;
-  return lanes * (5 * 4 + 32 + 2) + pairs * 16 + uint64_t{tiles} * 10 + pixels * 16 +
+  return lanes * (5 * 4 + 32 + 8 + 2) + pairs * 16 + uint64_t{tiles} * 10 + pixels * 16 +
          uint64_t{views} * (sizeof(Camera) + sizeof(ViewInfo) + 4) + status + (1u << 20);
 }
 
 void Batch::release() {
   release_graph();
   for (DeviceBuffer* d : {&ctrl_, &cams_, &views_, &tile_view_, &view_visible_, &survivor_view_, &cull_keys_,
-                          &keys_[0], &keys_[1], &order_[0], &order_[1], &projected_, &scan_status_, &sort_status_,
+                          &keys_[0], &keys_[1], &order_[0], &order_[1], &projected_, &rects_, &scan_status_, &sort_status_,
                           &sort_counts_, &hist_, &pair_keys_[0], &pair_keys_[1], &pair_vals_[0], &pair_vals_[1], &tile_ranges_,
                           &image_, &ranges_, &starts_})
     d->reset();

@@ -185,7 +185,7 @@ __device__ __forceinline__ void reveal_glow(const Camera& cam, float height, uin
 
 template <int DEG, bool KEY16>
 __device__ __forceinline__ bool cull_lane(const Camera& cam, const World& w, uint32_t index,
-                                          uint32_t& key, uint32_t rec[8]) {
+                                          uint32_t& key, uint32_t rec[8], const float* byte255) {
   const float px = w.px[index], py = w.py[index], pz = w.pz[index];
   const uint32_t rgba = w.rgba8[index];
   const uint32_t lodw = w.lod_alpha[index];
@@ -202,7 +202,7 @@ __device__ __forceinline__ bool cull_lane(const Camera& cam, const World& w, uin
   }
   if (!(cam.reveal[0] - py >= 0.0f)) return false;
   const bool lod = lodw != 0u;
-  const float ca = static_cast<float>(rgba >> 24) / 255.0f;
+  const float ca = byte255[rgba >> 24];
   const float alpha = lod ? f32(lodw) : ca;
   if (alpha < kQuadMinAlpha) return false;
   if (v[2] >= 0.0f) return false;
@@ -267,7 +267,7 @@ __device__ __forceinline__ bool cull_lane(const Camera& cam, const World& w, uin
   const float a2x = e2x * s2, a2y = e2y * s2;
 
   const float ratio = mx(alpha / kQuadMinAlpha, 1.0f);
-  const float lg = static_cast<float>(log_d(static_cast<double>(ratio)));
+  const float lg = log_f(ratio);
   float radius = mn(lod ? kLodBoundsRadius : kBoundsRadius, sqrtf(2.0f * lg));
   const float sigma = sqrtf(mx(a1x * a1x + a1y * a1y, a2x * a2x + a2y * a2y));
   radius = mn(radius, kMaxPixelRadius / mx(sigma, 1e-6f));
@@ -282,9 +282,7 @@ __device__ __forceinline__ bool cull_lane(const Camera& cam, const World& w, uin
   const float sv = (0.5f * (a + d) + sqrtf(mx(hd * hd + b * b, 0.0f))) - kLowPassVariance;
   if (!(sqrtf(mx(sv, 0.0f)) * radius >= cam.min_pixel_radius)) return false;
 
-  float rgb[3] = {static_cast<float>(rgba & 0xffu) / 255.0f,
-                  static_cast<float>((rgba >> 8) & 0xffu) / 255.0f,
-                  static_cast<float>((rgba >> 16) & 0xffu) / 255.0f};
+  float rgb[3] = {byte255[rgba & 0xffu], byte255[(rgba >> 8) & 0xffu], byte255[(rgba >> 16) & 0xffu]};
   if constexpr (DEG >= 1) {
     constexpr uint32_t kStride = ((DEG + 1) * (DEG + 1) - 1) * 3 / 2 + (((DEG + 1) * (DEG + 1) - 1) * 3) % 2;
     uint32_t words[kStride];
@@ -383,6 +381,75 @@ __global__ __launch_bounds__(256) void k_prologue(Ctrl* ctrl, uint32_t* hist, ui
   for (uint32_t i = gid; i < extra_words; i += stride) extra[i] = 0;
 }
 
+// log2 of a record's half peak opacity, clamped like splatDepth (SplatProjection.metalh:169-171).
+__device__ __forceinline__ float log2_peak(uint32_t w5) {
+  const float a0 = hi_half(w5);
+  const float l2 = log_f(mx(a0, kHalfMin)) * kInvLn2F;
+  return mn(mx(l2, -kMaxLog2Alpha), kMaxLog2Alpha);
+}
+
+// Footprint half-extents in pixels (the oracle's footprint, same specification): the smaller of the
+// quad's box r(|a1| + |a2|) and the cutoff disc's box sqrt(L / K^2) sqrt(a1^2 + a2^2) per axis, the
+// latter with slack. False when no pixel can pass the cutoff.
+__device__ __forceinline__ bool footprint(const uint32_t* p, float log2a0, float& ex, float& ey) {
+  const float cut = log2a0 - kLog2MinAlpha;
+  if (!(cut >= 0.0f)) return false;
+  const float rho = sqrtf(cut * kInvGaussianScale2) * kFootprintSlack;
+  const float a1x = lo_half(p[2]), a1y = hi_half(p[2]);
+  const float a2x = lo_half(p[3]), a2y = hi_half(p[3]);
+  const float r = f32(p[6]);
+  const float qx = r * (fabsf(a1x) + fabsf(a2x));
+  const float qy = r * (fabsf(a1y) + fabsf(a2y));
+  const float sx = rho * sqrtf(a1x * a1x + a2x * a2x);
+  const float sy = rho * sqrtf(a1y * a1y + a2y * a2y);
+  ex = mn(qx, sx);
+  ey = mn(qy, sy);
+  return true;
+}
+
+// Tile rectangle of a projected record (the CPU oracle's orc_tile_rect, same specification).
+struct Rect {
+  int32_t tx0, ty0, tx1, ty1;
+};
+__device__ __forceinline__ uint32_t tile_rect(const uint32_t* p, uint32_t width, uint32_t height,
+                                              uint32_t tile, Rect& r) {
+  const float cx = (f32(p[0]) * 0.5f + 0.5f) * static_cast<float>(width);
+  const float cy = (0.5f - f32(p[1]) * 0.5f) * static_cast<float>(height);
+  r.tx0 = r.ty0 = 0;
+  r.tx1 = r.ty1 = -1;
+  float ex = 0.0f, ey = 0.0f;
+  if (!footprint(p, log2_peak(p[5]), ex, ey)) return 0;
+  float x0 = ((cx - ex) - 0.5f) - kBinPad, x1 = ((cx + ex) - 0.5f) + kBinPad;
+  float y0 = ((cy - ey) - 0.5f) - kBinPad, y1 = ((cy + ey) - 0.5f) + kBinPad;
+  if (!(x0 <= x1) || !(y0 <= y1)) return 0;
+  x0 = mx(x0, -1.0f);
+  x1 = mn(x1, static_cast<float>(width));
+  y0 = mx(y0, -1.0f);
+  y1 = mn(y1, static_cast<float>(height));
+  int32_t px0 = static_cast<int32_t>(ceilf(x0)), px1 = static_cast<int32_t>(floorf(x1));
+  int32_t py0 = static_cast<int32_t>(ceilf(y0)), py1 = static_cast<int32_t>(floorf(y1));
+  px0 = max(px0, 0);
+  py0 = max(py0, 0);
+  px1 = min(px1, static_cast<int32_t>(width) - 1);
+  py1 = min(py1, static_cast<int32_t>(height) - 1);
+  if (px0 > px1 || py0 > py1) return 0;
+  r.tx0 = px0 / static_cast<int32_t>(tile);
+  r.ty0 = py0 / static_cast<int32_t>(tile);
+  r.tx1 = px1 / static_cast<int32_t>(tile);
+  r.ty1 = py1 / static_cast<int32_t>(tile);
+  return static_cast<uint32_t>((r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1));
+}
+
+// A survivor's tile rectangle packed for the binning: tx0 | ty0 << 12 | tx1 << 24 | ty1 << 36 |
+// view << 48 (12 bits per tile coordinate: up to 4096 tiles per axis), all ones when it covers none.
+constexpr uint64_t kNoRect = ~0ull;
+__device__ __forceinline__ uint64_t pack_rect(const Rect& r, uint32_t count, uint32_t view) {
+  if (count == 0u) return kNoRect;
+  return static_cast<uint64_t>(r.tx0) | (static_cast<uint64_t>(r.ty0) << 12) |
+         (static_cast<uint64_t>(r.tx1) << 24) | (static_cast<uint64_t>(r.ty1) << 36) |
+         (static_cast<uint64_t>(view) << 48);
+}
+
 // 256 threads, kCullItems lanes each: lane t = tile * 1024 + k * 256 + thread. The tile's
 // survivors are ranked in lane order (k, wave, lane), so the compaction keeps lane order. Blocks
 // past the tiles this frame needs exit before they take a ticket.
@@ -393,11 +460,13 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
   __shared__ uint32_t s_ticket;
   __shared__ uint32_t s_seg[kItems * 4];
   __shared__ uint32_t s_prefix;
+  __shared__ float s_byte255[256];  // b / 255 for every byte: the divisions of packRgba8's inverse
   const uint32_t tid = threadIdx.x, wave = tid >> 6;
   const uint32_t total = INDEXED ? min(*a.active_count, a.capacity) : a.total;
   const uint32_t view_tiles = (total + kTile - 1u) / kTile;
   const uint32_t blocks = view_tiles * a.view_count;
   if (blockIdx.x >= blocks) return;
+  s_byte255[tid] = static_cast<float>(tid) / 255.0f;
   if (tid == 0) s_ticket = atomicAdd(&a.ctrl->ticket[kTicketCull], 1u);
   __syncthreads();
   const uint32_t b = s_ticket;
@@ -421,7 +490,7 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
         const uint32_t r = find_range(a.range_starts, a.range_count, t);
         index = a.ranges[2 * r] + (t - a.range_starts[r]);
       }
-      visible = cull_lane<DEG, KEY16>(cam, a.world, index, key[k], rec[k]);
+      visible = cull_lane<DEG, KEY16>(cam, a.world, index, key[k], rec[k], s_byte255);
     }
     ballot[k] = __ballot(visible);
     if (lane_id() == 0) s_seg[k * 4 + wave] = static_cast<uint32_t>(__popcll(ballot[k]));
@@ -458,6 +527,12 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
       dst[0] = make_uint4(rec[k][0], rec[k][1], rec[k][2], rec[k][3]);
       dst[1] = make_uint4(rec[k][4], rec[k][5], rec[k][6], rec[k][7]);
       if (a.survivor_view != nullptr) a.survivor_view[off] = static_cast<uint16_t>(view);
+      // The binning's tile rectangle, from the record just written (read back in depth order, a
+      // gather of 8 bytes instead of the 32-byte record).
+      Rect r;
+      const uint32_t tiles = tile_rect(rec[k], static_cast<uint32_t>(cam.screen_size[0]),
+                                       static_cast<uint32_t>(cam.screen_size[1]), a.tile, r);
+      a.rects[off] = pack_rect(r, tiles, view);
     }
 #pragma unroll
     for (uint32_t w = 0; w < 4; ++w) before += s_seg[k * 4 + w];
@@ -678,65 +753,6 @@ __global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ count
   }
 }
 
-// log2 of a record's half peak opacity, clamped like splatDepth (SplatProjection.metalh:169-171).
-__device__ __forceinline__ float log2_peak(uint32_t w5) {
-  const float a0 = hi_half(w5);
-  const float l2 = static_cast<float>(log_d(static_cast<double>(mx(a0, kHalfMin))) / kLn2);
-  return mn(mx(l2, -kMaxLog2Alpha), kMaxLog2Alpha);
-}
-
-// Footprint half-extents in pixels (the oracle's footprint, same specification): the smaller of the
-// quad's box r(|a1| + |a2|) and the cutoff disc's box sqrt(L / K^2) sqrt(a1^2 + a2^2) per axis, the
-// latter with slack. False when no pixel can pass the cutoff.
-__device__ __forceinline__ bool footprint(const uint32_t* p, float log2a0, float& ex, float& ey) {
-  const float cut = log2a0 - kLog2MinAlpha;
-  if (!(cut >= 0.0f)) return false;
-  const float rho = sqrtf(cut * kInvGaussianScale2) * kFootprintSlack;
-  const float a1x = lo_half(p[2]), a1y = hi_half(p[2]);
-  const float a2x = lo_half(p[3]), a2y = hi_half(p[3]);
-  const float r = f32(p[6]);
-  const float qx = r * (fabsf(a1x) + fabsf(a2x));
-  const float qy = r * (fabsf(a1y) + fabsf(a2y));
-  const float sx = rho * sqrtf(a1x * a1x + a2x * a2x);
-  const float sy = rho * sqrtf(a1y * a1y + a2y * a2y);
-  ex = mn(qx, sx);
-  ey = mn(qy, sy);
-  return true;
-}
-
-// Tile rectangle of a projected record (the CPU oracle's orc_tile_rect, same specification).
-struct Rect {
-  int32_t tx0, ty0, tx1, ty1;
-};
-__device__ __forceinline__ uint32_t tile_rect(const uint32_t* p, uint32_t width, uint32_t height,
-                                              uint32_t tile, Rect& r) {
-  const float cx = (f32(p[0]) * 0.5f + 0.5f) * static_cast<float>(width);
-  const float cy = (0.5f - f32(p[1]) * 0.5f) * static_cast<float>(height);
-  r.tx0 = r.ty0 = 0;
-  r.tx1 = r.ty1 = -1;
-  float ex = 0.0f, ey = 0.0f;
-  if (!footprint(p, log2_peak(p[5]), ex, ey)) return 0;
-  float x0 = ((cx - ex) - 0.5f) - kBinPad, x1 = ((cx + ex) - 0.5f) + kBinPad;
-  float y0 = ((cy - ey) - 0.5f) - kBinPad, y1 = ((cy + ey) - 0.5f) + kBinPad;
-  if (!(x0 <= x1) || !(y0 <= y1)) return 0;
-  x0 = mx(x0, -1.0f);
-  x1 = mn(x1, static_cast<float>(width));
-  y0 = mx(y0, -1.0f);
-  y1 = mn(y1, static_cast<float>(height));
-  int32_t px0 = static_cast<int32_t>(ceilf(x0)), px1 = static_cast<int32_t>(floorf(x1));
-  int32_t py0 = static_cast<int32_t>(ceilf(y0)), py1 = static_cast<int32_t>(floorf(y1));
-  px0 = max(px0, 0);
-  py0 = max(py0, 0);
-  px1 = min(px1, static_cast<int32_t>(width) - 1);
-  py1 = min(py1, static_cast<int32_t>(height) - 1);
-  if (px0 > px1 || py0 > py1) return 0;
-  r.tx0 = px0 / static_cast<int32_t>(tile);
-  r.ty0 = py0 / static_cast<int32_t>(tile);
-  r.tx1 = px1 / static_cast<int32_t>(tile);
-  r.ty1 = py1 / static_cast<int32_t>(tile);
-  return static_cast<uint32_t>((r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1));
-}
-
 // Tile pairs of the depth-ordered survivors: counts, a block scan plus look-back for the global
 // offset, then the block emits its pairs cooperatively (pair e of the block belongs to the last
 // survivor whose exclusive offset is <= e), so writes are contiguous whatever the footprints.
@@ -763,16 +779,18 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
     Rect r{0, 0, -1, -1};
     if (i < n) {
       value = order[i];
-      const uint4* src = reinterpret_cast<const uint4*>(a.projected + static_cast<size_t>(value) * 8u);
-      const uint4 w0 = src[0], w1 = src[1];
-      const uint32_t p[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
-      if (a.survivor_view != nullptr) {
-        const ViewInfo& vi = a.views[a.survivor_view[value]];
-        base = vi.tile_base;
-        tiles_x = vi.tiles_x;
-        count = tile_rect(p, vi.width, vi.height, a.tile, r);
-      } else {
-        count = tile_rect(p, a.width, a.height, a.tile, r);
+      const uint64_t rc = a.rects[value];
+      if (rc != kNoRect) {
+        r.tx0 = static_cast<int32_t>(rc & 4095u);
+        r.ty0 = static_cast<int32_t>((rc >> 12) & 4095u);
+        r.tx1 = static_cast<int32_t>((rc >> 24) & 4095u);
+        r.ty1 = static_cast<int32_t>((rc >> 36) & 4095u);
+        count = static_cast<uint32_t>((r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1));
+        if (a.views != nullptr) {
+          const ViewInfo& vi = a.views[rc >> 48];
+          base = vi.tile_base;
+          tiles_x = vi.tiles_x;
+        }
       }
     }
     uint32_t row_total;
@@ -819,12 +837,9 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
     const uint32_t total = sat_add(s_prefix, aggregate);
     a.ctrl->pairs = total;
     a.ctrl->overflow = total > a.pair_capacity ? 1u : 0u;
+    // The tile sort's count: zero after an overflow (the frame is dropped).
+    a.ctrl->sort_pairs = total > a.pair_capacity ? 0u : total;
   }
-}
-
-// Sort count for the tile pairs: zero when they overflowed the capacity (the frame is dropped).
-__global__ void k_pair_count(Ctrl* ctrl) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) ctrl->sort_pairs = ctrl->overflow ? 0u : ctrl->pairs;
 }
 
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
@@ -936,20 +951,32 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
       const float det = a1x * a2y - a2x * a1y;
       const float sc = kGaussianScale / det;
       const float log2a0 = log2_peak(w1.y);
+      const float g00 = a2y * sc, g01 = -(a2x * sc), g10 = -(a1y * sc), g11 = a1x * sc;
+      const float rk = kGaussianScale * f32(w1.z);
       float ex = 0.0f, ey = 0.0f;
       if (footprint(p, log2a0, ex, ey)) {
         const float x0 = (cx - ex) - kBinPad, x1 = (cx + ex) + kBinPad;
         const float y0 = (cy - ey) - kBinPad, y1 = (cy + ey) + kBinPad;
+        // The quad's two slabs |gu|, |gv| <= K r against the region: gu varies by at most
+        // (|g00| + |g01|) h around the region's centre, h its half span plus the binning pad; the
+        // slack covers the fp32 rounding of the per-pixel rule.
+        constexpr float kHalf = 0.5f * static_cast<float>(S::kRegion - 1u) + kBinPad;
+        const float hu = (fabsf(g00) + fabsf(g01)) * kHalf, hv = (fabsf(g10) + fabsf(g11)) * kHalf;
+        const float lim_u = (rk + hu) * kFootprintSlack, lim_v = (rk + hv) * kFootprintSlack;
 #pragma unroll
         for (uint32_t q = 0; q < kW; ++q) {
           const float qx = static_cast<float>(ox + (q % S::kSide) * S::kRegion) + 0.5f;
           const float qy = static_cast<float>(oy + (q / S::kSide) * S::kRegion) + 0.5f;
           constexpr float kSpan = static_cast<float>(S::kRegion - 1u);
-          if (x1 >= qx && x0 <= qx + kSpan && y1 >= qy && y0 <= qy + kSpan) regions |= 1u << q;
+          const float dxc = (qx + 0.5f * kSpan) - cx, dyc = cy - (qy + 0.5f * kSpan);
+          const float uc = fmaf(g00, dxc, g01 * dyc), vc = fmaf(g10, dxc, g11 * dyc);
+          if (x1 >= qx && x0 <= qx + kSpan && y1 >= qy && y0 <= qy + kSpan && fabsf(uc) <= lim_u &&
+              fabsf(vc) <= lim_v)
+            regions |= 1u << q;
         }
       }
-      s_a[tid] = make_float4(cx, cy, a2y * sc, -(a2x * sc));
-      s_b[tid] = make_float4(-(a1y * sc), a1x * sc, kGaussianScale * f32(w1.z), log2a0);
+      s_a[tid] = make_float4(cx, cy, g00, g01);
+      s_b[tid] = make_float4(g10, g11, rk, log2a0);
       s_c[tid] = make_float4(mn(mx(lo_half(w1.x), 0.0f), kMaxColorScale),
                              mn(mx(hi_half(w1.x), 0.0f), kMaxColorScale),
                              mn(mx(lo_half(w1.y), 0.0f), kMaxColorScale), 0.0f);
@@ -1056,6 +1083,10 @@ uint32_t grid_for(uint32_t items, uint32_t per_block) {
   return b == 0u ? 1u : b;
 }
 
+// Blocks of the histogram kernel: enough to stream the keys, few enough that the per-block global
+// adds into each of the histogram's words stay short (each word takes one add per block).
+uint32_t hist_grid(uint32_t capacity) { return std::min(std::max(grid_for(capacity, 16384u), 16u), 256u); }
+
 // One LSD pass of 8 bits at `shift` in the chosen variant; `counts` holds 256 words per 4096-key
 // tile of `capacity` (reduce then scan only).
 void sort_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, const uint32_t* n_ptr,
@@ -1099,7 +1130,7 @@ hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], cons
   // vals[(passes - 1) & 1].
   hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, stream, ctrl, hist, 4u * 256u, nullptr, 0u, nullptr,
                      0u);
-  hipLaunchKernelGGL(k_hist, dim3(min(grid_for(capacity, 256), 1024u)), dim3(256), 0, stream, keys[0],
+  hipLaunchKernelGGL(k_hist, dim3(hist_grid(capacity)), dim3(256), 0, stream, keys[0],
                      n_dev, hist, passes);
   for (int p = 0; p < passes; ++p) {
     const uint32_t* kin = keys[p & 1];
@@ -1135,7 +1166,7 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes
   if (marks) (void)hipEventRecord(marks[2], s);
   // Depth sort of the survivors: cull keys -> keys[0] -> keys[1] -> keys[0] ...
   const uint32_t* visible = &a.ctrl->visible;
-  hipLaunchKernelGGL(k_hist, dim3(min(grid_for(a.capacity, 256), 1024u)), dim3(256), 0, s, a.cull_keys,
+  hipLaunchKernelGGL(k_hist, dim3(hist_grid(a.capacity)), dim3(256), 0, s, a.cull_keys,
                      visible, a.hist, depth_passes);
   for (int p = 0; p < depth_passes; ++p) {
     const uint32_t* kin = p == 0 ? a.cull_keys : a.keys[(p - 1) & 1];
@@ -1148,9 +1179,8 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes
   if (marks) (void)hipEventRecord(marks[3], s);
   hipLaunchKernelGGL(k_bin, dim3(grid_for(a.capacity, 256u * kBinItems)), dim3(256), 0, s, a, order);
   const uint32_t* sort_n = &a.ctrl->sort_pairs;
-  hipLaunchKernelGGL(k_pair_count, dim3(1), dim3(64), 0, s, a.ctrl);
   uint32_t* tile_hist = a.hist + 4 * 256;
-  hipLaunchKernelGGL(k_hist, dim3(min(grid_for(a.pair_capacity, 256), 2048u)), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_hist, dim3(hist_grid(a.pair_capacity)), dim3(256), 0, s,
                      a.pair_keys[0], sort_n, tile_hist, tile_passes);
   for (int p = 0; p < tile_passes; ++p) {
     sort_pass(a.pair_keys[p & 1], a.pair_vals[p & 1], a.pair_keys[(p + 1) & 1], a.pair_vals[(p + 1) & 1],

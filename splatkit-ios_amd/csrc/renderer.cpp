@@ -227,7 +227,7 @@ struct skm_renderer {
   uint32_t capacity = 0;       // survivor slots
   uint32_t pair_capacity = 0;  // tile pairs
   uint32_t tile_capacity = 0;  // tiles
-  DeviceBuffer ctrl, cull_keys, keys[2], order[2], projected, scan_status, sort_status, sort_counts, hist;
+  DeviceBuffer ctrl, cull_keys, keys[2], order[2], projected, rects, scan_status, sort_status, sort_counts, hist;
   int sort_variant = skm::kSortReduceScan;
   DeviceBuffer pair_keys[2], pair_vals[2], tile_ranges;
   size_t sort_status_blocks = 0;
@@ -349,8 +349,9 @@ struct skm_renderer {
   skm_status ensure_capacity(uint32_t cap) {
     cap = std::max(cap, 1u);
     if (cap <= capacity) return SKM_OK;
-    DeviceBuffer ck, k0, k1, o0, o1, pr, ss;
+    DeviceBuffer ck, k0, k1, o0, o1, pr, rc, ss;
     const size_t n = size_t{cap};
+    if (auto st = check(rc.allocate(n * 8), "rects"); st != SKM_OK) return st;
     if (auto st = check(ck.allocate(n * 4), "keys"); st != SKM_OK) return st;
     if (auto st = check(k0.allocate(n * 4), "keys"); st != SKM_OK) return st;
     if (auto st = check(k1.allocate(n * 4), "keys"); st != SKM_OK) return st;
@@ -366,6 +367,7 @@ struct skm_renderer {
     order[0] = std::move(o0);
     order[1] = std::move(o1);
     projected = std::move(pr);
+    rects = std::move(rc);
     scan_status = std::move(ss);
     capacity = cap;
     return SKM_OK;
@@ -837,6 +839,7 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
     a.pair_vals[k] = r->pair_vals[k].as<uint32_t>();
   }
   a.projected = r->projected.as<uint32_t>();
+  a.rects = r->rects.as<uint64_t>();
   a.scan_status = r->scan_status.as<uint64_t>();
   a.sort_status = r->sort_status.as<uint64_t>();
   a.hist = r->hist.as<uint32_t>();

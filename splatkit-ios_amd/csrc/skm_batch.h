@@ -78,7 +78,7 @@ class Batch {
 
   // Device scratch.
   DeviceBuffer ctrl_, cams_, views_, tile_view_, view_visible_, survivor_view_;
-  DeviceBuffer cull_keys_, keys_[2], order_[2], projected_, scan_status_, sort_status_, sort_counts_, hist_;
+  DeviceBuffer cull_keys_, keys_[2], order_[2], projected_, rects_, scan_status_, sort_status_, sort_counts_, hist_;
   DeviceBuffer pair_keys_[2], pair_vals_[2], tile_ranges_, image_, ranges_, starts_;
   PinnedBuffer staging_, host_;
   size_t staging_bytes_ = 0, host_bytes_ = 0;

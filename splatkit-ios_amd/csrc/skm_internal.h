@@ -116,6 +116,7 @@ struct FrameArgs {
   uint32_t* keys[2];    // sort ping-pong
   uint32_t* order[2];
   uint32_t* projected;  // 8 words per survivor
+  uint64_t* rects;      // per survivor: its packed tile rectangle (written by the cull, read by the binning)
   uint64_t* scan_status;  // one word per 1024-lane tile
   uint64_t* sort_status;
   uint32_t* hist;       // [2][4][256]: depth digits, tile digits

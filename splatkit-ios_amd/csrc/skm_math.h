@@ -27,31 +27,18 @@ __device__ __forceinline__ float half_to_float(uint32_t h16) {
 }
 __device__ __forceinline__ float lo_half(uint32_t w) { return half_to_float(w & 0xffffu); }
 __device__ __forceinline__ float hi_half(uint32_t w) { return half_to_float(w >> 16); }
-// IEEE binary16, round to nearest even, overflow to infinity: the shader's half(float). Written
-// with integer operations: a float conversion instruction would let the compiler fuse a preceding
-// multiply into v_fma_mix (one rounding from the exact product instead of two), which changes the
-// result at fp32 values that lie exactly between two halves.
+// IEEE binary16, round to nearest even, overflow to infinity: the shader's half(float), by the
+// conversion instruction. The empty asm makes the fp32 value a register first: without it the
+// compiler may fuse a preceding multiply into v_fma_mix (one rounding from the exact product
+// instead of two), which changes the result at fp32 values that lie exactly between two halves.
+// NaN becomes the quiet NaN 0x7e00 with its sign (the oracle's half_rne).
 __device__ __forceinline__ uint32_t half_rne(float value) {
-  const uint32_t b = bits(value);
-  const uint32_t sign = (b >> 16) & 0x8000u;
-  const uint32_t biased = (b >> 23) & 0xffu;
-  uint32_t mant = b & 0x7fffffu;
-  if (biased == 0xffu) return sign | (mant ? 0x7e00u : 0x7c00u);
-  const int32_t e = static_cast<int32_t>(biased) - 112;
-  if (e >= 31) return sign | 0x7c00u;
-  if (e <= 0) {
-    if (e < -10) return sign;
-    mant |= 0x800000u;
-    const uint32_t shift = static_cast<uint32_t>(14 - e);
-    const uint32_t q = mant >> shift;
-    const uint32_t rem = mant & ((1u << shift) - 1u);
-    const uint32_t mid = 1u << (shift - 1u);
-    return sign | (q + ((rem > mid || (rem == mid && (q & 1u))) ? 1u : 0u));
-  }
-  uint32_t h = (static_cast<uint32_t>(e) << 10) | (mant >> 13);
-  const uint32_t rem = mant & 0x1fffu;
-  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;  // may carry into infinity: IEEE
-  return sign | h;
+  if (value != value) return ((bits(value) >> 16) & 0x8000u) | 0x7e00u;
+  asm volatile("" : "+v"(value));
+  const _Float16 h = static_cast<_Float16>(value);
+  uint16_t out;
+  __builtin_memcpy(&out, &h, 2);
+  return out;
 }
 __device__ __forceinline__ uint32_t pack_rne(float a, float b) { return half_rne(a) | (half_rne(b) << 16); }
 __device__ __forceinline__ float round_half(float f) { return half_to_float(half_rne(f)); }
@@ -122,6 +109,32 @@ __device__ __forceinline__ double exp_d(double x) {
   double p = 1.0;
   for (int n = 20; n >= 1; --n) p = 1.0 + (r / static_cast<double>(n)) * p;
   return __builtin_ldexp(p, static_cast<int>(k));
+}
+
+// Natural logarithm in fp32 with a fixed operation order (the oracle's log_f, same
+// specification): x = m 2^e, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m-1)/(m+1),
+// |s| <= 0.1716, by the odd series of atanh to s^11 (Abramowitz & Stegun 4.5.64; the next term is
+// below 2^-33 relative), e ln 2 with a split constant. For x normal and positive.
+constexpr float kLn2HiF = 0.693145751953125f;   // 15 significant bits: e * kLn2HiF is exact
+constexpr float kLn2LoF = 1.428606765330187e-06f;
+constexpr float kInvLn2F = 1.44269504088896341f;
+__device__ __forceinline__ float log_f(float x) {
+  int e = 0;
+  float m = frexpf(x, &e);
+  if (m < 0.707106781f) {
+    m = m * 2.0f;
+    e = e - 1;
+  }
+  const float s = (m - 1.0f) / (m + 1.0f);
+  const float z = s * s;
+  float p = 0.0909090936f;  // 1/11
+  p = 0.111111112f + z * p;   // 1/9
+  p = 0.142857149f + z * p;   // 1/7
+  p = 0.200000003f + z * p;   // 1/5
+  p = 0.333333343f + z * p;   // 1/3
+  p = 1.0f + z * p;
+  const float fe = static_cast<float>(e);
+  return fe * kLn2HiF + (fe * kLn2LoF + (2.0f * s) * p);
 }
 
 __device__ __forceinline__ float pow_f(float x, float y) {

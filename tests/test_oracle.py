@@ -129,6 +129,22 @@ def test_log_exp_against_libm(oracle):
         assert abs(L.orc_exp_d(float(y)) / math.exp(y) - 1.0) <= 2e-15
 
 
+def test_fp32_log_against_libm(oracle):
+    """log_f, the fp32 logarithm of the radius and of the raster's log2 peak opacity: within 3 ulp of
+    the correctly rounded log over its domain (ratios 1..1e4, half opacities 2^-14..1e3), exact at 1."""
+    L = oracle.lib()
+    xs = np.concatenate([np.float32(2.0) ** np.arange(-14, 17, dtype=np.float32),
+                         rng.uniform(1.0, 1.0e4, 20000).astype(np.float32),
+                         rng.uniform(6.1035156e-05, 1.0, 20000).astype(np.float32),
+                         np.nextafter(np.float32(1.0), np.float32([0.0, 2.0]))])
+    assert L.orc_log_f(1.0) == 0.0
+    for x in xs:
+        want = np.float32(math.log(float(x)))
+        got = np.float32(L.orc_log_f(float(x)))
+        ulp = np.spacing(np.abs(want)) if want != 0 else np.float32(1e-45)
+        assert abs(float(got) - math.log(float(x))) <= 3 * float(ulp) + 1e-12, x
+
+
 # ---------------------------------------------------------------- camera (MetalFrameCompute.mm:141-164)
 
 def test_camera_uniform_closed_form(oracle, ref):
@@ -360,6 +376,22 @@ def test_binning_misses_no_accepted_pixel(oracle, skm_mod, w, h, n, log_scale, l
         if x0 <= x1 and y0 <= y1:
             loose += (x1 // 16 - x0 // 16 + 1) * (y1 // 16 - y0 // 16 + 1)
     assert o["pairs"] <= loose
+
+
+@pytest.mark.parametrize("region", [8, 16])
+@pytest.mark.parametrize("w,h,n,log_scale,lod_alpha,seed", [
+    (320, 180, 3000, -2.5, False, 21), (256, 200, 2000, -2.0, True, 22), (160, 90, 5000, -4.0, True, 23)])
+def test_raster_region_lists_drop_no_accepted_pixel(oracle, skm_mod, region, w, h, n, log_scale, lod_alpha, seed):
+    """The raster's per-region record test (quad slabs widened over the region, footprint box) keeps
+    every record that the coverage rule accepts at some pixel of the region."""
+    cloud = skm_mod.make_cloud(n, 1, seed=seed, layout=0, spread=0.8, log_scale_mean=log_scale)
+    if lod_alpha:
+        cloud["alphas"][::3] = np.float32(1.5) + np.arange(len(cloud["alphas"][::3]), dtype=np.float32) % 40
+    eye = np.array([0.2, 0.3, 3.5], np.float32)
+    o = oracle.frame(cloud, skm_mod.look_at(eye), skm_mod.perspective(math.pi / 3, w / h, 0.2, 100.0), eye, w, h,
+                     image=False)
+    missed, accepted = oracle.region_misses(o["projected"], o["count"], w, h, region)
+    assert accepted > 0 and missed == 0
 
 
 def test_raster_against_numpy_brute_force(oracle, skm_mod):
