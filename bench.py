@@ -88,7 +88,8 @@ def batched_vs_loop(r, skm, batches, loops):
             "per_view_loop_frames_per_s": round(64 / loop_s, 1), "per_view_loop_ms": round(loop_s * 1000, 3),
             "speedup": round(loop_s / batch_s, 3), "visible_per_batch": st["visible"], "pairs_per_batch": st["pairs"],
             "batches": batches, "loops": loops, "autotuned": best,
-            "autotune_ms": [{"tile": t, "raster_px": p, "sort_kernel": k, "gpu_ms": round(ms, 3)} for t, p, k, ms in table]}
+            "autotune_ms": [{"tile": t, "raster_px": p, "sort_kernel": k, "radix_bits": rb, "gpu_ms": round(ms, 3)}
+                            for t, p, k, rb, ms in table]}
 
 
 SERVING_SIZES = (500_000, 1_000_000, 2_000_000, 3_000_000, 5_000_000, 6_000_000, 8_000_000, 10_000_000)
@@ -283,7 +284,8 @@ def main():
         ((HEIGHT + plain_best["tile_size"] - 1) // plain_best["tile_size"])
     names = ["cull", "depth_sort", "bin_tile_sort_ranges", "raster"]
     ms = {n: tot["stage_ms"][k + 1] / n_frames for k, n in enumerate(names)}
-    by = stage_bytes(N_SPLATS, V, P, T, 4, 2 if T <= 65536 else 3)
+    tile_bits = max(1, (T - 1).bit_length())
+    by = stage_bytes(N_SPLATS, V, P, T, 4, -(-tile_bits // plain_best["radix_bits"]))
     stages = {}
     for n in names:
         gbs = by[n] / (ms[n] * 1e-3) / 1e9 if ms[n] > 0 else 0.0
@@ -333,9 +335,10 @@ def main():
                    "splats": N_SPLATS, "sh_degree": SH_DEGREE, "width": WIDTH, "height": HEIGHT,
                    "tile": plain_best["tile_size"], "raster_px": plain_best["raster_px"],
                    "sort_kernel": ["reduce_then_scan", "onesweep_lookback"][plain_best["sort_kernel"]],
+                   "tile_radix_bits": plain_best["radix_bits"],
                    "parallelism": "single GPU"},
-        "autotune_ms": [{"tile": t, "raster_px": p, "sort_kernel": k, "gpu_ms_4_views": round(ms, 3)}
-                        for t, p, k, ms in plain_table],
+        "autotune_ms": [{"tile": t, "raster_px": p, "sort_kernel": k, "radix_bits": rb, "gpu_ms_4_views": round(ms, 3)}
+                        for t, p, k, rb, ms in plain_table],
         "msplats_per_s": round(N_SPLATS * fps / 1e6, 1),
         "visible_per_frame": int(V), "pairs_per_frame": int(P),
         "gpu_ms_per_frame": round(tot["stage_ms"][5] / n_frames, 4),

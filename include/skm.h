@@ -266,7 +266,8 @@ typedef struct skm_view {
 
 typedef struct skm_batch_tuning {
   uint32_t tile_size;       /* raster tile edge in pixels, as skm_config (0 = 16) */
-  uint32_t tile_radix_bits; /* digit width of the tile sort (0 = 8) */
+  uint32_t tile_radix_bits; /* digit width of the tile sort's passes: 6, 7 or 8 (0 = 8); keys of k bits
+                             * take ceil(k / width) passes of even widths */
   uint32_t use_graph;       /* 1: capture the batch's kernels in a hipGraph and replay it */
   uint32_t sort_kernel;     /* radix pass variant, as skm_config.sort_kernel */
   uint32_t raster_px;       /* raster pixels per lane side, as skm_config.raster_px */
@@ -289,11 +290,11 @@ skm_status skm_draw_views(skm_renderer* r, const skm_view* views, uint32_t count
 skm_status skm_read_batch_stats(skm_renderer* r, skm_batch_stats* out);
 
 /* The batch autotuner: times every supported configuration (raster tile and block shape, radix pass
- * variant) on this batch of views, `repeats` timed launches each after one untimed launch, and makes
+ * variant, tile-sort digit width of 8, 7 or 6 bits) on this batch of views, `repeats` timed launches each after one untimed launch, and makes
  * the fastest the renderer's batch tuning (used when skm_draw_views gets tuning NULL). results gets
  * up to `capacity` candidates with their best GPU milliseconds; *best the choice. Every candidate
  * gives the same pixels: the composite of a pixel does not depend on tiles or block shape. */
-/* Applies a tuning (tile size, raster pixels per lane, radix pass variant) to this renderer's
+/* Applies a tuning (tile size, raster pixels per lane, radix pass variant, tile-sort digit width) to this renderer's
  * single-view draws as well; the next frame uses it. */
 skm_status skm_set_tuning(skm_renderer* r, const skm_batch_tuning* tuning);
 

@@ -127,9 +127,10 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
   }
   const uint32_t kTile = tuning.tile_size ? tuning.tile_size : 16u;
   const uint32_t px = tuning.raster_px ? tuning.raster_px : (kTile == 32u ? 2u : 1u);
-  if (count == 0 || count > kMaxViews || !raster_shape_supported(kTile, px) ||
-      (tuning.tile_radix_bits != 0 && tuning.tile_radix_bits != 8) || tuning.sort_kernel > 1) {
-    *error = "batch: 1 to 4096 views, a supported raster shape and 8-bit tile digits";
+  const uint32_t radix_bits = tuning.tile_radix_bits ? tuning.tile_radix_bits : 8u;
+  if (count == 0 || count > kMaxViews || !raster_shape_supported(kTile, px) || radix_bits < 6 || radix_bits > 8 ||
+      tuning.sort_kernel > 1) {
+    *error = "batch: 1 to 4096 views, a supported raster shape and 6- to 8-bit tile digits";
     return hipErrorInvalidValue;
   }
   uint64_t total = 0;
@@ -251,8 +252,10 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
   a.sort_variant = tuning.sort_kernel == 1 ? kSortOnesweep : kSortReduceScan;
   a.raster_px = px;
   a.sort_counts = sort_counts_.as<uint32_t>();
-  depth_passes_ = policy.key16 ? 2 : 4;
-  tile_passes_ = static_cast<int>(digit_passes(static_cast<uint32_t>(tiles - 1)));
+  const DigitPlan depth_plan = make_digit_plan(policy.key16 ? 16u : 32u, 8u);
+  const DigitPlan tile_plan = make_digit_plan(key_bits_for(static_cast<uint32_t>(tiles - 1)), radix_bits);
+  depth_passes_ = depth_plan.passes;
+  tile_passes_ = tile_plan.passes;
   (void)world_count;
   (void)sh_degree;
 
@@ -261,7 +264,7 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
     // What the captured kernels bake in: sizes, passes, addresses (reset on reallocation), the
     // background and the world. Cameras, views and ranges are read from device memory.
     std::vector<uint64_t> key = {count, total, range_count, uint64_t(world.sh_degree), uint64_t(policy.key16),
-                                 uint64_t(a.sort_variant), kTile, px,
+                                 uint64_t(a.sort_variant), kTile, px, radix_bits,
                                  tiles, uint64_t(depth_passes_), uint64_t(tile_passes_),
                                  reinterpret_cast<uint64_t>(world.px), world.capacity, pair_capacity_,
                                  lane_capacity_};
@@ -276,7 +279,7 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
       hipGraph_t g = nullptr;
       e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
       if (e == hipSuccess) {
-        const hipError_t le = launch_frame(a, nullptr, depth_passes_, tile_passes_, nullptr, stream);
+        const hipError_t le = launch_frame(a, nullptr, depth_plan, tile_plan, nullptr, stream);
         e = hipStreamEndCapture(stream, &g);
         if (e == hipSuccess) e = le;
       }
@@ -293,7 +296,7 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
     if (e == hipSuccess) e = hipGraphLaunch(exec_, stream);
   } else {
     e = hipEventRecord(start_, stream);
-    if (e == hipSuccess) e = launch_frame(a, nullptr, depth_passes_, tile_passes_, nullptr, stream);
+    if (e == hipSuccess) e = launch_frame(a, nullptr, depth_plan, tile_plan, nullptr, stream);
   }
   if (e == hipSuccess) e = hipEventRecord(end_, stream);
   if (e == hipSuccess) e = hipMemcpyAsync(host_.ptr, ctrl_.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost, stream);

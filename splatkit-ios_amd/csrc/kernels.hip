@@ -542,15 +542,17 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
 
 // Global digit histograms of every pass in one read of the keys (onesweep's upfront histogram).
 __global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
-                                              uint32_t* hist, int passes) {
+                                              uint32_t* hist, DigitPlan plan) {
   __shared__ uint32_t s_h[4][4][256];
   const uint32_t tid = threadIdx.x, wave = tid >> 6;
+  const int passes = plan.passes;
   for (uint32_t i = tid; i < 4 * 4 * 256; i += 256) (&s_h[0][0][0])[i] = 0;
   __syncthreads();
   const uint32_t n = *n_ptr;
   for (uint32_t i = blockIdx.x * 256u + tid; i < n; i += gridDim.x * 256u) {
     const uint32_t k = keys[i];
-    for (int p = 0; p < passes; ++p) atomicAdd(&s_h[wave][p][(k >> (8 * p)) & 255u], 1u);
+    for (int p = 0; p < passes; ++p)
+      atomicAdd(&s_h[wave][p][(k >> plan.shift[p]) & ((1u << plan.bits[p]) - 1u)], 1u);
   }
   __syncthreads();
   for (int p = 0; p < passes; ++p) {
@@ -569,13 +571,13 @@ constexpr uint32_t kLookback = 16;
 //     status words (decoupled look-back), one kernel per pass;
 //   !LOOKBACK (reduce then scan): k_tile_counts and k_digit_scan have already written the start of
 //     digit d of tile t at offsets[d * stride + t].
-template <bool LOOKBACK>
+template <bool LOOKBACK, int BITS>
 __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ keys_in,
                                                   const uint32_t* __restrict__ vals_in,
                                                   uint32_t* __restrict__ keys_out,
                                                   uint32_t* __restrict__ vals_out, const uint32_t* n_ptr,
                                                   const uint32_t* __restrict__ hist, uint32_t shift,
-                                                  uint64_t* status, Ctrl* ctrl, uint32_t ticket,
+                                                  uint32_t mask, uint64_t* status, Ctrl* ctrl, uint32_t ticket,
                                                   uint32_t tag_pass, const uint32_t* __restrict__ offsets,
                                                   uint32_t stride) {
   __shared__ uint32_t s_ticket;
@@ -608,10 +610,10 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
   for (uint32_t j = 0; j < kSortItems; ++j) {
     const uint32_t pos = base + wave * 1024u + j * 64u + lane;
     const bool valid = pos < n;
-    const uint32_t d = (key[j] >> shift) & 255u;
+    const uint32_t d = (key[j] >> shift) & mask;  // mask < 2^BITS
     uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (uint32_t bit = 0; bit < 8; ++bit) {
+    for (uint32_t bit = 0; bit < static_cast<uint32_t>(BITS); ++bit) {
       const bool set = (d >> bit) & 1u;
       const uint64_t vote = __ballot(set);
       peers &= set ? vote : ~vote;
@@ -679,7 +681,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
   for (uint32_t j = 0; j < kSortItems; ++j) {
     const uint32_t pos = base + wave * 1024u + j * 64u + lane;
     if (pos < n) {
-      const uint32_t d = (key[j] >> shift) & 255u;
+      const uint32_t d = (key[j] >> shift) & mask;
       const uint32_t local = s_digit_start[d] + s_wave_count[wave][d] + rank[j];
       s_keys[local] = key[j];
       s_vals[local] = val[j];
@@ -689,7 +691,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
   const uint32_t in_tile = min(kSortTile, n - base);
   for (uint32_t i = tid; i < in_tile; i += 256u) {
     const uint32_t k = s_keys[i];
-    const uint32_t dst = s_base[(k >> shift) & 255u] + i;
+    const uint32_t dst = s_base[(k >> shift) & mask] + i;
     if (keys_out != nullptr) keys_out[dst] = k;
     vals_out[dst] = s_vals[i];
   }
@@ -698,7 +700,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
 // Reduce then scan, step 1: the digit counts of each 4096-key tile, digit-major
 // (counts[d * stride + tile]), from per-wave LDS histograms.
 __global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
-                                                     uint32_t shift, uint32_t* __restrict__ counts,
+                                                     uint32_t shift, uint32_t mask, uint32_t* __restrict__ counts,
                                                      uint32_t stride) {
   __shared__ uint32_t s_h[4][kRadix];
   const uint32_t tid = threadIdx.x, wave = tid >> 6;
@@ -711,7 +713,7 @@ __global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict_
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) {
     const uint32_t pos = base + j * 256u + tid;
-    if (pos < n) atomicAdd(&s_h[wave][(keys[pos] >> shift) & 255u], 1u);
+    if (pos < n) atomicAdd(&s_h[wave][(keys[pos] >> shift) & mask], 1u);
   }
   __syncthreads();
   counts[static_cast<size_t>(tid) * stride + blockIdx.x] = s_h[0][tid] + s_h[1][tid] + s_h[2][tid] + s_h[3][tid];
@@ -858,15 +860,17 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ key
 // transmittance is already below kTransmittanceStop takes no more splats, so a pixel's value does not
 // depend on which other pixels share its wave. No branch: a wave runs the same instructions whatever
 // its lanes decide.
-__device__ __forceinline__ void blend_step(const float4 A, const float4 B, const float4 Cc, float fx,
-                                           float fy, float& T, float& C0, float& C1, float& C2) {
+__device__ __forceinline__ bool coverage(const float4 A, const float4 B, float fx, float fy, float T, float& e) {
   const float dx = fx - A.x;
   const float dy = A.y - fy;
   const float gu = fmaf(A.z, dx, A.w * dy);
   const float gv = fmaf(B.x, dx, B.y * dy);
-  const float e = B.w - fmaf(gu, gu, gv * gv);
-  const bool take = static_cast<int>(fabsf(gu) <= B.z) & static_cast<int>(fabsf(gv) <= B.z) &
-                    static_cast<int>(e >= kLog2MinAlpha) & static_cast<int>(T >= kTransmittanceStop);
+  e = B.w - fmaf(gu, gu, gv * gv);
+  return static_cast<int>(fabsf(gu) <= B.z) & static_cast<int>(fabsf(gv) <= B.z) &
+         static_cast<int>(e >= kLog2MinAlpha) & static_cast<int>(T >= kTransmittanceStop);
+}
+__device__ __forceinline__ void blend(bool take, float e, const float4 Cc, float& T, float& C0, float& C1,
+                                      float& C2) {
   // e is a number when take holds, so fminf agrees with the oracle's min; 2^-inf = 0.
   const float alpha = __builtin_amdgcn_exp2f(take ? fminf(e, 0.0f) : -__builtin_inff());
   const float w = T * alpha;
@@ -1010,9 +1014,20 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
         for (uint32_t u = 0; u < 4; ++u) {
           if (i + u < n) {
             const uint32_t k = (four >> (8u * u)) & 255u;
-            const float4 A = s_a[k], B = s_b[k], Cc = s_c[k];
+            const float4 A = s_a[k], B = s_b[k];
+            float e[kP];
+            bool take[kP], any = false;
 #pragma unroll
-            for (uint32_t q = 0; q < kP; ++q) blend_step(A, B, Cc, fx[q], fy[q], T[q], C0[q], C1[q], C2[q]);
+            for (uint32_t q = 0; q < kP; ++q) {
+              take[q] = coverage(A, B, fx[q], fy[q], T[q], e[q]);
+              any |= take[q];
+            }
+            // A record no pixel of the wave takes leaves the wave's pixels exactly as they are:
+            // skip its blend (a wave-uniform branch).
+            if (__ballot(any) == 0) continue;
+            const float4 Cc = s_c[k];
+#pragma unroll
+            for (uint32_t q = 0; q < kP; ++q) blend(take[q], e[q], Cc, T[q], C0[q], C1[q], C2[q]);
           }
         }
         if (__ballot(live()) == 0) break;
@@ -1085,23 +1100,55 @@ uint32_t grid_for(uint32_t items, uint32_t per_block) {
 
 // Blocks of the histogram kernel: enough to stream the keys, few enough that the per-block global
 // adds into each of the histogram's words stay short (each word takes one add per block).
-uint32_t hist_grid(uint32_t capacity) { return std::min(std::max(grid_for(capacity, 16384u), 16u), 256u); }
+uint32_t hist_grid(uint32_t capacity) { return std::min(std::max(grid_for(capacity, 16384u), 16u), 2048u); }
 
-// One LSD pass of 8 bits at `shift` in the chosen variant; `counts` holds 256 words per 4096-key
-// tile of `capacity` (reduce then scan only).
+// One stable LSD pass over the digit of `bits` bits at `shift`, in the chosen variant. The kernel
+// ranks with BITS >= bits ballots (the digit's missing high bits are zero in every lane); `counts`
+// holds 256 words per 4096-key tile of `capacity` (reduce then scan only).
+template <bool LOOKBACK, int BITS>
+void sort_pass_bits(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
+                    const uint32_t* n_ptr, uint32_t tiles, const uint32_t* hist, uint32_t shift, uint32_t bits,
+                    uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass, uint32_t* counts,
+                    hipStream_t s) {
+  const uint32_t mask = (1u << bits) - 1u;
+  if constexpr (LOOKBACK) {
+    hipLaunchKernelGGL((k_onesweep<true, BITS>), dim3(tiles), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist,
+                       shift, mask, status, ctrl, ticket, tag_pass, nullptr, 0u);
+  } else {
+    hipLaunchKernelGGL(k_tile_counts, dim3(tiles), dim3(256), 0, s, kin, n_ptr, shift, mask, counts, tiles);
+    hipLaunchKernelGGL(k_digit_scan, dim3(1u << bits), dim3(256), 0, s, counts, n_ptr, hist, tiles);
+    hipLaunchKernelGGL((k_onesweep<false, BITS>), dim3(tiles), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist,
+                       shift, mask, status, ctrl, ticket, tag_pass, counts, tiles);
+  }
+}
+
+template <bool LOOKBACK>
+void sort_pass_variant(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
+                       const uint32_t* n_ptr, uint32_t tiles, const uint32_t* hist, uint32_t shift, uint32_t bits,
+                       uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass, uint32_t* counts,
+                       hipStream_t s) {
+  if (bits <= 6)
+    sort_pass_bits<LOOKBACK, 6>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
+                                tag_pass, counts, s);
+  else if (bits == 7)
+    sort_pass_bits<LOOKBACK, 7>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
+                                tag_pass, counts, s);
+  else
+    sort_pass_bits<LOOKBACK, 8>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
+                                tag_pass, counts, s);
+}
+
+// Pass p of a sort under its digit plan.
 void sort_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, const uint32_t* n_ptr,
-               uint32_t capacity, const uint32_t* hist, uint32_t shift, uint64_t* status, Ctrl* ctrl,
+               uint32_t capacity, const uint32_t* hist, const DigitPlan& plan, int p, uint64_t* status, Ctrl* ctrl,
                uint32_t ticket, uint32_t tag_pass, uint32_t* counts, int variant, hipStream_t s) {
   const uint32_t tiles = grid_for(capacity, kSortTile);
-  if (variant == kSortOnesweep) {
-    hipLaunchKernelGGL(k_onesweep<true>, dim3(tiles), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist, shift,
-                       status, ctrl, ticket, tag_pass, nullptr, 0u);
-  } else {
-    hipLaunchKernelGGL(k_tile_counts, dim3(tiles), dim3(256), 0, s, kin, n_ptr, shift, counts, tiles);
-    hipLaunchKernelGGL(k_digit_scan, dim3(kRadix), dim3(256), 0, s, counts, n_ptr, hist, tiles);
-    hipLaunchKernelGGL(k_onesweep<false>, dim3(tiles), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist, shift,
-                       status, ctrl, ticket, tag_pass, counts, tiles);
-  }
+  if (variant == kSortOnesweep)
+    sort_pass_variant<true>(kin, vin, kout, vout, n_ptr, tiles, hist, plan.shift[p], plan.bits[p], status, ctrl,
+                            ticket, tag_pass, counts, s);
+  else
+    sort_pass_variant<false>(kin, vin, kout, vout, n_ptr, tiles, hist, plan.shift[p], plan.bits[p], status, ctrl,
+                             ticket, tag_pass, counts, s);
 }
 
 }  // namespace
@@ -1124,26 +1171,27 @@ hipError_t launch_to_bgra8(const float* rgba, uint32_t* out, uint64_t pixels, hi
 }
 
 hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev,
-                          uint32_t capacity, int passes, uint32_t* hist, uint64_t* status, uint32_t* counts,
-                          int variant, hipStream_t stream) {
+                          uint32_t capacity, const DigitPlan& plan, uint32_t* hist, uint64_t* status,
+                          uint32_t* counts, int variant, hipStream_t stream) {
+  const int passes = plan.passes;
   // keys[0] holds the input; passes alternate keys[0] -> keys[1] -> keys[0] ...; the order ends in
   // vals[(passes - 1) & 1].
   hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, stream, ctrl, hist, 4u * 256u, nullptr, 0u, nullptr,
                      0u);
-  hipLaunchKernelGGL(k_hist, dim3(hist_grid(capacity)), dim3(256), 0, stream, keys[0],
-                     n_dev, hist, passes);
+  hipLaunchKernelGGL(k_hist, dim3(hist_grid(capacity)), dim3(256), 0, stream, keys[0], n_dev, hist, plan);
   for (int p = 0; p < passes; ++p) {
     const uint32_t* kin = keys[p & 1];
     uint32_t* kout = (p + 1 < passes) ? keys[(p + 1) & 1] : nullptr;
     const uint32_t* vin = p == 0 ? nullptr : vals[(p - 1) & 1];
-    sort_pass(kin, vin, kout, vals[p & 1], n_dev, capacity, hist + p * 256, 8u * p, status, ctrl,
+    sort_pass(kin, vin, kout, vals[p & 1], n_dev, capacity, hist + p * 256, plan, p, status, ctrl,
               kTicketDepthSort + p, 2u + p, counts, variant, stream);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes, int tile_passes,
-                        hipEvent_t* marks, hipStream_t s) {
+hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan& depth_plan,
+                        const DigitPlan& tile_plan, hipEvent_t* marks, hipStream_t s) {
+  const int depth_passes = depth_plan.passes, tile_passes = tile_plan.passes;
   const uint32_t tiles = a.total_tiles;
   if (marks) (void)hipEventRecord(marks[0], s);
   if (lod != nullptr) {
@@ -1166,13 +1214,13 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes
   if (marks) (void)hipEventRecord(marks[2], s);
   // Depth sort of the survivors: cull keys -> keys[0] -> keys[1] -> keys[0] ...
   const uint32_t* visible = &a.ctrl->visible;
-  hipLaunchKernelGGL(k_hist, dim3(hist_grid(a.capacity)), dim3(256), 0, s, a.cull_keys,
-                     visible, a.hist, depth_passes);
+  hipLaunchKernelGGL(k_hist, dim3(hist_grid(a.capacity)), dim3(256), 0, s, a.cull_keys, visible, a.hist,
+                     depth_plan);
   for (int p = 0; p < depth_passes; ++p) {
     const uint32_t* kin = p == 0 ? a.cull_keys : a.keys[(p - 1) & 1];
     uint32_t* kout = (p + 1 < depth_passes) ? a.keys[p & 1] : nullptr;
     const uint32_t* vin = p == 0 ? nullptr : a.order[(p - 1) & 1];
-    sort_pass(kin, vin, kout, a.order[p & 1], visible, a.capacity, a.hist + p * 256, 8u * p, a.sort_status,
+    sort_pass(kin, vin, kout, a.order[p & 1], visible, a.capacity, a.hist + p * 256, depth_plan, p, a.sort_status,
               a.ctrl, kTicketDepthSort + p, 2u + p, a.sort_counts, a.sort_variant, s);
   }
   const uint32_t* order = a.order[(depth_passes - 1) & 1];
@@ -1180,11 +1228,11 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes
   hipLaunchKernelGGL(k_bin, dim3(grid_for(a.capacity, 256u * kBinItems)), dim3(256), 0, s, a, order);
   const uint32_t* sort_n = &a.ctrl->sort_pairs;
   uint32_t* tile_hist = a.hist + 4 * 256;
-  hipLaunchKernelGGL(k_hist, dim3(hist_grid(a.pair_capacity)), dim3(256), 0, s,
-                     a.pair_keys[0], sort_n, tile_hist, tile_passes);
+  hipLaunchKernelGGL(k_hist, dim3(hist_grid(a.pair_capacity)), dim3(256), 0, s, a.pair_keys[0], sort_n, tile_hist,
+                     tile_plan);
   for (int p = 0; p < tile_passes; ++p) {
     sort_pass(a.pair_keys[p & 1], a.pair_vals[p & 1], a.pair_keys[(p + 1) & 1], a.pair_vals[(p + 1) & 1],
-              sort_n, a.pair_capacity, tile_hist + p * 256, 8u * p, a.sort_status, a.ctrl, kTicketTileSort + p,
+              sort_n, a.pair_capacity, tile_hist + p * 256, tile_plan, p, a.sort_status, a.ctrl, kTicketTileSort + p,
               6u + p, a.sort_counts, a.sort_variant, s);
   }
   const int fin = tile_passes & 1;

@@ -27,7 +27,6 @@ using skm::align_up;
 using skm::Camera;
 using skm::Ctrl;
 using skm::DeviceBuffer;
-using skm::digit_passes;
 using skm::fill_camera;
 using skm::PinnedBuffer;
 using skm::sh_stride;
@@ -88,6 +87,7 @@ struct Slot {
   uint32_t lod_limit = 0;
   uint32_t width = 0, height = 0, tile = 16;
   int depth_passes = 4, tile_passes = 2;
+  skm::DigitPlan depth_plan{}, tile_plan{};
 };
 
 struct Policy {
@@ -206,6 +206,7 @@ struct skm_renderer {
   std::string description;
   uint32_t tile = 16;
   uint32_t raster_px = 1;
+  uint32_t tile_radix_bits = 8;  // digit width of the tile sort
   uint32_t pair_factor = 8;
 
   float render_scale = 1.0f;
@@ -853,9 +854,11 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   a.sort_counts = r->sort_counts.as<uint32_t>();
   a.raster_px = r->raster_px;
   s.tile = r->tile;
-  s.depth_passes = r->policy.key16 ? 2 : 4;
-  s.tile_passes = static_cast<int>(digit_passes(tiles_x * tiles_y - 1));
-  e = skm::launch_frame(a, lod_frame ? &lod : nullptr, s.depth_passes, s.tile_passes, s.marks, r->stream);
+  s.depth_plan = skm::make_digit_plan(r->policy.key16 ? 16u : 32u, 8u);
+  s.tile_plan = skm::make_digit_plan(skm::key_bits_for(tiles_x * tiles_y - 1), r->tile_radix_bits);
+  s.depth_passes = s.depth_plan.passes;
+  s.tile_passes = s.tile_plan.passes;
+  e = skm::launch_frame(a, lod_frame ? &lod : nullptr, s.depth_plan, s.tile_plan, s.marks, r->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(s.ctrl_host.ptr, r->ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost, r->stream);
   if (e == hipSuccess && lod_frame) {
@@ -1069,18 +1072,18 @@ skm_status skm_argsort_u32_device(skm_renderer* r, void* keys_dev, uint32_t n, i
     r->sort_capacity = static_cast<uint32_t>(cap);
   }
   if (auto st = r->maybe_clear_status(); st != SKM_OK) return st;
-  const int passes = key_bits == 16 ? 2 : 4;
   uint32_t* keys[2] = {static_cast<uint32_t*>(keys_dev), r->sort_keys.as<uint32_t>()};
-  // The order ends in vals[(passes - 1) & 1] = vals[1].
+  // Two or four passes of 8 bits: the order ends in vals[(passes - 1) & 1] = vals[1].
   uint32_t* vals[2] = {r->sort_vals.as<uint32_t>(), static_cast<uint32_t*>(order_dev)};
   hipEvent_t t0 = r->slots[0].marks[0], t1 = r->slots[0].marks[4];
   hipError_t e = hipMemcpyAsync(r->sort_n.ptr, &n, 4, hipMemcpyHostToDevice, r->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
   if (e == hipSuccess) e = hipEventRecord(t0, r->stream);
   if (e == hipSuccess)
-    e = skm::launch_argsort(r->ctrl.as<Ctrl>(), keys, vals, r->sort_n.as<uint32_t>(), std::max(n, 1u), passes,
-                            r->hist.as<uint32_t>(), r->sort_status.as<uint64_t>(),
-                            r->sort_counts.as<uint32_t>(), r->sort_variant, r->stream);
+    e = skm::launch_argsort(r->ctrl.as<Ctrl>(), keys, vals, r->sort_n.as<uint32_t>(), std::max(n, 1u),
+                            skm::make_digit_plan(static_cast<uint32_t>(key_bits), 8u), r->hist.as<uint32_t>(),
+                            r->sort_status.as<uint64_t>(), r->sort_counts.as<uint32_t>(), r->sort_variant,
+                            r->stream);
   if (e == hipSuccess) e = hipEventRecord(t1, r->stream);
   if (e == hipSuccess) e = hipEventSynchronize(t1);
   if (auto st = r->check(e, "sort"); st != SKM_OK) return st;
@@ -1168,7 +1171,8 @@ skm_status skm_autotune_views(skm_renderer* r, const skm_view* views, uint32_t c
   uint32_t n = 0;
   for (const auto& shape : kShapes) {
     for (uint32_t sort_kernel = 0; sort_kernel < 2; ++sort_kernel) {
-      const skm_batch_tuning t{shape[0], 8, 1, sort_kernel, shape[1], 0};
+     for (uint32_t radix_bits = 8; radix_bits >= 6; --radix_bits) {
+      const skm_batch_tuning t{shape[0], radix_bits, 1, sort_kernel, shape[1], 0};
       double ms = 0.0;
       for (uint32_t k = 0; k <= repeats; ++k) {
         skm_status st = skm_draw_views(r, views, count, ranges, range_count, &t);
@@ -1187,6 +1191,7 @@ skm_status skm_autotune_views(skm_renderer* r, const skm_view* views, uint32_t c
         chosen = t;
         chosen_ms = ms;
       }
+     }
     }
   }
   r->batch_tuning = chosen;
@@ -1199,15 +1204,18 @@ skm_status skm_set_tuning(skm_renderer* r, const skm_batch_tuning* t) {
   if (r == nullptr || t == nullptr) return SKM_ERR_INVALID;
   const uint32_t tile = t->tile_size ? t->tile_size : 16u;
   const uint32_t px = t->raster_px ? t->raster_px : (tile == 32u ? 2u : 1u);
-  if (!skm::raster_shape_supported(tile, px) || t->sort_kernel > 1 || (t->tile_radix_bits != 0 && t->tile_radix_bits != 8))
+  const uint32_t rb = t->tile_radix_bits ? t->tile_radix_bits : 8u;
+  if (!skm::raster_shape_supported(tile, px) || t->sort_kernel > 1 || rb < 6 || rb > 8)
     return r->fail(SKM_ERR_INVALID, "unsupported tuning");
   if (auto st = r->drain(); st != SKM_OK) return st;
   r->tile = tile;
   r->raster_px = px;
+  r->tile_radix_bits = rb;
   r->sort_variant = static_cast<int>(t->sort_kernel);
   r->batch_tuning = *t;
   r->batch_tuning.tile_size = tile;
   r->batch_tuning.raster_px = px;
+  r->batch_tuning.tile_radix_bits = rb;
   return SKM_OK;
 }
 

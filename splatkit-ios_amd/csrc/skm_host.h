@@ -1,5 +1,5 @@
 // Host-side helpers shared by the renderer, the view batches and the server: RAII device and
-// pinned buffers, the camera uniform and the radix pass count.
+// pinned buffers, the camera uniform and the BGRA8 conversion.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -107,13 +107,6 @@ inline void fill_camera(Camera& cam, const float* view, const float* proj, uint3
     cam.reveal[0] = HUGE_VALF;
     cam.reveal_tint[0] = cam.reveal_tint[1] = cam.reveal_tint[2] = 1.0f;
   }
-}
-
-// Radix passes of 8 bits that cover keys up to max_key.
-inline uint32_t digit_passes(uint32_t max_key) {
-  uint32_t passes = 1;
-  while (passes < 4 && (max_key >> (8 * passes)) != 0) ++passes;
-  return passes;
 }
 
 // MetalFrameCapture's BGRA8 (MetalFrameCapture.h:18) from the RGBA float composite.

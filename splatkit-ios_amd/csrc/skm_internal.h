@@ -79,6 +79,39 @@ enum Ticket : uint32_t {
 // (three kernels, no inter-block waiting), or onesweep's decoupled look-back (one kernel).
 enum SortVariant : int { kSortReduceScan = 0, kSortOnesweep = 1 };
 
+// The digits of an LSD radix sort: pass p sorts by the `bits[p]` bits at `shift[p]`, low passes
+// first; the widths sum to the key width. At most 4 passes of at most 8 bits.
+struct DigitPlan {
+  int passes;
+  uint32_t shift[4];
+  uint32_t bits[4];
+};
+
+// Even digits of at most `width` bits (6, 7 or 8) for keys of `key_bits` bits.
+inline DigitPlan make_digit_plan(uint32_t key_bits, uint32_t width) {
+  key_bits = key_bits == 0 ? 1 : key_bits;
+  width = width < 6 ? 6 : (width > 8 ? 8 : width);
+  DigitPlan p{};
+  p.passes = static_cast<int>((key_bits + width - 1) / width);
+  if (p.passes > 4) p.passes = 4;
+  const uint32_t base = key_bits / static_cast<uint32_t>(p.passes);
+  const uint32_t extra = key_bits % static_cast<uint32_t>(p.passes);
+  uint32_t shift = 0;
+  for (int k = 0; k < p.passes; ++k) {
+    p.shift[k] = shift;
+    p.bits[k] = base + (static_cast<uint32_t>(k) < extra ? 1u : 0u);
+    shift += p.bits[k];
+  }
+  return p;
+}
+
+// Bits needed to write `max_key`.
+inline uint32_t key_bits_for(uint32_t max_key) {
+  uint32_t b = 0;
+  while (b < 32 && (max_key >> b) != 0) ++b;
+  return b;
+}
+
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kCullItems = 4;  // lanes per thread of the cull (a 1024-lane tile per block)
 constexpr uint32_t kBinItems = 4;   // survivors per thread of the binning (1024 per block)
@@ -158,12 +191,12 @@ hipError_t launch_pack(const WorldOut& out, uint32_t offset, uint32_t count, con
 struct LodArgs;
 // marks: 6 events recorded at the start, after the LOD cut, the cull, the depth sort, the tile
 // binning/sort/ranges and the raster.
-hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, int depth_passes, int tile_passes,
-                        hipEvent_t* marks, hipStream_t stream);
+hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan& depth_plan,
+                        const DigitPlan& tile_plan, hipEvent_t* marks, hipStream_t stream);
 // BGRA8 bytes (one u32 per pixel) of an RGBA float image.
 hipError_t launch_to_bgra8(const float* rgba, uint32_t* out, uint64_t pixels, hipStream_t stream);
 hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev,
-                          uint32_t capacity, int passes, uint32_t* hist, uint64_t* status, uint32_t* counts,
-                          int variant, hipStream_t stream);
+                          uint32_t capacity, const DigitPlan& plan, uint32_t* hist, uint64_t* status,
+                          uint32_t* counts, int variant, hipStream_t stream);
 
 }  // namespace skm

@@ -571,25 +571,28 @@ class Renderer:
 
     def autotune_views(self, views, ranges=None, repeats=2):
         """Times every batch configuration on these views and keeps the fastest; returns
-        (best, [(tile, raster_px, sort_kernel, gpu_ms), ...])."""
+        (best, [(tile, raster_px, sort_kernel, radix_bits, gpu_ms), ...])."""
         arr = views if isinstance(views, C.Array) else self.make_views(views)
         rarr = None
         if ranges is not None:
             rarr = (Range * max(1, len(ranges)))(*[Range(o, c) for o, c in ranges])
-        res = (TuningResult * 16)()
+        res = (TuningResult * 64)()
         n = C.c_uint32()
         best = BatchTuning()
         self._check(self._lib.skm_autotune_views(self._h, arr, len(arr), rarr, 0 if ranges is None else len(ranges),
-                                                 repeats, res, 16, C.byref(n), C.byref(best)))
-        table = [(r.tuning.tile_size, r.tuning.raster_px, r.tuning.sort_kernel, r.gpu_ms) for r in res[:n.value]]
+                                                 repeats, res, 64, C.byref(n), C.byref(best)))
+        table = [(r.tuning.tile_size, r.tuning.raster_px, r.tuning.sort_kernel, r.tuning.tile_radix_bits, r.gpu_ms)
+                 for r in res[:n.value]]
         self._views = arr
-        best = {"tile_size": best.tile_size, "raster_px": best.raster_px, "sort_kernel": best.sort_kernel}
+        best = {"tile_size": best.tile_size, "raster_px": best.raster_px, "sort_kernel": best.sort_kernel,
+                "radix_bits": best.tile_radix_bits}
         self._view_tile = best["tile_size"]
         return best, table
 
-    def set_tuning(self, tile_size=16, raster_px=0, sort_kernel=0):
-        """Tile size, raster pixels per lane and radix pass variant for single draws and batches."""
-        t = BatchTuning(tile_size, 8, 1, sort_kernel, raster_px, 0)
+    def set_tuning(self, tile_size=16, raster_px=0, sort_kernel=0, radix_bits=8):
+        """Tile size, raster pixels per lane, radix pass variant and tile-sort digit width for single
+        draws and batches."""
+        t = BatchTuning(tile_size, radix_bits, 1, sort_kernel, raster_px, 0)
         self._check(self._lib.skm_set_tuning(self._h, C.byref(t)))
         self.tile = tile_size
 

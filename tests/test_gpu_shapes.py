@@ -30,8 +30,10 @@ def test_shapes_and_sort_variants_give_identical_pixels(skm_mod, oracle, scene):
     first = None
     for tile, px in SHAPES:
         for sort_kernel in (0, 1):
-            say(f"case: tile {tile} px {px} sort {sort_kernel}")
+            radix_bits = 8 - (tile // 8 + sort_kernel) % 3  # every digit width across the shapes
+            say(f"case: tile {tile} px {px} sort {sort_kernel} radix {radix_bits}")
             r = skm_mod.Renderer(0, sort_kernel=sort_kernel, tile=tile, raster_px=px)
+            r.set_tuning(tile, px, sort_kernel, radix_bits)
             try:
                 r.upload_world(cloud)
                 r.set_extent(w, h)
@@ -63,11 +65,14 @@ def test_autotuner_times_every_candidate(renderer, skm_mod, scene):
     views = skm_mod.views_c2(6, seed=82, radius=(3.0, 7.0), resolutions=((640, 360), (1280, 720), (333, 217)))
     best, table = renderer.autotune_views(views, repeats=1)
     say(f"best {best}; candidates {table}")
-    assert len(table) == 8 and all(ms > 0 for *_, ms in table)
-    assert (best["tile_size"], best["raster_px"], best["sort_kernel"]) in [(t, p, k) for t, p, k, _ in table]
+    assert len(table) == 24 and all(ms > 0 for *_, ms in table)
+    assert (best["tile_size"], best["raster_px"], best["sort_kernel"], best["radix_bits"]) in \
+        [(t, p, k, rb) for t, p, k, rb, _ in table]
     renderer.render_views(views)  # the default configuration
     ref = [renderer.read_view_frame(k).copy() for k in range(6)]
     for tile, px in SHAPES:
-        renderer.render_views(views, tile_size=tile, raster_px=px, sort_kernel=1)
-        for k in range(6):
-            assert np.array_equal(renderer.read_view_frame(k).view(np.uint32), ref[k].view(np.uint32)), (tile, px, k)
+        for rb in (6, 7):
+            renderer.render_views(views, tile_size=tile, raster_px=px, sort_kernel=rb % 2, radix_bits=rb)
+            for k in range(6):
+                assert np.array_equal(renderer.read_view_frame(k).view(np.uint32), ref[k].view(np.uint32)), \
+                    (tile, px, rb, k)
