@@ -563,8 +563,9 @@ __global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ keys,
 
 constexpr uint32_t kLookback = 16;
 
-// One stable LSD pass over 4096-key tiles. Keys of tile b sit at b*4096 + wave*1024 + j*64 + lane:
-// each wave ranks its 16 rows in order with 8 ballots per row (lanes with equal digits form a
+// One stable LSD pass over kSortTile-key tiles. Keys of tile b sit at b*kSortTile + wave*kSortItems*64 +
+// j*64 + lane:
+// each wave ranks its kSortItems rows in order with 8 ballots per row (lanes with equal digits form a
 // peer mask), so the in-tile order is position order and the pass is stable. The tile then learns
 // where each of its digits starts in the output:
 //   LOOKBACK (onesweep): tiles take tickets in order and look back across predecessors' per-digit
@@ -601,14 +602,14 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
   uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) {
-    const uint32_t pos = base + wave * 1024u + j * 64u + lane;
+    const uint32_t pos = base + wave * (kSortItems * 64u) + j * 64u + lane;
     const bool valid = pos < n;
     key[j] = valid ? keys_in[pos] : 0u;
     val[j] = valid ? (vals_in != nullptr ? vals_in[pos] : pos) : 0u;
   }
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) {
-    const uint32_t pos = base + wave * 1024u + j * 64u + lane;
+    const uint32_t pos = base + wave * (kSortItems * 64u) + j * 64u + lane;
     const bool valid = pos < n;
     const uint32_t d = (key[j] >> shift) & mask;  // mask < 2^BITS
     uint64_t peers = __ballot(valid);
@@ -679,7 +680,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
   // Stage the tile in digit order in LDS.
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) {
-    const uint32_t pos = base + wave * 1024u + j * 64u + lane;
+    const uint32_t pos = base + wave * (kSortItems * 64u) + j * 64u + lane;
     if (pos < n) {
       const uint32_t d = (key[j] >> shift) & mask;
       const uint32_t local = s_digit_start[d] + s_wave_count[wave][d] + rank[j];

@@ -115,8 +115,12 @@ inline uint32_t key_bits_for(uint32_t max_key) {
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kCullItems = 4;  // lanes per thread of the cull (a 1024-lane tile per block)
 constexpr uint32_t kBinItems = 4;   // survivors per thread of the binning (1024 per block)
-constexpr uint32_t kSortItems = 16;
-constexpr uint32_t kSortTile = kThreads * kSortItems;  // 4096 keys per onesweep block
+// Keys per thread of a radix pass (a block sorts 256 x kSortItems keys); a build may set it.
+#ifndef SKM_SORT_ITEMS
+#define SKM_SORT_ITEMS 16
+#endif
+constexpr uint32_t kSortItems = SKM_SORT_ITEMS;
+constexpr uint32_t kSortTile = kThreads * kSortItems;  // keys per radix-pass block (4096 at 16)
 constexpr uint32_t kRadix = 256;
 constexpr uint32_t kMaxRanges = 65536;  // MetalVisibilityPass.h:72
 constexpr float kTransmittanceStop = 1.0f / 4096.0f;

@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libskm.so")
+# SKM_LIB names another build of the library (e.g. one with a different radix tile size).
+LIB_PATH = os.environ.get("SKM_LIB") or os.path.join(HERE, "libskm.so")
 SCENE_PATH = os.path.join(HERE, "libskm_scene.so")
 
 SKM_OK, SKM_ERR_INVALID, SKM_ERR_DEVICE, SKM_ERR_MEMORY = 0, -1, -2, -3
