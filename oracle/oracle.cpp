@@ -435,6 +435,8 @@ bool visibility_lane(const orc_camera& cam, const uint32_t* splats, const uint32
 // [0, kMaxColorScale] (:146-149).
 struct Prep {
   float cx, cy, g00, g01, g10, g11, rk, log2a0;
+  // The quad coordinates are affine in the pixel centre: gu = g00 fx + h01 fy + u0 (likewise gv).
+  float h01, h11, u0, v0;
   float c[3];
   // FP64 copies for the oracle's values.
   double cx64, cy64, a1x, a1y, a2x, a2y, a0;
@@ -459,6 +461,10 @@ Prep prep(const uint32_t* p, uint32_t width, uint32_t height) {
   q.g01 = -(a2x * s);
   q.g10 = -(a1y * s);
   q.g11 = a1x * s;
+  q.h01 = -q.g01;
+  q.h11 = -q.g11;
+  q.u0 = std::fmaf(q.g01, q.cy, -(q.g00 * q.cx));
+  q.v0 = std::fmaf(q.g11, q.cy, -(q.g10 * q.cx));
   q.rk = kGaussianScale * f32(p[6]);
   q.log2a0 = log2_peak(p);
   const float a0 = hi_half(p[5]);
@@ -477,14 +483,15 @@ Prep prep(const uint32_t* p, uint32_t width, uint32_t height) {
 // The fp32 coverage rule: the pixel centre lies in the quad c + (u a1 + v a2), |u|,|v| <= r
 // (expandQuad, SplatProjection.metalh:155-165) and the Gaussian's log2 alpha is at or above
 // kLog2MinAlpha (splatFragmentUnder, SplatRaster.metal:40-44). In scaled units gu = K u,
-// gv = K v: |gu|,|gv| <= K r and log2 alpha = log2a0 - (gu^2 + gv^2), with two fused
-// multiply-adds per coordinate and one for the square sum.
+// gv = K v: |gu|,|gv| <= K r and log2 alpha = log2a0 - gu^2 - gv^2. With dx = fx - cx and
+// dy = cy - fy, gu = g00 dx + g01 dy = g00 fx + h01 fy + u0: two fused multiply-adds per
+// coordinate from the pixel centre (fx, fy), and two for log2 alpha.
 inline bool decide(const Prep& q, uint32_t px, uint32_t py, float* power) {
-  const float dx = (static_cast<float>(px) + 0.5f) - q.cx;
-  const float dy = q.cy - (static_cast<float>(py) + 0.5f);
-  const float gu = std::fmaf(q.g00, dx, q.g01 * dy);
-  const float gv = std::fmaf(q.g10, dx, q.g11 * dy);
-  *power = q.log2a0 - std::fmaf(gu, gu, gv * gv);
+  const float fx = static_cast<float>(px) + 0.5f;
+  const float fy = static_cast<float>(py) + 0.5f;
+  const float gu = std::fmaf(q.g00, fx, std::fmaf(q.h01, fy, q.u0));
+  const float gv = std::fmaf(q.g10, fx, std::fmaf(q.h11, fy, q.v0));
+  *power = std::fmaf(-gu, gu, std::fmaf(-gv, gv, q.log2a0));
   if (!(std::fabs(gu) <= q.rk && std::fabs(gv) <= q.rk)) return false;
   return *power >= kLog2MinAlpha;
 }

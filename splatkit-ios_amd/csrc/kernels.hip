@@ -856,24 +856,30 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ key
 }
 
 // One coverage-and-blend step of a pixel against a prepared record (the oracle's decide and fp32
-// composite): scaled quad coordinates from the pixel centre's offset, the quad and cutoff tests
-// select alpha = 2^min(log2 alpha, 0) or 0, then w = T alpha, C += w c, T -= w. A pixel whose
-// transmittance is already below kTransmittanceStop takes no more splats, so a pixel's value does not
-// depend on which other pixels share its wave. No branch: a wave runs the same instructions whatever
-// its lanes decide.
-__device__ __forceinline__ bool coverage(const float4 A, const float4 B, float fx, float fy, float T, float& e) {
-  const float dx = fx - A.x;
-  const float dy = A.y - fy;
-  const float gu = fmaf(A.z, dx, A.w * dy);
-  const float gv = fmaf(B.x, dx, B.y * dy);
-  e = B.w - fmaf(gu, gu, gv * gv);
+// composite): the scaled quad coordinates are affine in the pixel centre, gu = g00 fx + h01 fy + u0
+// (two fused multiply-adds per coordinate), log2 alpha = log2a0 - gu^2 - gv^2 (two more), the quad
+// and cutoff tests select alpha = 2^min(log2 alpha, 0) or 0, then w = T alpha, C += w c, T -= w. A
+// pixel whose transmittance is already below kTransmittanceStop takes no more splats, so a pixel's
+// value does not depend on which other pixels share its wave. No branch: a wave runs the same
+// instructions whatever its lanes decide. A = (g00, h01, u0, g10), B = (h11, v0, K r, log2a0).
+// shape_test is the quad and cutoff part, which does not depend on the transmittance.
+__device__ __forceinline__ bool shape_test(const float4 A, const float4 B, float fx, float fy, float& e) {
+  const float gu = fmaf(A.x, fx, fmaf(A.y, fy, A.z));
+  const float gv = fmaf(A.w, fx, fmaf(B.x, fy, B.y));
+  e = fmaf(-gu, gu, fmaf(-gv, gv, B.w));
   return static_cast<int>(fabsf(gu) <= B.z) & static_cast<int>(fabsf(gv) <= B.z) &
-         static_cast<int>(e >= kLog2MinAlpha) & static_cast<int>(T >= kTransmittanceStop);
+         static_cast<int>(e >= kLog2MinAlpha);
+}
+// 2^x clamped to [0, 1] by the instruction's output clamp. 2^x is monotonic with 2^0 = 1, so this is
+// 2^min(x, 0) for every x, and 0 for x = -inf.
+__device__ __forceinline__ float exp2_clamped(float x) {
+  float r;
+  asm("v_exp_f32_e64 %0, %1 clamp" : "=v"(r) : "v"(x));
+  return r;
 }
 __device__ __forceinline__ void blend(bool take, float e, const float4 Cc, float& T, float& C0, float& C1,
                                       float& C2) {
-  // e is a number when take holds, so fminf agrees with the oracle's min; 2^-inf = 0.
-  const float alpha = __builtin_amdgcn_exp2f(take ? fminf(e, 0.0f) : -__builtin_inff());
+  const float alpha = exp2_clamped(take ? e : -__builtin_inff());
   const float w = T * alpha;
   C0 = fmaf(w, Cc.x, C0);
   C1 = fmaf(w, Cc.y, C1);
@@ -905,8 +911,10 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
                                                                            const uint32_t* __restrict__ ranges) {
   using S = RasterShape<TILE, PX>;
   constexpr uint32_t kB = S::kThreads, kW = S::kWaves, kP = S::kPixels;
-  __shared__ float4 s_a[kB], s_b[kB], s_c[kB];
-  __shared__ __align__(16) uint8_t s_list[kW][kB];
+  // s_rec[0..2][k]: record k's A, B (coverage) and colour. The region lists hold the byte offset of
+  // s_rec[0][k] (16 k), which the LDS reads take as their address with no arithmetic.
+  __shared__ float4 s_rec[3][kB];
+  __shared__ __align__(16) uint32_t s_list[kW][kB];
   __shared__ uint32_t s_count[kW][kW];  // [region][wave]
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
   const uint32_t tile_id = blockIdx.x;  // global tile id
@@ -941,13 +949,26 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
     for (uint32_t q = 0; q < kP; ++q) any |= T[q] >= kTransmittanceStop;
     return any;
   };
+  // Every list entry ever written is a record offset, so a group of four entries read past a list's
+  // end still addresses records (whose results the group then ignores).
+  reinterpret_cast<uint4*>(&s_list[0][0])[tid % (kW * kB / 4u)] = make_uint4(0u, 0u, 0u, 0u);
+  // Two-stage prefetch of the pair records: while batch b is prepared and walked, the record of batch
+  // b + 1 and the pair value of batch b + 2 are in flight.
+  auto record = [&](uint32_t idx, uint4& w0, uint4& w1) {
+    const uint4* src = reinterpret_cast<const uint4*>(a.projected + static_cast<size_t>(idx) * 8u);
+    w0 = src[0];
+    w1 = src[1];
+  };
+  uint4 next0 = make_uint4(0u, 0u, 0u, 0u), next1 = next0;
+  if (start + tid < end) record(pair_vals[start + tid], next0, next1);
+  uint32_t next_idx = start + kB + tid < end ? pair_vals[start + kB + tid] : 0u;
   for (uint32_t s0 = start; s0 < end; s0 += kB) {
     const uint32_t j = s0 + tid;
     uint32_t regions = 0;
+    const uint4 w0 = next0, w1 = next1;
+    if (j + kB < end) record(next_idx, next0, next1);
+    next_idx = j + 2u * kB < end ? pair_vals[j + 2u * kB] : 0u;
     if (j < end) {
-      const uint32_t idx = pair_vals[j];
-      const uint4* src = reinterpret_cast<const uint4*>(a.projected + static_cast<size_t>(idx) * 8u);
-      const uint4 w0 = src[0], w1 = src[1];
       const uint32_t p[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
       const float cx = (f32(w0.x) * 0.5f + 0.5f) * fw;
       const float cy = (0.5f - f32(w0.y) * 0.5f) * fh;
@@ -963,8 +984,9 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
         const float x0 = (cx - ex) - kBinPad, x1 = (cx + ex) + kBinPad;
         const float y0 = (cy - ey) - kBinPad, y1 = (cy + ey) + kBinPad;
         // The quad's two slabs |gu|, |gv| <= K r against the region: gu varies by at most
-        // (|g00| + |g01|) h around the region's centre, h its half span plus the binning pad; the
-        // slack covers the fp32 rounding of the per-pixel rule.
+        // (|g00| + |g01|) h around the region's centre, h its half span plus the binning pad. The
+        // pad's share, (|g00| + |g01|) / 16, and the slack cover the fp32 rounding of the per-pixel
+        // rule, at most about 3 W 2^-24 (|g00| + |g01|) at image width W.
         constexpr float kHalf = 0.5f * static_cast<float>(S::kRegion - 1u) + kBinPad;
         const float hu = (fabsf(g00) + fabsf(g01)) * kHalf, hv = (fabsf(g10) + fabsf(g11)) * kHalf;
         const float lim_u = (rk + hu) * kFootprintSlack, lim_v = (rk + hv) * kFootprintSlack;
@@ -980,9 +1002,12 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
             regions |= 1u << q;
         }
       }
-      s_a[tid] = make_float4(cx, cy, g00, g01);
-      s_b[tid] = make_float4(g10, g11, rk, log2a0);
-      s_c[tid] = make_float4(mn(mx(lo_half(w1.x), 0.0f), kMaxColorScale),
+      // gu = g00 (fx - cx) + g01 (cy - fy) = g00 fx + h01 fy + u0 with h01 = -g01 and
+      // u0 = g01 cy - g00 cx; likewise gv.
+      const float u0 = fmaf(g01, cy, -(g00 * cx)), v0 = fmaf(g11, cy, -(g10 * cx));
+      s_rec[0][tid] = make_float4(g00, -g01, u0, g10);
+      s_rec[1][tid] = make_float4(-g11, v0, rk, log2a0);
+      s_rec[2][tid] = make_float4(mn(mx(lo_half(w1.x), 0.0f), kMaxColorScale),
                              mn(mx(hi_half(w1.x), 0.0f), kMaxColorScale),
                              mn(mx(lo_half(w1.y), 0.0f), kMaxColorScale), 0.0f);
     }
@@ -999,7 +1024,7 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
         uint32_t off = rank_below(ball[q]);
 #pragma unroll
         for (uint32_t w = 0; w + 1 < kW; ++w) off += (w < wave) ? s_count[q][w] : 0u;
-        s_list[q][off] = static_cast<uint8_t>(tid);
+        s_list[q][off] = tid * static_cast<uint32_t>(sizeof(float4));
       }
     }
     uint32_t n = 0;
@@ -1008,28 +1033,37 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
     n = __builtin_amdgcn_readfirstlane(n);
     __syncthreads();
     if (__ballot(live()) != 0) {
-      const uint32_t* list = reinterpret_cast<const uint32_t*>(s_list[wave]);
+      const uint4* list = reinterpret_cast<const uint4*>(s_list[wave]);
       for (uint32_t i = 0; i < n; i += 4u) {
-        const uint32_t four = list[i >> 2];
+        // A group of four records: their geometry does not depend on the transmittance, so all four
+        // are read and tested together (one LDS latency per group); the blends then run in order.
+        const uint4 four = list[i >> 2];
+        const uint32_t addr[4] = {four.x, four.y, four.z, four.w};
+        float e[4][kP];
+        bool hit[4][kP];
+        float4 Cc[4];
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) {
-          if (i + u < n) {
-            const uint32_t k = (four >> (8u * u)) & 255u;
-            const float4 A = s_a[k], B = s_b[k];
-            float e[kP];
-            bool take[kP], any = false;
+          const float4* rec =
+              reinterpret_cast<const float4*>(reinterpret_cast<const char*>(&s_rec[0][0]) + addr[u]);
+          const float4 A = rec[0], B = rec[kB];
+          Cc[u] = rec[2 * kB];
 #pragma unroll
-            for (uint32_t q = 0; q < kP; ++q) {
-              take[q] = coverage(A, B, fx[q], fy[q], T[q], e[q]);
-              any |= take[q];
-            }
-            // A record no pixel of the wave takes leaves the wave's pixels exactly as they are:
-            // skip its blend (a wave-uniform branch).
-            if (__ballot(any) == 0) continue;
-            const float4 Cc = s_c[k];
+          for (uint32_t q = 0; q < kP; ++q) hit[u][q] = shape_test(A, B, fx[q], fy[q], e[u][q]) && i + u < n;
+        }
 #pragma unroll
-            for (uint32_t q = 0; q < kP; ++q) blend(take[q], e[q], Cc, T[q], C0[q], C1[q], C2[q]);
+        for (uint32_t u = 0; u < 4; ++u) {
+          bool take[kP], any = false;
+#pragma unroll
+          for (uint32_t q = 0; q < kP; ++q) {
+            take[q] = hit[u][q] && T[q] >= kTransmittanceStop;
+            any |= take[q];
           }
+          // A record no pixel of the wave takes leaves the wave's pixels exactly as they are: skip
+          // its blend (a wave-uniform branch).
+          if (__ballot(any) == 0) continue;
+#pragma unroll
+          for (uint32_t q = 0; q < kP; ++q) blend(take[q], e[u][q], Cc[u], T[q], C0[q], C1[q], C2[q]);
         }
         if (__ballot(live()) == 0) break;
       }

@@ -394,6 +394,24 @@ def test_raster_region_lists_drop_no_accepted_pixel(oracle, skm_mod, region, w, 
     assert accepted > 0 and missed == 0
 
 
+@pytest.mark.parametrize("w,h,seed", [(3840, 2160, 24), (7680, 4320, 25)])
+def test_region_and_tile_lists_at_large_pixel_coordinates(oracle, skm_mod, w, h, seed):
+    """The coverage rule is affine in the pixel centre (gu = g00 fx + h01 fy + u0), so its fp32
+    rounding grows with the pixel coordinates (about 3 W 2^-24 (|g00| + |g01|) at width W). At 4K and
+    8K the region test's margins (the 1/16-pixel pad on its half span, then the slack) and the tile
+    rectangles' pad still keep every accepted (pixel, record)."""
+    cloud = skm_mod.make_cloud(400, 1, seed=seed, layout=0, spread=0.8, log_scale_mean=-4.5)
+    cloud["alphas"][::4] = np.float32(2.0)
+    eye = np.array([0.2, 0.3, 3.5], np.float32)
+    o = oracle.frame(cloud, skm_mod.look_at(eye), skm_mod.perspective(math.pi / 3, w / h, 0.2, 100.0), eye, w, h,
+                     image=False)
+    for region in (8, 16):
+        missed, accepted = oracle.region_misses(o["projected"], o["count"], w, h, region)
+        assert accepted > 0 and missed == 0
+    missed, accepted = oracle.missed_coverage(o["projected"], o["count"], o["pair_values"], o["ranges"], w, h)
+    assert accepted > 0 and missed == 0
+
+
 def test_raster_against_numpy_brute_force(oracle, skm_mod):
     """An independent FP64 composite in numpy over all survivors in depth order (exact 2x2 solve of
     the half axes, alpha = min(1, a0 exp(-q/2)), cutoff 1/255) matches the oracle within 1/255."""
