@@ -570,8 +570,8 @@ constexpr uint32_t kLookback = 16;
 // where each of its digits starts in the output:
 //   LOOKBACK (onesweep): tiles take tickets in order and look back across predecessors' per-digit
 //     status words (decoupled look-back), one kernel per pass;
-//   !LOOKBACK (reduce then scan): k_tile_counts and k_digit_scan have already written the start of
-//     digit d of tile t at offsets[d * stride + t].
+//   !LOOKBACK (reduce then scan): k_tile_counts and k_digit_scan have already written the offset of
+//     digit d of tile t within digit d at offsets[d * stride + t] and digit d's total at hist[d].
 template <bool LOOKBACK, int BITS>
 __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ keys_in,
                                                   const uint32_t* __restrict__ vals_in,
@@ -674,7 +674,10 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
     }
     s_base[tid] = gstart + excl - start;
   } else {
-    s_base[tid] = offsets[static_cast<size_t>(tid) * stride + b] - start;
+    // Global start of each digit from the digit totals (zero for digits above the mask).
+    uint32_t all;
+    const uint32_t gstart = block_exclusive_scan(tid <= mask ? hist[tid] : 0u, s_scan, &all);
+    s_base[tid] = gstart + offsets[static_cast<size_t>(tid) * stride + b] - start;
   }
   __syncthreads();
   // Stage the tile in digit order in LDS.
@@ -700,6 +703,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
 
 // Reduce then scan, step 1: the digit counts of each 4096-key tile, digit-major
 // (counts[d * stride + tile]), from per-wave LDS histograms.
+// Per-tile digit counts of one pass (reduce then scan): counts[d * stride + tile].
 __global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
                                                      uint32_t shift, uint32_t mask, uint32_t* __restrict__ counts,
                                                      uint32_t stride) {
@@ -722,19 +726,16 @@ __global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict_
 
 // Reduce then scan, step 2: one block per digit turns its row of tile counts into output starts:
 // the digit's global start (the pass histogram's exclusive scan) plus the exclusive scan over tiles.
+// Block d scans digit d's row of tile counts in place (exclusive, from 0) and stores the row's total
+// in totals[d]; the scatter adds the digit's start, the exclusive scan of the totals.
 __global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ counts, const uint32_t* n_ptr,
-                                                    const uint32_t* __restrict__ hist, uint32_t stride) {
+                                                    uint32_t* __restrict__ totals, uint32_t stride) {
   constexpr uint32_t kPer = 8;
   __shared__ uint32_t s_scan[4];
   const uint32_t tid = threadIdx.x, d = blockIdx.x;
   const uint32_t n = *n_ptr;
   const uint32_t tiles = (n + kSortTile - 1u) / kSortTile;
-  uint32_t total;
-  const uint32_t below = block_exclusive_scan(hist[tid], s_scan, &total);
-  __shared__ uint32_t s_start;
-  if (tid == d) s_start = below;
-  __syncthreads();
-  uint32_t carry = s_start;
+  uint32_t carry = 0;
   uint32_t* row = counts + static_cast<size_t>(d) * stride;
   for (uint32_t t0 = 0; t0 < tiles; t0 += 256u * kPer) {
     uint32_t v[kPer], sum = 0;
@@ -754,6 +755,7 @@ __global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ count
     }
     carry += chunk;
   }
+  if (tid == 0) totals[d] = carry;
 }
 
 // Tile pairs of the depth-ordered survivors: counts, a block scan plus look-back for the global
@@ -1142,7 +1144,7 @@ uint32_t hist_grid(uint32_t capacity) { return std::min(std::max(grid_for(capaci
 // holds 256 words per 4096-key tile of `capacity` (reduce then scan only).
 template <bool LOOKBACK, int BITS>
 void sort_pass_bits(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
-                    const uint32_t* n_ptr, uint32_t tiles, const uint32_t* hist, uint32_t shift, uint32_t bits,
+                    const uint32_t* n_ptr, uint32_t tiles, uint32_t* hist, uint32_t shift, uint32_t bits,
                     uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass, uint32_t* counts,
                     hipStream_t s) {
   const uint32_t mask = (1u << bits) - 1u;
@@ -1159,7 +1161,7 @@ void sort_pass_bits(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, ui
 
 template <bool LOOKBACK>
 void sort_pass_variant(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
-                       const uint32_t* n_ptr, uint32_t tiles, const uint32_t* hist, uint32_t shift, uint32_t bits,
+                       const uint32_t* n_ptr, uint32_t tiles, uint32_t* hist, uint32_t shift, uint32_t bits,
                        uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass, uint32_t* counts,
                        hipStream_t s) {
   if (bits <= 6)
@@ -1175,7 +1177,7 @@ void sort_pass_variant(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
 
 // Pass p of a sort under its digit plan.
 void sort_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, const uint32_t* n_ptr,
-               uint32_t capacity, const uint32_t* hist, const DigitPlan& plan, int p, uint64_t* status, Ctrl* ctrl,
+               uint32_t capacity, uint32_t* hist, const DigitPlan& plan, int p, uint64_t* status, Ctrl* ctrl,
                uint32_t ticket, uint32_t tag_pass, uint32_t* counts, int variant, hipStream_t s) {
   const uint32_t tiles = grid_for(capacity, kSortTile);
   if (variant == kSortOnesweep)
@@ -1213,7 +1215,10 @@ hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], cons
   // vals[(passes - 1) & 1].
   hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, stream, ctrl, hist, 4u * 256u, nullptr, 0u, nullptr,
                      0u);
-  hipLaunchKernelGGL(k_hist, dim3(hist_grid(capacity)), dim3(256), 0, stream, keys[0], n_dev, hist, plan);
+  // The reduce-then-scan passes take each digit's total from their own tile counts; the onesweep
+  // passes need every pass's histogram first.
+  if (variant == kSortOnesweep)
+    hipLaunchKernelGGL(k_hist, dim3(hist_grid(capacity)), dim3(256), 0, stream, keys[0], n_dev, hist, plan);
   for (int p = 0; p < passes; ++p) {
     const uint32_t* kin = keys[p & 1];
     uint32_t* kout = (p + 1 < passes) ? keys[(p + 1) & 1] : nullptr;
@@ -1249,8 +1254,9 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
   if (marks) (void)hipEventRecord(marks[2], s);
   // Depth sort of the survivors: cull keys -> keys[0] -> keys[1] -> keys[0] ...
   const uint32_t* visible = &a.ctrl->visible;
-  hipLaunchKernelGGL(k_hist, dim3(hist_grid(a.capacity)), dim3(256), 0, s, a.cull_keys, visible, a.hist,
-                     depth_plan);
+  if (a.sort_variant == kSortOnesweep)
+    hipLaunchKernelGGL(k_hist, dim3(hist_grid(a.capacity)), dim3(256), 0, s, a.cull_keys, visible, a.hist,
+                       depth_plan);
   for (int p = 0; p < depth_passes; ++p) {
     const uint32_t* kin = p == 0 ? a.cull_keys : a.keys[(p - 1) & 1];
     uint32_t* kout = (p + 1 < depth_passes) ? a.keys[p & 1] : nullptr;
@@ -1263,8 +1269,9 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
   hipLaunchKernelGGL(k_bin, dim3(grid_for(a.capacity, 256u * kBinItems)), dim3(256), 0, s, a, order);
   const uint32_t* sort_n = &a.ctrl->sort_pairs;
   uint32_t* tile_hist = a.hist + 4 * 256;
-  hipLaunchKernelGGL(k_hist, dim3(hist_grid(a.pair_capacity)), dim3(256), 0, s, a.pair_keys[0], sort_n, tile_hist,
-                     tile_plan);
+  if (a.sort_variant == kSortOnesweep)
+    hipLaunchKernelGGL(k_hist, dim3(hist_grid(a.pair_capacity)), dim3(256), 0, s, a.pair_keys[0], sort_n,
+                       tile_hist, tile_plan);
   for (int p = 0; p < tile_passes; ++p) {
     sort_pass(a.pair_keys[p & 1], a.pair_vals[p & 1], a.pair_keys[(p + 1) & 1], a.pair_vals[(p + 1) & 1],
               sort_n, a.pair_capacity, tile_hist + p * 256, tile_plan, p, a.sort_status, a.ctrl, kTicketTileSort + p,

@@ -76,9 +76,14 @@ struct WorldBuffers {
   }
 };
 
+// A frame's inputs travel in one copy: the camera at offset 0, the ranges at kInputRanges, their
+// prefix starts right after the ranges.
+constexpr size_t kInputRanges = 256;
+constexpr size_t kInputBytes = kInputRanges + size_t{skm::kMaxRanges} * 8 + size_t{skm::kMaxRanges + 1} * 4 + 16;
+
 struct Slot {
-  DeviceBuffer cam, ranges, starts, image;
-  PinnedBuffer cam_host, ranges_host, starts_host, ctrl_host, lod_host;
+  DeviceBuffer inputs, image;
+  PinnedBuffer inputs_host, ctrl_host, lod_host;
   hipEvent_t marks[6] = {};
   hipEvent_t done = nullptr;
   bool in_flight = false;
@@ -491,12 +496,7 @@ skm_status skm_renderer_create(const skm_config* config, skm_renderer** out) {
             hipMemset(r->ctrl.ptr, 0, sizeof(Ctrl)) == hipSuccess &&
             r->hist.allocate(2 * 4 * 256 * 4) == hipSuccess;
   for (Slot& s : r->slots) {
-    ok = ok && s.cam.allocate(sizeof(Camera)) == hipSuccess &&
-         s.ranges.allocate(size_t{skm::kMaxRanges} * 8) == hipSuccess &&
-         s.starts.allocate(size_t{skm::kMaxRanges + 1} * 4) == hipSuccess &&
-         s.cam_host.allocate(sizeof(Camera)) == hipSuccess &&
-         s.ranges_host.allocate(size_t{skm::kMaxRanges} * 8) == hipSuccess &&
-         s.starts_host.allocate(size_t{skm::kMaxRanges + 1} * 4) == hipSuccess &&
+    ok = ok && s.inputs.allocate(kInputBytes) == hipSuccess && s.inputs_host.allocate(kInputBytes) == hipSuccess &&
          s.ctrl_host.allocate(sizeof(Ctrl)) == hipSuccess &&
          s.lod_host.allocate(sizeof(skm::LodState) + 16) == hipSuccess &&
          hipEventCreate(&s.done) == hipSuccess;
@@ -789,37 +789,36 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   }
   if (auto st = r->maybe_clear_status(); st != SKM_OK) return st;
 
-  Camera& cam = *s.cam_host.as<Camera>();
+  char* inputs = s.inputs_host.as<char>();
+  Camera& cam = *reinterpret_cast<Camera*>(inputs);
   fill_camera(cam, f->view, f->proj, W, H, r->linear, r->policy.subpixel, f->camera_position, &f->reveal);
 
-  uint32_t* rh = s.ranges_host.as<uint32_t>();
-  uint32_t* sh = s.starts_host.as<uint32_t>();
+  const size_t starts_at = kInputRanges + align_up(size_t{f->range_count} * 8, 16);
+  uint32_t* rh = reinterpret_cast<uint32_t*>(inputs + kInputRanges);
+  uint32_t* sh = reinterpret_cast<uint32_t*>(inputs + starts_at);
   sh[0] = 0;
   for (uint32_t i = 0; i < f->range_count; ++i) {
     rh[2 * i] = f->ranges[i].offset;
     rh[2 * i + 1] = f->ranges[i].count;
     sh[i + 1] = sh[i] + f->ranges[i].count;
   }
-  hipError_t e = hipMemcpyAsync(s.cam.ptr, &cam, sizeof(Camera), hipMemcpyHostToDevice, r->stream);
-  if (e == hipSuccess && f->range_count > 0) {
-    e = hipMemcpyAsync(s.ranges.ptr, rh, size_t{f->range_count} * 8, hipMemcpyHostToDevice, r->stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(s.starts.ptr, sh, size_t{f->range_count + 1} * 4, hipMemcpyHostToDevice, r->stream);
-  }
+  const size_t input_bytes = f->range_count > 0 ? starts_at + size_t{f->range_count + 1} * 4 : sizeof(Camera);
+  hipError_t e = hipMemcpyAsync(s.inputs.ptr, inputs, input_bytes, hipMemcpyHostToDevice, r->stream);
   if (auto st = r->check(e, "frame inputs"); st != SKM_OK) return st;
+  const Camera* cam_dev = s.inputs.as<Camera>();
 
   const bool world = r->has_world;
   const bool lod_frame = world && r->lod_world;
   skm::LodArgs lod{};
   if (lod_frame) {
     s.lod_limit = r->policy.lod_splat_limit == 0 ? r->lod.capacity : std::min(r->policy.lod_splat_limit, r->lod.capacity);
-    lod = r->lod.args(s.cam.as<Camera>(), s.lod_limit, r->threshold.pixels, 4.0f, 1);
+    lod = r->lod.args(cam_dev, s.lod_limit, r->threshold.pixels, 4.0f, 1);
   }
   skm::FrameArgs a{};
-  a.cam = s.cam.as<Camera>();
+  a.cam = cam_dev;
   a.world = world ? r->world.view() : skm::World{};
-  a.ranges = s.ranges.as<uint32_t>();
-  a.range_starts = s.starts.as<uint32_t>();
+  a.ranges = reinterpret_cast<const uint32_t*>(s.inputs.as<char>() + kInputRanges);
+  a.range_starts = reinterpret_cast<const uint32_t*>(s.inputs.as<char>() + starts_at);
   a.range_count = world && !lod_frame ? f->range_count : 0;
   a.total = world && !lod_frame ? static_cast<uint32_t>(total) : 0;
   a.active = lod_frame ? lod.indices : nullptr;
