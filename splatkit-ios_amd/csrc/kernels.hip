@@ -758,23 +758,141 @@ __global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ count
   if (tid == 0) totals[d] = carry;
 }
 
-// Tile pairs of the depth-ordered survivors: counts, a block scan plus look-back for the global
-// offset, then the block emits its pairs cooperatively (pair e of the block belongs to the last
-// survivor whose exclusive offset is <= e), so writes are contiguous whatever the footprints.
-// 1024 survivors per block: position i = tile * 1024 + k * 256 + thread.
+// Tile pairs of a packed rectangle (k_cull's pack_rect): none for kNoRect.
+__device__ __forceinline__ uint32_t rect_pairs(uint64_t rc) {
+  if (rc == kNoRect) return 0u;
+  const uint32_t w = static_cast<uint32_t>((rc >> 24) & 4095u) - static_cast<uint32_t>(rc & 4095u) + 1u;
+  const uint32_t h = static_cast<uint32_t>((rc >> 36) & 4095u) - static_cast<uint32_t>((rc >> 12) & 4095u) + 1u;
+  return w * h;
+}
+
+__device__ __forceinline__ uint64_t wave_inclusive_scan64(uint64_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t lo = __shfl_up(static_cast<uint32_t>(v), d, 64);
+    const uint32_t hi = __shfl_up(static_cast<uint32_t>(v >> 32), d, 64);
+    if (lane >= d) v += (static_cast<uint64_t>(hi) << 32) | lo;
+  }
+  return v;
+}
+
+// Binning, first pass (reduce then scan) over the survivors in depth order, kBinItems * 256 per
+// block: a block gathers its survivors' tile rectangles into depth order (low words in keys[0], high
+// words in keys[1]: the depth sort is done with both) and stores its pair total in sort_counts[b].
+__global__ __launch_bounds__(256) void k_bin_count(FrameArgs a, const uint32_t* __restrict__ order) {
+  constexpr uint32_t kTile = 256u * kBinItems;
+  __shared__ uint32_t s_scan[4];
+  const uint32_t tid = threadIdx.x, b = blockIdx.x;
+  const uint32_t n = a.ctrl->visible;
+  const uint32_t blocks = (n + kTile - 1u) / kTile;
+  if (b >= blocks) return;
+  // All loads first (indices clamped to the last survivor), so a thread waits for two round trips
+  // rather than two per item.
+  uint32_t value[kBinItems];
+  uint64_t rc[kBinItems];
+#pragma unroll
+  for (uint32_t k = 0; k < kBinItems; ++k) value[k] = order[min(b * kTile + k * 256u + tid, n - 1u)];
+#pragma unroll
+  for (uint32_t k = 0; k < kBinItems; ++k) rc[k] = a.rects[value[k]];
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kBinItems; ++k) {
+    const uint32_t i = b * kTile + k * 256u + tid;
+    if (i < n) {
+      a.keys[0][i] = static_cast<uint32_t>(rc[k]);
+      a.keys[1][i] = static_cast<uint32_t>(rc[k] >> 32);
+      sum += rect_pairs(rc[k]);
+    }
+  }
+  uint32_t total;
+  (void)block_exclusive_scan(sum, s_scan, &total);
+  if (tid == 0) a.sort_counts[b] = total;
+}
+
+// Binning, the scan: one block turns the block totals into block starts in place (exclusive,
+// saturating at 2^32 - 1) and publishes the frame's pair count.
+__global__ __launch_bounds__(256) void k_bin_scan(FrameArgs a) {
+  constexpr uint32_t kTile = 256u * kBinItems, kPer = 8;
+  __shared__ uint64_t s_wave[4];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6;
+  const uint32_t n = a.ctrl->visible;
+  const uint32_t blocks = (n + kTile - 1u) / kTile;
+  uint64_t carry = 0;
+  for (uint32_t t0 = 0; t0 < blocks; t0 += 256u * kPer) {
+    uint32_t v[kPer];
+    uint64_t sum = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+      const uint32_t t = t0 + tid * kPer + i;
+      v[i] = t < blocks ? a.sort_counts[t] : 0u;
+      sum += v[i];
+    }
+    const uint64_t incl = wave_inclusive_scan64(sum);
+    if (lane_id() == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint64_t before = 0, all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) {
+      before += w < wave ? s_wave[w] : 0ull;
+      all += s_wave[w];
+    }
+    __syncthreads();
+    uint64_t run = carry + before + incl - sum;
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+      const uint32_t t = t0 + tid * kPer + i;
+      if (t < blocks) a.sort_counts[t] = run > 0xffffffffull ? 0xffffffffu : static_cast<uint32_t>(run);
+      run += v[i];
+    }
+    carry += all;
+  }
+  if (tid == 0) {
+    const uint32_t pairs = carry > 0xffffffffull ? 0xffffffffu : static_cast<uint32_t>(carry);
+    const bool over = carry > a.pair_capacity;
+    a.ctrl->pairs = pairs;
+    a.ctrl->overflow = over ? 1u : 0u;
+    // The tile sort's count: zero after an overflow (the frame is dropped).
+    a.ctrl->sort_pairs = over ? 0u : pairs;
+  }
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(v, d, 64);
+    if (lane >= d) v = max(v, t);
+  }
+  return v;
+}
+
+// Binning, second pass: a block reads its survivors' rectangles (in depth order) and values, scans
+// their pair counts into slot offsets, and writes its pairs from its block start. Pair e of the block
+// belongs to the last slot whose offset is <= e, and its tile is that slot's rectangle cell
+// k = e - offset, row-major, so pairs come out in depth order, survivor by survivor. Each wave takes a
+// contiguous run of 64-pair chunks: the slots of a chunk are a window of 64 slots from the previous
+// chunk's last owner, whose starts are marked in LDS and spread by a wave max-scan (a binary search
+// covers the first chunk, and any pair the window misses).
 __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __restrict__ order) {
   constexpr uint32_t kItems = kBinItems;
   constexpr uint32_t kTile = 256u * kItems;
-  __shared__ uint32_t s_ticket, s_prefix;
   __shared__ uint32_t s_scan[4];
-  __shared__ uint32_t s_excl[kTile], s_val[kTile], s_origin[kTile], s_width[kTile], s_base[kTile];
-  const uint32_t tid = threadIdx.x;
+  __shared__ uint32_t s_excl[kTile + 1u], s_val[kTile], s_origin[kTile], s_width[kTile], s_base[kTile];
+  __shared__ uint32_t s_mark[4][64];
+  const uint32_t tid = threadIdx.x, b = blockIdx.x, wave = tid >> 6, lane = tid & 63u;
   const uint32_t n = a.ctrl->visible;
   const uint32_t blocks = (n + kTile - 1u) / kTile;
-  if (blockIdx.x >= blocks) return;
-  if (tid == 0) s_ticket = atomicAdd(&a.ctrl->ticket[kTicketBin], 1u);
-  __syncthreads();
-  const uint32_t b = s_ticket;
+  if (b >= blocks) return;
+  s_mark[wave][lane] = 0u;
+  uint32_t values[kItems], lo_words[kItems], hi_words[kItems];
+#pragma unroll
+  for (uint32_t k = 0; k < kItems; ++k) {
+    const uint32_t i = min(b * kTile + k * 256u + tid, n - 1u);
+    values[k] = order[i];
+    lo_words[k] = a.keys[0][i];
+    hi_words[k] = a.keys[1][i];
+  }
   uint32_t aggregate = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kItems; ++k) {
@@ -783,8 +901,8 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
     uint32_t count = 0, value = 0, base = 0, tiles_x = a.tiles_x;
     Rect r{0, 0, -1, -1};
     if (i < n) {
-      value = order[i];
-      const uint64_t rc = a.rects[value];
+      value = values[k];
+      const uint64_t rc = static_cast<uint64_t>(lo_words[k]) | (static_cast<uint64_t>(hi_words[k]) << 32);
       if (rc != kNoRect) {
         r.tx0 = static_cast<int32_t>(rc & 4095u);
         r.ty0 = static_cast<int32_t>((rc >> 12) & 4095u);
@@ -807,43 +925,55 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
     s_base[slot] = base;
     aggregate += row_total;
   }
-  if ((tid >> 6) == 0) {
-    const uint32_t tag = make_tag(a.ctrl->epoch, 1);
-    uint32_t prefix = 0;
-    if (b == 0) {
-      if (lane_id() == 0) status_store(a.scan_status, status_word(tag, kFlagPrefix, aggregate));
-    } else {
-      if (lane_id() == 0) status_store(a.scan_status + b, status_word(tag, kFlagAggregate, aggregate));
-      prefix = wave_lookback(a.scan_status, b, tag);
-      if (lane_id() == 0)
-        status_store(a.scan_status + b, status_word(tag, kFlagPrefix, sat_add(prefix, aggregate)));
-    }
-    if (lane_id() == 0) s_prefix = prefix;
-  }
+  if (tid == 0) s_excl[kTile] = aggregate;
   __syncthreads();
-  const uint64_t prefix = s_prefix;
-  for (uint32_t e = tid; e < aggregate; e += 256u) {
+  const uint64_t prefix = a.sort_counts[b];
+  auto find = [&](uint32_t e) {  // last slot with s_excl[slot] <= e
     uint32_t lo = 0, hi = kTile - 1u;
-    while (lo < hi) {  // last slot with s_excl[slot] <= e
+    while (lo < hi) {
       const uint32_t mid = (lo + hi + 1u) / 2u;
       if (s_excl[mid] <= e) lo = mid; else hi = mid - 1u;
     }
-    const uint32_t k = e - s_excl[lo];
-    const uint32_t wdt = s_width[lo] & 0xffffu, stride = s_width[lo] >> 16;
-    const uint32_t tx = (s_origin[lo] & 0xffffu) + k % wdt;
-    const uint32_t ty = (s_origin[lo] >> 16) + k / wdt;
-    const uint64_t idx = prefix + e;
-    if (idx < a.pair_capacity) {
-      a.pair_keys[0][idx] = s_base[lo] + ty * stride + tx;
-      a.pair_vals[0][idx] = s_val[lo];
+    return lo;
+  };
+  const uint32_t chunks = (aggregate + 63u) / 64u;
+  const uint32_t c0 = chunks * wave / 4u, c1 = chunks * (wave + 1u) / 4u;
+  uint32_t s0 = c0 < c1 ? find(c0 * 64u) : 0u;
+  for (uint32_t c = c0; c < c1; ++c) {
+    const uint32_t first = c * 64u;
+    const uint32_t s = s0 + lane;
+    if (s < kTile) {
+      const uint32_t ex = s_excl[s], cnt = s_excl[s + 1u] - ex;
+      if (cnt != 0u && ex < first + 64u && ex + cnt > first) s_mark[wave][ex > first ? ex - first : 0u] = s + 1u;
     }
-  }
-  if (b == blocks - 1u && tid == 0) {
-    const uint32_t total = sat_add(s_prefix, aggregate);
-    a.ctrl->pairs = total;
-    a.ctrl->overflow = total > a.pair_capacity ? 1u : 0u;
-    // The tile sort's count: zero after an overflow (the frame is dropped).
-    a.ctrl->sort_pairs = total > a.pair_capacity ? 0u : total;
+    const uint32_t m = wave_inclusive_max(s_mark[wave][lane]);
+    s_mark[wave][lane] = 0u;
+    const uint32_t e = first + lane;
+    uint32_t owner = m != 0u ? m - 1u : 0u;
+    if (e < aggregate) {
+      if (m == 0u || e < s_excl[owner] || e >= s_excl[owner + 1u]) owner = find(e);
+      const uint32_t k = e - s_excl[owner];
+      const uint32_t wdt = s_width[owner] & 0xffffu, stride = s_width[owner] >> 16;
+      // k / wdt from a reciprocal estimate, corrected to the exact quotient.
+      uint32_t q = static_cast<uint32_t>(static_cast<float>(k) * __builtin_amdgcn_rcpf(static_cast<float>(wdt)));
+      int32_t rem = static_cast<int32_t>(k - q * wdt);
+      while (rem < 0) {
+        --q;
+        rem += static_cast<int32_t>(wdt);
+      }
+      while (rem >= static_cast<int32_t>(wdt)) {
+        ++q;
+        rem -= static_cast<int32_t>(wdt);
+      }
+      const uint32_t tx = (s_origin[owner] & 0xffffu) + static_cast<uint32_t>(rem);
+      const uint32_t ty = (s_origin[owner] >> 16) + q;
+      const uint64_t idx = prefix + e;
+      if (idx < a.pair_capacity) {
+        a.pair_keys[0][idx] = s_base[owner] + ty * stride + tx;
+        a.pair_vals[0][idx] = s_val[owner];
+      }
+    }
+    s0 = __shfl(owner, 63, 64);
   }
 }
 
@@ -1266,6 +1396,8 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
   }
   const uint32_t* order = a.order[(depth_passes - 1) & 1];
   if (marks) (void)hipEventRecord(marks[3], s);
+  hipLaunchKernelGGL(k_bin_count, dim3(grid_for(a.capacity, 256u * kBinItems)), dim3(256), 0, s, a, order);
+  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_bin, dim3(grid_for(a.capacity, 256u * kBinItems)), dim3(256), 0, s, a, order);
   const uint32_t* sort_n = &a.ctrl->sort_pairs;
   uint32_t* tile_hist = a.hist + 4 * 256;
