@@ -43,7 +43,7 @@ hipError_t Batch::ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_
   };
   if (ctrl_.ptr == nullptr) {
     need(ctrl_, sizeof(Ctrl));
-    need(hist_, 2 * 4 * 256 * 4);
+    need(hist_, kHistWords * 4);
     if (e == hipSuccess) e = hipMemset(ctrl_.ptr, 0, sizeof(Ctrl));
   }
   if (views > view_capacity_) {
@@ -73,7 +73,7 @@ hipError_t Batch::ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_
     if (e == hipSuccess) pair_capacity_ = pairs;
   }
   if (tiles > tile_capacity_) {
-    need(tile_ranges_, size_t{tiles} * 8);
+    need(tile_ranges_, size_t{tiles} * 12);  // ranges, then the raster's tile order
     need(tile_view_, size_t{tiles} * 2);
     if (e == hipSuccess) tile_capacity_ = tiles;
   }
@@ -241,6 +241,7 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
   a.hist = hist_.as<uint32_t>();
   a.pair_capacity = static_cast<uint32_t>(pair_capacity_);
   a.tile_ranges = tile_ranges_.as<uint32_t>();
+  a.tile_order = tile_ranges_.as<uint32_t>() + size_t{tile_capacity_} * 2;
   a.image = image_.as<float>();
   a.capacity = static_cast<uint32_t>(lane_capacity_);
   a.view_count = count;
@@ -345,7 +346,7 @@ uint64_t Batch::bytes_for(uint32_t views, uint64_t lanes, uint32_t tiles, uint64
   pixels = std::max(pixels, pixel_capacity_);
   views = std::max(views, view_capacity_);
   const uint64_t status = (lanes / 1024 + 2) * 8 + (std::max(lanes, pairs) / kSortTile + 2) * kRadix * 12;
-  return lanes * (5 * 4 + 32 + 8 + 2) + pairs * 16 + uint64_t{tiles} * 10 + pixels * 16 +
+  return lanes * (5 * 4 + 32 + 8 + 2) + pairs * 16 + uint64_t{tiles} * 14 + pixels * 16 +
          uint64_t{views} * (sizeof(Camera) + sizeof(ViewInfo) + 4) + status + (1u << 20);
 }
 

@@ -987,6 +987,45 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ key
   }
 }
 
+// The raster's tile order, heaviest first: a tile's bucket is the count of leading zero bits of its
+// pairs + 1, so bucket 0 holds the tiles with the most pairs and bucket 31 the empty ones. Counts per
+// bucket go to counters[0, 32), then each tile takes a slot in its bucket's range through the
+// cursors at counters[32, 64) (the order within a bucket does not matter: a tile's pixels do not
+// depend on when it runs).
+__device__ __forceinline__ uint32_t order_bucket(const uint32_t* ranges, uint32_t t) {
+  return static_cast<uint32_t>(__clz(static_cast<int>(ranges[2 * t + 1] - ranges[2 * t] + 1u)));
+}
+__global__ __launch_bounds__(256) void k_order_count(const uint32_t* __restrict__ ranges, uint32_t tiles,
+                                                     uint32_t* __restrict__ counters) {
+  __shared__ uint32_t s_count[32];
+  const uint32_t tid = threadIdx.x, t = blockIdx.x * 256u + tid;
+  if (tid < 32u) s_count[tid] = 0u;
+  __syncthreads();
+  if (t < tiles) atomicAdd(&s_count[order_bucket(ranges, t)], 1u);
+  __syncthreads();
+  if (tid < 32u && s_count[tid] != 0u) atomicAdd(&counters[tid], s_count[tid]);
+}
+__global__ __launch_bounds__(256) void k_order_scatter(const uint32_t* __restrict__ ranges, uint32_t tiles,
+                                                       uint32_t* __restrict__ counters, uint32_t* __restrict__ order) {
+  __shared__ uint32_t s_start[32], s_count[32], s_base[32];
+  const uint32_t tid = threadIdx.x, t = blockIdx.x * 256u + tid;
+  if (tid < 64u) {
+    const uint32_t v = tid < 32u ? counters[tid] : 0u;
+    const uint32_t incl = wave_inclusive_scan(v);
+    if (tid < 32u) {
+      s_start[tid] = incl - v;
+      s_count[tid] = 0u;
+    }
+  }
+  __syncthreads();
+  const uint32_t bucket = t < tiles ? order_bucket(ranges, t) : 0u;
+  const uint32_t local = t < tiles ? atomicAdd(&s_count[bucket], 1u) : 0u;
+  __syncthreads();
+  if (tid < 32u && s_count[tid] != 0u) s_base[tid] = s_start[tid] + atomicAdd(&counters[32u + tid], s_count[tid]);
+  __syncthreads();
+  if (t < tiles) order[s_base[bucket] + local] = t;
+}
+
 // One coverage-and-blend step of a pixel against a prepared record (the oracle's decide and fp32
 // composite): the scaled quad coordinates are affine in the pixel centre, gu = g00 fx + h01 fy + u0
 // (two fused multiply-adds per coordinate), log2 alpha = log2a0 - gu^2 - gv^2 (two more), the quad
@@ -1049,7 +1088,7 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
   __shared__ __align__(16) uint32_t s_list[kW][kB];
   __shared__ uint32_t s_count[kW][kW];  // [region][wave]
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
-  const uint32_t tile_id = blockIdx.x;  // global tile id
+  const uint32_t tile_id = a.tile_order != nullptr ? a.tile_order[blockIdx.x] : blockIdx.x;  // global tile id
   uint32_t width = a.width, height = a.height, tiles_x = a.tiles_x, local = tile_id;
   uint64_t pixel_base = 0;
   if (a.tile_view != nullptr) {
@@ -1370,7 +1409,7 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
   }
   if (marks) (void)hipEventRecord(marks[1], s);
   hipLaunchKernelGGL(k_prologue, dim3(min(grid_for(tiles * 2u, 256), 256u)), dim3(256), 0, s, a.ctrl,
-                     a.hist, 2u * 4u * 256u, a.tile_ranges, tiles * 2u, a.view_visible,
+                     a.hist, kHistWords, a.tile_ranges, tiles * 2u, a.view_visible,
                      a.view_visible != nullptr ? a.view_count : 0u);
   const uint32_t view_lanes = grid_for(a.total, 256u * kCullItems) * 256u * kCullItems;
   const uint32_t lanes = a.active != nullptr ? a.capacity : view_lanes * a.view_count;
@@ -1412,6 +1451,12 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
   const int fin = tile_passes & 1;
   hipLaunchKernelGGL(k_ranges, dim3(min(grid_for(a.pair_capacity, 256), 2048u)), dim3(256), 0, s,
                      a.pair_keys[fin], sort_n, a.tile_ranges);
+  if (a.tile_order != nullptr) {
+    uint32_t* counters = a.hist + kOrderCounters;
+    hipLaunchKernelGGL(k_order_count, dim3(grid_for(tiles, 256)), dim3(256), 0, s, a.tile_ranges, tiles, counters);
+    hipLaunchKernelGGL(k_order_scatter, dim3(grid_for(tiles, 256)), dim3(256), 0, s, a.tile_ranges, tiles, counters,
+                       a.tile_order);
+  }
   if (marks) (void)hipEventRecord(marks[4], s);
   if (const hipError_t e = launch_raster(a, a.pair_vals[fin], tiles, s); e != hipSuccess) return e;
   if (marks) (void)hipEventRecord(marks[5], s);

@@ -401,7 +401,8 @@ struct skm_renderer {
     if (tiles <= tile_capacity) return SKM_OK;
     if (auto st = drain(); st != SKM_OK) return st;  // a frame in flight may still read the ranges
     DeviceBuffer t;
-    if (auto st = check(t.allocate(size_t{tiles} * 8u), "tile ranges"); st != SKM_OK) return st;
+    // Ranges (2 words per tile), then the raster's tile order (1 word per tile).
+    if (auto st = check(t.allocate(size_t{tiles} * 12u), "tile ranges"); st != SKM_OK) return st;
     tile_ranges = std::move(t);
     tile_capacity = tiles;
     return SKM_OK;
@@ -494,7 +495,7 @@ skm_status skm_renderer_create(const skm_config* config, skm_renderer** out) {
   r->description = desc;
   bool ok = r->ctrl.allocate(sizeof(Ctrl)) == hipSuccess &&
             hipMemset(r->ctrl.ptr, 0, sizeof(Ctrl)) == hipSuccess &&
-            r->hist.allocate(2 * 4 * 256 * 4) == hipSuccess;
+            r->hist.allocate(skm::kHistWords * 4) == hipSuccess;
   for (Slot& s : r->slots) {
     ok = ok && s.inputs.allocate(kInputBytes) == hipSuccess && s.inputs_host.allocate(kInputBytes) == hipSuccess &&
          s.ctrl_host.allocate(sizeof(Ctrl)) == hipSuccess &&
@@ -845,6 +846,7 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   a.hist = r->hist.as<uint32_t>();
   a.pair_capacity = r->pair_capacity;
   a.tile_ranges = r->tile_ranges.as<uint32_t>();
+  a.tile_order = r->tile_ranges.as<uint32_t>() + size_t{r->tile_capacity} * 2;
   a.image = s.image.as<float>();
   a.capacity = r->capacity;
   a.view_count = 1;

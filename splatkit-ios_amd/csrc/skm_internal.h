@@ -123,6 +123,8 @@ constexpr uint32_t kSortItems = SKM_SORT_ITEMS;
 constexpr uint32_t kSortTile = kThreads * kSortItems;  // keys per radix-pass block (4096 at 16)
 constexpr uint32_t kRadix = 256;
 constexpr uint32_t kMaxRanges = 65536;  // MetalVisibilityPass.h:72
+constexpr uint32_t kOrderCounters = 2u * 4u * 256u;  // offset of the tile-order counters in hist
+constexpr uint32_t kHistWords = kOrderCounters + 64u;
 constexpr float kTransmittanceStop = 1.0f / 4096.0f;
 
 // One view of a batch: its extent, its tiles' place in the batch's global tile ids, and its first
@@ -161,6 +163,10 @@ struct FrameArgs {
   uint32_t* pair_vals[2];
   uint32_t pair_capacity;
   uint32_t* tile_ranges;  // 2 words per tile
+  // The raster's launch order of the tiles (one word per tile): the tiles with the most pairs first,
+  // so the longest blocks start earliest. Built per frame from the ranges, with 64 counter words
+  // at hist + kOrderCounters. Null: tile order.
+  uint32_t* tile_order;
   float* image;           // RGBA float, top-down rows
   uint32_t capacity;      // survivor capacity
   // Batches (view_count > 1): one camera per view at cam[v]; every view culls the same lanes
