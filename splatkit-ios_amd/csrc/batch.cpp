@@ -61,7 +61,6 @@ hipError_t Batch::ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_
     need(order_[1], n * 4);
     need(projected_, n * 32);
     need(rects_, n * 8);
-    need(survivor_view_, n * 2);
     if (e == hipSuccess) lane_capacity_ = lanes;
   }
   if (pairs > pair_capacity_) {
@@ -86,8 +85,8 @@ hipError_t Batch::ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_
     need(starts_, (std::max<size_t>(ranges, 1) + 1) * 4);
     if (e == hipSuccess) range_capacity_ = std::max(ranges, 1u);
   }
-  // Look-back status words: one per 1024-lane tile (cull, binning); 256 per 4096-key sort tile.
-  const size_t scan = static_cast<size_t>(lane_capacity_ / 1024 + 2);
+  // Look-back status words: one per cull tile and per binning block; 256 per 4096-key sort tile.
+  const size_t scan = static_cast<size_t>(lane_capacity_ / std::min(kCullTile, 256u * kBinItems) + 2);
   if (e == hipSuccess && scan > scan_words_) {
     need(scan_status_, scan * 8);
     if (e == hipSuccess) e = hipMemset(scan_status_.ptr, 0, scan_status_.bytes);
@@ -135,7 +134,7 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
   }
   uint64_t total = 0;
   for (uint32_t i = 0; i < range_count; ++i) total += ranges[i].count;
-  const uint64_t view_lanes = (total + 1023) / 1024 * 1024;
+  const uint64_t view_lanes = (total + kCullTile - 1) / kCullTile * kCullTile;
   const uint64_t lanes = view_lanes * count;
   info_.assign(count, ViewInfo{});
   uint64_t tiles = 0, pixels = 0;
@@ -237,6 +236,7 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
   a.projected = projected_.as<uint32_t>();
   a.rects = rects_.as<uint64_t>();
   a.scan_status = scan_status_.as<uint64_t>();
+  a.cull_counts = reinterpret_cast<uint32_t*>(scan_status_.ptr);
   a.sort_status = sort_status_.as<uint64_t>();
   a.hist = hist_.as<uint32_t>();
   a.pair_capacity = static_cast<uint32_t>(pair_capacity_);
@@ -246,7 +246,6 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
   a.capacity = static_cast<uint32_t>(lane_capacity_);
   a.view_count = count;
   a.views = views_.as<ViewInfo>();
-  a.survivor_view = survivor_view_.as<uint16_t>();
   a.view_visible = view_visible_.as<uint32_t>();
   a.tile_view = tile_view_.as<uint16_t>();
   a.total_tiles = static_cast<uint32_t>(tiles);
@@ -309,7 +308,7 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
     return e;
   }
   in_flight_ = true;
-  sv_ready_ = false;
+  st_ready_ = false;
   return hipSuccess;
 }
 
@@ -331,7 +330,7 @@ hipError_t Batch::wait(bool* dropped) {
 
 uint64_t Batch::device_bytes() const {
   uint64_t b = 0;
-  for (const DeviceBuffer* d : {&ctrl_, &cams_, &views_, &tile_view_, &view_visible_, &survivor_view_, &cull_keys_,
+  for (const DeviceBuffer* d : {&ctrl_, &cams_, &views_, &tile_view_, &view_visible_, &cull_keys_,
                                 &keys_[0], &keys_[1], &order_[0], &order_[1], &projected_, &rects_, &scan_status_,
                                 &sort_status_, &sort_counts_, &hist_, &pair_keys_[0], &pair_keys_[1], &pair_vals_[0],
                                 &pair_vals_[1], &tile_ranges_, &image_, &ranges_, &starts_})
@@ -345,14 +344,15 @@ uint64_t Batch::bytes_for(uint32_t views, uint64_t lanes, uint32_t tiles, uint64
   tiles = std::max(tiles, tile_capacity_);
   pixels = std::max(pixels, pixel_capacity_);
   views = std::max(views, view_capacity_);
-  const uint64_t status = (lanes / 1024 + 2) * 8 + (std::max(lanes, pairs) / kSortTile + 2) * kRadix * 12;
-  return lanes * (5 * 4 + 32 + 8 + 2) + pairs * 16 + uint64_t{tiles} * 14 + pixels * 16 +
+  const uint64_t status = (lanes / std::min(kCullTile, 256u * kBinItems) + 2) * 8 +
+                          (std::max(lanes, pairs) / kSortTile + 2) * kRadix * 12;
+  return lanes * (5 * 4 + 32 + 8) + pairs * 16 + uint64_t{tiles} * 14 + pixels * 16 +
          uint64_t{views} * (sizeof(Camera) + sizeof(ViewInfo) + 4) + status + (1u << 20);
 }
 
 void Batch::release() {
   release_graph();
-  for (DeviceBuffer* d : {&ctrl_, &cams_, &views_, &tile_view_, &view_visible_, &survivor_view_, &cull_keys_,
+  for (DeviceBuffer* d : {&ctrl_, &cams_, &views_, &tile_view_, &view_visible_, &cull_keys_,
                           &keys_[0], &keys_[1], &order_[0], &order_[1], &projected_, &rects_, &scan_status_, &sort_status_,
                           &sort_counts_, &hist_, &pair_keys_[0], &pair_keys_[1], &pair_vals_[0], &pair_vals_[1], &tile_ranges_,
                           &image_, &ranges_, &starts_})
@@ -369,55 +369,62 @@ hipError_t Batch::read_image(uint32_t v, float* rgba) const {
                    hipMemcpyDeviceToHost);
 }
 
-hipError_t Batch::survivor_views() const {
-  if (sv_ready_) return hipSuccess;
-  const uint32_t total = ctrl_host()->visible;
-  sv_view_.resize(total);
-  sv_local_.resize(total);
-  if (total) {
-    if (hipError_t e = hipMemcpy(sv_view_.data(), survivor_view_.ptr, size_t{total} * 2, hipMemcpyDeviceToHost);
+hipError_t Batch::survivor_tiles() const {
+  if (st_ready_) return hipSuccess;
+  const uint32_t blocks = ctrl_host()->cull_lanes / kCullTile;
+  st_count_.assign(blocks, 0u);
+  st_local_.assign(blocks, 0u);
+  if (blocks) {
+    if (hipError_t e = hipMemcpy(st_count_.data(), scan_status_.ptr, size_t{blocks} * 4, hipMemcpyDeviceToHost);
         e != hipSuccess)
       return e;
   }
   std::vector<uint32_t> next(info_.size(), 0);
-  for (uint32_t i = 0; i < total; ++i) sv_local_[i] = next[sv_view_[i]]++;
-  sv_ready_ = true;
+  for (uint32_t b = 0; b < blocks; ++b) {
+    st_local_[b] = next[tile_view_of(b)];
+    next[tile_view_of(b)] += st_count_[b];
+  }
+  st_ready_ = true;
   return hipSuccess;
 }
 
 hipError_t Batch::read_visible(uint32_t v, uint32_t* keys, uint32_t* projected, uint32_t capacity) const {
-  if (hipError_t e = survivor_views(); e != hipSuccess) return e;
-  const uint32_t total = ctrl_host()->visible;
-  constexpr uint32_t kChunk = 1u << 22;
+  if (hipError_t e = survivor_tiles(); e != hipSuccess) return e;
+  // View v's survivors: the segments of its cull tiles, in tile order, read in chunks of tiles.
+  constexpr uint32_t kChunkTiles = (1u << 22) / kCullTile;
+  const uint32_t blocks = static_cast<uint32_t>(st_count_.size());
   std::vector<uint32_t> kc, pc;
   uint32_t k = 0;
-  for (uint32_t c0 = 0; c0 < total && k < capacity; c0 += kChunk) {
-    const uint32_t n = std::min(kChunk, total - c0);
+  for (uint32_t b0 = 0; b0 < blocks && k < capacity; b0 += kChunkTiles) {
+    const uint32_t b1 = std::min(blocks, b0 + kChunkTiles);
     bool any = false;
-    for (uint32_t i = c0; i < c0 + n && !any; ++i) any = sv_view_[i] == v;
+    for (uint32_t b = b0; b < b1 && !any; ++b) any = tile_view_of(b) == v && st_count_[b] != 0u;
     if (!any) continue;
+    const size_t lanes = size_t{b1 - b0} * kCullTile, first = size_t{b0} * kCullTile;
     hipError_t e = hipSuccess;
     if (keys) {
-      kc.resize(n);
-      e = hipMemcpy(kc.data(), cull_keys_.as<uint32_t>() + c0, size_t{n} * 4, hipMemcpyDeviceToHost);
+      kc.resize(lanes);
+      e = hipMemcpy(kc.data(), cull_keys_.as<uint32_t>() + first, lanes * 4, hipMemcpyDeviceToHost);
     }
     if (e == hipSuccess && projected) {
-      pc.resize(size_t{n} * 8);
-      e = hipMemcpy(pc.data(), projected_.as<uint32_t>() + size_t{c0} * 8, size_t{n} * 32, hipMemcpyDeviceToHost);
+      pc.resize(lanes * 8);
+      e = hipMemcpy(pc.data(), projected_.as<uint32_t>() + first * 8, lanes * 32, hipMemcpyDeviceToHost);
     }
     if (e != hipSuccess) return e;
-    for (uint32_t i = 0; i < n && k < capacity; ++i) {
-      if (sv_view_[c0 + i] != v) continue;
-      if (keys) keys[k] = kc[i];
-      if (projected) std::memcpy(projected + size_t{k} * 8, pc.data() + size_t{i} * 8, 32);
-      ++k;
+    for (uint32_t b = b0; b < b1; ++b) {
+      if (tile_view_of(b) != v) continue;
+      for (uint32_t j = 0; j < st_count_[b] && k < capacity; ++j, ++k) {
+        const size_t i = size_t{b - b0} * kCullTile + j;
+        if (keys) keys[k] = kc[i];
+        if (projected) std::memcpy(projected + size_t{k} * 8, pc.data() + i * 8, 32);
+      }
     }
   }
   return hipSuccess;
 }
 
 hipError_t Batch::read_order(uint32_t v, uint32_t* order, uint32_t capacity) const {
-  if (hipError_t e = survivor_views(); e != hipSuccess) return e;
+  if (hipError_t e = survivor_tiles(); e != hipSuccess) return e;
   const uint32_t total = ctrl_host()->visible, n = view_visible(v);
   std::vector<uint32_t> all(total);
   const uint32_t* src = order_[(depth_passes_ - 1) & 1].as<uint32_t>();
@@ -427,7 +434,7 @@ hipError_t Batch::read_order(uint32_t v, uint32_t* order, uint32_t capacity) con
   }
   uint32_t k = 0;
   for (uint32_t i = 0; i < total && k < std::min(n, capacity); ++i)
-    if (sv_view_[all[i]] == v) order[k++] = sv_local_[all[i]];
+    if (tile_view_of(all[i] / kCullTile) == v) order[k++] = local_index(all[i]);
   return hipSuccess;
 }
 
@@ -461,9 +468,9 @@ hipError_t Batch::read_tiles(uint32_t v, uint64_t* pairs, uint32_t* pair_tiles, 
     for (uint64_t i = 0; e == hipSuccess && i < n; ++i) pair_tiles[i] -= vi.tile_base;
   }
   if (e == hipSuccess && pair_values && n) {
-    if (e = survivor_views(); e != hipSuccess) return e;
+    if (e = survivor_tiles(); e != hipSuccess) return e;
     e = hipMemcpy(pair_values, pair_vals_[fin].as<uint32_t>() + p0, n * 4, hipMemcpyDeviceToHost);
-    for (uint64_t i = 0; e == hipSuccess && i < n; ++i) pair_values[i] = sv_local_[pair_values[i]];
+    for (uint64_t i = 0; e == hipSuccess && i < n; ++i) pair_values[i] = local_index(pair_values[i]);
   }
   return e;
 }

@@ -3,7 +3,8 @@
 //   k_pack      SplatCloud floats -> resident SoA records (GpuLayout.cpp:92-137)
 //   k_prologue  per-frame control block: epoch, counters, histograms, tile ranges
 //   k_cull      visibility, projection, SH colour, depth key (SplatVisibility.metal:39-106,
-//               SplatProjection.metalh:78-141), compacted in lane order by a decoupled look-back
+//               SplatProjection.metalh:78-141), each cull tile's survivors in lane order from
+//               its own base (a gapped layout the depth sort's first pass compacts)
 //   k_hist      onesweep global digit histograms of a key array, all passes in one read
 //   k_onesweep  one LSD radix pass, stable (MetalRadixSort.h:13-19): wave64 ballot ranking,
 //               LDS-staged digit histograms and scatter, per-digit decoupled look-back
@@ -37,10 +38,6 @@ __device__ __forceinline__ void status_store(uint64_t* p, uint64_t v) {
 __device__ __forceinline__ uint64_t status_load(uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
-  const uint32_t s = a + b;
-  return s < a ? 0xffffffffu : s;
-}
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 // Number of set bits of `mask` below this lane.
@@ -56,12 +53,6 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
     const uint32_t t = __shfl_up(v, d, 64);
     if (lane >= d) v += t;
   }
-  return v;
-}
-
-__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
   return v;
 }
 
@@ -81,40 +72,6 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
   __syncthreads();
   *total = all;
   return before + incl - v;
-}
-
-// Decoupled look-back over single-value status words by one full wave: the 64 lanes read 64
-// predecessors at once, nearest first, until one of them holds an inclusive prefix. Every
-// predecessor took its ticket earlier and publishes without waiting on later blocks, so the wait
-// ends. Returns the exclusive prefix (saturating) in every lane.
-__device__ uint32_t wave_lookback(uint64_t* status, uint32_t block, uint32_t tag) {
-  const uint32_t lane = lane_id();
-  uint64_t excl = 0;
-  int32_t k = static_cast<int32_t>(block) - 1;
-  while (k >= 0) {
-    const int32_t idx = k - static_cast<int32_t>(lane);
-    uint32_t flag = kFlagPrefix, value = 0;
-    bool ready = true;
-    if (idx >= 0) {
-      const uint64_t s = status_load(status + idx);
-      flag = static_cast<uint32_t>(s >> 32) & 3u;
-      value = static_cast<uint32_t>(s);
-      ready = (static_cast<uint32_t>(s >> 34) == tag) && flag != 0u;
-    }
-    if (__ballot(!ready) != 0ull) {
-      __builtin_amdgcn_s_sleep(1);
-      continue;
-    }
-    const uint64_t prefixes = __ballot(flag == kFlagPrefix);
-    if (prefixes != 0ull) {
-      const uint32_t first = static_cast<uint32_t>(__builtin_ctzll(prefixes));
-      excl += wave_sum64(lane <= first ? value : 0u);
-      break;
-    }
-    excl += wave_sum64(value);
-    k -= 64;
-  }
-  return excl > 0xffffffffull ? 0xffffffffu : static_cast<uint32_t>(excl);
 }
 
 __device__ __forceinline__ void mat_vec(const float* m, float x, float y, float z, float w, float out[4]) {
@@ -372,6 +329,7 @@ __global__ __launch_bounds__(256) void k_prologue(Ctrl* ctrl, uint32_t* hist, ui
     ctrl->pairs = 0;
     ctrl->overflow = 0;
     ctrl->sort_pairs = 0;
+    ctrl->cull_lanes = 0;
     for (int k = 0; k < 16; ++k) ctrl->ticket[k] = 0;
     for (int k = 0; k < 16; ++k) ctrl->lod[k] = 0;
   }
@@ -450,16 +408,16 @@ __device__ __forceinline__ uint64_t pack_rect(const Rect& r, uint32_t count, uin
          (static_cast<uint64_t>(view) << 48);
 }
 
-// 256 threads, kCullItems lanes each: lane t = tile * 1024 + k * 256 + thread. The tile's
-// survivors are ranked in lane order (k, wave, lane), so the compaction keeps lane order. Blocks
-// past the tiles this frame needs exit before they take a ticket.
+// 256 threads, kCullItems lanes each: lane t = tile * kCullTile + k * 256 + thread. Block b's
+// survivors are ranked in lane order (k, wave, lane) and written from position b * kCullTile, with
+// their count in cull_counts[b]: no block waits for another, and ordering by position is lane order
+// (compaction order). Block 0 publishes the layout's extent, every block adds its count to
+// ctrl->visible.
 template <int DEG, bool INDEXED, bool KEY16>
 __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
   constexpr uint32_t kItems = kCullItems;
   constexpr uint32_t kTile = 256u * kItems;
-  __shared__ uint32_t s_ticket;
   __shared__ uint32_t s_seg[kItems * 4];
-  __shared__ uint32_t s_prefix;
   __shared__ float s_byte255[256];  // b / 255 for every byte: the divisions of packRgba8's inverse
   const uint32_t tid = threadIdx.x, wave = tid >> 6;
   const uint32_t total = INDEXED ? min(*a.active_count, a.capacity) : a.total;
@@ -467,9 +425,8 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
   const uint32_t blocks = view_tiles * a.view_count;
   if (blockIdx.x >= blocks) return;
   s_byte255[tid] = static_cast<float>(tid) / 255.0f;
-  if (tid == 0) s_ticket = atomicAdd(&a.ctrl->ticket[kTicketCull], 1u);
   __syncthreads();
-  const uint32_t b = s_ticket;
+  const uint32_t b = blockIdx.x;
   const uint32_t view = a.view_count > 1u ? b % a.view_count : 0u;
   const uint32_t lt = a.view_count > 1u ? b / a.view_count : b;  // the lane tile
   const Camera& cam = a.cam[view];
@@ -499,22 +456,15 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
   uint32_t aggregate = 0;
 #pragma unroll
   for (uint32_t g = 0; g < kItems * 4; ++g) aggregate += s_seg[g];
-  if (wave == 0) {
-    const uint32_t tag = make_tag(a.ctrl->epoch, 0);
-    uint32_t prefix = 0;
-    if (b == 0) {
-      if (lane_id() == 0) status_store(a.scan_status, status_word(tag, kFlagPrefix, aggregate));
-    } else {
-      if (lane_id() == 0) status_store(a.scan_status + b, status_word(tag, kFlagAggregate, aggregate));
-      prefix = wave_lookback(a.scan_status, b, tag);
-      if (lane_id() == 0)
-        status_store(a.scan_status + b, status_word(tag, kFlagPrefix, sat_add(prefix, aggregate)));
+  if (tid == 0) {
+    a.cull_counts[b] = aggregate;
+    if (b == 0) a.ctrl->cull_lanes = blocks * kTile;
+    if (aggregate != 0u) {
+      atomicAdd(&a.ctrl->visible, aggregate);
+      if (a.view_visible != nullptr) atomicAdd(&a.view_visible[view], aggregate);
     }
-    if (lane_id() == 0) s_prefix = prefix;
-    if (lane_id() == 0 && aggregate != 0u && a.view_visible != nullptr) atomicAdd(&a.view_visible[view], aggregate);
   }
-  __syncthreads();
-  uint32_t before = s_prefix;
+  uint32_t before = b * kTile;
 #pragma unroll
   for (uint32_t k = 0; k < kItems; ++k) {
     uint32_t seg = before;
@@ -526,7 +476,6 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
       uint4* dst = reinterpret_cast<uint4*>(a.projected + static_cast<size_t>(off) * 8u);
       dst[0] = make_uint4(rec[k][0], rec[k][1], rec[k][2], rec[k][3]);
       dst[1] = make_uint4(rec[k][4], rec[k][5], rec[k][6], rec[k][7]);
-      if (a.survivor_view != nullptr) a.survivor_view[off] = static_cast<uint16_t>(view);
       // The binning's tile rectangle, from the record just written (read back in depth order, a
       // gather of 8 bytes instead of the 32-byte record).
       Rect r;
@@ -537,12 +486,17 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
 #pragma unroll
     for (uint32_t w = 0; w < 4; ++w) before += s_seg[k * 4 + w];
   }
-  if (b == blocks - 1u && tid == 0) a.ctrl->visible = s_prefix + aggregate;
 }
 
 // Global digit histograms of every pass in one read of the keys (onesweep's upfront histogram).
+// Whether position pos of a sort input holds a key: below n and, in the cull's gapped layout (seg
+// non-null: survivors per cull tile), within its cull tile's survivors.
+__device__ __forceinline__ bool sort_valid(uint32_t pos, uint32_t n, const uint32_t* seg) {
+  return pos < n && (seg == nullptr || pos % kCullTile < seg[pos / kCullTile]);
+}
+
 __global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
-                                              uint32_t* hist, DigitPlan plan) {
+                                              const uint32_t* __restrict__ seg, uint32_t* hist, DigitPlan plan) {
   __shared__ uint32_t s_h[4][4][256];
   const uint32_t tid = threadIdx.x, wave = tid >> 6;
   const int passes = plan.passes;
@@ -550,6 +504,7 @@ __global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ keys,
   __syncthreads();
   const uint32_t n = *n_ptr;
   for (uint32_t i = blockIdx.x * 256u + tid; i < n; i += gridDim.x * 256u) {
+    if (!sort_valid(i, n, seg)) continue;
     const uint32_t k = keys[i];
     for (int p = 0; p < passes; ++p)
       atomicAdd(&s_h[wave][p][(k >> plan.shift[p]) & ((1u << plan.bits[p]) - 1u)], 1u);
@@ -564,7 +519,8 @@ __global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ keys,
 constexpr uint32_t kLookback = 16;
 
 // One stable LSD pass over kSortTile-key tiles. Keys of tile b sit at b*kSortTile + wave*kSortItems*64 +
-// j*64 + lane:
+// j*64 + lane (positions sort_valid rejects hold none: the first depth pass reads the cull's gapped
+// layout, with a key's position as its value, and writes the survivors packed):
 // each wave ranks its kSortItems rows in order with 8 ballots per row (lanes with equal digits form a
 // peer mask), so the in-tile order is position order and the pass is stable. The tile then learns
 // where each of its digits starts in the output:
@@ -580,7 +536,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
                                                   const uint32_t* __restrict__ hist, uint32_t shift,
                                                   uint32_t mask, uint64_t* status, Ctrl* ctrl, uint32_t ticket,
                                                   uint32_t tag_pass, const uint32_t* __restrict__ offsets,
-                                                  uint32_t stride) {
+                                                  uint32_t stride, const uint32_t* __restrict__ seg) {
   __shared__ uint32_t s_ticket;
   __shared__ uint32_t s_wave_count[4][kRadix];
   __shared__ uint32_t s_digit_start[kRadix];
@@ -600,17 +556,18 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
   const uint32_t base = b * kSortTile;
 
   uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+  uint32_t valid_rows = 0;  // bit j: row j's position holds a key
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) {
     const uint32_t pos = base + wave * (kSortItems * 64u) + j * 64u + lane;
-    const bool valid = pos < n;
+    const bool valid = sort_valid(pos, n, seg);
+    valid_rows |= valid ? 1u << j : 0u;
     key[j] = valid ? keys_in[pos] : 0u;
     val[j] = valid ? (vals_in != nullptr ? vals_in[pos] : pos) : 0u;
   }
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) {
-    const uint32_t pos = base + wave * (kSortItems * 64u) + j * 64u + lane;
-    const bool valid = pos < n;
+    const bool valid = (valid_rows >> j) & 1u;
     const uint32_t d = (key[j] >> shift) & mask;  // mask < 2^BITS
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -683,8 +640,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
   // Stage the tile in digit order in LDS.
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) {
-    const uint32_t pos = base + wave * (kSortItems * 64u) + j * 64u + lane;
-    if (pos < n) {
+    if ((valid_rows >> j) & 1u) {
       const uint32_t d = (key[j] >> shift) & mask;
       const uint32_t local = s_digit_start[d] + s_wave_count[wave][d] + rank[j];
       s_keys[local] = key[j];
@@ -692,8 +648,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
     }
   }
   __syncthreads();
-  const uint32_t in_tile = min(kSortTile, n - base);
-  for (uint32_t i = tid; i < in_tile; i += 256u) {
+  for (uint32_t i = tid; i < tile_total; i += 256u) {
     const uint32_t k = s_keys[i];
     const uint32_t dst = s_base[(k >> shift) & mask] + i;
     if (keys_out != nullptr) keys_out[dst] = k;
@@ -701,12 +656,11 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
   }
 }
 
-// Reduce then scan, step 1: the digit counts of each 4096-key tile, digit-major
+// Reduce then scan, step 1: the digit counts of each kSortTile-key tile, digit-major
 // (counts[d * stride + tile]), from per-wave LDS histograms.
-// Per-tile digit counts of one pass (reduce then scan): counts[d * stride + tile].
 __global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
                                                      uint32_t shift, uint32_t mask, uint32_t* __restrict__ counts,
-                                                     uint32_t stride) {
+                                                     uint32_t stride, const uint32_t* __restrict__ seg) {
   __shared__ uint32_t s_h[4][kRadix];
   const uint32_t tid = threadIdx.x, wave = tid >> 6;
   const uint32_t n = *n_ptr;
@@ -718,16 +672,15 @@ __global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict_
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) {
     const uint32_t pos = base + j * 256u + tid;
-    if (pos < n) atomicAdd(&s_h[wave][(keys[pos] >> shift) & mask], 1u);
+    if (sort_valid(pos, n, seg)) atomicAdd(&s_h[wave][(keys[pos] >> shift) & mask], 1u);
   }
   __syncthreads();
   counts[static_cast<size_t>(tid) * stride + blockIdx.x] = s_h[0][tid] + s_h[1][tid] + s_h[2][tid] + s_h[3][tid];
 }
 
-// Reduce then scan, step 2: one block per digit turns its row of tile counts into output starts:
-// the digit's global start (the pass histogram's exclusive scan) plus the exclusive scan over tiles.
-// Block d scans digit d's row of tile counts in place (exclusive, from 0) and stores the row's total
-// in totals[d]; the scatter adds the digit's start, the exclusive scan of the totals.
+// Reduce then scan, step 2: block d scans digit d's row of tile counts in place (exclusive, from 0)
+// and stores the row's total in totals[d]; the scatter adds the digit's start, the exclusive scan of
+// the totals.
 __global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ counts, const uint32_t* n_ptr,
                                                     uint32_t* __restrict__ totals, uint32_t stride) {
   constexpr uint32_t kPer = 8;
@@ -1315,16 +1268,16 @@ template <bool LOOKBACK, int BITS>
 void sort_pass_bits(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
                     const uint32_t* n_ptr, uint32_t tiles, uint32_t* hist, uint32_t shift, uint32_t bits,
                     uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass, uint32_t* counts,
-                    hipStream_t s) {
+                    const uint32_t* seg, hipStream_t s) {
   const uint32_t mask = (1u << bits) - 1u;
   if constexpr (LOOKBACK) {
     hipLaunchKernelGGL((k_onesweep<true, BITS>), dim3(tiles), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist,
-                       shift, mask, status, ctrl, ticket, tag_pass, nullptr, 0u);
+                       shift, mask, status, ctrl, ticket, tag_pass, nullptr, 0u, seg);
   } else {
-    hipLaunchKernelGGL(k_tile_counts, dim3(tiles), dim3(256), 0, s, kin, n_ptr, shift, mask, counts, tiles);
+    hipLaunchKernelGGL(k_tile_counts, dim3(tiles), dim3(256), 0, s, kin, n_ptr, shift, mask, counts, tiles, seg);
     hipLaunchKernelGGL(k_digit_scan, dim3(1u << bits), dim3(256), 0, s, counts, n_ptr, hist, tiles);
     hipLaunchKernelGGL((k_onesweep<false, BITS>), dim3(tiles), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist,
-                       shift, mask, status, ctrl, ticket, tag_pass, counts, tiles);
+                       shift, mask, status, ctrl, ticket, tag_pass, counts, tiles, seg);
   }
 }
 
@@ -1332,29 +1285,31 @@ template <bool LOOKBACK>
 void sort_pass_variant(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
                        const uint32_t* n_ptr, uint32_t tiles, uint32_t* hist, uint32_t shift, uint32_t bits,
                        uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass, uint32_t* counts,
-                       hipStream_t s) {
+                       const uint32_t* seg, hipStream_t s) {
   if (bits <= 6)
     sort_pass_bits<LOOKBACK, 6>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
-                                tag_pass, counts, s);
+                                tag_pass, counts, seg, s);
   else if (bits == 7)
     sort_pass_bits<LOOKBACK, 7>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
-                                tag_pass, counts, s);
+                                tag_pass, counts, seg, s);
   else
     sort_pass_bits<LOOKBACK, 8>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
-                                tag_pass, counts, s);
+                                tag_pass, counts, seg, s);
 }
 
-// Pass p of a sort under its digit plan.
+// Pass p of a sort under its digit plan; `seg` is the cull's gapped layout (first depth pass) or
+// null.
 void sort_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, const uint32_t* n_ptr,
                uint32_t capacity, uint32_t* hist, const DigitPlan& plan, int p, uint64_t* status, Ctrl* ctrl,
-               uint32_t ticket, uint32_t tag_pass, uint32_t* counts, int variant, hipStream_t s) {
+               uint32_t ticket, uint32_t tag_pass, uint32_t* counts, int variant, const uint32_t* seg,
+               hipStream_t s) {
   const uint32_t tiles = grid_for(capacity, kSortTile);
   if (variant == kSortOnesweep)
     sort_pass_variant<true>(kin, vin, kout, vout, n_ptr, tiles, hist, plan.shift[p], plan.bits[p], status, ctrl,
-                            ticket, tag_pass, counts, s);
+                            ticket, tag_pass, counts, seg, s);
   else
     sort_pass_variant<false>(kin, vin, kout, vout, n_ptr, tiles, hist, plan.shift[p], plan.bits[p], status, ctrl,
-                             ticket, tag_pass, counts, s);
+                             ticket, tag_pass, counts, seg, s);
 }
 
 }  // namespace
@@ -1387,13 +1342,14 @@ hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], cons
   // The reduce-then-scan passes take each digit's total from their own tile counts; the onesweep
   // passes need every pass's histogram first.
   if (variant == kSortOnesweep)
-    hipLaunchKernelGGL(k_hist, dim3(hist_grid(capacity)), dim3(256), 0, stream, keys[0], n_dev, hist, plan);
+    hipLaunchKernelGGL(k_hist, dim3(hist_grid(capacity)), dim3(256), 0, stream, keys[0], n_dev, nullptr, hist,
+                       plan);
   for (int p = 0; p < passes; ++p) {
     const uint32_t* kin = keys[p & 1];
     uint32_t* kout = (p + 1 < passes) ? keys[(p + 1) & 1] : nullptr;
     const uint32_t* vin = p == 0 ? nullptr : vals[(p - 1) & 1];
     sort_pass(kin, vin, kout, vals[p & 1], n_dev, capacity, hist + p * 256, plan, p, status, ctrl,
-              kTicketDepthSort + p, 2u + p, counts, variant, stream);
+              kTicketDepthSort + p, 2u + p, counts, variant, nullptr, stream);
   }
   return hipGetLastError();
 }
@@ -1421,17 +1377,21 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
     default: launch_cull_deg<3>(a, cull_blocks, s); break;
   }
   if (marks) (void)hipEventRecord(marks[2], s);
-  // Depth sort of the survivors: cull keys -> keys[0] -> keys[1] -> keys[0] ...
+  // Depth sort of the survivors: cull keys -> keys[0] -> keys[1] -> keys[0] ... The first pass reads
+  // the cull's gapped layout (its extent and per-tile counts) and writes the survivors packed, with
+  // their positions as values.
   const uint32_t* visible = &a.ctrl->visible;
+  const uint32_t* cull_lanes = &a.ctrl->cull_lanes;
   if (a.sort_variant == kSortOnesweep)
-    hipLaunchKernelGGL(k_hist, dim3(hist_grid(a.capacity)), dim3(256), 0, s, a.cull_keys, visible, a.hist,
-                       depth_plan);
+    hipLaunchKernelGGL(k_hist, dim3(hist_grid(a.capacity)), dim3(256), 0, s, a.cull_keys, cull_lanes, a.cull_counts,
+                       a.hist, depth_plan);
   for (int p = 0; p < depth_passes; ++p) {
     const uint32_t* kin = p == 0 ? a.cull_keys : a.keys[(p - 1) & 1];
     uint32_t* kout = (p + 1 < depth_passes) ? a.keys[p & 1] : nullptr;
     const uint32_t* vin = p == 0 ? nullptr : a.order[(p - 1) & 1];
-    sort_pass(kin, vin, kout, a.order[p & 1], visible, a.capacity, a.hist + p * 256, depth_plan, p, a.sort_status,
-              a.ctrl, kTicketDepthSort + p, 2u + p, a.sort_counts, a.sort_variant, s);
+    sort_pass(kin, vin, kout, a.order[p & 1], p == 0 ? cull_lanes : visible, a.capacity, a.hist + p * 256, depth_plan, p,
+              a.sort_status, a.ctrl, kTicketDepthSort + p, 2u + p, a.sort_counts, a.sort_variant,
+              p == 0 ? a.cull_counts : nullptr, s);
   }
   const uint32_t* order = a.order[(depth_passes - 1) & 1];
   if (marks) (void)hipEventRecord(marks[3], s);
@@ -1441,12 +1401,12 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
   const uint32_t* sort_n = &a.ctrl->sort_pairs;
   uint32_t* tile_hist = a.hist + 4 * 256;
   if (a.sort_variant == kSortOnesweep)
-    hipLaunchKernelGGL(k_hist, dim3(hist_grid(a.pair_capacity)), dim3(256), 0, s, a.pair_keys[0], sort_n,
+    hipLaunchKernelGGL(k_hist, dim3(hist_grid(a.pair_capacity)), dim3(256), 0, s, a.pair_keys[0], sort_n, nullptr,
                        tile_hist, tile_plan);
   for (int p = 0; p < tile_passes; ++p) {
     sort_pass(a.pair_keys[p & 1], a.pair_vals[p & 1], a.pair_keys[(p + 1) & 1], a.pair_vals[(p + 1) & 1],
               sort_n, a.pair_capacity, tile_hist + p * 256, tile_plan, p, a.sort_status, a.ctrl, kTicketTileSort + p,
-              6u + p, a.sort_counts, a.sort_variant, s);
+              6u + p, a.sort_counts, a.sort_variant, nullptr, s);
   }
   const int fin = tile_passes & 1;
   hipLaunchKernelGGL(k_ranges, dim3(min(grid_for(a.pair_capacity, 256), 2048u)), dim3(256), 0, s,

@@ -353,7 +353,8 @@ struct skm_renderer {
 
   // Survivor scratch for `cap` lanes. Transactional: nothing changes unless every buffer fits.
   skm_status ensure_capacity(uint32_t cap) {
-    cap = std::max(cap, 1u);
+    // Whole cull tiles: the cull writes survivors in a gapped layout of kCullTile lanes per tile.
+    cap = static_cast<uint32_t>(align_up(std::max(cap, 1u), skm::kCullTile));
     if (cap <= capacity) return SKM_OK;
     DeviceBuffer ck, k0, k1, o0, o1, pr, rc, ss;
     const size_t n = size_t{cap};
@@ -842,6 +843,7 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   a.projected = r->projected.as<uint32_t>();
   a.rects = r->rects.as<uint64_t>();
   a.scan_status = r->scan_status.as<uint64_t>();
+  a.cull_counts = reinterpret_cast<uint32_t*>(r->scan_status.ptr);
   a.sort_status = r->sort_status.as<uint64_t>();
   a.hist = r->hist.as<uint32_t>();
   a.pair_capacity = r->pair_capacity;
@@ -995,6 +997,27 @@ skm_status skm_read_frame(skm_renderer* r, int format, void* dst, size_t dst_byt
   return SKM_OK;
 }
 
+// The last frame's gapped survivor layout: the packed index of each cull tile's first survivor
+// (bases[b], with bases[tiles] the survivor count) from the per-tile counts.
+static skm_status survivor_bases(skm_renderer* r, const Ctrl& c, std::vector<uint32_t>* counts,
+                                 std::vector<uint32_t>* bases) {
+  const uint32_t tiles = c.cull_lanes / skm::kCullTile;
+  counts->assign(tiles, 0u);
+  bases->assign(size_t{tiles} + 1, 0u);
+  if (tiles)
+    if (auto st = r->check(hipMemcpy(counts->data(), r->scan_status.ptr, size_t{tiles} * 4, hipMemcpyDeviceToHost),
+                           "read");
+        st != SKM_OK)
+      return st;
+  for (uint32_t b = 0; b < tiles; ++b) (*bases)[b + 1] = (*bases)[b] + (*counts)[b];
+  return SKM_OK;
+}
+
+// A survivor's packed index (compaction order) from its position in the gapped layout.
+static uint32_t packed_index(const std::vector<uint32_t>& bases, uint32_t pos) {
+  return bases[pos / skm::kCullTile] + pos % skm::kCullTile;
+}
+
 skm_status skm_read_visible(skm_renderer* r, uint32_t* count, uint32_t* keys, uint32_t* projected,
                             uint32_t capacity) {
   if (r == nullptr || count == nullptr) return SKM_ERR_INVALID;
@@ -1004,12 +1027,25 @@ skm_status skm_read_visible(skm_renderer* r, uint32_t* count, uint32_t* keys, ui
     return st;
   *count = c.visible;
   const uint32_t n = std::min(c.visible, capacity);
-  if (keys && n)
-    if (auto st = r->check(hipMemcpy(keys, r->cull_keys.ptr, size_t{n} * 4, hipMemcpyDeviceToHost), "read");
+  if (n == 0 || (keys == nullptr && projected == nullptr)) return SKM_OK;
+  std::vector<uint32_t> counts, bases;
+  if (auto st = survivor_bases(r, c, &counts, &bases); st != SKM_OK) return st;
+  const size_t lanes = c.cull_lanes;
+  std::vector<uint32_t> gk(keys ? lanes : 0), gp(projected ? lanes * 8 : 0);
+  if (keys)
+    if (auto st = r->check(hipMemcpy(gk.data(), r->cull_keys.ptr, lanes * 4, hipMemcpyDeviceToHost), "read");
         st != SKM_OK)
       return st;
-  if (projected && n)
-    return r->check(hipMemcpy(projected, r->projected.ptr, size_t{n} * 32, hipMemcpyDeviceToHost), "read");
+  if (projected)
+    if (auto st = r->check(hipMemcpy(gp.data(), r->projected.ptr, lanes * 32, hipMemcpyDeviceToHost), "read");
+        st != SKM_OK)
+      return st;
+  for (uint32_t b = 0; b < counts.size(); ++b)
+    for (uint32_t j = 0; j < counts[b] && bases[b] + j < n; ++j) {
+      const size_t g = size_t{b} * skm::kCullTile + j, k = bases[b] + j;
+      if (keys) keys[k] = gk[g];
+      if (projected) std::memcpy(projected + k * 8, gp.data() + g * 8, 32);
+    }
   return SKM_OK;
 }
 
@@ -1023,7 +1059,14 @@ skm_status skm_read_order(skm_renderer* r, uint32_t* order, uint32_t capacity) {
   const int passes = r->slots[r->last_slot].depth_passes;
   const uint32_t n = std::min(c.visible, capacity);
   if (n == 0) return SKM_OK;
-  return r->check(hipMemcpy(order, r->order[(passes - 1) & 1].ptr, size_t{n} * 4, hipMemcpyDeviceToHost), "read");
+  std::vector<uint32_t> counts, bases;
+  if (auto st = survivor_bases(r, c, &counts, &bases); st != SKM_OK) return st;
+  if (auto st = r->check(hipMemcpy(order, r->order[(passes - 1) & 1].ptr, size_t{n} * 4, hipMemcpyDeviceToHost),
+                         "read");
+      st != SKM_OK)
+    return st;
+  for (uint32_t i = 0; i < n; ++i) order[i] = packed_index(bases, order[i]);
+  return SKM_OK;
 }
 
 skm_status skm_read_tiles(skm_renderer* r, uint64_t* pairs, uint32_t* pair_tiles, uint32_t* pair_values,
@@ -1042,10 +1085,14 @@ skm_status skm_read_tiles(skm_renderer* r, uint64_t* pairs, uint32_t* pair_tiles
     if (auto st = r->check(hipMemcpy(pair_tiles, r->pair_keys[fin].ptr, n * 4, hipMemcpyDeviceToHost), "read");
         st != SKM_OK)
       return st;
-  if (n > 0 && pair_values)
+  if (n > 0 && pair_values) {
+    std::vector<uint32_t> counts, bases;
+    if (auto st = survivor_bases(r, c, &counts, &bases); st != SKM_OK) return st;
     if (auto st = r->check(hipMemcpy(pair_values, r->pair_vals[fin].ptr, n * 4, hipMemcpyDeviceToHost), "read");
         st != SKM_OK)
       return st;
+    for (uint64_t i = 0; i < n; ++i) pair_values[i] = packed_index(bases, pair_values[i]);
+  }
   const uint32_t tiles = ((s.width + s.tile - 1) / s.tile) * ((s.height + s.tile - 1) / s.tile);
   if (ranges && tile_capacity)
     return r->check(hipMemcpy(ranges, r->tile_ranges.ptr, size_t{std::min(tiles, tile_capacity)} * 8,

@@ -162,7 +162,7 @@ struct skm_server {
       tiles += ((w + 15) / 16) * ((h + 15) / 16);
       pixels += w * h;
     }
-    const uint64_t lanes = (uint64_t{sc.count} + 1023) / 1024 * 1024 * batch.size();
+    const uint64_t lanes = (uint64_t{sc.count} + skm::kCullTile - 1) / skm::kCullTile * skm::kCullTile * batch.size();
     const uint64_t need = b->bytes_for(static_cast<uint32_t>(batch.size()), lanes, static_cast<uint32_t>(tiles),
                                        pixels) + std::max<uint64_t>(pixels * 4, sc.bgra.bytes);
     const uint64_t have = b->device_bytes() + sc.bgra.bytes;

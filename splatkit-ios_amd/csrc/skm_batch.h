@@ -67,17 +67,21 @@ class Batch {
  private:
   const Ctrl* ctrl_host() const { return host_.as<Ctrl>(); }
   const uint32_t* counts_host() const { return reinterpret_cast<const uint32_t*>(host_.as<char>() + sizeof(Ctrl)); }
-  // The view of every survivor and its index among its view's survivors (read back on demand).
-  hipError_t survivor_views() const;
-  mutable std::vector<uint16_t> sv_view_;
-  mutable std::vector<uint32_t> sv_local_;
-  mutable bool sv_ready_ = false;
+  // The cull's gapped layout of the last batch (read back on demand): survivors per cull tile, and
+  // the index among its view's survivors of each tile's first survivor. Cull tile b belongs to view
+  // b % views (tile-major tickets).
+  hipError_t survivor_tiles() const;
+  uint32_t tile_view_of(uint32_t b) const { return info_.size() > 1 ? b % static_cast<uint32_t>(info_.size()) : 0u; }
+  // A survivor's index among its view's survivors, from its position in the gapped layout.
+  uint32_t local_index(uint32_t pos) const { return st_local_[pos / kCullTile] + pos % kCullTile; }
+  mutable std::vector<uint32_t> st_count_, st_local_;
+  mutable bool st_ready_ = false;
   hipError_t ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_t tiles, uint64_t pixels,
                     uint32_t ranges);
   void release_graph();
 
   // Device scratch.
-  DeviceBuffer ctrl_, cams_, views_, tile_view_, view_visible_, survivor_view_;
+  DeviceBuffer ctrl_, cams_, views_, tile_view_, view_visible_;
   DeviceBuffer cull_keys_, keys_[2], order_[2], projected_, rects_, scan_status_, sort_status_, sort_counts_, hist_;
   DeviceBuffer pair_keys_[2], pair_vals_[2], tile_ranges_, image_, ranges_, starts_;
   PinnedBuffer staging_, host_;

@@ -63,7 +63,8 @@ struct Ctrl {
   uint32_t pairs;     // tile pairs, saturated at 2^32 - 1
   uint32_t overflow;  // nonzero when the pairs exceeded the pair capacity
   uint32_t sort_pairs;  // pairs the tile sort orders: `pairs`, or 0 after an overflow
-  uint32_t reserved[3];
+  uint32_t cull_lanes;  // cull tiles x kCullTile: the extent of the cull's gapped survivor layout
+  uint32_t reserved[2];
   uint32_t ticket[16];
   uint32_t lod[16];   // LOD selection counters
 };
@@ -113,7 +114,12 @@ inline uint32_t key_bits_for(uint32_t max_key) {
 }
 
 constexpr uint32_t kThreads = 256;
-constexpr uint32_t kCullItems = 4;  // lanes per thread of the cull (a 1024-lane tile per block)
+// Lanes per thread of the cull (a kCullTile-lane tile per block); a build may set it.
+#ifndef SKM_CULL_ITEMS
+#define SKM_CULL_ITEMS 4
+#endif
+constexpr uint32_t kCullItems = SKM_CULL_ITEMS;
+constexpr uint32_t kCullTile = kThreads * kCullItems;
 constexpr uint32_t kBinItems = 4;   // survivors per thread of the binning (1024 per block)
 // Keys per thread of a radix pass (a block sorts 256 x kSortItems keys); a build may set it.
 #ifndef SKM_SORT_ITEMS
@@ -156,7 +162,11 @@ struct FrameArgs {
   uint32_t* order[2];
   uint32_t* projected;  // 8 words per survivor
   uint64_t* rects;      // per survivor: its packed tile rectangle (written by the cull, read by the binning)
-  uint64_t* scan_status;  // one word per 1024-lane tile
+  // Survivors sit in a gapped layout: cull tile b writes its survivors, in lane order, from position
+  // b * kCullTile and its count to cull_counts[b]. Ordering by position is compaction order; the
+  // depth sort's first pass skips the gaps, so the sorted order holds each survivor's position.
+  uint32_t* cull_counts;
+  uint64_t* scan_status;
   uint64_t* sort_status;
   uint32_t* hist;       // [2][4][256]: depth digits, tile digits
   uint32_t* pair_keys[2];
@@ -170,13 +180,12 @@ struct FrameArgs {
   float* image;           // RGBA float, top-down rows
   uint32_t capacity;      // survivor capacity
   // Batches (view_count > 1): one camera per view at cam[v]; every view culls the same lanes
-  // [0, total). Cull tiles go tile-major (the views of one 1024-lane tile take consecutive
-  // tickets, so its records come from HBM once and from cache for the other views); survivors are
-  // compacted in (tile, view, lane) order, which is lane order within each view. Null for a single
-  // view.
+  // [0, total). Cull tiles go tile-major: block b culls lane tile b / views for view b % views, so
+  // the views of one lane tile run as neighbouring blocks and its records come from HBM once and
+  // from cache for the other views. Ordering survivors by position is (tile, view, lane) order,
+  // which is lane order within each view. Null for a single view.
   uint32_t view_count;
   const ViewInfo* views;
-  uint16_t* survivor_view;     // the view of each survivor
   uint32_t* view_visible;      // survivors per view
   const uint16_t* tile_view;   // the view of each global tile
   uint32_t total_tiles;        // tiles of all views
