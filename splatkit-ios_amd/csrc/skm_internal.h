@@ -116,7 +116,7 @@ inline uint32_t key_bits_for(uint32_t max_key) {
 constexpr uint32_t kThreads = 256;
 // Lanes per thread of the cull (a kCullTile-lane tile per block); a build may set it.
 #ifndef SKM_CULL_ITEMS
-#define SKM_CULL_ITEMS 4
+#define SKM_CULL_ITEMS 2
 #endif
 constexpr uint32_t kCullItems = SKM_CULL_ITEMS;
 constexpr uint32_t kCullTile = kThreads * kCullItems;
