@@ -820,113 +820,138 @@ __device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t v) {
   return v;
 }
 
-// Binning, second pass: a block reads its survivors' rectangles (in depth order) and values, scans
-// their pair counts into slot offsets, and writes its pairs from its block start. Pair e of the block
-// belongs to the last slot whose offset is <= e, and its tile is that slot's rectangle cell
-// k = e - offset, row-major, so pairs come out in depth order, survivor by survivor. Each wave takes a
-// contiguous run of 64-pair chunks: the slots of a chunk are a window of 64 slots from the previous
-// chunk's last owner, whose starts are marked in LDS and spread by a wave max-scan (a binary search
-// covers the first chunk, and any pair the window misses).
+// Binning, second pass, balanced by output: emission block k writes pairs [k C, (k + 1) C) of the
+// frame (C = kEmitChunk), whatever survivors they come from (the nearest survivors come first in
+// depth order and have the largest footprints). It finds the survivor block holding its first pair
+// (a 64-way search over the block starts), then, for each survivor block its range meets, reads
+// that block's rectangles (in depth order) and values, scans their pair counts into slot offsets in
+// LDS and writes its share. Pair e of a survivor block belongs to the last slot whose offset is <= e,
+// and its tile is that slot's rectangle cell k = e - offset, row-major, so pairs come out in depth
+// order, survivor by survivor. Each wave takes a contiguous run of 64-pair chunks: the slots of a
+// chunk are a window of 64 slots from the previous chunk's last owner, whose starts are marked in
+// LDS and spread by a wave max-scan (a binary search covers the first chunk and any pair the window
+// misses).
+constexpr uint32_t kEmitChunk = 4096;
 __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __restrict__ order) {
   constexpr uint32_t kItems = kBinItems;
   constexpr uint32_t kTile = 256u * kItems;
   __shared__ uint32_t s_scan[4];
   __shared__ uint32_t s_excl[kTile + 1u], s_val[kTile], s_origin[kTile], s_width[kTile], s_base[kTile];
   __shared__ uint32_t s_mark[4][64];
-  const uint32_t tid = threadIdx.x, b = blockIdx.x, wave = tid >> 6, lane = tid & 63u;
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
+  const uint32_t total = a.ctrl->sort_pairs;
+  const uint32_t e0 = blockIdx.x * kEmitChunk;
+  if (e0 >= total) return;
+  const uint32_t e1 = min(e0 + kEmitChunk, total);
   const uint32_t n = a.ctrl->visible;
   const uint32_t blocks = (n + kTile - 1u) / kTile;
-  if (b >= blocks) return;
-  s_mark[wave][lane] = 0u;
-  uint32_t values[kItems], lo_words[kItems], hi_words[kItems];
-#pragma unroll
-  for (uint32_t k = 0; k < kItems; ++k) {
-    const uint32_t i = min(b * kTile + k * 256u + tid, n - 1u);
-    values[k] = order[i];
-    lo_words[k] = a.keys[0][i];
-    hi_words[k] = a.keys[1][i];
+  const uint32_t* starts = a.sort_counts;  // exclusive pair start of each survivor block
+  // The survivor block of pair e0: the last block whose start is <= e0.
+  uint32_t lo = 0, hi = blocks;
+  while (hi - lo > 1u) {
+    const uint32_t step = (hi - lo + 63u) / 64u;
+    const uint32_t probe = lo + lane * step;
+    const uint64_t ok = __ballot(probe < hi && starts[probe] <= e0);
+    lo += (63u - static_cast<uint32_t>(__builtin_clzll(ok))) * step;
+    hi = min(hi, lo + step);
   }
-  uint32_t aggregate = 0;
+  s_mark[wave][lane] = 0u;
+  for (uint32_t b = lo; b < blocks; ++b) {
+    const uint32_t block_start = starts[b];
+    if (block_start >= e1) break;
+    uint32_t values[kItems], lo_words[kItems], hi_words[kItems];
 #pragma unroll
-  for (uint32_t k = 0; k < kItems; ++k) {
-    const uint32_t slot = k * 256u + tid;
-    const uint32_t i = b * kTile + slot;
-    uint32_t count = 0, value = 0, base = 0, tiles_x = a.tiles_x;
-    Rect r{0, 0, -1, -1};
-    if (i < n) {
-      value = values[k];
-      const uint64_t rc = static_cast<uint64_t>(lo_words[k]) | (static_cast<uint64_t>(hi_words[k]) << 32);
-      if (rc != kNoRect) {
-        r.tx0 = static_cast<int32_t>(rc & 4095u);
-        r.ty0 = static_cast<int32_t>((rc >> 12) & 4095u);
-        r.tx1 = static_cast<int32_t>((rc >> 24) & 4095u);
-        r.ty1 = static_cast<int32_t>((rc >> 36) & 4095u);
-        count = static_cast<uint32_t>((r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1));
-        if (a.views != nullptr) {
-          const ViewInfo& vi = a.views[rc >> 48];
-          base = vi.tile_base;
-          tiles_x = vi.tiles_x;
+    for (uint32_t k = 0; k < kItems; ++k) {
+      const uint32_t i = min(b * kTile + k * 256u + tid, n - 1u);
+      values[k] = order[i];
+      lo_words[k] = a.keys[0][i];
+      hi_words[k] = a.keys[1][i];
+    }
+    __syncthreads();  // the previous block's table is no longer read
+    uint32_t aggregate = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; ++k) {
+      const uint32_t slot = k * 256u + tid;
+      const uint32_t i = b * kTile + slot;
+      uint32_t count = 0, value = 0, base = 0, tiles_x = a.tiles_x;
+      Rect r{0, 0, -1, -1};
+      if (i < n) {
+        value = values[k];
+        const uint64_t rc = static_cast<uint64_t>(lo_words[k]) | (static_cast<uint64_t>(hi_words[k]) << 32);
+        if (rc != kNoRect) {
+          r.tx0 = static_cast<int32_t>(rc & 4095u);
+          r.ty0 = static_cast<int32_t>((rc >> 12) & 4095u);
+          r.tx1 = static_cast<int32_t>((rc >> 24) & 4095u);
+          r.ty1 = static_cast<int32_t>((rc >> 36) & 4095u);
+          count = static_cast<uint32_t>((r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1));
+          if (a.views != nullptr) {
+            const ViewInfo& vi = a.views[rc >> 48];
+            base = vi.tile_base;
+            tiles_x = vi.tiles_x;
+          }
         }
       }
+      uint32_t row_total;
+      const uint32_t excl = block_exclusive_scan(count, s_scan, &row_total);
+      s_excl[slot] = aggregate + excl;
+      s_val[slot] = value;
+      s_origin[slot] = static_cast<uint32_t>(r.tx0) | (static_cast<uint32_t>(r.ty0) << 16);
+      s_width[slot] = static_cast<uint32_t>(r.tx1 - r.tx0 + 1) | (tiles_x << 16);
+      s_base[slot] = base;
+      aggregate += row_total;
     }
-    uint32_t row_total;
-    const uint32_t excl = block_exclusive_scan(count, s_scan, &row_total);
-    s_excl[slot] = aggregate + excl;
-    s_val[slot] = value;
-    s_origin[slot] = static_cast<uint32_t>(r.tx0) | (static_cast<uint32_t>(r.ty0) << 16);
-    s_width[slot] = static_cast<uint32_t>(r.tx1 - r.tx0 + 1) | (tiles_x << 16);
-    s_base[slot] = base;
-    aggregate += row_total;
-  }
-  if (tid == 0) s_excl[kTile] = aggregate;
-  __syncthreads();
-  const uint64_t prefix = a.sort_counts[b];
-  auto find = [&](uint32_t e) {  // last slot with s_excl[slot] <= e
-    uint32_t lo = 0, hi = kTile - 1u;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1u) / 2u;
-      if (s_excl[mid] <= e) lo = mid; else hi = mid - 1u;
-    }
-    return lo;
-  };
-  const uint32_t chunks = (aggregate + 63u) / 64u;
-  const uint32_t c0 = chunks * wave / 4u, c1 = chunks * (wave + 1u) / 4u;
-  uint32_t s0 = c0 < c1 ? find(c0 * 64u) : 0u;
-  for (uint32_t c = c0; c < c1; ++c) {
-    const uint32_t first = c * 64u;
-    const uint32_t s = s0 + lane;
-    if (s < kTile) {
-      const uint32_t ex = s_excl[s], cnt = s_excl[s + 1u] - ex;
-      if (cnt != 0u && ex < first + 64u && ex + cnt > first) s_mark[wave][ex > first ? ex - first : 0u] = s + 1u;
-    }
-    const uint32_t m = wave_inclusive_max(s_mark[wave][lane]);
-    s_mark[wave][lane] = 0u;
-    const uint32_t e = first + lane;
-    uint32_t owner = m != 0u ? m - 1u : 0u;
-    if (e < aggregate) {
-      if (m == 0u || e < s_excl[owner] || e >= s_excl[owner + 1u]) owner = find(e);
-      const uint32_t k = e - s_excl[owner];
-      const uint32_t wdt = s_width[owner] & 0xffffu, stride = s_width[owner] >> 16;
-      // k / wdt from a reciprocal estimate, corrected to the exact quotient.
-      uint32_t q = static_cast<uint32_t>(static_cast<float>(k) * __builtin_amdgcn_rcpf(static_cast<float>(wdt)));
-      int32_t rem = static_cast<int32_t>(k - q * wdt);
-      while (rem < 0) {
-        --q;
-        rem += static_cast<int32_t>(wdt);
+    if (tid == 0) s_excl[kTile] = aggregate;
+    __syncthreads();
+    auto find = [&](uint32_t e) {  // last slot with s_excl[slot] <= e
+      uint32_t l = 0, h = kTile - 1u;
+      while (l < h) {
+        const uint32_t mid = (l + h + 1u) / 2u;
+        if (s_excl[mid] <= e) l = mid; else h = mid - 1u;
       }
-      while (rem >= static_cast<int32_t>(wdt)) {
-        ++q;
-        rem -= static_cast<int32_t>(wdt);
+      return l;
+    };
+    // This block's pairs in [e0, e1), as block-local indices [first_pair, end_pair).
+    const uint32_t first_pair = e0 > block_start ? e0 - block_start : 0u;
+    const uint32_t end_pair = min(e1 - block_start, aggregate);
+    const uint32_t c_first = first_pair / 64u, c_end = (end_pair + 63u) / 64u;
+    const uint32_t chunks = c_end > c_first ? c_end - c_first : 0u;
+    const uint32_t c0 = c_first + chunks * wave / 4u, c1 = c_first + chunks * (wave + 1u) / 4u;
+    uint32_t s0 = c0 < c1 ? find(c0 * 64u) : 0u;
+    for (uint32_t c = c0; c < c1; ++c) {
+      const uint32_t first = c * 64u;
+      const uint32_t s = s0 + lane;
+      if (s < kTile) {
+        const uint32_t ex = s_excl[s], cnt = s_excl[s + 1u] - ex;
+        if (cnt != 0u && ex < first + 64u && ex + cnt > first)
+          s_mark[wave][ex > first ? ex - first : 0u] = s + 1u;
       }
-      const uint32_t tx = (s_origin[owner] & 0xffffu) + static_cast<uint32_t>(rem);
-      const uint32_t ty = (s_origin[owner] >> 16) + q;
-      const uint64_t idx = prefix + e;
-      if (idx < a.pair_capacity) {
+      const uint32_t m = wave_inclusive_max(s_mark[wave][lane]);
+      s_mark[wave][lane] = 0u;
+      const uint32_t e = first + lane;
+      uint32_t owner = m != 0u ? m - 1u : 0u;
+      if (e >= first_pair && e < end_pair) {
+        if (m == 0u || e < s_excl[owner] || e >= s_excl[owner + 1u]) owner = find(e);
+        const uint32_t k = e - s_excl[owner];
+        const uint32_t wdt = s_width[owner] & 0xffffu, stride = s_width[owner] >> 16;
+        // k / wdt from a reciprocal estimate, corrected to the exact quotient.
+        uint32_t q = static_cast<uint32_t>(static_cast<float>(k) * __builtin_amdgcn_rcpf(static_cast<float>(wdt)));
+        int32_t rem = static_cast<int32_t>(k - q * wdt);
+        while (rem < 0) {
+          --q;
+          rem += static_cast<int32_t>(wdt);
+        }
+        while (rem >= static_cast<int32_t>(wdt)) {
+          ++q;
+          rem -= static_cast<int32_t>(wdt);
+        }
+        const uint32_t tx = (s_origin[owner] & 0xffffu) + static_cast<uint32_t>(rem);
+        const uint32_t ty = (s_origin[owner] >> 16) + q;
+        const uint32_t idx = block_start + e;  // < total <= pair capacity
         a.pair_keys[0][idx] = s_base[owner] + ty * stride + tx;
         a.pair_vals[0][idx] = s_val[owner];
       }
+      s0 = __shfl(owner, 63, 64);
     }
-    s0 = __shfl(owner, 63, 64);
   }
 }
 
@@ -1397,7 +1422,7 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
   if (marks) (void)hipEventRecord(marks[3], s);
   hipLaunchKernelGGL(k_bin_count, dim3(grid_for(a.capacity, 256u * kBinItems)), dim3(256), 0, s, a, order);
   hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_bin, dim3(grid_for(a.capacity, 256u * kBinItems)), dim3(256), 0, s, a, order);
+  hipLaunchKernelGGL(k_bin, dim3(grid_for(a.pair_capacity, kEmitChunk)), dim3(256), 0, s, a, order);
   const uint32_t* sort_n = &a.ctrl->sort_pairs;
   uint32_t* tile_hist = a.hist + 4 * 256;
   if (a.sort_variant == kSortOnesweep)
