@@ -489,10 +489,18 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
 }
 
 // Global digit histograms of every pass in one read of the keys (onesweep's upfront histogram).
-// Whether position pos of a sort input holds a key: below n and, in the cull's gapped layout (seg
-// non-null: survivors per cull tile), within its cull tile's survivors.
-__device__ __forceinline__ bool sort_valid(uint32_t pos, uint32_t n, const uint32_t* seg) {
-  return pos < n && (seg == nullptr || pos % kCullTile < seg[pos / kCullTile]);
+// Sort inputs in the cull's gapped layout (seg non-null: survivors per cull tile): position pos holds
+// a key when pos < n and pos % kCullTile is below its cull tile's count. A wave's 64 consecutive
+// positions from `row` lie in one cull tile (kCullTile is a multiple of 64), so the count is one
+// scalar load per row; it is ~0 for a packed input and for rows at or past n (which hold no key).
+static_assert(kCullTile % 64u == 0u, "a wave row lies in one cull tile");
+__device__ __forceinline__ uint32_t row_limit(const uint32_t* seg, uint32_t row, uint32_t n) {
+  const uint32_t r = __builtin_amdgcn_readfirstlane(row);
+  if (seg == nullptr || r >= n) return ~0u;
+  return seg[r / kCullTile];
+}
+__device__ __forceinline__ bool sort_valid(uint32_t pos, uint32_t n, uint32_t limit) {
+  return pos < n && pos % kCullTile < limit;
 }
 
 __global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
@@ -504,8 +512,9 @@ __global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ keys,
   __syncthreads();
   const uint32_t n = *n_ptr;
   for (uint32_t i = blockIdx.x * 256u + tid; i < n; i += gridDim.x * 256u) {
-    if (!sort_valid(i, n, seg)) continue;
+    const uint32_t limit = row_limit(seg, i - lane_id(), n);
     const uint32_t k = keys[i];
+    if (!sort_valid(i, n, limit)) continue;
     for (int p = 0; p < passes; ++p)
       atomicAdd(&s_h[wave][p][(k >> plan.shift[p]) & ((1u << plan.bits[p]) - 1u)], 1u);
   }
@@ -556,14 +565,21 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
   const uint32_t base = b * kSortTile;
 
   uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+  // Every load first (positions clamped to n - 1), so a thread waits for one round trip.
+#pragma unroll
+  for (uint32_t j = 0; j < kSortItems; ++j) {
+    const uint32_t pos = min(base + wave * (kSortItems * 64u) + j * 64u + lane, n - 1u);
+    key[j] = keys_in[pos];
+    val[j] = vals_in != nullptr ? vals_in[pos] : pos;
+  }
   uint32_t valid_rows = 0;  // bit j: row j's position holds a key
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) {
-    const uint32_t pos = base + wave * (kSortItems * 64u) + j * 64u + lane;
-    const bool valid = sort_valid(pos, n, seg);
+    const uint32_t row = base + wave * (kSortItems * 64u) + j * 64u;
+    const bool valid = sort_valid(row + lane, n, row_limit(seg, row, n));
     valid_rows |= valid ? 1u << j : 0u;
-    key[j] = valid ? keys_in[pos] : 0u;
-    val[j] = valid ? (vals_in != nullptr ? vals_in[pos] : pos) : 0u;
+    key[j] = valid ? key[j] : 0u;
+    val[j] = valid ? val[j] : 0u;
   }
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) {
@@ -669,10 +685,13 @@ __global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict_
   for (uint32_t w = 0; w < 4; ++w) s_h[w][tid] = 0;
   __syncthreads();
   const uint32_t base = blockIdx.x * kSortTile;
+  uint32_t key[kSortItems];
+#pragma unroll
+  for (uint32_t j = 0; j < kSortItems; ++j) key[j] = keys[min(base + j * 256u + tid, n - 1u)];
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) {
-    const uint32_t pos = base + j * 256u + tid;
-    if (sort_valid(pos, n, seg)) atomicAdd(&s_h[wave][(keys[pos] >> shift) & mask], 1u);
+    const uint32_t row = base + j * 256u + wave * 64u;
+    if (sort_valid(row + (tid & 63u), n, row_limit(seg, row, n))) atomicAdd(&s_h[wave][(key[j] >> shift) & mask], 1u);
   }
   __syncthreads();
   counts[static_cast<size_t>(tid) * stride + blockIdx.x] = s_h[0][tid] + s_h[1][tid] + s_h[2][tid] + s_h[3][tid];
