@@ -17,6 +17,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "splatkit-ios_amd"))
 
@@ -72,6 +74,32 @@ def tune_single_draws(r, frames, draws=12):
         table.append((t, p, k, rb, (time.perf_counter() - t0) * 1e3 / draws))
     t, p, k, rb, _ = min(table, key=lambda row: row[4])
     return {"tile_size": t, "raster_px": p, "sort_kernel": k, "radix_bits": rb}, table
+
+
+def radix_sort(r, skm, sizes):
+    """The stable LSD argsort alone (skm_argsort_u32_device) on uniformly random 32-bit keys resident
+    in HBM: the depth sort's algorithm at the frame's pair count and above. Algorithmic bytes in the
+    reference's units: n (2 (k + 4) passes + k) with k = 4 and 4 passes."""
+    out = []
+    rng = np.random.default_rng(0x5EED0005)
+    for n in sizes:
+        keys = rng.integers(0, 2**32, n, dtype=np.uint32)
+        src, work, order = skm.DeviceBuffer(4 * n), skm.DeviceBuffer(4 * n), skm.DeviceBuffer(4 * n)
+        src.upload(keys)
+        ms = []
+        for _ in range(4):
+            work.copy_from(src, 4 * n)  # the sort overwrites its keys
+            ms.append(r.argsort_device(work.ptr.value, n, 32, order.ptr.value))
+        if n <= 16_000_000:
+            assert np.array_equal(order.download(np.uint32, n), np.argsort(keys, kind="stable"))
+        t = min(ms[1:])
+        byt = n * (2 * (4 + 4) * 4 + 4)
+        out.append({"keys": n, "key_bits": 32, "ms": round(t, 4), "GB_s": round(byt / t / 1e6, 1),
+                    "hbm_frac": round(byt / t / 1e6 / (HBM_PEAK / 1e9), 4)})
+        for b in (src, work, order):
+            b.close()
+    return {"workload": "stable argsort of uniformly random 32-bit keys (4 passes of 8 bits), keys resident in HBM",
+            "runs": out}
 
 
 def batched_vs_loop(r, skm, batches, loops, single_tuning):
@@ -251,6 +279,8 @@ def main():
     ap.add_argument("--serving-rates", default="150,300,600,1200,2400")
     ap.add_argument("--lod-leaves", type=int, default=20_000_000, help="LOD leg's leaves (0 skips it)")
     ap.add_argument("--lod-frames", type=int, default=120)
+    ap.add_argument("--sort-sizes", default="7000000,16000000,154000000",
+                    help="radix-sort leg: key counts (empty skips it)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a known-size kernel for the HBM "
                          "byte counters of a profiler)")
@@ -331,6 +361,8 @@ def main():
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                     "frac": round(gbs * 1e9 / HBM_PEAK, 4), "traffic": None}
 
+    sizes = [int(x) for x in args.sort_sizes.split(",") if x]
+    sort_leg = radix_sort(r, skm, sizes) if sizes else None
     batched = batched_vs_loop(r, skm, args.batches, 2, plain_best) if args.batches > 0 else None
     r.close()
     lod = lod_flythrough(skm, args.lod_frames, args.lod_leaves) if args.lod_leaves > 0 else None
@@ -363,7 +395,8 @@ def main():
         "msplats_per_s": round(N_SPLATS * fps / 1e6, 1),
         "visible_per_frame": int(V), "pairs_per_frame": int(P),
         "gpu_ms_per_frame": round(tot["stage_ms"][5] / n_frames, 4),
-        "stages": stages, "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "serving": served, "lod": lod,
+        "stages": stages, "roofline": roofline, "cpu_baseline": cpu, "radix_sort": sort_leg, "batched": batched,
+        "serving": served, "lod": lod,
         "device": device, "scene_generation_s": round(gen_s, 2),
     }
     print(json.dumps(out), flush=True)

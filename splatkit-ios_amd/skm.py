@@ -382,6 +382,49 @@ class SkmError(RuntimeError):
         self.status = status
 
 
+class DeviceBuffer:
+    """Device memory from the HIP runtime libskm.so itself links (not a second runtime such as the one
+    a framework may bundle), for the device-pointer entry points in tests and benchmarks."""
+
+    def __init__(self, nbytes):
+        lib()  # loads libskm.so, and with it its HIP runtime
+        self._hip = C.CDLL("libamdhip64.so.7")
+        self._hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self._hip.hipFree.argtypes = [C.c_void_p]
+        self._hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.ptr = C.c_void_p()
+        self.nbytes = nbytes
+        if self._hip.hipMalloc(C.byref(self.ptr), max(1, nbytes)) != 0:
+            raise MemoryError(f"hipMalloc of {nbytes} bytes failed")
+
+    def upload(self, array):
+        a = np.ascontiguousarray(array)
+        assert a.nbytes <= self.nbytes
+        if self._hip.hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, 1) != 0:  # host to device
+            raise RuntimeError("hipMemcpy to device failed")
+
+    def copy_from(self, other, nbytes):
+        if self._hip.hipMemcpy(self.ptr, other.ptr, nbytes, 3) != 0:  # device to device
+            raise RuntimeError("hipMemcpy on device failed")
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype)
+        if self._hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes, 2) != 0:  # device to host
+            raise RuntimeError("hipMemcpy to host failed")
+        return out
+
+    def close(self):
+        if self.ptr:
+            self._hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Renderer:
     def __init__(self, device=0, pair_factor=0, sort_kernel=0, tile=TILE, raster_px=0):
         self._lib = lib()
