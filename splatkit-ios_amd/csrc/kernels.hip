@@ -1055,6 +1055,13 @@ __device__ __forceinline__ void blend(bool take, float e, const float4 Cc, float
   T = T - w;
 }
 
+// Records the raster tests per LDS round trip (a build may set it; it divides every batch size).
+#ifndef SKM_RASTER_GROUP
+#define SKM_RASTER_GROUP 8
+#endif
+constexpr uint32_t kGroup = SKM_RASTER_GROUP;
+static_assert(64u % kGroup == 0u, "group size");
+
 // Raster block shapes: a TILE x TILE tile per block, cut into square regions of 8 * PX pixels, one
 // wave per region, PX x PX pixels per lane (lane (x, y) of the 8 x 8 lane grid takes the pixels
 // (x + 8 i, y + 8 j)).
@@ -1201,26 +1208,24 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
     n = __builtin_amdgcn_readfirstlane(n);
     __syncthreads();
     if (__ballot(live()) != 0) {
-      const uint4* list = reinterpret_cast<const uint4*>(s_list[wave]);
-      for (uint32_t i = 0; i < n; i += 4u) {
-        // A group of four records: their geometry does not depend on the transmittance, so all four
-        // are read and tested together (one LDS latency per group); the blends then run in order.
-        const uint4 four = list[i >> 2];
-        const uint32_t addr[4] = {four.x, four.y, four.z, four.w};
-        float e[4][kP];
-        bool hit[4][kP];
-        float4 Cc[4];
+      for (uint32_t i = 0; i < n; i += kGroup) {
+        // A group of kGroup records: their geometry does not depend on the transmittance, so all of
+        // them are read and tested together (one LDS latency per group); the blends then run in order.
+        uint32_t addr[kGroup];
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) {
+        for (uint32_t u = 0; u < kGroup; ++u) addr[u] = s_list[wave][i + u];
+        float e[kGroup][kP];
+        bool hit[kGroup][kP];
+#pragma unroll
+        for (uint32_t u = 0; u < kGroup; ++u) {
           const float4* rec =
               reinterpret_cast<const float4*>(reinterpret_cast<const char*>(&s_rec[0][0]) + addr[u]);
           const float4 A = rec[0], B = rec[kB];
-          Cc[u] = rec[2 * kB];
 #pragma unroll
           for (uint32_t q = 0; q < kP; ++q) hit[u][q] = shape_test(A, B, fx[q], fy[q], e[u][q]) && i + u < n;
         }
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) {
+        for (uint32_t u = 0; u < kGroup; ++u) {
           bool take[kP], any = false;
 #pragma unroll
           for (uint32_t q = 0; q < kP; ++q) {
@@ -1230,8 +1235,11 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
           // A record no pixel of the wave takes leaves the wave's pixels exactly as they are: skip
           // its blend (a wave-uniform branch).
           if (__ballot(any) == 0) continue;
+          // The colour is read only for a record the wave blends.
+          const float4 Cc =
+              reinterpret_cast<const float4*>(reinterpret_cast<const char*>(&s_rec[0][0]) + addr[u])[2 * kB];
 #pragma unroll
-          for (uint32_t q = 0; q < kP; ++q) blend(take[q], e[u][q], Cc[u], T[q], C0[q], C1[q], C2[q]);
+          for (uint32_t q = 0; q < kP; ++q) blend(take[q], e[u][q], Cc, T[q], C0[q], C1[q], C2[q]);
         }
         if (__ballot(live()) == 0) break;
       }
