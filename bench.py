@@ -373,10 +373,15 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu_fps, cpu_stages = cpu_baseline(args.cpu_frames, 1)
+        threads = min(16, os.cpu_count() or 1)  # the host share of one GPU
+        mt_fps, mt_stages = cpu_baseline(args.cpu_frames, threads)
         cpu = {"value": round(cpu_fps, 4), "unit": "frames/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_frames} orbit frames of the 3M SH3 1920x1080 workload, whole frame "
                          "(cull, stable sort, bin, fp32 raster) on one host thread",
                "stage_ms": dict(zip(["cull", "sort", "bin", "raster"], cpu_stages)),
+               "threaded": {"value": round(mt_fps, 4), "cores": threads,
+                            "note": "cull and raster over std::thread workers; sort and bin on one thread",
+                            "stage_ms": dict(zip(["cull", "sort", "bin", "raster"], mt_stages))},
                "host_threads_available": os.cpu_count()}
 
     out = {

@@ -877,46 +877,76 @@ uint32_t orc_tile_rect(const uint32_t* p, uint32_t width, uint32_t height, uint3
 
 // (tile | depth) order: pairs emitted in depth order (row-major over each rectangle), then a
 // stable sort by tile; equal tiles keep depth order, equal depths keep compaction order. Tile
-// ranges are [start, end) per tile, zero for an empty tile.
-uint64_t orc_bin(const uint32_t* projected, const uint32_t* order, uint32_t n, uint32_t width,
-                 uint32_t height, uint32_t tile, uint32_t* pair_tiles, uint32_t* pair_values,
-                 uint64_t capacity, uint32_t* ranges) {
+// ranges are [start, end) per tile, zero for an empty tile. The sort is a counting sort: per-tile
+// counts, their exclusive scan, then each pair at its tile's cursor.
+static void bin_pairs(const uint32_t* projected, const uint32_t* order, uint32_t n, uint32_t width,
+                      uint32_t height, uint32_t tile, std::vector<uint32_t>* pair_tiles,
+                      std::vector<uint32_t>* pair_values, uint32_t* ranges) {
   const uint32_t tiles_x = (width + tile - 1) / tile;
   const uint32_t tiles_y = (height + tile - 1) / tile;
   const uint32_t T = tiles_x * tiles_y;
-  uint64_t total = 0;
+  std::vector<int32_t> rects(size_t{n} * 4);
+  std::vector<uint64_t> cursor(size_t{T} + 1, 0);
   for (uint32_t i = 0; i < n; ++i) {
-    int32_t rect[4];
-    total += orc_tile_rect(projected + size_t{order[i]} * 8u, width, height, tile, rect);
-  }
-  if (ranges != nullptr) std::memset(ranges, 0, size_t{T} * 8u);
-  if (total > capacity || pair_tiles == nullptr) return total;
-  std::vector<uint32_t> tiles, values;
-  tiles.reserve(total);
-  values.reserve(total);
-  for (uint32_t i = 0; i < n; ++i) {
-    int32_t rect[4];
-    if (!orc_tile_rect(projected + size_t{order[i]} * 8u, width, height, tile, rect)) continue;
+    int32_t* rect = &rects[size_t{i} * 4];
+    if (!orc_tile_rect(projected + size_t{order[i]} * 8u, width, height, tile, rect)) {
+      rect[0] = 0, rect[1] = 0, rect[2] = -1, rect[3] = -1;
+      continue;
+    }
     for (int32_t ty = rect[1]; ty <= rect[3]; ++ty)
-      for (int32_t tx = rect[0]; tx <= rect[2]; ++tx) {
-        tiles.push_back(static_cast<uint32_t>(ty) * tiles_x + static_cast<uint32_t>(tx));
-        values.push_back(order[i]);
-      }
+      for (int32_t tx = rect[0]; tx <= rect[2]; ++tx)
+        ++cursor[size_t{static_cast<uint32_t>(ty) * tiles_x + static_cast<uint32_t>(tx)} + 1];
   }
-  std::vector<uint32_t> perm(total);
-  std::iota(perm.begin(), perm.end(), 0u);
-  std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return tiles[a] < tiles[b]; });
-  for (uint64_t j = 0; j < total; ++j) {
-    pair_tiles[j] = tiles[perm[j]];
-    pair_values[j] = values[perm[j]];
-  }
+  for (uint32_t t = 0; t < T; ++t) cursor[t + 1] += cursor[t];
   if (ranges != nullptr) {
-    for (uint64_t j = 0; j < total; ++j) {
-      const uint32_t t = pair_tiles[j];
-      if (j == 0 || pair_tiles[j - 1] != t) ranges[2 * t] = static_cast<uint32_t>(j);
-      if (j + 1 == total || pair_tiles[j + 1] != t) ranges[2 * t + 1] = static_cast<uint32_t>(j + 1);
+    std::memset(ranges, 0, size_t{T} * 8u);
+    for (uint32_t t = 0; t < T; ++t) {
+      if (cursor[t + 1] == cursor[t]) continue;
+      ranges[2 * t] = static_cast<uint32_t>(cursor[t]);
+      ranges[2 * t + 1] = static_cast<uint32_t>(cursor[t + 1]);
     }
   }
+  pair_tiles->resize(cursor[T]);
+  pair_values->resize(cursor[T]);
+  for (uint32_t i = 0; i < n; ++i) {
+    const int32_t* rect = &rects[size_t{i} * 4];
+    for (int32_t ty = rect[1]; ty <= rect[3]; ++ty)
+      for (int32_t tx = rect[0]; tx <= rect[2]; ++tx) {
+        const uint32_t t = static_cast<uint32_t>(ty) * tiles_x + static_cast<uint32_t>(tx);
+        const uint64_t j = cursor[t]++;
+        (*pair_tiles)[j] = t;
+        (*pair_values)[j] = order[i];
+      }
+  }
+}
+
+uint64_t orc_bin(const uint32_t* projected, const uint32_t* order, uint32_t n, uint32_t width,
+                 uint32_t height, uint32_t tile, uint32_t* pair_tiles, uint32_t* pair_values,
+                 uint64_t capacity, uint32_t* ranges) {
+  if (pair_tiles == nullptr) {
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      int32_t rect[4];
+      total += orc_tile_rect(projected + size_t{order[i]} * 8u, width, height, tile, rect);
+    }
+    if (ranges != nullptr) {
+      const uint32_t T = ((width + tile - 1) / tile) * ((height + tile - 1) / tile);
+      std::memset(ranges, 0, size_t{T} * 8u);
+    }
+    return total;
+  }
+  std::vector<uint32_t> pt, pv;
+  bin_pairs(projected, order, n, width, height, tile, &pt, &pv, ranges);
+  const uint64_t total = pt.size();
+  if (total > capacity) {
+    if (ranges != nullptr) {
+      const uint32_t T = ((width + tile - 1) / tile) * ((height + tile - 1) / tile);
+      std::memset(ranges, 0, size_t{T} * 8u);
+    }
+    return total;
+  }
+  std::copy(pt.begin(), pt.end(), pair_tiles);
+  std::copy(pv.begin(), pv.end(), pair_values);
   return total;
 }
 
@@ -1066,11 +1096,9 @@ int orc_render_frame(const orc_camera* cam, const uint32_t* splats, const uint32
   auto t2 = clock::now();
   const uint32_t tiles_x = (width + tile - 1) / tile, tiles_y = (height + tile - 1) / tile;
   std::vector<uint32_t> ranges(size_t{tiles_x} * tiles_y * 2u);
-  const uint64_t pairs = orc_bin(projected.data(), order.data(), visible, width, height, tile,
-                                 nullptr, nullptr, 0, nullptr);
-  std::vector<uint32_t> pt(pairs), pv(pairs);
-  orc_bin(projected.data(), order.data(), visible, width, height, tile, pt.data(), pv.data(), pairs,
-          ranges.data());
+  std::vector<uint32_t> pt, pv;
+  bin_pairs(projected.data(), order.data(), visible, width, height, tile, &pt, &pv, ranges.data());
+  const uint64_t pairs = pt.size();
   auto t3 = clock::now();
   orc_raster32(projected.data(), pv.data(), ranges.data(), width, height, tile, background3,
                1.0f / 4096.0f, threads, out_rgba);
