@@ -1035,8 +1035,8 @@ __device__ __forceinline__ bool shape_test(const float4 A, const float4 B, float
   const float gu = fmaf(A.x, fx, fmaf(A.y, fy, A.z));
   const float gv = fmaf(A.w, fx, fmaf(B.x, fy, B.y));
   e = fmaf(-gu, gu, fmaf(-gv, gv, B.w));
-  return static_cast<int>(fabsf(gu) <= B.z) & static_cast<int>(fabsf(gv) <= B.z) &
-         static_cast<int>(e >= kLog2MinAlpha);
+  // One compare for the quad: a NaN or infinite coordinate fails the cutoff test through e.
+  return static_cast<int>(fmaxf(fabsf(gu), fabsf(gv)) <= B.z) & static_cast<int>(e >= kLog2MinAlpha);
 }
 // 2^x clamped to [0, 1] by the instruction's output clamp. 2^x is monotonic with 2^0 = 1, so this is
 // 2^min(x, 0) for every x, and 0 for x = -inf.
@@ -1088,8 +1088,11 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
   constexpr uint32_t kB = S::kThreads, kW = S::kWaves, kP = S::kPixels;
   // s_rec[0..2][k]: record k's A, B (coverage) and colour. The region lists hold the byte offset of
   // s_rec[0][k] (16 k), which the LDS reads take as their address with no arithmetic.
-  __shared__ float4 s_rec[3][kB];
-  __shared__ __align__(16) uint32_t s_list[kW][kB];
+  // Slot kB of each record array is a null record that no pixel takes (radius -1): region lists are
+  // padded with it to whole groups.
+  constexpr uint32_t kS = kB + 1u;
+  __shared__ float4 s_rec[3][kS];
+  __shared__ __align__(16) uint32_t s_list[kW][kB + kGroup];
   __shared__ uint32_t s_count[kW][kW];  // [region][wave]
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
   const uint32_t tile_id = a.tile_order != nullptr ? a.tile_order[blockIdx.x] : blockIdx.x;  // global tile id
@@ -1124,9 +1127,11 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
     for (uint32_t q = 0; q < kP; ++q) any |= T[q] >= kTransmittanceStop;
     return any;
   };
-  // Every list entry ever written is a record offset, so a group of four entries read past a list's
-  // end still addresses records (whose results the group then ignores).
-  reinterpret_cast<uint4*>(&s_list[0][0])[tid % (kW * kB / 4u)] = make_uint4(0u, 0u, 0u, 0u);
+  if (tid == 0) {
+    s_rec[0][kB] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    s_rec[1][kB] = make_float4(0.0f, 0.0f, -1.0f, -__builtin_inff());
+    s_rec[2][kB] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
   // Two-stage prefetch of the pair records: while batch b is prepared and walked, the record of batch
   // b + 1 and the pair value of batch b + 2 are in flight.
   auto record = [&](uint32_t idx, uint4& w0, uint4& w1) {
@@ -1207,6 +1212,7 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
     for (uint32_t w = 0; w < kW; ++w) n += s_count[wave][w];
     n = __builtin_amdgcn_readfirstlane(n);
     __syncthreads();
+    if (lane < kGroup) s_list[wave][n + lane] = kB * static_cast<uint32_t>(sizeof(float4));  // the null record
     if (__ballot(live()) != 0) {
       for (uint32_t i = 0; i < n; i += kGroup) {
         // A group of kGroup records: their geometry does not depend on the transmittance, so all of
@@ -1220,9 +1226,9 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
         for (uint32_t u = 0; u < kGroup; ++u) {
           const float4* rec =
               reinterpret_cast<const float4*>(reinterpret_cast<const char*>(&s_rec[0][0]) + addr[u]);
-          const float4 A = rec[0], B = rec[kB];
+          const float4 A = rec[0], B = rec[kS];
 #pragma unroll
-          for (uint32_t q = 0; q < kP; ++q) hit[u][q] = shape_test(A, B, fx[q], fy[q], e[u][q]) && i + u < n;
+          for (uint32_t q = 0; q < kP; ++q) hit[u][q] = shape_test(A, B, fx[q], fy[q], e[u][q]);
         }
 #pragma unroll
         for (uint32_t u = 0; u < kGroup; ++u) {
@@ -1237,7 +1243,7 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
           if (__ballot(any) == 0) continue;
           // The colour is read only for a record the wave blends.
           const float4 Cc =
-              reinterpret_cast<const float4*>(reinterpret_cast<const char*>(&s_rec[0][0]) + addr[u])[2 * kB];
+              reinterpret_cast<const float4*>(reinterpret_cast<const char*>(&s_rec[0][0]) + addr[u])[2 * kS];
 #pragma unroll
           for (uint32_t q = 0; q < kP; ++q) blend(take[q], e[u][q], Cc, T[q], C0[q], C1[q], C2[q]);
         }
