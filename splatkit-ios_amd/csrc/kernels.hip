@@ -688,10 +688,23 @@ __global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict_
   uint32_t key[kSortItems];
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) key[j] = keys[min(base + j * 256u + tid, n - 1u)];
+  // Runs of equal digits in a wave row (common: pairs come out a survivor at a time, and high digits
+  // change slowly) add once per run: a lane starts a run when its digit differs from the previous
+  // lane's, and the run ends at the next start or at the row's last key (a row's keys are a prefix
+  // of its lanes, in both the packed and the gapped layout).
+  const uint32_t lane = tid & 63u;
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) {
     const uint32_t row = base + j * 256u + wave * 64u;
-    if (sort_valid(row + (tid & 63u), n, row_limit(seg, row, n))) atomicAdd(&s_h[wave][(key[j] >> shift) & mask], 1u);
+    const bool valid = sort_valid(row + lane, n, row_limit(seg, row, n));
+    const uint32_t d = (key[j] >> shift) & mask;
+    const uint32_t prev = __shfl_up(d, 1u, 64);
+    const bool start = valid && (lane == 0u || prev != d);
+    const uint64_t starts = __ballot(start);
+    const uint32_t keys_in_row = static_cast<uint32_t>(__popcll(__ballot(valid)));
+    const uint64_t later = starts & ~((2ull << lane) - 1ull);  // starts after this lane
+    const uint32_t end = later != 0ull ? static_cast<uint32_t>(__builtin_ctzll(later)) : keys_in_row;
+    if (start) atomicAdd(&s_h[wave][d], end - lane);
   }
   __syncthreads();
   counts[static_cast<size_t>(tid) * stride + blockIdx.x] = s_h[0][tid] + s_h[1][tid] + s_h[2][tid] + s_h[3][tid];
