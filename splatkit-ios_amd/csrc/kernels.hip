@@ -1334,7 +1334,7 @@ uint32_t hist_grid(uint32_t capacity) { return std::min(std::max(grid_for(capaci
 
 // One stable LSD pass over the digit of `bits` bits at `shift`, in the chosen variant. The kernel
 // ranks with BITS >= bits ballots (the digit's missing high bits are zero in every lane); `counts`
-// holds 256 words per 4096-key tile of `capacity` (reduce then scan only).
+// holds 256 words per kSortTile-key tile of `capacity` (reduce then scan only).
 template <bool LOOKBACK, int BITS>
 void sort_pass_bits(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
                     const uint32_t* n_ptr, uint32_t tiles, uint32_t* hist, uint32_t shift, uint32_t bits,

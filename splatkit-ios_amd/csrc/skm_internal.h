@@ -123,10 +123,10 @@ constexpr uint32_t kCullTile = kThreads * kCullItems;
 constexpr uint32_t kBinItems = 4;   // survivors per thread of the binning (1024 per block)
 // Keys per thread of a radix pass (a block sorts 256 x kSortItems keys); a build may set it.
 #ifndef SKM_SORT_ITEMS
-#define SKM_SORT_ITEMS 16
+#define SKM_SORT_ITEMS 12
 #endif
 constexpr uint32_t kSortItems = SKM_SORT_ITEMS;
-constexpr uint32_t kSortTile = kThreads * kSortItems;  // keys per radix-pass block (4096 at 16)
+constexpr uint32_t kSortTile = kThreads * kSortItems;  // keys per radix-pass block (3072 at 12)
 constexpr uint32_t kRadix = 256;
 constexpr uint32_t kMaxRanges = 65536;  // MetalVisibilityPass.h:72
 constexpr uint32_t kOrderCounters = 2u * 4u * 256u;  // offset of the tile-order counters in hist
@@ -189,7 +189,7 @@ struct FrameArgs {
   uint32_t* view_visible;      // survivors per view
   const uint16_t* tile_view;   // the view of each global tile
   uint32_t total_tiles;        // tiles of all views
-  // Radix sort variant and its tile-count scratch (256 words per 4096-key tile of the larger of the
+  // Radix sort variant and its tile-count scratch (256 words per kSortTile-key tile of the larger of the
   // survivor and pair capacities).
   int sort_variant;
   uint32_t* sort_counts;
