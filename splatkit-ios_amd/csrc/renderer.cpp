@@ -1,5 +1,6 @@
-// The C-ABI renderer (include/skm.h): resident worlds in HBM, two frames in flight on one HIP
-// stream, the frame pipeline of kernels.hip, per-stage HIP-event timing and readback.
+// The C-ABI renderer (include/skm.h): resident worlds in HBM, two frames in flight (each frame
+// slot with its own HIP stream and scratch, so consecutive frames overlap on the GPU), the frame
+// pipeline of kernels.hip, per-stage HIP-event timing and readback.
 //
 // Mirrors MetalSplatRenderer (packages/splatkit-ios/Sources/SplatKitCore/rendering/metal/
 // MetalSplatRenderer.mm): transactional uploads (:166-208), range validation before any GPU work
@@ -81,7 +82,22 @@ struct WorldBuffers {
 constexpr size_t kInputRanges = 256;
 constexpr size_t kInputBytes = kInputRanges + size_t{skm::kMaxRanges} * 8 + size_t{skm::kMaxRanges + 1} * 4 + 16;
 
+// Device memory the frame pipeline writes: control block, survivor arrays, tile pairs and ranges.
+// Each frame slot owns one, so frames on the two slots' streams overlap; the standalone sort owns
+// another.
+struct Scratch {
+  uint32_t capacity = 0;       // survivor lanes
+  uint32_t pair_capacity = 0;  // tile pairs
+  uint32_t tile_capacity = 0;  // tiles
+  DeviceBuffer ctrl, hist, cull_keys, keys[2], order[2], projected, rects, scan_status, sort_status, sort_counts;
+  DeviceBuffer pair_keys[2], pair_vals[2], tile_ranges;
+  size_t sort_status_blocks = 0;
+  uint64_t pipelines = 0;  // prologue launches since the status arrays were last cleared
+};
+
 struct Slot {
+  hipStream_t stream = nullptr;
+  Scratch x;
   DeviceBuffer inputs, image;
   PinnedBuffer inputs_host, ctrl_host, lod_host;
   hipEvent_t marks[6] = {};
@@ -229,18 +245,16 @@ struct skm_renderer {
   uint32_t last_selected = 0;
   uint32_t lod_stats[6] = {};
 
-  // Frame scratch, shared by the frames on the stream.
-  uint32_t capacity = 0;       // survivor slots
-  uint32_t pair_capacity = 0;  // tile pairs
-  uint32_t tile_capacity = 0;  // tiles
-  DeviceBuffer ctrl, cull_keys, keys[2], order[2], projected, rects, scan_status, sort_status, sort_counts, hist;
+  // Sizes every slot's scratch reaches before its next frame: survivor lanes (whole cull tiles)
+  // and tile pairs (grown after a frame overflows).
+  uint32_t capacity = 0;
+  uint64_t pair_target = 0;
   int sort_variant = skm::kSortReduceScan;
-  DeviceBuffer pair_keys[2], pair_vals[2], tile_ranges;
-  size_t sort_status_blocks = 0;
-  uint64_t pipelines = 0;  // prologue launches since the status arrays were last cleared
 
-  // Standalone sort scratch.
+  // Standalone sort scratch, on `stream`.
+  Scratch sort;
   DeviceBuffer sort_keys, sort_vals, sort_n;
+  hipEvent_t sort_marks[2] = {};
   uint32_t sort_capacity = 0;
 
   Slot slots[kSlots];
@@ -338,24 +352,36 @@ struct skm_renderer {
     return st;
   }
 
-  skm_status ensure_status(uint32_t items) {
+  skm_status ensure_status(Scratch& x, uint32_t items, hipStream_t st_) {
     const size_t blocks = (size_t{items} + skm::kSortTile - 1) / skm::kSortTile + 1;
-    if (blocks <= sort_status_blocks) return SKM_OK;
+    if (blocks <= x.sort_status_blocks) return SKM_OK;
     DeviceBuffer b, c;
     if (auto st = check(b.allocate(blocks * skm::kRadix * 8u), "sort status"); st != SKM_OK) return st;
     if (auto st = check(c.allocate(blocks * skm::kRadix * 4u), "sort counts"); st != SKM_OK) return st;
-    if (auto st = check(hipMemsetAsync(b.ptr, 0, b.bytes, stream), "clear"); st != SKM_OK) return st;
-    sort_status = std::move(b);
-    sort_counts = std::move(c);
-    sort_status_blocks = blocks;
+    if (auto st = check(hipMemsetAsync(b.ptr, 0, b.bytes, st_), "clear"); st != SKM_OK) return st;
+    x.sort_status = std::move(b);
+    x.sort_counts = std::move(c);
+    x.sort_status_blocks = blocks;
     return SKM_OK;
   }
 
-  // Survivor scratch for `cap` lanes. Transactional: nothing changes unless every buffer fits.
-  skm_status ensure_capacity(uint32_t cap) {
-    // Whole cull tiles: the cull writes survivors in a gapped layout of kCullTile lanes per tile.
-    cap = static_cast<uint32_t>(align_up(std::max(cap, 1u), skm::kCullTile));
-    if (cap <= capacity) return SKM_OK;
+  // The control block and histograms every pipeline starts from.
+  skm_status ensure_ctrl(Scratch& x, hipStream_t st_) {
+    if (x.ctrl.ptr != nullptr) return SKM_OK;
+    DeviceBuffer c, h;
+    if (auto st = check(c.allocate(sizeof(Ctrl)), "control"); st != SKM_OK) return st;
+    if (auto st = check(h.allocate(skm::kHistWords * 4), "histograms"); st != SKM_OK) return st;
+    if (auto st = check(hipMemsetAsync(c.ptr, 0, sizeof(Ctrl), st_), "clear"); st != SKM_OK) return st;
+    x.ctrl = std::move(c);
+    x.hist = std::move(h);
+    return SKM_OK;
+  }
+
+  // Survivor scratch for `cap` lanes in one slot, which has no frame in flight. Transactional:
+  // nothing changes unless every buffer fits.
+  skm_status ensure_capacity(Slot& s, uint32_t cap) {
+    Scratch& x = s.x;
+    if (cap <= x.capacity) return SKM_OK;
     DeviceBuffer ck, k0, k1, o0, o1, pr, rc, ss;
     const size_t n = size_t{cap};
     if (auto st = check(rc.allocate(n * 8), "rects"); st != SKM_OK) return st;
@@ -366,58 +392,81 @@ struct skm_renderer {
     if (auto st = check(o1.allocate(n * 4), "order"); st != SKM_OK) return st;
     if (auto st = check(pr.allocate(n * 32), "projected"); st != SKM_OK) return st;
     if (auto st = check(ss.allocate(((n + 255) / 256 + 1) * 8), "scan status"); st != SKM_OK) return st;
-    if (auto st = check(hipMemsetAsync(ss.ptr, 0, ss.bytes, stream), "clear"); st != SKM_OK) return st;
-    if (auto st = ensure_status(cap); st != SKM_OK) return st;
-    cull_keys = std::move(ck);
-    keys[0] = std::move(k0);
-    keys[1] = std::move(k1);
-    order[0] = std::move(o0);
-    order[1] = std::move(o1);
-    projected = std::move(pr);
-    rects = std::move(rc);
-    scan_status = std::move(ss);
-    capacity = cap;
+    if (auto st = check(hipMemsetAsync(ss.ptr, 0, ss.bytes, s.stream), "clear"); st != SKM_OK) return st;
+    if (auto st = ensure_status(x, cap, s.stream); st != SKM_OK) return st;
+    x.cull_keys = std::move(ck);
+    x.keys[0] = std::move(k0);
+    x.keys[1] = std::move(k1);
+    x.order[0] = std::move(o0);
+    x.order[1] = std::move(o1);
+    x.projected = std::move(pr);
+    x.rects = std::move(rc);
+    x.scan_status = std::move(ss);
+    x.capacity = cap;
     return SKM_OK;
   }
 
-  skm_status ensure_pairs(uint64_t want) {
-    want = std::min<uint64_t>(std::max<uint64_t>(want, uint64_t{1} << 22), 0xfffff000ull);
-    if (want <= pair_capacity) return SKM_OK;
+  // Every slot's survivor scratch; the caller has drained the frames.
+  skm_status ensure_capacity(uint32_t cap) {
+    // Whole cull tiles: the cull writes survivors in a gapped layout of kCullTile lanes per tile.
+    cap = static_cast<uint32_t>(align_up(std::max(cap, 1u), skm::kCullTile));
+    for (Slot& s : slots)
+      if (auto st = ensure_capacity(s, cap); st != SKM_OK) return st;
+    capacity = std::max(capacity, cap);
+    return SKM_OK;
+  }
+
+  static uint64_t clamp_pairs(uint64_t want) {
+    return std::min<uint64_t>(std::max<uint64_t>(want, uint64_t{1} << 22), 0xfffff000ull);
+  }
+
+  skm_status ensure_pairs(Slot& s, uint64_t want) {
+    Scratch& x = s.x;
+    want = clamp_pairs(want);
+    if (want <= x.pair_capacity) return SKM_OK;
     DeviceBuffer k0, k1, v0, v1;
     const size_t n = static_cast<size_t>(want);
     if (auto st = check(k0.allocate(n * 4), "pair keys"); st != SKM_OK) return st;
     if (auto st = check(k1.allocate(n * 4), "pair keys"); st != SKM_OK) return st;
     if (auto st = check(v0.allocate(n * 4), "pair values"); st != SKM_OK) return st;
     if (auto st = check(v1.allocate(n * 4), "pair values"); st != SKM_OK) return st;
-    if (auto st = ensure_status(static_cast<uint32_t>(want)); st != SKM_OK) return st;
-    pair_keys[0] = std::move(k0);
-    pair_keys[1] = std::move(k1);
-    pair_vals[0] = std::move(v0);
-    pair_vals[1] = std::move(v1);
-    pair_capacity = static_cast<uint32_t>(want);
+    if (auto st = ensure_status(x, static_cast<uint32_t>(want), s.stream); st != SKM_OK) return st;
+    x.pair_keys[0] = std::move(k0);
+    x.pair_keys[1] = std::move(k1);
+    x.pair_vals[0] = std::move(v0);
+    x.pair_vals[1] = std::move(v1);
+    x.pair_capacity = static_cast<uint32_t>(want);
     return SKM_OK;
   }
 
-  skm_status ensure_tiles(uint32_t tiles) {
-    if (tiles <= tile_capacity) return SKM_OK;
-    if (auto st = drain(); st != SKM_OK) return st;  // a frame in flight may still read the ranges
+  // Every slot's pair scratch; the caller has drained the frames.
+  skm_status ensure_pairs(uint64_t want) {
+    for (Slot& s : slots)
+      if (auto st = ensure_pairs(s, want); st != SKM_OK) return st;
+    pair_target = std::max(pair_target, clamp_pairs(want));
+    return SKM_OK;
+  }
+
+  skm_status ensure_tiles(Slot& s, uint32_t tiles) {
+    Scratch& x = s.x;
+    if (tiles <= x.tile_capacity) return SKM_OK;
     DeviceBuffer t;
     // Ranges (2 words per tile), then the raster's tile order (1 word per tile).
     if (auto st = check(t.allocate(size_t{tiles} * 12u), "tile ranges"); st != SKM_OK) return st;
-    tile_ranges = std::move(t);
-    tile_capacity = tiles;
+    x.tile_ranges = std::move(t);
+    x.tile_capacity = tiles;
     return SKM_OK;
   }
 
-  // Clears look-back status words well before the 30-bit tags they carry could repeat.
-  skm_status maybe_clear_status() {
-    if (++pipelines < (1ull << 24)) return SKM_OK;
-    if (auto st = drain(); st != SKM_OK) return st;
-    pipelines = 1;
-    if (auto st = check(hipMemsetAsync(sort_status.ptr, 0, sort_status.bytes, stream), "clear");
-        st != SKM_OK)
+  // Clears look-back status words well before the 30-bit tags they carry could repeat. The
+  // scratch has no pipeline in flight.
+  skm_status maybe_clear_status(Scratch& x, hipStream_t st_) {
+    if (++x.pipelines < (1ull << 24)) return SKM_OK;
+    x.pipelines = 1;
+    if (auto st = check(hipMemsetAsync(x.sort_status.ptr, 0, x.sort_status.bytes, st_), "clear"); st != SKM_OK)
       return st;
-    return check(hipMemsetAsync(scan_status.ptr, 0, scan_status.bytes, stream), "clear");
+    if (x.scan_status.ptr == nullptr) return SKM_OK;
+    return check(hipMemsetAsync(x.scan_status.ptr, 0, x.scan_status.bytes, st_), "clear");
   }
 
   skm_status upload_range(WorldBuffers& w, uint32_t offset, const skm_cloud_view* c, int src_degree) {
@@ -494,18 +543,18 @@ skm_status skm_renderer_create(const skm_config* config, skm_renderer** out) {
   std::snprintf(desc, sizeof(desc), "%s (%s, %d CUs), HIP %d.%d", prop.name[0] ? prop.name : "AMD GPU", prop.gcnArchName,
                 prop.multiProcessorCount, runtime / 10000000, (runtime / 100000) % 100);
   r->description = desc;
-  bool ok = r->ctrl.allocate(sizeof(Ctrl)) == hipSuccess &&
-            hipMemset(r->ctrl.ptr, 0, sizeof(Ctrl)) == hipSuccess &&
-            r->hist.allocate(skm::kHistWords * 4) == hipSuccess;
+  bool ok = hipEventCreate(&r->sort_marks[0]) == hipSuccess && hipEventCreate(&r->sort_marks[1]) == hipSuccess;
   for (Slot& s : r->slots) {
-    ok = ok && s.inputs.allocate(kInputBytes) == hipSuccess && s.inputs_host.allocate(kInputBytes) == hipSuccess &&
+    ok = ok && hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) == hipSuccess &&
+         r->ensure_ctrl(s.x, s.stream) == SKM_OK && r->ensure_tiles(s, 1) == SKM_OK &&
+         s.inputs.allocate(kInputBytes) == hipSuccess && s.inputs_host.allocate(kInputBytes) == hipSuccess &&
          s.ctrl_host.allocate(sizeof(Ctrl)) == hipSuccess &&
          s.lod_host.allocate(sizeof(skm::LodState) + 16) == hipSuccess &&
          hipEventCreate(&s.done) == hipSuccess;
     for (auto& m : s.marks) ok = ok && hipEventCreate(&m) == hipSuccess;
   }
-  ok = ok && r->ensure_capacity(1) == SKM_OK && r->ensure_pairs(0) == SKM_OK &&
-       r->ensure_tiles(1) == SKM_OK && hipStreamSynchronize(r->stream) == hipSuccess;
+  ok = ok && r->ensure_capacity(1) == SKM_OK && r->ensure_pairs(0) == SKM_OK;
+  for (Slot& s : r->slots) ok = ok && hipStreamSynchronize(s.stream) == hipSuccess;
   if (!ok) {
     skm_renderer_destroy(r);
     return SKM_ERR_DEVICE;
@@ -518,8 +567,13 @@ void skm_renderer_destroy(skm_renderer* r) {
   if (r == nullptr) return;
   (void)hipSetDevice(r->device);
   if (r->stream) (void)hipStreamSynchronize(r->stream);
+  for (Slot& s : r->slots)
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
   r->batch.reset();
+  for (auto& m : r->sort_marks)
+    if (m) (void)hipEventDestroy(m);
   for (Slot& s : r->slots) {
+    if (s.stream) (void)hipStreamDestroy(s.stream);
     if (s.done) (void)hipEventDestroy(s.done);
     for (auto& m : s.marks)
       if (m) (void)hipEventDestroy(m);
@@ -774,14 +828,14 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   const uint32_t tiles_x = (W + r->tile - 1) / r->tile, tiles_y = (H + r->tile - 1) / r->tile;
   const uint32_t slot_id = static_cast<uint32_t>(r->submitted % kSlots);
   Slot& s = r->slots[slot_id];
+  // The slot's previous frame ends here; its scratch is free from then on.
   if (auto st = r->harvest(s); st != SKM_OK) return st;
   if (r->grow_pairs) {
-    if (auto st = r->drain(); st != SKM_OK) return st;
-    const uint64_t want = std::max<uint64_t>(r->pairs_needed + r->pairs_needed / 4, uint64_t{r->pair_capacity} * 2);
-    if (auto st = r->ensure_pairs(want); st != SKM_OK) return st;
+    r->pair_target = skm_renderer::clamp_pairs(std::max<uint64_t>(r->pairs_needed + r->pairs_needed / 4, r->pair_target * 2));
     r->grow_pairs = false;
   }
-  if (auto st = r->ensure_tiles(tiles_x * tiles_y); st != SKM_OK) return st;
+  if (auto st = r->ensure_pairs(s, r->pair_target); st != SKM_OK) return st;
+  if (auto st = r->ensure_tiles(s, tiles_x * tiles_y); st != SKM_OK) return st;
   if (s.width != W || s.height != H) {
     DeviceBuffer img;
     if (auto st = r->check(img.allocate(size_t{W} * H * 16u), "image"); st != SKM_OK) return st;
@@ -789,7 +843,8 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
     s.width = W;
     s.height = H;
   }
-  if (auto st = r->maybe_clear_status(); st != SKM_OK) return st;
+  Scratch& x = s.x;
+  if (auto st = r->maybe_clear_status(x, s.stream); st != SKM_OK) return st;
 
   char* inputs = s.inputs_host.as<char>();
   Camera& cam = *reinterpret_cast<Camera*>(inputs);
@@ -805,7 +860,7 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
     sh[i + 1] = sh[i] + f->ranges[i].count;
   }
   const size_t input_bytes = f->range_count > 0 ? starts_at + size_t{f->range_count + 1} * 4 : sizeof(Camera);
-  hipError_t e = hipMemcpyAsync(s.inputs.ptr, inputs, input_bytes, hipMemcpyHostToDevice, r->stream);
+  hipError_t e = hipMemcpyAsync(s.inputs.ptr, inputs, input_bytes, hipMemcpyHostToDevice, s.stream);
   if (auto st = r->check(e, "frame inputs"); st != SKM_OK) return st;
   const Camera* cam_dev = s.inputs.as<Camera>();
 
@@ -813,6 +868,10 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   const bool lod_frame = world && r->lod_world;
   skm::LodArgs lod{};
   if (lod_frame) {
+    // The tree's selection buffers are one set: a cut waits for the other slot's frame.
+    const Slot& other = r->slots[(slot_id + 1) % kSlots];
+    if (other.in_flight)
+      if (auto st = r->check(hipStreamWaitEvent(s.stream, other.done, 0), "frame order"); st != SKM_OK) return st;
     s.lod_limit = r->policy.lod_splat_limit == 0 ? r->lod.capacity : std::min(r->policy.lod_splat_limit, r->lod.capacity);
     lod = r->lod.args(cam_dev, s.lod_limit, r->threshold.pixels, 4.0f, 1);
   }
@@ -832,45 +891,45 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   a.tiles_x = tiles_x;
   a.tiles_y = tiles_y;
   for (int c = 0; c < 3; ++c) a.background[c] = static_cast<float>(r->policy.background[c]);
-  a.ctrl = r->ctrl.as<Ctrl>();
-  a.cull_keys = r->cull_keys.as<uint32_t>();
+  a.ctrl = x.ctrl.as<Ctrl>();
+  a.cull_keys = x.cull_keys.as<uint32_t>();
   for (int k = 0; k < 2; ++k) {
-    a.keys[k] = r->keys[k].as<uint32_t>();
-    a.order[k] = r->order[k].as<uint32_t>();
-    a.pair_keys[k] = r->pair_keys[k].as<uint32_t>();
-    a.pair_vals[k] = r->pair_vals[k].as<uint32_t>();
+    a.keys[k] = x.keys[k].as<uint32_t>();
+    a.order[k] = x.order[k].as<uint32_t>();
+    a.pair_keys[k] = x.pair_keys[k].as<uint32_t>();
+    a.pair_vals[k] = x.pair_vals[k].as<uint32_t>();
   }
-  a.projected = r->projected.as<uint32_t>();
-  a.rects = r->rects.as<uint64_t>();
-  a.scan_status = r->scan_status.as<uint64_t>();
-  a.cull_counts = reinterpret_cast<uint32_t*>(r->scan_status.ptr);
-  a.sort_status = r->sort_status.as<uint64_t>();
-  a.hist = r->hist.as<uint32_t>();
-  a.pair_capacity = r->pair_capacity;
-  a.tile_ranges = r->tile_ranges.as<uint32_t>();
-  a.tile_order = r->tile_ranges.as<uint32_t>() + size_t{r->tile_capacity} * 2;
+  a.projected = x.projected.as<uint32_t>();
+  a.rects = x.rects.as<uint64_t>();
+  a.scan_status = x.scan_status.as<uint64_t>();
+  a.cull_counts = reinterpret_cast<uint32_t*>(x.scan_status.ptr);
+  a.sort_status = x.sort_status.as<uint64_t>();
+  a.hist = x.hist.as<uint32_t>();
+  a.pair_capacity = x.pair_capacity;
+  a.tile_ranges = x.tile_ranges.as<uint32_t>();
+  a.tile_order = x.tile_ranges.as<uint32_t>() + size_t{x.tile_capacity} * 2;
   a.image = s.image.as<float>();
-  a.capacity = r->capacity;
+  a.capacity = x.capacity;
   a.view_count = 1;
   a.total_tiles = tiles_x * tiles_y;
   a.sort_variant = r->sort_variant;
-  a.sort_counts = r->sort_counts.as<uint32_t>();
+  a.sort_counts = x.sort_counts.as<uint32_t>();
   a.raster_px = r->raster_px;
   s.tile = r->tile;
   s.depth_plan = skm::make_digit_plan(r->policy.key16 ? 16u : 32u, 8u);
   s.tile_plan = skm::make_digit_plan(skm::key_bits_for(tiles_x * tiles_y - 1), r->tile_radix_bits);
   s.depth_passes = s.depth_plan.passes;
   s.tile_passes = s.tile_plan.passes;
-  e = skm::launch_frame(a, lod_frame ? &lod : nullptr, s.depth_plan, s.tile_plan, s.marks, r->stream);
+  e = skm::launch_frame(a, lod_frame ? &lod : nullptr, s.depth_plan, s.tile_plan, s.marks, s.stream);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(s.ctrl_host.ptr, r->ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost, r->stream);
+    e = hipMemcpyAsync(s.ctrl_host.ptr, x.ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost, s.stream);
   if (e == hipSuccess && lod_frame) {
-    e = hipMemcpyAsync(s.lod_host.ptr, lod.state + lod.rounds, sizeof(skm::LodState), hipMemcpyDeviceToHost, r->stream);
+    e = hipMemcpyAsync(s.lod_host.ptr, lod.state + lod.rounds, sizeof(skm::LodState), hipMemcpyDeviceToHost, s.stream);
     if (e == hipSuccess)
       e = hipMemcpyAsync(s.lod_host.as<char>() + sizeof(skm::LodState), &lod.ctrl->final_limited, 4,
-                         hipMemcpyDeviceToHost, r->stream);
+                         hipMemcpyDeviceToHost, s.stream);
   }
-  if (e == hipSuccess) e = hipEventRecord(s.done, r->stream);
+  if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
   if (auto st = r->check(e, "frame launch"); st != SKM_OK) return st;
   s.in_flight = true;
   s.with_world = world;
@@ -999,13 +1058,13 @@ skm_status skm_read_frame(skm_renderer* r, int format, void* dst, size_t dst_byt
 
 // The last frame's gapped survivor layout: the packed index of each cull tile's first survivor
 // (bases[b], with bases[tiles] the survivor count) from the per-tile counts.
-static skm_status survivor_bases(skm_renderer* r, const Ctrl& c, std::vector<uint32_t>* counts,
+static skm_status survivor_bases(skm_renderer* r, const Scratch& x, const Ctrl& c, std::vector<uint32_t>* counts,
                                  std::vector<uint32_t>* bases) {
   const uint32_t tiles = c.cull_lanes / skm::kCullTile;
   counts->assign(tiles, 0u);
   bases->assign(size_t{tiles} + 1, 0u);
   if (tiles)
-    if (auto st = r->check(hipMemcpy(counts->data(), r->scan_status.ptr, size_t{tiles} * 4, hipMemcpyDeviceToHost),
+    if (auto st = r->check(hipMemcpy(counts->data(), x.scan_status.ptr, size_t{tiles} * 4, hipMemcpyDeviceToHost),
                            "read");
         st != SKM_OK)
       return st;
@@ -1022,22 +1081,25 @@ skm_status skm_read_visible(skm_renderer* r, uint32_t* count, uint32_t* keys, ui
                             uint32_t capacity) {
   if (r == nullptr || count == nullptr) return SKM_ERR_INVALID;
   if (auto st = r->drain(); st != SKM_OK) return st;
+  *count = 0;
+  if (r->last_slot < 0) return SKM_OK;
+  const Scratch& x = r->slots[r->last_slot].x;
   Ctrl c;
-  if (auto st = r->check(hipMemcpy(&c, r->ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost), "read"); st != SKM_OK)
+  if (auto st = r->check(hipMemcpy(&c, x.ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost), "read"); st != SKM_OK)
     return st;
   *count = c.visible;
   const uint32_t n = std::min(c.visible, capacity);
   if (n == 0 || (keys == nullptr && projected == nullptr)) return SKM_OK;
   std::vector<uint32_t> counts, bases;
-  if (auto st = survivor_bases(r, c, &counts, &bases); st != SKM_OK) return st;
+  if (auto st = survivor_bases(r, x, c, &counts, &bases); st != SKM_OK) return st;
   const size_t lanes = c.cull_lanes;
   std::vector<uint32_t> gk(keys ? lanes : 0), gp(projected ? lanes * 8 : 0);
   if (keys)
-    if (auto st = r->check(hipMemcpy(gk.data(), r->cull_keys.ptr, lanes * 4, hipMemcpyDeviceToHost), "read");
+    if (auto st = r->check(hipMemcpy(gk.data(), x.cull_keys.ptr, lanes * 4, hipMemcpyDeviceToHost), "read");
         st != SKM_OK)
       return st;
   if (projected)
-    if (auto st = r->check(hipMemcpy(gp.data(), r->projected.ptr, lanes * 32, hipMemcpyDeviceToHost), "read");
+    if (auto st = r->check(hipMemcpy(gp.data(), x.projected.ptr, lanes * 32, hipMemcpyDeviceToHost), "read");
         st != SKM_OK)
       return st;
   for (uint32_t b = 0; b < counts.size(); ++b)
@@ -1053,15 +1115,16 @@ skm_status skm_read_order(skm_renderer* r, uint32_t* order, uint32_t capacity) {
   if (r == nullptr || order == nullptr) return SKM_ERR_INVALID;
   if (auto st = r->drain(); st != SKM_OK) return st;
   if (r->last_slot < 0) return r->fail(SKM_ERR_STATE, "no frame drawn");
+  const Scratch& x = r->slots[r->last_slot].x;
   Ctrl c;
-  if (auto st = r->check(hipMemcpy(&c, r->ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost), "read"); st != SKM_OK)
+  if (auto st = r->check(hipMemcpy(&c, x.ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost), "read"); st != SKM_OK)
     return st;
   const int passes = r->slots[r->last_slot].depth_passes;
   const uint32_t n = std::min(c.visible, capacity);
   if (n == 0) return SKM_OK;
   std::vector<uint32_t> counts, bases;
-  if (auto st = survivor_bases(r, c, &counts, &bases); st != SKM_OK) return st;
-  if (auto st = r->check(hipMemcpy(order, r->order[(passes - 1) & 1].ptr, size_t{n} * 4, hipMemcpyDeviceToHost),
+  if (auto st = survivor_bases(r, x, c, &counts, &bases); st != SKM_OK) return st;
+  if (auto st = r->check(hipMemcpy(order, x.order[(passes - 1) & 1].ptr, size_t{n} * 4, hipMemcpyDeviceToHost),
                          "read");
       st != SKM_OK)
     return st;
@@ -1074,28 +1137,29 @@ skm_status skm_read_tiles(skm_renderer* r, uint64_t* pairs, uint32_t* pair_tiles
   if (r == nullptr || pairs == nullptr) return SKM_ERR_INVALID;
   if (auto st = r->drain(); st != SKM_OK) return st;
   if (r->last_slot < 0) return r->fail(SKM_ERR_STATE, "no frame drawn");
+  const Scratch& x = r->slots[r->last_slot].x;
   Ctrl c;
-  if (auto st = r->check(hipMemcpy(&c, r->ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost), "read"); st != SKM_OK)
+  if (auto st = r->check(hipMemcpy(&c, x.ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost), "read"); st != SKM_OK)
     return st;
   *pairs = c.pairs;
   const Slot& s = r->slots[r->last_slot];
   const int fin = s.tile_passes & 1;
   const uint64_t n = std::min<uint64_t>(c.sort_pairs, capacity);
   if (n > 0 && pair_tiles)
-    if (auto st = r->check(hipMemcpy(pair_tiles, r->pair_keys[fin].ptr, n * 4, hipMemcpyDeviceToHost), "read");
+    if (auto st = r->check(hipMemcpy(pair_tiles, x.pair_keys[fin].ptr, n * 4, hipMemcpyDeviceToHost), "read");
         st != SKM_OK)
       return st;
   if (n > 0 && pair_values) {
     std::vector<uint32_t> counts, bases;
-    if (auto st = survivor_bases(r, c, &counts, &bases); st != SKM_OK) return st;
-    if (auto st = r->check(hipMemcpy(pair_values, r->pair_vals[fin].ptr, n * 4, hipMemcpyDeviceToHost), "read");
+    if (auto st = survivor_bases(r, x, c, &counts, &bases); st != SKM_OK) return st;
+    if (auto st = r->check(hipMemcpy(pair_values, x.pair_vals[fin].ptr, n * 4, hipMemcpyDeviceToHost), "read");
         st != SKM_OK)
       return st;
     for (uint64_t i = 0; i < n; ++i) pair_values[i] = packed_index(bases, pair_values[i]);
   }
   const uint32_t tiles = ((s.width + s.tile - 1) / s.tile) * ((s.height + s.tile - 1) / s.tile);
   if (ranges && tile_capacity)
-    return r->check(hipMemcpy(ranges, r->tile_ranges.ptr, size_t{std::min(tiles, tile_capacity)} * 8,
+    return r->check(hipMemcpy(ranges, x.tile_ranges.ptr, size_t{std::min(tiles, tile_capacity)} * 8,
                               hipMemcpyDeviceToHost),
                     "read");
   return SKM_OK;
@@ -1106,31 +1170,35 @@ skm_status skm_argsort_u32_device(skm_renderer* r, void* keys_dev, uint32_t n, i
   if (r == nullptr || (n > 0 && (keys_dev == nullptr || order_dev == nullptr))) return SKM_ERR_INVALID;
   if (key_bits != 16 && key_bits != 32) return r->fail(SKM_ERR_INVALID, "key_bits must be 16 or 32");
   if (auto st = r->check(hipSetDevice(r->device), "device"); st != SKM_OK) return st;
-  if (auto st = r->drain(); st != SKM_OK) return st;
+  // The sort has its own scratch on the renderer's stream, apart from the frames in flight; only its
+  // own previous sort (and any batch) is ordered before it.
+  if (auto st = r->check(hipStreamSynchronize(r->stream), "sort"); st != SKM_OK) return st;
+  Scratch& x = r->sort;
+  if (auto st = r->ensure_ctrl(x, r->stream); st != SKM_OK) return st;
   if (n > r->sort_capacity || r->sort_n.ptr == nullptr) {
     DeviceBuffer k, v, c;
     const size_t cap = std::max<size_t>(n, 1);
     if (auto st = r->check(k.allocate(cap * 4), "sort keys"); st != SKM_OK) return st;
     if (auto st = r->check(v.allocate(cap * 4), "sort values"); st != SKM_OK) return st;
     if (auto st = r->check(c.allocate(256), "sort count"); st != SKM_OK) return st;
-    if (auto st = r->ensure_status(static_cast<uint32_t>(cap)); st != SKM_OK) return st;
+    if (auto st = r->ensure_status(x, static_cast<uint32_t>(cap), r->stream); st != SKM_OK) return st;
     r->sort_keys = std::move(k);
     r->sort_vals = std::move(v);
     r->sort_n = std::move(c);
     r->sort_capacity = static_cast<uint32_t>(cap);
   }
-  if (auto st = r->maybe_clear_status(); st != SKM_OK) return st;
+  if (auto st = r->maybe_clear_status(x, r->stream); st != SKM_OK) return st;
   uint32_t* keys[2] = {static_cast<uint32_t*>(keys_dev), r->sort_keys.as<uint32_t>()};
   // Two or four passes of 8 bits: the order ends in vals[(passes - 1) & 1] = vals[1].
   uint32_t* vals[2] = {r->sort_vals.as<uint32_t>(), static_cast<uint32_t*>(order_dev)};
-  hipEvent_t t0 = r->slots[0].marks[0], t1 = r->slots[0].marks[4];
+  hipEvent_t t0 = r->sort_marks[0], t1 = r->sort_marks[1];
   hipError_t e = hipMemcpyAsync(r->sort_n.ptr, &n, 4, hipMemcpyHostToDevice, r->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
   if (e == hipSuccess) e = hipEventRecord(t0, r->stream);
   if (e == hipSuccess)
-    e = skm::launch_argsort(r->ctrl.as<Ctrl>(), keys, vals, r->sort_n.as<uint32_t>(), std::max(n, 1u),
-                            skm::make_digit_plan(static_cast<uint32_t>(key_bits), 8u), r->hist.as<uint32_t>(),
-                            r->sort_status.as<uint64_t>(), r->sort_counts.as<uint32_t>(), r->sort_variant,
+    e = skm::launch_argsort(x.ctrl.as<Ctrl>(), keys, vals, r->sort_n.as<uint32_t>(), std::max(n, 1u),
+                            skm::make_digit_plan(static_cast<uint32_t>(key_bits), 8u), x.hist.as<uint32_t>(),
+                            x.sort_status.as<uint64_t>(), x.sort_counts.as<uint32_t>(), r->sort_variant,
                             r->stream);
   if (e == hipSuccess) e = hipEventRecord(t1, r->stream);
   if (e == hipSuccess) e = hipEventSynchronize(t1);
