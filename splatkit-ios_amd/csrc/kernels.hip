@@ -527,6 +527,21 @@ __global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ keys,
 
 constexpr uint32_t kLookback = 16;
 
+// Workgroups reach the 8 XCDs of an MI355X round-robin in launch order (block i on XCD i % 8), and
+// each XCD has its own L2. A radix pass's neighbouring tiles write neighbouring memory (the digit
+// runs of consecutive tiles abut), so block i of g takes tile xcd_tile(i, g): XCD x works through
+// one contiguous range of tiles in order, and the partial lines that adjacent tiles write meet in
+// one L2 instead of eight.
+#ifndef SKM_XCD_REMAP
+#define SKM_XCD_REMAP 1
+#endif
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t i, uint32_t g) {
+  if (!SKM_XCD_REMAP) return i;
+  const uint32_t x = i % kXcds, q = g / kXcds, r = g % kXcds;
+  return x * q + min(x, r) + i / kXcds;
+}
+
 // One stable LSD pass over kSortTile-key tiles. Keys of tile b sit at b*kSortTile + wave*kSortItems*64 +
 // j*64 + lane (positions sort_valid rejects hold none: the first depth pass reads the cull's gapped
 // layout, with a key's position as its value, and writes the survivors packed):
@@ -561,7 +576,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
 #pragma unroll
   for (uint32_t w = 0; w < 4; ++w) s_wave_count[w][tid] = 0;
   __syncthreads();
-  const uint32_t b = LOOKBACK ? s_ticket : blockIdx.x;
+  const uint32_t b = LOOKBACK ? s_ticket : xcd_tile(blockIdx.x, blocks);
   const uint32_t base = b * kSortTile;
 
   uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
@@ -680,11 +695,13 @@ __global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict_
   __shared__ uint32_t s_h[4][kRadix];
   const uint32_t tid = threadIdx.x, wave = tid >> 6;
   const uint32_t n = *n_ptr;
-  if (blockIdx.x >= (n + kSortTile - 1u) / kSortTile) return;
+  const uint32_t blocks = (n + kSortTile - 1u) / kSortTile;
+  if (blockIdx.x >= blocks) return;
 #pragma unroll
   for (uint32_t w = 0; w < 4; ++w) s_h[w][tid] = 0;
   __syncthreads();
-  const uint32_t base = blockIdx.x * kSortTile;
+  const uint32_t b = xcd_tile(blockIdx.x, blocks);
+  const uint32_t base = b * kSortTile;
   uint32_t key[kSortItems];
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) key[j] = keys[min(base + j * 256u + tid, n - 1u)];
@@ -707,38 +724,54 @@ __global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict_
     if (start) atomicAdd(&s_h[wave][d], end - lane);
   }
   __syncthreads();
-  counts[static_cast<size_t>(tid) * stride + blockIdx.x] = s_h[0][tid] + s_h[1][tid] + s_h[2][tid] + s_h[3][tid];
+  counts[static_cast<size_t>(tid) * stride + b] = s_h[0][tid] + s_h[1][tid] + s_h[2][tid] + s_h[3][tid];
 }
 
 // Reduce then scan, step 2: block d scans digit d's row of tile counts in place (exclusive, from 0)
-// and stores the row's total in totals[d]; the scatter adds the digit's start, the exclusive scan of
-// the totals.
+// and stores the row's total in totals[d]. Per round, wave w takes kPer rows of 64 consecutive tiles
+// (lanes read consecutive words, every load in flight together) and scans them in order with one
+// wave-wide scan per row; the waves' runs are then offset by the runs before them.
 __global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ counts, const uint32_t* n_ptr,
                                                     uint32_t* __restrict__ totals, uint32_t stride) {
-  constexpr uint32_t kPer = 8;
-  __shared__ uint32_t s_scan[4];
-  const uint32_t tid = threadIdx.x, d = blockIdx.x;
+  constexpr uint32_t kPer = 16;
+  constexpr uint32_t kRound = 256u * kPer;
+  __shared__ uint32_t s_run[4];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id(), d = blockIdx.x;
   const uint32_t n = *n_ptr;
   const uint32_t tiles = (n + kSortTile - 1u) / kSortTile;
   uint32_t carry = 0;
   uint32_t* row = counts + static_cast<size_t>(d) * stride;
-  for (uint32_t t0 = 0; t0 < tiles; t0 += 256u * kPer) {
-    uint32_t v[kPer], sum = 0;
+  for (uint32_t t0 = 0; t0 < tiles; t0 += kRound) {
+    const uint32_t w0 = t0 + wave * (kPer * 64u);
+    uint32_t v[kPer];
 #pragma unroll
     for (uint32_t i = 0; i < kPer; ++i) {
-      const uint32_t t = t0 + tid * kPer + i;
+      const uint32_t t = w0 + i * 64u + lane;
       v[i] = t < tiles ? row[t] : 0u;
-      sum += v[i];
     }
-    uint32_t chunk;
-    uint32_t run = carry + block_exclusive_scan(sum, s_scan, &chunk);
+    uint32_t run = 0;  // this wave's sum over its rows so far
 #pragma unroll
     for (uint32_t i = 0; i < kPer; ++i) {
-      const uint32_t t = t0 + tid * kPer + i;
-      if (t < tiles) row[t] = run;
-      run += v[i];
+      const uint32_t incl = wave_inclusive_scan(v[i]);
+      const uint32_t excl = run + incl - v[i];
+      run += static_cast<uint32_t>(__shfl(incl, 63, 64));
+      v[i] = excl;
     }
-    carry += chunk;
+    if (lane == 0u) s_run[wave] = run;
+    __syncthreads();
+    uint32_t before = carry, all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) {
+      before += w < wave ? s_run[w] : 0u;
+      all += s_run[w];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+      const uint32_t t = w0 + i * 64u + lane;
+      if (t < tiles) row[t] = before + v[i];
+    }
+    carry += all;
   }
   if (tid == 0) totals[d] = carry;
 }

@@ -38,7 +38,10 @@ static_assert(sizeof(skm_view) == 152 && sizeof(skm_batch_tuning) == 24 && sizeo
 
 namespace {
 
-constexpr uint32_t kSlots = 2;             // frames in flight (MetalFrameSlots.h:8)
+#ifndef SKM_FRAME_SLOTS
+#define SKM_FRAME_SLOTS 2
+#endif
+constexpr uint32_t kSlots = SKM_FRAME_SLOTS;  // frames in flight (MetalFrameSlots.h:8)
 constexpr uint32_t kMaxExtent = 16384;     // per axis
 constexpr uint32_t kUploadChunk = 1u << 20;  // splats per staging copy
 
@@ -868,8 +871,9 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   const bool lod_frame = world && r->lod_world;
   skm::LodArgs lod{};
   if (lod_frame) {
-    // The tree's selection buffers are one set: a cut waits for the other slot's frame.
-    const Slot& other = r->slots[(slot_id + 1) % kSlots];
+    // The tree's selection buffers are one set: a cut waits for the previous frame, which waited
+    // for its own predecessor.
+    const Slot& other = r->slots[(slot_id + kSlots - 1) % kSlots];
     if (other.in_flight)
       if (auto st = r->check(hipStreamWaitEvent(s.stream, other.done, 0), "frame order"); st != SKM_OK) return st;
     s.lod_limit = r->policy.lod_splat_limit == 0 ? r->lod.capacity : std::min(r->policy.lod_splat_limit, r->lod.capacity);
