@@ -41,7 +41,8 @@ typedef struct skm_config {
   uint32_t sort_kernel;    /* radix pass: 0 = reduce then scan of tile counts, 1 = onesweep look-back */
   uint32_t raster_px;      /* raster pixels per lane side: 1 or 2 (0 = 1, or 2 at 32-pixel tiles);
                             * supported (tile, px): (8, 1), (16, 1), (16, 2), (32, 2) */
-  uint32_t reserved;
+  uint32_t frames_in_flight; /* frames a draw may queue before it waits, each with its own HIP stream
+                              * and scratch: 1 to 3 (0 = 2, the reference's count) */
 } skm_config;
 
 /* SplatRenderer::Range (SplatRenderer.h:100-103). */

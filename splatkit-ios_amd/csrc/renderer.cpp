@@ -38,10 +38,10 @@ static_assert(sizeof(skm_view) == 152 && sizeof(skm_batch_tuning) == 24 && sizeo
 
 namespace {
 
-#ifndef SKM_FRAME_SLOTS
-#define SKM_FRAME_SLOTS 2
-#endif
-constexpr uint32_t kSlots = SKM_FRAME_SLOTS;  // frames in flight (MetalFrameSlots.h:8)
+// Frames in flight: the reference's two (kFramesInFlight, MetalFrameSlots.h:8) unless the config
+// asks for one or three.
+constexpr uint32_t kDefaultSlots = 2;
+constexpr uint32_t kMaxSlots = 3;
 constexpr uint32_t kMaxExtent = 16384;     // per axis
 constexpr uint32_t kUploadChunk = 1u << 20;  // splats per staging copy
 
@@ -86,7 +86,7 @@ constexpr size_t kInputRanges = 256;
 constexpr size_t kInputBytes = kInputRanges + size_t{skm::kMaxRanges} * 8 + size_t{skm::kMaxRanges + 1} * 4 + 16;
 
 // Device memory the frame pipeline writes: control block, survivor arrays, tile pairs and ranges.
-// Each frame slot owns one, so frames on the two slots' streams overlap; the standalone sort owns
+// Each frame slot owns one, so frames on the slots' streams overlap; the standalone sort owns
 // another.
 struct Scratch {
   uint32_t capacity = 0;       // survivor lanes
@@ -260,7 +260,14 @@ struct skm_renderer {
   hipEvent_t sort_marks[2] = {};
   uint32_t sort_capacity = 0;
 
-  Slot slots[kSlots];
+  Slot slots[kMaxSlots];
+  uint32_t slot_count = kDefaultSlots;
+  struct Slots {
+    Slot *b, *e;
+    Slot* begin() const { return b; }
+    Slot* end() const { return e; }
+  };
+  Slots active() { return {slots, slots + slot_count}; }
   uint64_t submitted = 0, completed = 0;
   int last_slot = -1;
   bool gpu_failed = false;
@@ -342,7 +349,7 @@ struct skm_renderer {
 
   skm_status drain() {
     skm_status st = SKM_OK;
-    for (Slot& s : slots) {
+    for (Slot& s : active()) {
       const skm_status k = harvest(s);
       if (k != SKM_OK) st = k;
     }
@@ -413,7 +420,7 @@ struct skm_renderer {
   skm_status ensure_capacity(uint32_t cap) {
     // Whole cull tiles: the cull writes survivors in a gapped layout of kCullTile lanes per tile.
     cap = static_cast<uint32_t>(align_up(std::max(cap, 1u), skm::kCullTile));
-    for (Slot& s : slots)
+    for (Slot& s : active())
       if (auto st = ensure_capacity(s, cap); st != SKM_OK) return st;
     capacity = std::max(capacity, cap);
     return SKM_OK;
@@ -444,7 +451,7 @@ struct skm_renderer {
 
   // Every slot's pair scratch; the caller has drained the frames.
   skm_status ensure_pairs(uint64_t want) {
-    for (Slot& s : slots)
+    for (Slot& s : active())
       if (auto st = ensure_pairs(s, want); st != SKM_OK) return st;
     pair_target = std::max(pair_target, clamp_pairs(want));
     return SKM_OK;
@@ -525,7 +532,7 @@ skm_status skm_renderer_create(const skm_config* config, skm_renderer** out) {
   const uint32_t tile = config && config->tile_size ? config->tile_size : 16u;
   const uint32_t px = config && config->raster_px ? config->raster_px : (tile == 32u ? 2u : 1u);
   if (!skm::raster_shape_supported(tile, px)) return SKM_ERR_UNSUPPORTED;
-  if (config && config->sort_kernel > 1) return SKM_ERR_INVALID;
+  if (config && (config->sort_kernel > 1 || config->frames_in_flight > kMaxSlots)) return SKM_ERR_INVALID;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return SKM_ERR_DEVICE;
   auto* r = new skm_renderer();
@@ -534,6 +541,7 @@ skm_status skm_renderer_create(const skm_config* config, skm_renderer** out) {
   r->raster_px = px;
   if (config && config->pair_factor) r->pair_factor = config->pair_factor;
   if (config) r->sort_variant = static_cast<int>(config->sort_kernel);
+  if (config && config->frames_in_flight) r->slot_count = config->frames_in_flight;
   hipDeviceProp_t prop;
   if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
       hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -547,7 +555,7 @@ skm_status skm_renderer_create(const skm_config* config, skm_renderer** out) {
                 prop.multiProcessorCount, runtime / 10000000, (runtime / 100000) % 100);
   r->description = desc;
   bool ok = hipEventCreate(&r->sort_marks[0]) == hipSuccess && hipEventCreate(&r->sort_marks[1]) == hipSuccess;
-  for (Slot& s : r->slots) {
+  for (Slot& s : r->active()) {
     ok = ok && hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) == hipSuccess &&
          r->ensure_ctrl(s.x, s.stream) == SKM_OK && r->ensure_tiles(s, 1) == SKM_OK &&
          s.inputs.allocate(kInputBytes) == hipSuccess && s.inputs_host.allocate(kInputBytes) == hipSuccess &&
@@ -557,7 +565,7 @@ skm_status skm_renderer_create(const skm_config* config, skm_renderer** out) {
     for (auto& m : s.marks) ok = ok && hipEventCreate(&m) == hipSuccess;
   }
   ok = ok && r->ensure_capacity(1) == SKM_OK && r->ensure_pairs(0) == SKM_OK;
-  for (Slot& s : r->slots) ok = ok && hipStreamSynchronize(s.stream) == hipSuccess;
+  for (Slot& s : r->active()) ok = ok && hipStreamSynchronize(s.stream) == hipSuccess;
   if (!ok) {
     skm_renderer_destroy(r);
     return SKM_ERR_DEVICE;
@@ -570,12 +578,12 @@ void skm_renderer_destroy(skm_renderer* r) {
   if (r == nullptr) return;
   (void)hipSetDevice(r->device);
   if (r->stream) (void)hipStreamSynchronize(r->stream);
-  for (Slot& s : r->slots)
+  for (Slot& s : r->active())
     if (s.stream) (void)hipStreamSynchronize(s.stream);
   r->batch.reset();
   for (auto& m : r->sort_marks)
     if (m) (void)hipEventDestroy(m);
-  for (Slot& s : r->slots) {
+  for (Slot& s : r->active()) {
     if (s.stream) (void)hipStreamDestroy(s.stream);
     if (s.done) (void)hipEventDestroy(s.done);
     for (auto& m : s.marks)
@@ -692,7 +700,7 @@ int skm_ready(const skm_renderer* r) { return r && r->surface_w > 0 && !r->gpu_f
 int skm_gpu_failed(skm_renderer* r) { return r && r->gpu_failed ? 1 : 0; }
 int skm_has_completed_world_frame(skm_renderer* r) {
   if (r == nullptr) return 0;
-  for (Slot& s : r->slots)
+  for (Slot& s : r->active())
     if (s.in_flight && hipEventQuery(s.done) == hipSuccess) (void)r->harvest(s);
   return r->completed_world_frame ? 1 : 0;
 }
@@ -829,7 +837,7 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   if (auto st = r->check(hipSetDevice(r->device), "device"); st != SKM_OK) return st;
   const uint32_t W = r->draw_w(), H = r->draw_h();
   const uint32_t tiles_x = (W + r->tile - 1) / r->tile, tiles_y = (H + r->tile - 1) / r->tile;
-  const uint32_t slot_id = static_cast<uint32_t>(r->submitted % kSlots);
+  const uint32_t slot_id = static_cast<uint32_t>(r->submitted % r->slot_count);
   Slot& s = r->slots[slot_id];
   // The slot's previous frame ends here; its scratch is free from then on.
   if (auto st = r->harvest(s); st != SKM_OK) return st;
@@ -873,7 +881,7 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   if (lod_frame) {
     // The tree's selection buffers are one set: a cut waits for the previous frame, which waited
     // for its own predecessor.
-    const Slot& other = r->slots[(slot_id + kSlots - 1) % kSlots];
+    const Slot& other = r->slots[(slot_id + r->slot_count - 1) % r->slot_count];
     if (other.in_flight)
       if (auto st = r->check(hipStreamWaitEvent(s.stream, other.done, 0), "frame order"); st != SKM_OK) return st;
     s.lod_limit = r->policy.lod_splat_limit == 0 ? r->lod.capacity : std::min(r->policy.lod_splat_limit, r->lod.capacity);
@@ -911,7 +919,10 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   a.hist = x.hist.as<uint32_t>();
   a.pair_capacity = x.pair_capacity;
   a.tile_ranges = x.tile_ranges.as<uint32_t>();
-  a.tile_order = x.tile_ranges.as<uint32_t>() + size_t{x.tile_capacity} * 2;
+#ifndef SKM_TILE_ORDER
+#define SKM_TILE_ORDER 1
+#endif
+  a.tile_order = SKM_TILE_ORDER ? x.tile_ranges.as<uint32_t>() + size_t{x.tile_capacity} * 2 : nullptr;
   a.image = s.image.as<float>();
   a.capacity = x.capacity;
   a.view_count = 1;
@@ -958,7 +969,7 @@ uint64_t skm_submitted_frames(const skm_renderer* r) { return r ? r->submitted :
 uint64_t skm_completed_frames(skm_renderer* r) {
   if (r == nullptr) return 0;
   if (r->gpu_failed) return 0;
-  for (Slot& s : r->slots)
+  for (Slot& s : r->active())
     if (s.in_flight && hipEventQuery(s.done) == hipSuccess) (void)r->harvest(s);
   return r->completed;
 }

@@ -29,7 +29,7 @@ _u32p = C.POINTER(C.c_uint32)
 
 class Config(C.Structure):
     _fields_ = [("device", C.c_int32), ("tile_size", C.c_uint32), ("pair_factor", C.c_uint32),
-                ("sort_kernel", C.c_uint32), ("raster_px", C.c_uint32), ("reserved", C.c_uint32)]
+                ("sort_kernel", C.c_uint32), ("raster_px", C.c_uint32), ("frames_in_flight", C.c_uint32)]
 
 
 class Range(C.Structure):
@@ -426,11 +426,11 @@ class DeviceBuffer:
 
 
 class Renderer:
-    def __init__(self, device=0, pair_factor=0, sort_kernel=0, tile=TILE, raster_px=0):
+    def __init__(self, device=0, pair_factor=0, sort_kernel=0, tile=TILE, raster_px=0, frames_in_flight=0):
         self._lib = lib()
         self._h = _R()
         self.tile = tile
-        cfg = Config(device, tile, pair_factor, sort_kernel, raster_px, 0)
+        cfg = Config(device, tile, pair_factor, sort_kernel, raster_px, frames_in_flight)
         rc = self._lib.skm_renderer_create(C.byref(cfg), C.byref(self._h))
         if rc != SKM_OK:
             raise SkmError(rc, "renderer creation failed (no HIP device?)")

@@ -231,6 +231,35 @@ def test_frames_in_flight_give_the_same_image(renderer, skm_mod):
     assert st["completed"] == st["submitted"] and st["gpu_ms"] > 0
 
 
+def test_one_two_and_three_frames_in_flight_agree(skm_mod):
+    """Each frame slot has its own stream and scratch: overlapping frames give every frame's counts and
+    the last image exactly as frames drawn one at a time."""
+    say("case: frames_in_flight 1, 2, 3 over 8 orbit frames")
+    cloud = skm_mod.make_cloud(300_000, 3, seed=37, layout=1, spread=4.0)
+    frames = [skm_mod.orbit_camera(k * 7, 640, 480) for k in range(8)]
+    results = []
+    for fif in (1, 2, 3):
+        r = skm_mod.Renderer(frames_in_flight=fif)
+        try:
+            r.upload_world(cloud)
+            r.set_extent(640, 480)
+            r.apply_policy(sort_depth=32)
+            r.frame_totals(reset=True)
+            for view, proj, eye in frames:
+                r.draw(r.make_frame(view, proj, eye))
+            r.finish()
+            tot = r.frame_totals(reset=True)
+            results.append((tot["frames"], tot["visible"], tot["pairs"], r.read_frame()))
+        finally:
+            r.close()
+    for fif, res in zip((2, 3), results[1:]):
+        assert res[:3] == results[0][:3], fif
+        assert np.array_equal(res[3], results[0][3]), fif
+    assert results[0][0] == 8
+    with pytest.raises(skm_mod.SkmError):
+        skm_mod.Renderer(frames_in_flight=4)
+
+
 def test_c1_frame_matches_oracle(renderer, skm_mod, oracle):
     """The 3M-splat SH3 scene at 1920x1080: order bit-exact, pixels within 1/255 of FP64."""
     say("case: C1 3M SH3 1920x1080 frame 0")
