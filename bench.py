@@ -284,6 +284,9 @@ def main():
     ap.add_argument("--lod-frames", type=int, default=120)
     ap.add_argument("--sort-sizes", default="7000000,16000000,154000000",
                     help="radix-sort leg: key counts (empty skips it)")
+    ap.add_argument("--tuning", default="",
+                    help="tile,raster_px,sort_kernel,radix_bits: skip the single-draw tuner (for counter runs, "
+                         "whose serialised dispatches would time the candidates wrongly)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a known-size kernel for the HBM "
                          "byte counters of a profiler)")
@@ -316,7 +319,11 @@ def main():
     frames = [skm.orbit_camera(f, WIDTH, HEIGHT) for f in range(60)]
     # Tuning for single draws: every candidate of the autotuner (tile size, raster block shape, radix
     # pass variant, tile digit width) times 12 single draws of the orbit; the fastest is kept.
-    plain_best, plain_table = tune_single_draws(r, frames)
+    if args.tuning:
+        t, p, k, rb = (int(x) for x in args.tuning.split(","))
+        plain_best, plain_table = {"tile_size": t, "raster_px": p, "sort_kernel": k, "radix_bits": rb}, []
+    else:
+        plain_best, plain_table = tune_single_draws(r, frames)
     r.set_tuning(**plain_best)
     for i in range(args.warmup):
         r.draw(r.make_frame(*frames[i % 60]))
@@ -330,6 +337,13 @@ def main():
     elapsed = time.perf_counter() - t0
     tot = r.frame_totals(reset=True)
     fps = args.steps / elapsed
+
+    # The same frames one at a time (each drawn and finished before the next): kernel durations with
+    # the GPU to themselves, beside the overlapped ones of the timed region.
+    for i in range(20):
+        r.draw(r.make_frame(*frames[i % 60]))
+        r.finish()
+    alone = r.frame_totals(reset=True)
 
     n_frames = max(1, tot["frames"])
     V = tot["visible"] / n_frames
@@ -350,19 +364,22 @@ def main():
     raster_tflops = evals * RASTER_FLOPS_PER_EVAL / (ms["raster"] * 1e-3) / 1e12 if ms["raster"] > 0 else 0.0
     stages["raster"]["pair_evals"] = int(evals)
     stages["raster"]["coverage_TFLOP_s"] = round(raster_tflops, 2)
-    dominant = max(names, key=lambda n: ms[n])
+    # The roofline's kernel is the raster: the longest single kernel of the frame in the kernel trace
+    # (profiles/k_raster_3M_sh3_1080p_kernel_stats.md). Stage spans are not kernels: the binning
+    # stage holds eleven launches, and with frames overlapping each span also holds other frames' work.
     traffic = measured_traffic(f"k_raster<{tile}, {plain_best['raster_px']}>")
-    if dominant == "raster":
-        roofline = {"bound": "valu", "kernel": f"k_raster<{tile}, {plain_best['raster_px']}>",
-                    "achieved": round(raster_tflops, 3),
-                    "peak": VALU_PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(raster_tflops * 1e12 / VALU_PEAK_FP32, 4),
-                    "traffic": traffic, "note": "fp32 VALU work of the coverage test over tile^2 x pairs evaluations; "
-                                                "traffic = HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE counter runs "
-                                                "in profiles/hbm_traffic_3M_sh3_1080p.md"}
-    else:
-        gbs = stages[dominant]["GB_s"]
-        roofline = {"bound": "hbm", "kernel": dominant, "achieved": gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                    "frac": round(gbs * 1e9 / HBM_PEAK, 4), "traffic": None}
+    raster_alone_ms = alone["stage_ms"][4] / max(1, alone["frames"])
+    alone_tflops = evals * RASTER_FLOPS_PER_EVAL / (raster_alone_ms * 1e-3) / 1e12 if raster_alone_ms > 0 else 0.0
+    roofline = {"bound": "valu", "kernel": f"k_raster<{tile}, {plain_best['raster_px']}>",
+                "achieved": round(raster_tflops, 3),
+                "peak": VALU_PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(raster_tflops * 1e12 / VALU_PEAK_FP32, 4),
+                "traffic": traffic,
+                "alone": {"ms": round(raster_alone_ms, 4), "achieved": round(alone_tflops, 3),
+                          "frac": round(alone_tflops * 1e12 / VALU_PEAK_FP32, 4), "frames": alone["frames"]},
+                "note": "fp32 VALU work of the coverage test over tile^2 x pairs evaluations, over the raster's "
+                        f"average duration in the timed region, where {FRAMES_IN_FLIGHT} frames overlap and share "
+                        "the GPU; 'alone' times the same frames drawn one at a time. traffic = HBM bytes per launch "
+                        "from the FETCH_SIZE/WRITE_SIZE counter runs in profiles/hbm_traffic_3M_sh3_1080p.md"}
 
     sizes = [int(x) for x in args.sort_sizes.split(",") if x]
     sort_leg = radix_sort(r, skm, sizes) if sizes else None
