@@ -321,7 +321,8 @@ __global__ __launch_bounds__(256) void k_pack(WorldOut w, uint32_t offset, uint3
 
 __global__ __launch_bounds__(256) void k_prologue(Ctrl* ctrl, uint32_t* hist, uint32_t hist_words,
                                                   uint32_t* ranges, uint32_t range_words,
-                                                  uint32_t* extra, uint32_t extra_words) {
+                                                  uint32_t* extra, uint32_t extra_words,
+                                                  const LodState* lod_final, const uint32_t* lod_limited) {
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid == 0) {
     ctrl->epoch = ctrl->epoch + 1u;
@@ -331,7 +332,13 @@ __global__ __launch_bounds__(256) void k_prologue(Ctrl* ctrl, uint32_t* hist, ui
     ctrl->sort_pairs = 0;
     ctrl->cull_lanes = 0;
     for (int k = 0; k < 16; ++k) ctrl->ticket[k] = 0;
-    for (int k = 0; k < 16; ++k) ctrl->lod[k] = 0;
+    // The frame's LOD cut, kept with the frame's counters: the selection buffers are free for the
+    // next frame's cut once this frame's cull has read the indices.
+    static_assert(sizeof(LodState) == 8 * sizeof(uint32_t), "LodState fits ctrl->lod[0..7]");
+    const uint32_t* st = reinterpret_cast<const uint32_t*>(lod_final);
+    for (int k = 0; k < 8; ++k) ctrl->lod[k] = st != nullptr ? st[k] : 0u;
+    ctrl->lod[8] = lod_limited != nullptr ? *lod_limited : 0u;
+    for (int k = 9; k < 16; ++k) ctrl->lod[k] = 0;
   }
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = gid; i < hist_words; i += stride) hist[i] = 0;
@@ -1442,7 +1449,7 @@ hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], cons
   // keys[0] holds the input; passes alternate keys[0] -> keys[1] -> keys[0] ...; the order ends in
   // vals[(passes - 1) & 1].
   hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, stream, ctrl, hist, 4u * 256u, nullptr, 0u, nullptr,
-                     0u);
+                     0u, nullptr, nullptr);
   // The reduce-then-scan passes take each digit's total from their own tile counts; the onesweep
   // passes need every pass's histogram first.
   if (variant == kSortOnesweep)
@@ -1470,7 +1477,9 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
   if (marks) (void)hipEventRecord(marks[1], s);
   hipLaunchKernelGGL(k_prologue, dim3(min(grid_for(tiles * 2u, 256), 256u)), dim3(256), 0, s, a.ctrl,
                      a.hist, kHistWords, a.tile_ranges, tiles * 2u, a.view_visible,
-                     a.view_visible != nullptr ? a.view_count : 0u);
+                     a.view_visible != nullptr ? a.view_count : 0u,
+                     lod != nullptr ? lod->state + lod->rounds : nullptr,
+                     lod != nullptr ? &lod->ctrl->final_limited : nullptr);
   const uint32_t view_lanes = grid_for(a.total, 256u * kCullItems) * 256u * kCullItems;
   const uint32_t lanes = a.active != nullptr ? a.capacity : view_lanes * a.view_count;
   const uint32_t cull_blocks = grid_for(lanes, 256u * kCullItems);

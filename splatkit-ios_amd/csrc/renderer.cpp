@@ -102,7 +102,7 @@ struct Slot {
   hipStream_t stream = nullptr;
   Scratch x;
   DeviceBuffer inputs, image;
-  PinnedBuffer inputs_host, ctrl_host, lod_host;
+  PinnedBuffer inputs_host, ctrl_host;
   hipEvent_t marks[6] = {};
   hipEvent_t done = nullptr;
   bool in_flight = false;
@@ -320,8 +320,11 @@ struct skm_renderer {
     (void)hipEventElapsedTime(&ms[5], s.marks[0], s.marks[5]);
     for (int k = 0; k < 6; ++k) stage_ms[k] = ms[k];
     if (s.with_lod) {
-      const auto* st = s.lod_host.as<skm::LodState>();
-      const uint32_t limited = *reinterpret_cast<const uint32_t*>(s.lod_host.as<char>() + sizeof(skm::LodState));
+      // k_prologue copied the cut's final state and denied refinements into ctrl->lod.
+      skm::LodState cut;
+      std::memcpy(&cut, c->lod, sizeof(cut));
+      const skm::LodState* st = &cut;
+      const uint32_t limited = c->lod[8];
       last_selected = st->count;
       lod_stats[0] = st->count;
       lod_stats[1] = st->active;
@@ -560,7 +563,6 @@ skm_status skm_renderer_create(const skm_config* config, skm_renderer** out) {
          r->ensure_ctrl(s.x, s.stream) == SKM_OK && r->ensure_tiles(s, 1) == SKM_OK &&
          s.inputs.allocate(kInputBytes) == hipSuccess && s.inputs_host.allocate(kInputBytes) == hipSuccess &&
          s.ctrl_host.allocate(sizeof(Ctrl)) == hipSuccess &&
-         s.lod_host.allocate(sizeof(skm::LodState) + 16) == hipSuccess &&
          hipEventCreate(&s.done) == hipSuccess;
     for (auto& m : s.marks) ok = ok && hipEventCreate(&m) == hipSuccess;
   }
@@ -879,11 +881,12 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   const bool lod_frame = world && r->lod_world;
   skm::LodArgs lod{};
   if (lod_frame) {
-    // The tree's selection buffers are one set: a cut waits for the previous frame, which waited
-    // for its own predecessor.
+    // The tree's selection buffers are one set: a cut waits until the previous frame's cull (marks[2])
+    // has read that frame's indices; the previous frame waited for its own predecessor's cull.
     const Slot& other = r->slots[(slot_id + r->slot_count - 1) % r->slot_count];
     if (other.in_flight)
-      if (auto st = r->check(hipStreamWaitEvent(s.stream, other.done, 0), "frame order"); st != SKM_OK) return st;
+      if (auto st = r->check(hipStreamWaitEvent(s.stream, other.marks[2], 0), "frame order"); st != SKM_OK)
+        return st;
     s.lod_limit = r->policy.lod_splat_limit == 0 ? r->lod.capacity : std::min(r->policy.lod_splat_limit, r->lod.capacity);
     lod = r->lod.args(cam_dev, s.lod_limit, r->threshold.pixels, 4.0f, 1);
   }
@@ -938,12 +941,6 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   e = skm::launch_frame(a, lod_frame ? &lod : nullptr, s.depth_plan, s.tile_plan, s.marks, s.stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(s.ctrl_host.ptr, x.ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost, s.stream);
-  if (e == hipSuccess && lod_frame) {
-    e = hipMemcpyAsync(s.lod_host.ptr, lod.state + lod.rounds, sizeof(skm::LodState), hipMemcpyDeviceToHost, s.stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(s.lod_host.as<char>() + sizeof(skm::LodState), &lod.ctrl->final_limited, 4,
-                         hipMemcpyDeviceToHost, s.stream);
-  }
   if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
   if (auto st = r->check(e, "frame launch"); st != SKM_OK) return st;
   s.in_flight = true;
