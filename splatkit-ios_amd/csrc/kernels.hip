@@ -40,6 +40,18 @@ __device__ __forceinline__ uint64_t status_load(uint64_t* p) {
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+// Workgroups reach the 8 XCDs of an MI355X round-robin in launch order (block i on XCD i % 8), and
+// each XCD has its own L2. Block i of g takes logical block xcd_tile(i, g): XCD x works through one
+// contiguous range of logical blocks in order. A radix pass's neighbouring tiles write neighbouring
+// memory (the digit runs of consecutive tiles abut), so their partial lines meet in one L2 instead
+// of eight; a batch's cull runs the views of one lane tile as consecutive logical blocks, so the
+// tile's splat records come from HBM once and from that XCD's L2 for the other views.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t i, uint32_t g) {
+  const uint32_t x = i % kXcds, q = g / kXcds, r = g % kXcds;
+  return x * q + min(x, r) + i / kXcds;
+}
 // Number of set bits of `mask` below this lane.
 __device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
@@ -415,7 +427,8 @@ __device__ __forceinline__ uint64_t pack_rect(const Rect& r, uint32_t count, uin
          (static_cast<uint64_t>(view) << 48);
 }
 
-// 256 threads, kCullItems lanes each: lane t = tile * kCullTile + k * 256 + thread. Block b's
+// 256 threads, kCullItems lanes each: lane t = tile * kCullTile + k * 256 + thread. Logical block b
+// (xcd_tile of the launch index) culls lane tile b / views for view b % views; its
 // survivors are ranked in lane order (k, wave, lane) and written from position b * kCullTile, with
 // their count in cull_counts[b]: no block waits for another, and ordering by position is lane order
 // (compaction order). Block 0 publishes the layout's extent, every block adds its count to
@@ -433,7 +446,7 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
   if (blockIdx.x >= blocks) return;
   s_byte255[tid] = static_cast<float>(tid) / 255.0f;
   __syncthreads();
-  const uint32_t b = blockIdx.x;
+  const uint32_t b = xcd_tile(blockIdx.x, blocks);
   const uint32_t view = a.view_count > 1u ? b % a.view_count : 0u;
   const uint32_t lt = a.view_count > 1u ? b / a.view_count : b;  // the lane tile
   const Camera& cam = a.cam[view];
@@ -534,20 +547,6 @@ __global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ keys,
 
 constexpr uint32_t kLookback = 16;
 
-// Workgroups reach the 8 XCDs of an MI355X round-robin in launch order (block i on XCD i % 8), and
-// each XCD has its own L2. A radix pass's neighbouring tiles write neighbouring memory (the digit
-// runs of consecutive tiles abut), so block i of g takes tile xcd_tile(i, g): XCD x works through
-// one contiguous range of tiles in order, and the partial lines that adjacent tiles write meet in
-// one L2 instead of eight.
-#ifndef SKM_XCD_REMAP
-#define SKM_XCD_REMAP 1
-#endif
-constexpr uint32_t kXcds = 8;
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t i, uint32_t g) {
-  if (!SKM_XCD_REMAP) return i;
-  const uint32_t x = i % kXcds, q = g / kXcds, r = g % kXcds;
-  return x * q + min(x, r) + i / kXcds;
-}
 
 // One stable LSD pass over kSortTile-key tiles. Keys of tile b sit at b*kSortTile + wave*kSortItems*64 +
 // j*64 + lane (positions sort_valid rejects hold none: the first depth pass reads the cull's gapped

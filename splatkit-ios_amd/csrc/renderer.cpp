@@ -849,13 +849,15 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   }
   if (auto st = r->ensure_pairs(s, r->pair_target); st != SKM_OK) return st;
   if (auto st = r->ensure_tiles(s, tiles_x * tiles_y); st != SKM_OK) return st;
-  if (s.width != W || s.height != H) {
+  // The slot's image grows to the largest extent drawn (draws that alternate extents reallocate
+  // nothing).
+  if (size_t{W} * H * 16u > s.image.bytes) {
     DeviceBuffer img;
     if (auto st = r->check(img.allocate(size_t{W} * H * 16u), "image"); st != SKM_OK) return st;
     s.image = std::move(img);
-    s.width = W;
-    s.height = H;
   }
+  s.width = W;
+  s.height = H;
   Scratch& x = s.x;
   if (auto st = r->maybe_clear_status(x, s.stream); st != SKM_OK) return st;
 
@@ -922,10 +924,7 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   a.hist = x.hist.as<uint32_t>();
   a.pair_capacity = x.pair_capacity;
   a.tile_ranges = x.tile_ranges.as<uint32_t>();
-#ifndef SKM_TILE_ORDER
-#define SKM_TILE_ORDER 1
-#endif
-  a.tile_order = SKM_TILE_ORDER ? x.tile_ranges.as<uint32_t>() + size_t{x.tile_capacity} * 2 : nullptr;
+  a.tile_order = x.tile_ranges.as<uint32_t>() + size_t{x.tile_capacity} * 2;
   a.image = s.image.as<float>();
   a.capacity = x.capacity;
   a.view_count = 1;
