@@ -79,30 +79,32 @@ def tune_single_draws(r, frames, draws=12):
     return {"tile_size": t, "raster_px": p, "sort_kernel": k, "radix_bits": rb}, table
 
 
-def radix_sort(r, skm, sizes):
-    """The stable LSD argsort alone (skm_argsort_u32_device) on uniformly random 32-bit keys resident
-    in HBM: the depth sort's algorithm at the frame's pair count and above. Algorithmic bytes in the
-    reference's units: n (2 (k + 4) passes + k) with k = 4 and 4 passes."""
+def radix_sort(r, skm, runs):
+    """The stable LSD argsort alone (skm_argsort_u32_device) on uniformly random keys resident in HBM,
+    for (n, key bits, label) runs: the depth sort's algorithm at and above the frame's sizes, and the
+    tile sort at the frame's pair count P with the frame's tile-id width. Algorithmic bytes in the
+    reference's units, passes actually executed: n (2 (k + 4) passes + k) with k = 4."""
     out = []
     rng = np.random.default_rng(0x5EED0005)
-    for n in sizes:
-        keys = rng.integers(0, 2**32, n, dtype=np.uint32)
+    for n, bits, label in runs:
+        keys = rng.integers(0, 2**bits, n, dtype=np.uint64).astype(np.uint32)
         src, work, order = skm.DeviceBuffer(4 * n), skm.DeviceBuffer(4 * n), skm.DeviceBuffer(4 * n)
         src.upload(keys)
         ms = []
         for _ in range(4):
             work.copy_from(src, 4 * n)  # the sort overwrites its keys
-            ms.append(r.argsort_device(work.ptr.value, n, 32, order.ptr.value))
+            ms.append(r.argsort_device(work.ptr.value, n, bits, order.ptr.value))
         if n <= 16_000_000:
             assert np.array_equal(order.download(np.uint32, n), np.argsort(keys, kind="stable"))
         t = min(ms[1:])
-        byt = n * (2 * (4 + 4) * 4 + 4)
-        out.append({"keys": n, "key_bits": 32, "ms": round(t, 4), "GB_s": round(byt / t / 1e6, 1),
-                    "hbm_frac": round(byt / t / 1e6 / (HBM_PEAK / 1e9), 4)})
+        passes = -(-bits // 8)
+        byt = n * (2 * (4 + 4) * passes + 4)
+        out.append({"label": label, "keys": n, "key_bits": bits, "passes": passes, "ms": round(t, 4),
+                    "GB_s": round(byt / t / 1e6, 1), "hbm_frac": round(byt / t / 1e6 / (HBM_PEAK / 1e9), 4)})
         for b in (src, work, order):
             b.close()
-    return {"workload": "stable argsort of uniformly random 32-bit keys (4 passes of 8 bits), keys resident in HBM",
-            "runs": out}
+    return {"workload": "stable argsort of uniformly random keys resident in HBM (ceil(bits / 8) passes, bits split "
+                        "evenly)", "runs": out}
 
 
 def batched_vs_loop(r, skm, batches, loops, single_tuning):
@@ -382,7 +384,8 @@ def main():
                         "from the FETCH_SIZE/WRITE_SIZE counter runs in profiles/hbm_traffic_3M_sh3_1080p.md"}
 
     sizes = [int(x) for x in args.sort_sizes.split(",") if x]
-    sort_leg = radix_sort(r, skm, sizes) if sizes else None
+    runs = [(int(P), tile_bits, "tile sort at the frame's P")] + [(n, 32, "depth-key sort") for n in sizes]
+    sort_leg = radix_sort(r, skm, runs) if sizes else None
     batched = batched_vs_loop(r, skm, args.batches, 2, plain_best) if args.batches > 0 else None
     r.close()
     lod = lod_flythrough(skm, args.lod_frames, args.lod_leaves) if args.lod_leaves > 0 else None

@@ -1179,7 +1179,7 @@ skm_status skm_read_tiles(skm_renderer* r, uint64_t* pairs, uint32_t* pair_tiles
 skm_status skm_argsort_u32_device(skm_renderer* r, void* keys_dev, uint32_t n, int key_bits,
                                   void* order_dev, double* ms) {
   if (r == nullptr || (n > 0 && (keys_dev == nullptr || order_dev == nullptr))) return SKM_ERR_INVALID;
-  if (key_bits != 16 && key_bits != 32) return r->fail(SKM_ERR_INVALID, "key_bits must be 16 or 32");
+  if (key_bits < 1 || key_bits > 32) return r->fail(SKM_ERR_INVALID, "key_bits must be 1 to 32");
   if (auto st = r->check(hipSetDevice(r->device), "device"); st != SKM_OK) return st;
   // The sort has its own scratch on the renderer's stream, apart from the frames in flight; only its
   // own previous sort (and any batch) is ordered before it.
@@ -1200,15 +1200,17 @@ skm_status skm_argsort_u32_device(skm_renderer* r, void* keys_dev, uint32_t n, i
   }
   if (auto st = r->maybe_clear_status(x, r->stream); st != SKM_OK) return st;
   uint32_t* keys[2] = {static_cast<uint32_t*>(keys_dev), r->sort_keys.as<uint32_t>()};
-  // Two or four passes of 8 bits: the order ends in vals[(passes - 1) & 1] = vals[1].
-  uint32_t* vals[2] = {r->sort_vals.as<uint32_t>(), static_cast<uint32_t*>(order_dev)};
+  // The order ends in vals[(passes - 1) & 1]: that slot is the caller's buffer.
+  const skm::DigitPlan plan = skm::make_digit_plan(static_cast<uint32_t>(key_bits), 8u);
+  uint32_t* vals[2] = {r->sort_vals.as<uint32_t>(), r->sort_vals.as<uint32_t>()};
+  vals[(plan.passes - 1) & 1] = static_cast<uint32_t*>(order_dev);
   hipEvent_t t0 = r->sort_marks[0], t1 = r->sort_marks[1];
   hipError_t e = hipMemcpyAsync(r->sort_n.ptr, &n, 4, hipMemcpyHostToDevice, r->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
   if (e == hipSuccess) e = hipEventRecord(t0, r->stream);
   if (e == hipSuccess)
     e = skm::launch_argsort(x.ctrl.as<Ctrl>(), keys, vals, r->sort_n.as<uint32_t>(), std::max(n, 1u),
-                            skm::make_digit_plan(static_cast<uint32_t>(key_bits), 8u), x.hist.as<uint32_t>(),
+                            plan, x.hist.as<uint32_t>(),
                             x.sort_status.as<uint64_t>(), x.sort_counts.as<uint32_t>(), r->sort_variant,
                             r->stream);
   if (e == hipSuccess) e = hipEventRecord(t1, r->stream);

@@ -186,6 +186,16 @@ def test_radix_sort_matches_stable_sort(renderer, n, mask, seed, bits):
     assert np.array_equal(renderer.argsort(keys, bits), np.argsort(keys, kind="stable"))
 
 
+@pytest.mark.parametrize("n,bits,seed", [(7_400_000, 13, 21), (300_001, 20, 22), (65_537, 5, 23), (4097, 24, 24),
+                                         (3, 1, 25)])
+def test_radix_sort_with_one_to_four_passes(renderer, n, bits, seed):
+    """Keys below 2^bits take ceil(bits / 8) passes (1, 2, 3); the order lands in the caller's buffer for
+    odd and even pass counts. 7.4M 13-bit keys are the tile sort of a 1080p frame's pairs."""
+    say(f"case: argsort n={n} bits={bits}")
+    keys = np.random.default_rng(seed).integers(0, 2**bits, n, dtype=np.uint64).astype(np.uint32)
+    assert np.array_equal(renderer.argsort(keys, bits), np.argsort(keys, kind="stable"))
+
+
 def test_slab_tiles_and_world_info(renderer, skm_mod, oracle):
     """createSlab/uploadTile (MetalWorld.mm:59-100): a slab filled by tiles renders like the world."""
     say("case: slab of 3 tiles")
