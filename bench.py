@@ -279,7 +279,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=2)
     ap.add_argument("--batches", type=int, default=5, help="timed 64-view batches (0 skips the batch leg)")
-    ap.add_argument("--serving-requests", type=int, default=300,
+    ap.add_argument("--serving-requests", type=int, default=1000,
                     help="requests per offered rate in the serving leg (0 skips it)")
     ap.add_argument("--serving-rates", default="150,300,600,1200,2400")
     ap.add_argument("--lod-leaves", type=int, default=20_000_000, help="LOD leg's leaves (0 skips it)")

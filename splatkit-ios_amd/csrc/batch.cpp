@@ -17,6 +17,14 @@ hipError_t grow(DeviceBuffer& b, size_t bytes) {
   return b.allocate(bytes);
 }
 
+// A capacity that has to grow grows by at least half: batches of changing shape (a served stream)
+// then reallocate a few times rather than at every new maximum, and every reallocation frees the old
+// buffers, which waits for the whole device.
+uint64_t grown(uint64_t want, uint64_t have, uint64_t limit) {
+  if (want <= have) return want;
+  return std::min(std::max(want, have + have / 2), std::max(want, limit));
+}
+
 }  // namespace
 
 Batch::~Batch() {
@@ -33,7 +41,11 @@ void Batch::release_graph() {
 }
 
 hipError_t Batch::ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_t tiles, uint64_t pixels,
-                         uint32_t ranges) {
+                         uint32_t ranges, hipStream_t stream) {
+  lanes = grown(lanes, lane_capacity_, kLaneLimit) / kCullTile * kCullTile;
+  pairs = grown(pairs, pair_capacity_, kLaneLimit);
+  tiles = static_cast<uint32_t>(grown(tiles, tile_capacity_, 1u << 24));
+  pixels = grown(pixels, pixel_capacity_, ~0ull);
   hipError_t e = hipSuccess;
   bool moved = false;
   auto need = [&](DeviceBuffer& b, size_t bytes) {
@@ -44,7 +56,7 @@ hipError_t Batch::ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_
   if (ctrl_.ptr == nullptr) {
     need(ctrl_, sizeof(Ctrl));
     need(hist_, kHistWords * 4);
-    if (e == hipSuccess) e = hipMemset(ctrl_.ptr, 0, sizeof(Ctrl));
+    if (e == hipSuccess) e = hipMemsetAsync(ctrl_.ptr, 0, sizeof(Ctrl), stream);
   }
   if (views > view_capacity_) {
     need(cams_, size_t{views} * sizeof(Camera));
@@ -89,18 +101,19 @@ hipError_t Batch::ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_
   const size_t scan = static_cast<size_t>(lane_capacity_ / std::min(kCullTile, 256u * kBinItems) + 2);
   if (e == hipSuccess && scan > scan_words_) {
     need(scan_status_, scan * 8);
-    if (e == hipSuccess) e = hipMemset(scan_status_.ptr, 0, scan_status_.bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(scan_status_.ptr, 0, scan_status_.bytes, stream);
     if (e == hipSuccess) scan_words_ = scan;
   }
   const size_t sort = static_cast<size_t>(std::max(lane_capacity_, pair_capacity_) / kSortTile + 2) * kRadix;
   if (e == hipSuccess && sort > sort_words_) {
     need(sort_status_, sort * 8);
     need(sort_counts_, sort * 4);
-    if (e == hipSuccess) e = hipMemset(sort_status_.ptr, 0, sort_status_.bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(sort_status_.ptr, 0, sort_status_.bytes, stream);
     if (e == hipSuccess) sort_words_ = sort;
   }
+  // Staged inputs: cameras, view descriptors, ranges and their starts, then the view of each tile.
   const size_t stage = size_t{views} * (sizeof(Camera) + sizeof(ViewInfo)) + size_t{ranges} * 8 +
-                       (size_t{ranges} + 1) * 4 + 64;
+                       (size_t{ranges} + 1) * 4 + 64 + size_t{tiles} * 2 + 16;
   if (e == hipSuccess && stage > staging_bytes_) {
     e = staging_.allocate(stage * 2);
     if (e == hipSuccess) staging_bytes_ = stage * 2;
@@ -160,7 +173,7 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
   }
   const uint64_t want_pairs = std::min<uint64_t>(
       std::max<uint64_t>({pair_want_, lanes * 3, uint64_t{1} << 22}), kLaneLimit);
-  if (hipError_t e = ensure(count, lanes, want_pairs, static_cast<uint32_t>(tiles), pixels, range_count);
+  if (hipError_t e = ensure(count, lanes, want_pairs, static_cast<uint32_t>(tiles), pixels, range_count, stream);
       e != hipSuccess) {
     *error = "batch: scratch allocation failed";
     return e;
@@ -191,13 +204,17 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
     if (e == hipSuccess)
       e = hipMemcpyAsync(starts_.ptr, sh, (size_t{range_count} + 1) * 4, hipMemcpyHostToDevice, stream);
   }
-  // The view of each global tile: rebuilt when the extents change.
+  // The view of each global tile, uploaded on the stream (no device-wide wait) when the extents change.
   std::vector<uint16_t> tv(static_cast<size_t>(tiles));
   for (uint32_t v = 0; v < count; ++v)
     std::fill(tv.begin() + info_[v].tile_base, tv.begin() + info_[v].tile_base + info_[v].tiles_x * info_[v].tiles_y,
               static_cast<uint16_t>(v));
   if (e == hipSuccess && tv != tile_view_host_) {
-    e = hipMemcpy(tile_view_.ptr, tv.data(), tv.size() * 2, hipMemcpyHostToDevice);
+    const size_t at = align_up(size_t{count} * (sizeof(Camera) + sizeof(ViewInfo)) + size_t{range_count} * 8 +
+                                   (size_t{range_count} + 1) * 4,
+                               16);
+    std::memcpy(st + at, tv.data(), tv.size() * 2);
+    e = hipMemcpyAsync(tile_view_.ptr, st + at, tv.size() * 2, hipMemcpyHostToDevice, stream);
     tile_view_host_ = std::move(tv);
   }
   if (e == hipSuccess && ++pipelines_ >= (1ull << 24)) {
@@ -339,10 +356,12 @@ uint64_t Batch::device_bytes() const {
 }
 
 uint64_t Batch::bytes_for(uint32_t views, uint64_t lanes, uint32_t tiles, uint64_t pixels) const {
-  lanes = std::max(lanes, lane_capacity_);
-  const uint64_t pairs = std::max<uint64_t>({pair_want_, pair_capacity_, lanes * 3, uint64_t{1} << 22});
-  tiles = std::max(tiles, tile_capacity_);
-  pixels = std::max(pixels, pixel_capacity_);
+  lanes = std::max(grown(lanes, lane_capacity_, kLaneLimit) / kCullTile * kCullTile, lane_capacity_);
+  const uint64_t pairs = std::max(
+      grown(std::max<uint64_t>({pair_want_, lanes * 3, uint64_t{1} << 22}), pair_capacity_, kLaneLimit),
+      pair_capacity_);
+  tiles = std::max(static_cast<uint32_t>(grown(tiles, tile_capacity_, 1u << 24)), tile_capacity_);
+  pixels = std::max(grown(pixels, pixel_capacity_, ~0ull), pixel_capacity_);
   views = std::max(views, view_capacity_);
   const uint64_t status = (lanes / std::min(kCullTile, 256u * kBinItems) + 2) * 8 +
                           (std::max(lanes, pairs) / kSortTile + 2) * kRadix * 12;

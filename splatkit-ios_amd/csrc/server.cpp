@@ -32,6 +32,12 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// What one launch may take past its first request: 2^26 cull lanes (views x the scene's lanes per
+// view; about 4 GB of survivor scratch and as much again of tile pairs) and 2^25 pixels (four
+// 3840x2160 views; 512 MB of image).
+constexpr uint64_t kLaunchLanes = uint64_t{1} << 26;
+constexpr uint64_t kLaunchPixels = uint64_t{1} << 25;
+
 double ms_between(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
 }
@@ -87,7 +93,9 @@ struct Scene {
 
 struct skm_server {
   skm_server_config cfg{};
-  skm_batch_tuning tuning{16, 8, 1, 0, 1, 0};
+  // Eager launches: a served batch's shape (request count, each request's resolution) changes from
+  // launch to launch, so a captured graph would be captured again almost every time.
+  skm_batch_tuning tuning{16, 8, 0, 0, 1, 0};
   int device = 0;
   uint64_t budget = 0;
   std::string error;
@@ -237,9 +245,21 @@ struct skm_server {
       if (head == nullptr) return;
       Scene& sc = *scenes[head->req.scene];
       tried[head->req.scene] = true;
+      // At most kLaunchLanes cull lanes and kLaunchPixels pixels per launch (past the first request):
+      // a launch of that size already fills the device for many milliseconds, and bigger ones would
+      // only grow every executor's scratch (each reallocation waits for the whole device).
+      const uint64_t lanes_per_view = (uint64_t{sc.count} + skm::kCullTile - 1) / skm::kCullTile * skm::kCullTile;
       std::vector<Request*> batch;
-      for (Request* q : pending)
-        if (q->req.scene == head->req.scene && batch.size() < std::max(1u, cfg.max_batch_views)) batch.push_back(q);
+      uint64_t pixels = 0;
+      for (Request* q : pending) {
+        if (q->req.scene != head->req.scene || batch.size() >= std::max(1u, cfg.max_batch_views)) continue;
+        const uint64_t px = uint64_t{q->req.view.width} * q->req.view.height;
+        if (!batch.empty() &&
+            ((batch.size() + 1) * lanes_per_view > kLaunchLanes || pixels + px > kLaunchPixels))
+          break;
+        batch.push_back(q);
+        pixels += px;
+      }
       const uint64_t base = resident() + scratch();
       while (!batch.empty() && base + growth(sc, batch) > budget) batch.pop_back();
       if (batch.empty()) {

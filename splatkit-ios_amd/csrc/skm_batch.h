@@ -77,7 +77,7 @@ class Batch {
   mutable std::vector<uint32_t> st_count_, st_local_;
   mutable bool st_ready_ = false;
   hipError_t ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_t tiles, uint64_t pixels,
-                    uint32_t ranges);
+                    uint32_t ranges, hipStream_t stream);
   void release_graph();
 
   // Device scratch.
