@@ -222,7 +222,7 @@ def lod_flythrough(skm, frames, leaves):
     tree = skm.LodTree.build(cloud, 21, 256, threads=min(16, os.cpu_count() or 1))
     build_s = time.time() - t0
     del cloud
-    r = skm.Renderer(0)
+    r = skm.Renderer(0, frames_in_flight=FRAMES_IN_FLIGHT)
     r.upload_lod_world(tree, 1)
     r.set_extent(WIDTH, HEIGHT)
     cams = [fly_camera(f, frames) for f in range(frames)]

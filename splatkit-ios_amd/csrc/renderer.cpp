@@ -98,9 +98,17 @@ struct Scratch {
   uint64_t pipelines = 0;  // prologue launches since the status arrays were last cleared
 };
 
+// One cut's working state: the selection's control block, per-round states, frontiers, costs,
+// packets, look-back status words and the selected indices. Each frame slot owns one, so the cuts of
+// consecutive frames overlap on the GPU.
+struct LodCut {
+  DeviceBuffer state, ctrl, frontier[2], cost, packets, indices, status[3];
+};
+
 struct Slot {
   hipStream_t stream = nullptr;
   Scratch x;
+  LodCut cut;  // a tree world's cut for this slot's frames
   DeviceBuffer inputs, image;
   PinnedBuffer inputs_host, ctrl_host;
   hipEvent_t marks[6] = {};
@@ -137,9 +145,10 @@ struct LodThreshold {
   }
 };
 
-// Device buffers of one tree's cut (MetalLodSelection::upload, MetalLodSelection.mm:60-108).
+// Device buffers of one tree (MetalLodSelection::upload, MetalLodSelection.mm:60-108): clusters and
+// leaf references, read by every cut.
 struct LodBuffers {
-  DeviceBuffer clusters, leaves, state, ctrl, frontier[2], cost, packets, indices, status[3];
+  DeviceBuffer clusters, leaves;
   uint32_t cluster_count = 0, rounds = 0, capacity = 0, frontier_capacity = 0;
 
   hipError_t upload(const skm_lod_tree_view* t, uint32_t cap, uint32_t n_rounds, hipStream_t stream) {
@@ -147,31 +156,37 @@ struct LodBuffers {
     rounds = n_rounds;
     capacity = cap;
     frontier_capacity = std::max(1u, std::min(cap, cluster_count));
-    const size_t fc = frontier_capacity;
-    const size_t blocks = (fc + 255) / 256 + 1;
     hipError_t e = clusters.allocate(std::max<size_t>(t->cluster_count, 1) * 64);
     if (e == hipSuccess) e = leaves.allocate(std::max<size_t>(t->leaf_ref_count, 1) * 4);
-    if (e == hipSuccess) e = state.allocate((size_t{rounds} + 1) * sizeof(skm::LodState));
-    if (e == hipSuccess) e = ctrl.allocate(sizeof(skm::LodCtrl));
-    if (e == hipSuccess) e = frontier[0].allocate(fc * 4);
-    if (e == hipSuccess) e = frontier[1].allocate(fc * 4);
-    if (e == hipSuccess) e = cost.allocate(fc * 8);
-    if (e == hipSuccess) e = packets.allocate(fc * 16);
-    if (e == hipSuccess) e = indices.allocate(std::max<size_t>(cap, 1) * 4);
-    for (auto& st : status)
-      if (e == hipSuccess) e = st.allocate(blocks * 8);
     if (e == hipSuccess && t->cluster_count)
       e = hipMemcpyAsync(clusters.ptr, t->clusters, t->cluster_count * 64, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess && t->leaf_ref_count)
       e = hipMemcpyAsync(leaves.ptr, t->leaves, t->leaf_ref_count * 4, hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) e = hipMemsetAsync(ctrl.ptr, 0, sizeof(skm::LodCtrl), stream);
-    for (auto& st : status)
-      if (e == hipSuccess) e = hipMemsetAsync(st.ptr, 0, st.bytes, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     return e;
   }
 
-  skm::LodArgs args(const Camera* cam, uint32_t limit, float pixel_limit, float color_weight, int cull) const {
+  // A cut's state sized for this tree, cleared on `stream`.
+  hipError_t allocate_cut(LodCut& c, hipStream_t stream) const {
+    const size_t fc = frontier_capacity;
+    const size_t blocks = (fc + 255) / 256 + 1;
+    hipError_t e = c.state.allocate((size_t{rounds} + 1) * sizeof(skm::LodState));
+    if (e == hipSuccess) e = c.ctrl.allocate(sizeof(skm::LodCtrl));
+    if (e == hipSuccess) e = c.frontier[0].allocate(fc * 4);
+    if (e == hipSuccess) e = c.frontier[1].allocate(fc * 4);
+    if (e == hipSuccess) e = c.cost.allocate(fc * 8);
+    if (e == hipSuccess) e = c.packets.allocate(fc * 16);
+    if (e == hipSuccess) e = c.indices.allocate(std::max<size_t>(capacity, 1) * 4);
+    for (auto& st : c.status)
+      if (e == hipSuccess) e = st.allocate(blocks * 8);
+    if (e == hipSuccess) e = hipMemsetAsync(c.ctrl.ptr, 0, sizeof(skm::LodCtrl), stream);
+    for (auto& st : c.status)
+      if (e == hipSuccess) e = hipMemsetAsync(st.ptr, 0, st.bytes, stream);
+    return e;
+  }
+
+  skm::LodArgs args(const Camera* cam, const LodCut& c, uint32_t limit, float pixel_limit, float color_weight,
+                    int cull) const {
     skm::LodArgs a{};
     a.cam = cam;
     a.clusters = clusters.as<skm::LodCluster>();
@@ -183,14 +198,14 @@ struct LodBuffers {
     a.pixel_limit = pixel_limit;
     a.color_weight = color_weight;
     a.cull = cull;
-    a.ctrl = ctrl.as<skm::LodCtrl>();
-    a.state = state.as<skm::LodState>();
-    a.frontier[0] = frontier[0].as<uint32_t>();
-    a.frontier[1] = frontier[1].as<uint32_t>();
-    a.cost = cost.as<uint32_t>();
-    a.packets = packets.as<uint4>();
-    a.indices = indices.as<uint32_t>();
-    for (int k = 0; k < 3; ++k) a.status[k] = status[k].as<uint64_t>();
+    a.ctrl = c.ctrl.as<skm::LodCtrl>();
+    a.state = c.state.as<skm::LodState>();
+    a.frontier[0] = c.frontier[0].as<uint32_t>();
+    a.frontier[1] = c.frontier[1].as<uint32_t>();
+    a.cost = c.cost.as<uint32_t>();
+    a.packets = c.packets.as<uint4>();
+    a.indices = c.indices.as<uint32_t>();
+    for (int k = 0; k < 3; ++k) a.status[k] = c.status[k].as<uint64_t>();
     return a;
   }
 };
@@ -350,11 +365,25 @@ struct skm_renderer {
     return SKM_OK;
   }
 
+  // The slot of the k-th oldest frame that may be in flight (frame f uses slot f % slot_count).
+  Slot& slot_by_age(uint32_t k) { return slots[(submitted + k) % slot_count]; }
+
+  // Takes the finished frames in submission order, stopping at the first still running (frames on
+  // different slots may finish out of order; the last-frame statistics follow the newest in order).
+  void poll() {
+    for (uint32_t k = 0; k < slot_count; ++k) {
+      Slot& s = slot_by_age(k);
+      if (!s.in_flight) continue;
+      if (hipEventQuery(s.done) != hipSuccess) break;
+      (void)harvest(s);
+    }
+  }
+
   skm_status drain() {
     skm_status st = SKM_OK;
-    for (Slot& s : active()) {
-      const skm_status k = harvest(s);
-      if (k != SKM_OK) st = k;
+    for (uint32_t k = 0; k < slot_count; ++k) {
+      const skm_status h = harvest(slot_by_age(k));
+      if (h != SKM_OK) st = h;
     }
     if (batch) {
       bool dropped = false;
@@ -702,8 +731,7 @@ int skm_ready(const skm_renderer* r) { return r && r->surface_w > 0 && !r->gpu_f
 int skm_gpu_failed(skm_renderer* r) { return r && r->gpu_failed ? 1 : 0; }
 int skm_has_completed_world_frame(skm_renderer* r) {
   if (r == nullptr) return 0;
-  for (Slot& s : r->active())
-    if (s.in_flight && hipEventQuery(s.done) == hipSuccess) (void)r->harvest(s);
+  r->poll();
   return r->completed_world_frame ? 1 : 0;
 }
 void skm_draw_extent(const skm_renderer* r, uint32_t* width, uint32_t* height) {
@@ -760,10 +788,15 @@ skm_status skm_upload_lod_world(skm_renderer* r, const skm_lod_tree_view* tree, 
   if (auto st = r->upload_range(w, 0, cloud, degree > 0 ? cloud->sh_degree : 0); st != SKM_OK) return st;
   LodBuffers lod;
   if (auto st = r->check(lod.upload(tree, cap, rounds, r->stream), "LOD upload"); st != SKM_OK) return st;
+  // One cut state per frame slot, each cleared on its slot's stream.
+  LodCut cuts[kMaxSlots];
+  for (uint32_t k = 0; k < r->slot_count; ++k)
+    if (auto st = r->check(lod.allocate_cut(cuts[k], r->slots[k].stream), "LOD cut"); st != SKM_OK) return st;
   if (auto st = r->ensure_capacity(cap); st != SKM_OK) return st;
   if (auto st = r->ensure_pairs(uint64_t{cap} * r->pair_factor); st != SKM_OK) return st;
   r->world = std::move(w);
   r->lod = std::move(lod);
+  for (uint32_t k = 0; k < r->slot_count; ++k) r->slots[k].cut = std::move(cuts[k]);
   r->has_world = true;
   r->lod_world = true;
   r->threshold.reset(r->policy.lod_error_pixels);
@@ -883,14 +916,9 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   const bool lod_frame = world && r->lod_world;
   skm::LodArgs lod{};
   if (lod_frame) {
-    // The tree's selection buffers are one set: a cut waits until the previous frame's cull (marks[2])
-    // has read that frame's indices; the previous frame waited for its own predecessor's cull.
-    const Slot& other = r->slots[(slot_id + r->slot_count - 1) % r->slot_count];
-    if (other.in_flight)
-      if (auto st = r->check(hipStreamWaitEvent(s.stream, other.marks[2], 0), "frame order"); st != SKM_OK)
-        return st;
+    // The slot's own cut state: the cut overlaps the other slots' frames.
     s.lod_limit = r->policy.lod_splat_limit == 0 ? r->lod.capacity : std::min(r->policy.lod_splat_limit, r->lod.capacity);
-    lod = r->lod.args(cam_dev, s.lod_limit, r->threshold.pixels, 4.0f, 1);
+    lod = r->lod.args(cam_dev, s.cut, s.lod_limit, r->threshold.pixels, 4.0f, 1);
   }
   skm::FrameArgs a{};
   a.cam = cam_dev;
@@ -965,8 +993,7 @@ uint64_t skm_submitted_frames(const skm_renderer* r) { return r ? r->submitted :
 uint64_t skm_completed_frames(skm_renderer* r) {
   if (r == nullptr) return 0;
   if (r->gpu_failed) return 0;
-  for (Slot& s : r->active())
-    if (s.in_flight && hipEventQuery(s.done) == hipSuccess) (void)r->harvest(s);
+  r->poll();
   return r->completed;
 }
 double skm_last_gpu_millis(const skm_renderer* r) { return r ? r->stage_ms[5] : 0; }
@@ -991,7 +1018,9 @@ skm_status skm_read_lod_cut(skm_renderer* r, uint32_t* count, uint32_t* indices,
   *count = r->lod_stats[0];
   if (stats6) std::memcpy(stats6, r->lod_stats, sizeof(r->lod_stats));
   const uint32_t n = std::min(r->lod_stats[0], capacity);
-  if (indices && n) return r->check(hipMemcpy(indices, r->lod.indices.ptr, size_t{n} * 4, hipMemcpyDeviceToHost), "read");
+  if (r->last_slot < 0) return SKM_OK;
+  const LodCut& cut = r->slots[r->last_slot].cut;
+  if (indices && n) return r->check(hipMemcpy(indices, cut.indices.ptr, size_t{n} * 4, hipMemcpyDeviceToHost), "read");
   return SKM_OK;
 }
 
@@ -1008,12 +1037,14 @@ skm_status skm_lod_select(skm_renderer* r, const skm_lod_tree_view* tree, const 
   const uint64_t leaves = std::max<uint64_t>(tree->leaf_count, 1);
   const uint32_t cap = std::max(1u, static_cast<uint32_t>(std::min<uint64_t>({capacity == 0 ? leaves : capacity, leaves, 1ull << 27})));
   LodBuffers lod;
+  LodCut cut;
   if (auto st = r->check(lod.upload(tree, cap, rounds, r->stream), "LOD upload"); st != SKM_OK) return st;
+  if (auto st = r->check(lod.allocate_cut(cut, r->stream), "LOD cut"); st != SKM_OK) return st;
   DeviceBuffer cam_dev;
   Camera cam;
   fill_camera(cam, view16, proj16, width, height, false, 0.0f, nullptr, nullptr);
   if (auto st = r->check(cam_dev.allocate(sizeof(Camera)), "camera"); st != SKM_OK) return st;
-  const skm::LodArgs a = lod.args(cam_dev.as<Camera>(), cap, pixel_limit, color_weight, cull);
+  const skm::LodArgs a = lod.args(cam_dev.as<Camera>(), cut, cap, pixel_limit, color_weight, cull);
   skm::LodState final_state{};
   uint32_t limited = 0;
   hipError_t e = hipMemcpyAsync(cam_dev.ptr, &cam, sizeof(Camera), hipMemcpyHostToDevice, r->stream);
@@ -1028,7 +1059,7 @@ skm_status skm_lod_select(skm_renderer* r, const skm_lod_tree_view* tree, const 
     std::memcpy(stats6, v, sizeof(v));
   }
   const uint32_t n = std::min(final_state.count, cap);
-  if (indices && n) return r->check(hipMemcpy(indices, lod.indices.ptr, size_t{n} * 4, hipMemcpyDeviceToHost), "read");
+  if (indices && n) return r->check(hipMemcpy(indices, cut.indices.ptr, size_t{n} * 4, hipMemcpyDeviceToHost), "read");
   return SKM_OK;
 }
 

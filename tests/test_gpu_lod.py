@@ -91,3 +91,34 @@ def test_lod_world_frame_matches_oracle(renderer, skm_mod, oracle):
     g = gpu_frame(renderer, skm_mod, None, view, proj, eye, 960, 540, upload=False, lod_error=1.5)
     o = oracle.frame(tree.nodes, view, proj, eye, 960, 540, active=o_cut)
     assert_parity(g, o, label="LOD frame")
+
+
+def test_lod_frames_in_flight_match_one_at_a_time(skm_mod):
+    """Each frame slot has its own cut state: tree-world frames drawn with three in flight give every
+    frame's counts and the last cut and image exactly as frames drawn one at a time (the budget is
+    never reached, so the threshold stays at the policy's and does not depend on completion order)."""
+    say("case: LOD world, 6 orbit frames, frames_in_flight 1 and 3")
+    cloud = skm_mod.make_cloud(150_000, 1, seed=67, layout=1, spread=10.0, area=30.0, log_scale_mean=-3.5)
+    tree = skm_mod.LodTree.build(cloud, 21, 256, threads=16)
+    proj = skm_mod.perspective(np.pi / 3, 960 / 540, 0.2, 500.0)
+    eyes = [np.array([6.0 * np.sin(k), 2.0, 18.0 * np.cos(0.3 * k)], np.float32) for k in range(6)]
+    results = []
+    for fif in (1, 3):
+        r = skm_mod.Renderer(frames_in_flight=fif)
+        try:
+            r.upload_lod_world(tree)
+            r.set_extent(960, 540)
+            r.apply_policy(sort_depth=32, lod_error=1.5)
+            r.frame_totals(reset=True)
+            for eye in eyes:
+                r.draw(r.make_frame(skm_mod.look_at(eye), proj, eye))
+            r.finish()
+            tot = r.frame_totals(reset=True)
+            cut, stats = r.read_lod_cut()
+            results.append((tot["frames"], tot["visible"], tot["pairs"], cut, r.read_frame()))
+        finally:
+            r.close()
+    assert results[0][:3] == results[1][:3]
+    assert np.array_equal(results[0][3], results[1][3])
+    assert np.array_equal(results[0][4], results[1][4])
+    assert results[0][0] == 6
