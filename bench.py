@@ -211,7 +211,7 @@ def fly_camera(frame, frames, width=1920, height=1080):
     return skm.look_at(eye, ahead), skm.perspective(math.pi / 3, width / height, 0.2, 400.0), eye
 
 
-def lod_flythrough(skm, frames, leaves):
+def lod_flythrough(skm, frames, leaves, frames_in_flight=FRAMES_IN_FLIGHT):
     """The LOD world: `leaves` splats at SH1 drawn like the 3M scene over a 200 x 200 m square, an
     octree with at most 256 leaves per cluster, a cut per frame at 1.5 px error (color weight 4,
     culling on), 1920x1080, over a 120-frame fly-through, with 16-bit and 32-bit depth keys."""
@@ -222,7 +222,7 @@ def lod_flythrough(skm, frames, leaves):
     tree = skm.LodTree.build(cloud, 21, 256, threads=min(16, os.cpu_count() or 1))
     build_s = time.time() - t0
     del cloud
-    r = skm.Renderer(0, frames_in_flight=FRAMES_IN_FLIGHT)
+    r = skm.Renderer(0, frames_in_flight=frames_in_flight)
     r.upload_lod_world(tree, 1)
     r.set_extent(WIDTH, HEIGHT)
     cams = [fly_camera(f, frames) for f in range(frames)]
@@ -286,6 +286,8 @@ def main():
     ap.add_argument("--lod-frames", type=int, default=120)
     ap.add_argument("--sort-sizes", default="7000000,16000000,154000000",
                     help="radix-sort leg: key counts (empty skips it)")
+    ap.add_argument("--frames-in-flight", type=int, default=FRAMES_IN_FLIGHT,
+                    help="frame slots of the plain and LOD renderers (1 to 3)")
     ap.add_argument("--tuning", default="",
                     help="tile,raster_px,sort_kernel,radix_bits: skip the single-draw tuner (for counter runs, "
                          "whose serialised dispatches would time the candidates wrongly)")
@@ -313,7 +315,7 @@ def main():
     t0 = time.time()
     cloud = skm.scene_c1(N_SPLATS, SH_DEGREE)
     gen_s = time.time() - t0
-    r = skm.Renderer(0, frames_in_flight=FRAMES_IN_FLIGHT)
+    r = skm.Renderer(0, frames_in_flight=args.frames_in_flight)
     device = r.description()
     r.upload_world(cloud)
     r.set_extent(WIDTH, HEIGHT)
@@ -379,7 +381,7 @@ def main():
                 "alone": {"ms": round(raster_alone_ms, 4), "achieved": round(alone_tflops, 3),
                           "frac": round(alone_tflops * 1e12 / VALU_PEAK_FP32, 4), "frames": alone["frames"]},
                 "note": "fp32 VALU work of the coverage test over tile^2 x pairs evaluations, over the raster's "
-                        f"average duration in the timed region, where {FRAMES_IN_FLIGHT} frames overlap and share "
+                        f"average duration in the timed region, where {args.frames_in_flight} frames overlap and share "
                         "the GPU; 'alone' times the same frames drawn one at a time. traffic = HBM bytes per launch "
                         "from the FETCH_SIZE/WRITE_SIZE counter runs in profiles/hbm_traffic_3M_sh3_1080p.md"}
 
@@ -388,7 +390,8 @@ def main():
     sort_leg = radix_sort(r, skm, runs) if sizes else None
     batched = batched_vs_loop(r, skm, args.batches, 2, plain_best) if args.batches > 0 else None
     r.close()
-    lod = lod_flythrough(skm, args.lod_frames, args.lod_leaves) if args.lod_leaves > 0 else None
+    lod = (lod_flythrough(skm, args.lod_frames, args.lod_leaves, args.frames_in_flight) if args.lod_leaves > 0
+           else None)
     served = None
     if args.serving_requests > 0:
         served = serving(skm, [float(x) for x in args.serving_rates.split(",")], args.serving_requests)
@@ -416,7 +419,7 @@ def main():
                    "splats": N_SPLATS, "sh_degree": SH_DEGREE, "width": WIDTH, "height": HEIGHT,
                    "tile": plain_best["tile_size"], "raster_px": plain_best["raster_px"],
                    "sort_kernel": ["reduce_then_scan", "onesweep_lookback"][plain_best["sort_kernel"]],
-                   "tile_radix_bits": plain_best["radix_bits"], "frames_in_flight": FRAMES_IN_FLIGHT,
+                   "tile_radix_bits": plain_best["radix_bits"], "frames_in_flight": args.frames_in_flight,
                    "parallelism": "single GPU"},
         "autotune_ms": [{"tile": t, "raster_px": p, "sort_kernel": k, "radix_bits": rb, "ms_per_draw": round(ms, 4)}
                         for t, p, k, rb, ms in plain_table],

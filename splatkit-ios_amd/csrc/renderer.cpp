@@ -41,7 +41,7 @@ namespace {
 // Frames in flight: the reference's two (kFramesInFlight, MetalFrameSlots.h:8) unless the config
 // asks for one or three.
 constexpr uint32_t kDefaultSlots = 2;
-constexpr uint32_t kMaxSlots = 3;
+constexpr uint32_t kMaxSlots = 3;  // with the renderer's own stream, the process's 4 hardware queues
 constexpr uint32_t kMaxExtent = 16384;     // per axis
 constexpr uint32_t kUploadChunk = 1u << 20;  // splats per staging copy
 
