@@ -760,6 +760,8 @@ skm_status skm_upload_world(skm_renderer* r, const skm_cloud_view* cloud, int ma
   r->world = std::move(w);
   r->has_world = true;
   r->lod_world = false;
+  r->lod = LodBuffers{};
+  for (Slot& sl : r->active()) sl.cut = LodCut{};  // a flat world keeps no cut state
   r->completed_world_frame = false;
   return r->check(hipStreamSynchronize(r->stream), "upload");
 }
@@ -819,6 +821,8 @@ skm_status skm_create_slab(skm_renderer* r, uint32_t capacity, int sh_degree) {
   r->world = std::move(w);
   r->has_world = true;
   r->lod_world = false;
+  r->lod = LodBuffers{};
+  for (Slot& sl : r->active()) sl.cut = LodCut{};  // a flat world keeps no cut state
   r->completed_world_frame = false;
   return r->check(hipStreamSynchronize(r->stream), "slab");
 }
