@@ -1108,8 +1108,11 @@ __device__ __forceinline__ void blend(bool take, float e, const float4 Cc, float
 }
 
 // Records the raster tests per LDS round trip (a build may set it; it divides every batch size).
+// Four keeps k_raster<16, 1> at 73 VGPRs (six waves per SIMD); eight tests more records per round
+// trip at 113 VGPRs (four waves), which is faster for one frame alone and slower once frames
+// overlap or views are batched.
 #ifndef SKM_RASTER_GROUP
-#define SKM_RASTER_GROUP 8
+#define SKM_RASTER_GROUP 4
 #endif
 constexpr uint32_t kGroup = SKM_RASTER_GROUP;
 static_assert(64u % kGroup == 0u, "group size");
