@@ -245,6 +245,19 @@ skm_status skm_argsort_u32(skm_renderer* r, const uint32_t* keys, uint32_t n, in
 skm_status skm_argsort_u32_device(skm_renderer* r, void* keys_dev, uint32_t n, int key_bits,
                                   void* order_dev, double* ms);
 
+/* ---- the loader's spatial order (reorderSpatially, packages/splat-core/src/sorting/SpatialOrder.cpp:48-92) ---- */
+/* mortonCode (SpatialOrder.cpp:39-44): 30-bit Morton code of a point quantised to 1024 cells per
+ * axis of the box [lo, hi] (an axis where the box is flat quantises to 0). */
+uint32_t skm_morton_code(const float position[3], const float lo[3], const float hi[3]);
+/* Permutes a host cloud in place along the Morton curve over [lo, hi] (the cloud's bounds), on the
+ * GPU: a stable sort of the codes, so equal codes keep their input order, then every attribute array
+ * gathered in that order (positions 3, covariances 6, colours 3, alphas 1 and sh_floats SH floats per
+ * splat; sh may be NULL when sh_floats is 0). order (may be NULL) receives the source index of each
+ * splat. */
+skm_status skm_reorder_spatially(skm_renderer* r, float* positions, float* covariances, float* colors,
+                                 float* alphas, float* sh, uint32_t sh_floats, uint32_t count, const float lo[3],
+                                 const float hi[3], uint32_t* order);
+
 /* ---- the LOD cut on its own (MetalLodSelection, MetalLodSelection.h:13-48) ---- */
 /* Uploads the tree's clusters and leaf references, selects one cut for this camera and reads it
  * back: capacity is the most nodes the cut may hold, pixel_limit 0 refines every cluster. */

@@ -822,6 +822,53 @@ void orc_argsort(const uint32_t* keys, uint32_t n, uint32_t* order) {
   std::stable_sort(order, order + n, [keys](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
 }
 
+// spreadBits (SpatialOrder.cpp:12-18): the low 10 bits of v, one every third bit.
+static uint32_t spread_bits(uint32_t v) {
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// quantise (SpatialOrder.cpp:20-26): 1024 cells over [lo, hi], 0 for a flat axis.
+static uint32_t quantise(float value, float lo, float hi) {
+  const float extent = hi - lo;
+  if (!(extent > 0.0f)) return 0u;
+  const float t = (value - lo) / extent;
+  const float scaled = std::floor(t * 1024.0f);
+  return static_cast<uint32_t>(std::clamp(scaled, 0.0f, 1023.0f));
+}
+
+uint32_t orc_morton_code(const float* p, const float* lo, const float* hi) {
+  return spread_bits(quantise(p[0], lo[0], hi[0])) | (spread_bits(quantise(p[1], lo[1], hi[1])) << 1) |
+         (spread_bits(quantise(p[2], lo[2], hi[2])) << 2);
+}
+
+// reorderSpatially's permutation (SpatialOrder.cpp:48-84): LSD radix sort of the 30-bit codes in
+// three stable counting passes of 10 bits.
+void orc_spatial_order(const float* positions, uint32_t n, const float* lo, const float* hi, uint32_t* order) {
+  std::vector<uint32_t> keys(n), keys2(n), ord2(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    keys[i] = orc_morton_code(positions + size_t{i} * 3, lo, hi);
+    order[i] = i;
+  }
+  uint32_t *kin = keys.data(), *kout = keys2.data(), *oin = order, *oout = ord2.data();
+  for (int shift = 0; shift < 30; shift += 10) {
+    std::vector<uint32_t> start(1025, 0u);
+    for (uint32_t i = 0; i < n; ++i) ++start[((kin[i] >> shift) & 0x3ffu) + 1];
+    for (int d = 0; d < 1024; ++d) start[d + 1] += start[d];
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t dst = start[(kin[i] >> shift) & 0x3ffu]++;
+      kout[dst] = kin[i];
+      oout[dst] = oin[i];
+    }
+    std::swap(kin, kout);
+    std::swap(oin, oout);
+  }
+  if (oin != order) std::copy(oin, oin + n, order);
+}
+
 // Footprint half-extents in pixels: every pixel centre the coverage rule accepts lies within
 // (ex, ey) of the centre. The quad bounds the offset by r(|a1| + |a2|) per axis (expandQuad,
 // SplatProjection.metalh:155-165); the cutoff bounds gu^2 + gv^2 by L = log2a0 - kLog2MinAlpha,

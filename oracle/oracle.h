@@ -68,6 +68,12 @@ uint32_t orc_depth_key16(const orc_camera* cam, float depth);
 /* Stable ascending argsort of u32 keys (std::stable_sort). */
 void orc_argsort(const uint32_t* keys, uint32_t n, uint32_t* order);
 
+/* mortonCode (SpatialOrder.cpp:39-44) over the box [lo, hi]. */
+uint32_t orc_morton_code(const float* p3, const float* lo3, const float* hi3);
+/* reorderSpatially's permutation (SpatialOrder.cpp:48-84): order[i] is the source index of splat i
+ * after three stable 10-bit counting passes over the Morton codes. */
+void orc_spatial_order(const float* positions, uint32_t n, const float* lo3, const float* hi3, uint32_t* order);
+
 /* Tile binning in depth order, then stable by tile. ranges: 2 words per tile.
  * Returns the pair count; pairs are written only when it fits `capacity`. */
 uint64_t orc_bin(const uint32_t* projected, const uint32_t* order, uint32_t n, uint32_t width,

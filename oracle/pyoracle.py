@@ -42,6 +42,8 @@ _API = {
     "orc_project": (C.c_int, [C.POINTER(Camera), _u32p, _u32p, C.c_int, C.c_uint32, _u32p]),
     "orc_depth_key16": (C.c_uint32, [C.POINTER(Camera), C.c_float]),
     "orc_argsort": (None, [_u32p, C.c_uint32, _u32p]),
+    "orc_morton_code": (C.c_uint32, [_fp, _fp, _fp]),
+    "orc_spatial_order": (None, [_fp, C.c_uint32, _fp, _fp, _u32p]),
     "orc_bin": (C.c_uint64, [_u32p, _u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _u32p, _u32p,
                              C.c_uint64, _u32p]),
     "orc_tile_rect": (C.c_uint32, [_u32p, C.c_uint32, C.c_uint32, C.c_uint32, _i32p]),
@@ -66,6 +68,8 @@ _API = {
 
 _REF_API = {
     "ref_to_half": (C.c_uint16, [C.c_float]),
+    "ref_morton_code": (C.c_uint32, [_fp, _fp, _fp]),
+    "ref_spatial_order": (None, [_fp, C.c_uint32, _fp, _fp, _u32p]),
     "ref_from_half": (C.c_float, [C.c_uint16]),
     "ref_pack_splats": (None, [C.c_uint32, _fp, _fp, _fp, _fp, _u32p]),
     "ref_sh_stride": (C.c_uint32, [C.c_int]),
@@ -174,6 +178,21 @@ def argsort(keys):
     order = np.zeros(max(len(keys), 1), np.uint32)
     lib().orc_argsort(up(keys), len(keys), up(order))
     return order[:len(keys)]
+
+
+def morton_code(p, lo, hi, f=None):
+    """mortonCode (SpatialOrder.cpp:39-44); f: another library with the same entry (the reference's)."""
+    args = [np.ascontiguousarray(x, np.float32) for x in (p, lo, hi)]
+    return int((f or lib().orc_morton_code)(*[fp(a) for a in args]))
+
+
+def spatial_order(positions, lo, hi, f=None):
+    """reorderSpatially's permutation (SpatialOrder.cpp:48-84): source index of each splat."""
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    lo, hi = np.ascontiguousarray(lo, np.float32), np.ascontiguousarray(hi, np.float32)
+    order = np.zeros(max(len(pos), 1), np.uint32)
+    (f or lib().orc_spatial_order)(fp(pos), len(pos), fp(lo), fp(hi), up(order))
+    return order[:len(pos)]
 
 
 def bin_tiles(projected, order, width, height, tile=TILE):

@@ -11,6 +11,7 @@
 #include "splat/math/Covariance.h"
 #include "splat/math/Half.h"
 #include "splat/math/Mat4.h"
+#include "splat/sorting/SpatialOrder.h"
 #include "splatkit/rendering/GpuLayout.h"
 #include "splatkit/rendering/LodThreshold.h"
 
@@ -33,6 +34,30 @@ splat::SplatCloud make_cloud(uint32_t n, int sh_degree, const float* positions,
 }  // namespace
 
 extern "C" {
+
+uint32_t ref_morton_code(const float* p3, const float* lo3, const float* hi3) {
+  splat::Bounds b;
+  for (int k = 0; k < 3; ++k) {
+    b.min[k] = lo3[k];
+    b.max[k] = hi3[k];
+  }
+  return splat::mortonCode(p3, b);
+}
+
+// reorderSpatially on a cloud whose alphas are the source indices: the permutation it applied.
+void ref_spatial_order(const float* positions, uint32_t n, const float* lo3, const float* hi3, uint32_t* order) {
+  splat::SplatCloud c;
+  c.positions.assign(positions, positions + size_t{n} * 3);
+  c.covariances.assign(size_t{n} * 6, 0.0f);
+  c.colors.assign(size_t{n} * 3, 0.0f);
+  for (uint32_t i = 0; i < n; ++i) c.alphas.push_back(static_cast<float>(i));
+  for (int k = 0; k < 3; ++k) {
+    c.bounds.min[k] = lo3[k];
+    c.bounds.max[k] = hi3[k];
+  }
+  splat::reorderSpatially(c);
+  for (uint32_t i = 0; i < n; ++i) order[i] = static_cast<uint32_t>(c.alphas[i]);
+}
 
 uint16_t ref_to_half(float v) { return splat::toHalf(v); }
 float ref_from_half(uint16_t h) { return splat::fromHalf(h); }

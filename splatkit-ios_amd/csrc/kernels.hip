@@ -1436,6 +1436,47 @@ hipError_t launch_pack(const WorldOut& out, uint32_t offset, uint32_t count, con
   return hipGetLastError();
 }
 
+namespace {
+__global__ __launch_bounds__(256) void k_morton(const float* __restrict__ pos, uint32_t n, float lo0, float lo1,
+                                                float lo2, float hi0, float hi1, float hi2,
+                                                uint32_t* __restrict__ codes) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const float p[3] = {pos[3 * static_cast<size_t>(i)], pos[3 * static_cast<size_t>(i) + 1],
+                      pos[3 * static_cast<size_t>(i) + 2]};
+  const float lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+  codes[i] = morton_code(p, lo, hi);
+}
+
+__global__ __launch_bounds__(256) void k_gather_rows(const float* __restrict__ in, float* __restrict__ out,
+                                                     const uint32_t* __restrict__ order, uint32_t n,
+                                                     uint32_t stride) {
+  const uint64_t total = static_cast<uint64_t>(n) * stride;
+  for (uint64_t e = static_cast<uint64_t>(blockIdx.x) * 256u + threadIdx.x; e < total;
+       e += static_cast<uint64_t>(gridDim.x) * 256u) {
+    const uint64_t row = e / stride, col = e - row * stride;
+    out[e] = in[static_cast<uint64_t>(order[row]) * stride + col];
+  }
+}
+}  // namespace
+
+hipError_t launch_morton(const float* positions, uint32_t n, const float lo[3], const float hi[3], uint32_t* codes,
+                         hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_morton, dim3(grid_for(n, 256)), dim3(256), 0, stream, positions, n, lo[0], lo[1], lo[2],
+                     hi[0], hi[1], hi[2], codes);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_rows(const float* in, float* out, const uint32_t* order, uint32_t n, uint32_t stride,
+                              hipStream_t stream) {
+  if (n == 0 || stride == 0) return hipSuccess;
+  const uint64_t total = static_cast<uint64_t>(n) * stride;
+  const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((total + 255) / 256, 8192));
+  hipLaunchKernelGGL(k_gather_rows, dim3(blocks), dim3(256), 0, stream, in, out, order, n, stride);
+  return hipGetLastError();
+}
+
 hipError_t launch_to_bgra8(const float* rgba, uint32_t* out, uint64_t pixels, hipStream_t stream) {
   if (pixels == 0) return hipSuccess;
   const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((pixels + 255) / 256, 4096));

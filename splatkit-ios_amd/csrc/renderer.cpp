@@ -1257,6 +1257,57 @@ skm_status skm_argsort_u32_device(skm_renderer* r, void* keys_dev, uint32_t n, i
   return SKM_OK;
 }
 
+uint32_t skm_morton_code(const float position[3], const float lo[3], const float hi[3]) {
+  if (position == nullptr || lo == nullptr || hi == nullptr) return 0;
+  return skm::morton_code(position, lo, hi);
+}
+
+// reorderSpatially (SpatialOrder.cpp:48-92): codes on the GPU, the stable argsort of their 30 bits
+// (equal codes keep decode order, as the reference's three stable counting passes do), then each
+// attribute array gathered in that order and copied back over the host array.
+skm_status skm_reorder_spatially(skm_renderer* r, float* positions, float* covariances, float* colors,
+                                 float* alphas, float* sh, uint32_t sh_floats, uint32_t count, const float lo[3],
+                                 const float hi[3], uint32_t* order) {
+  if (r == nullptr || lo == nullptr || hi == nullptr) return SKM_ERR_INVALID;
+  if (count > 0 && (positions == nullptr || covariances == nullptr || colors == nullptr || alphas == nullptr ||
+                    (sh_floats > 0 && sh == nullptr)))
+    return r->fail(SKM_ERR_INVALID, "incomplete cloud");
+  if (count < 2) {
+    if (order && count == 1) order[0] = 0;
+    return SKM_OK;
+  }
+  if (auto st = r->check(hipSetDevice(r->device), "device"); st != SKM_OK) return st;
+  DeviceBuffer codes, idx, in, out;
+  const size_t n = count;
+  const size_t widest = std::max<size_t>(6, sh_floats);
+  if (auto st = r->check(codes.allocate(n * 4), "codes"); st != SKM_OK) return st;
+  if (auto st = r->check(idx.allocate(n * 4), "order"); st != SKM_OK) return st;
+  if (auto st = r->check(in.allocate(n * widest * 4), "reorder"); st != SKM_OK) return st;
+  if (auto st = r->check(out.allocate(n * widest * 4), "reorder"); st != SKM_OK) return st;
+  hipError_t e = hipMemcpy(in.ptr, positions, n * 12, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = skm::launch_morton(in.as<float>(), count, lo, hi, codes.as<uint32_t>(), r->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  if (auto st = r->check(e, "morton codes"); st != SKM_OK) return st;
+  if (auto st = skm_argsort_u32_device(r, codes.ptr, count, 30, idx.ptr, nullptr); st != SKM_OK) return st;
+  struct Rows {
+    float* host;
+    size_t stride;
+  };
+  const Rows rows[] = {{positions, 3}, {covariances, 6}, {colors, 3}, {alphas, 1}, {sh, sh_floats}};
+  for (const Rows& a : rows) {
+    if (a.stride == 0 || a.host == nullptr) continue;
+    e = hipMemcpy(in.ptr, a.host, n * a.stride * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = skm::launch_gather_rows(in.as<float>(), out.as<float>(), idx.as<uint32_t>(), count,
+                                  static_cast<uint32_t>(a.stride), r->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+    if (e == hipSuccess) e = hipMemcpy(a.host, out.ptr, n * a.stride * 4, hipMemcpyDeviceToHost);
+    if (auto st = r->check(e, "reorder"); st != SKM_OK) return st;
+  }
+  if (order) return r->check(hipMemcpy(order, idx.ptr, n * 4, hipMemcpyDeviceToHost), "order");
+  return SKM_OK;
+}
+
 skm_status skm_argsort_u32(skm_renderer* r, const uint32_t* keys, uint32_t n, int key_bits, uint32_t* order) {
   if (r == nullptr || (n > 0 && (keys == nullptr || order == nullptr))) return SKM_ERR_INVALID;
   if (n == 0) return SKM_OK;

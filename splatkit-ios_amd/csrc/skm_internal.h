@@ -217,5 +217,32 @@ hipError_t launch_to_bgra8(const float* rgba, uint32_t* out, uint64_t pixels, hi
 hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev,
                           uint32_t capacity, const DigitPlan& plan, uint32_t* hist, uint64_t* status,
                           uint32_t* counts, int variant, hipStream_t stream);
+// The 30-bit Morton code of each of n points (xyz interleaved) over the box [lo, hi]
+// (mortonCode, SpatialOrder.cpp:39-44).
+hipError_t launch_morton(const float* positions, uint32_t n, const float lo[3], const float hi[3], uint32_t* codes,
+                         hipStream_t stream);
+// out row i = in row order[i], rows of `stride` floats.
+hipError_t launch_gather_rows(const float* in, float* out, const uint32_t* order, uint32_t n, uint32_t stride,
+                              hipStream_t stream);
+
+// mortonCode (SpatialOrder.cpp:10-44): each coordinate quantised to 1024 cells of the box (0 when the
+// box is flat on that axis), bits interleaved x, y, z from the lowest.
+__host__ __device__ inline uint32_t morton_spread(uint32_t v) {
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__host__ __device__ inline uint32_t morton_cell(float value, float lo, float hi) {
+  const float extent = hi - lo;
+  if (!(extent > 0.0f)) return 0u;
+  const float scaled = floorf(((value - lo) / extent) * 1024.0f);
+  return static_cast<uint32_t>(scaled < 0.0f ? 0.0f : (scaled > 1023.0f ? 1023.0f : scaled));
+}
+__host__ __device__ inline uint32_t morton_code(const float p[3], const float lo[3], const float hi[3]) {
+  return morton_spread(morton_cell(p[0], lo[0], hi[0])) | (morton_spread(morton_cell(p[1], lo[1], hi[1])) << 1) |
+         (morton_spread(morton_cell(p[2], lo[2], hi[2])) << 2);
+}
 
 }  // namespace skm

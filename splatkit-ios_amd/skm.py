@@ -214,6 +214,8 @@ API = {
     "skm_read_view_tiles": (C.c_int, [_R, C.c_uint32, C.POINTER(C.c_uint64), _u32p, _u32p, C.c_uint64, _u32p,
                                       C.c_uint32]),
     "skm_argsort_u32": (C.c_int, [_R, _u32p, C.c_uint32, C.c_int, _u32p]),
+    "skm_morton_code": (C.c_uint32, [_fp, _fp, _fp]),
+    "skm_reorder_spatially": (C.c_int, [_R, _fp, _fp, _fp, _fp, _fp, C.c_uint32, C.c_uint32, _fp, _fp, _u32p]),
     "skm_argsort_u32_device": (C.c_int, [_R, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p,
                                          C.POINTER(C.c_double)]),
 }
@@ -736,6 +738,27 @@ class Renderer:
         order = np.empty(max(1, len(keys)), np.uint32)
         self._check(self._lib.skm_argsort_u32(self._h, uptr(keys), len(keys), key_bits, uptr(order)))
         return order[:len(keys)]
+
+    def reorder_spatially(self, cloud, lo=None, hi=None):
+        """The loader's spatial order (reorderSpatially, SpatialOrder.cpp:48-92) on the GPU: permutes the
+        cloud's arrays in place along the Morton curve over [lo, hi] (default: the positions' bounds)
+        and returns the source index of each splat."""
+        pos = cloud["positions"]
+        n = len(pos)
+        arrays = [cloud[k] for k in ("positions", "covariances", "colors", "alphas")]
+        sh = cloud.get("sh")
+        for a in arrays + ([sh] if sh is not None else []):
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float32 and a.flags.c_contiguous and len(a) == n):
+                raise ValueError("reorder_spatially needs contiguous float32 arrays of one length")
+        lo = np.asarray(pos.min(0) if lo is None else lo, np.float32) if n else np.zeros(3, np.float32)
+        hi = np.asarray(pos.max(0) if hi is None else hi, np.float32) if n else np.zeros(3, np.float32)
+        order = np.zeros(max(n, 1), np.uint32)
+        sh_floats = sh.shape[1] if sh is not None and n else 0
+        ptr = lambda a: a.ctypes.data_as(_fp)
+        self._check(self._lib.skm_reorder_spatially(self._h, ptr(arrays[0]), ptr(arrays[1]), ptr(arrays[2]),
+                                                     ptr(arrays[3]), ptr(sh) if sh_floats else None, sh_floats, n,
+                                                     ptr(lo), ptr(hi), order.ctypes.data_as(_u32p)))
+        return order[:n]
 
     def argsort_device(self, keys_ptr, n, key_bits, order_ptr):
         ms = C.c_double()
