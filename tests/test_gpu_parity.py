@@ -271,9 +271,13 @@ def test_one_two_and_three_frames_in_flight_agree(skm_mod):
 
 
 def test_c1_frame_matches_oracle(renderer, skm_mod, oracle):
-    """The 3M-splat SH3 scene at 1920x1080: order bit-exact, pixels within 1/255 of FP64."""
-    say("case: C1 3M SH3 1920x1080 frame 0")
+    """The 3M-splat SH3 scene at 1920x1080, in the loader's spatial order as the benchmark draws it:
+    order bit-exact, pixels within 1/255 of FP64."""
+    say("case: C1 3M SH3 1920x1080 frame 0, spatial order")
     cloud = skm_mod.scene_c1()
+    order = renderer.reorder_spatially(cloud)
+    assert np.array_equal(order, oracle.spatial_order(cloud["positions"][np.argsort(order)],
+                                                      cloud["positions"].min(0), cloud["positions"].max(0)))
     view, proj, eye = skm_mod.orbit_camera(0, 1920, 1080)
     g = gpu_frame(renderer, skm_mod, cloud, view, proj, eye, 1920, 1080)
     o = oracle.frame(cloud, view, proj, eye, 1920, 1080)
