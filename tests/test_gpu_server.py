@@ -129,3 +129,23 @@ def test_poisson_stream_reports_latency(skm_mod, scenes):
         assert st["completed"] == 60 and 0 < st["p50_ms"] <= st["p99_ms"]
     finally:
         srv.close()
+
+
+def test_launches_stay_within_the_pixel_cap(skm_mod, scenes):
+    """A launch takes at most 2^25 pixels past its first request: 3840x2160 requests (8.3M pixels)
+    go at most four to a launch even when 64 are allowed and twelve are waiting."""
+    say("case: 12 requests at 3840x2160 for one scene, max_batch_views 64")
+    srv = skm_mod.Server(max_batch_views=64, max_in_flight=1)
+    try:
+        srv.add_scene(scenes[0])
+        views = skm_mod.views_c2(12, seed=81, radius=(3.0, 6.0), resolutions=((3840, 2160),))
+        tickets = [srv.submit(0, *v) for v in views]
+        done = srv.wait_all(tickets)
+        assert all(c["status"] == 0 for c in done.values())
+        sizes = [c["batch_views"] for c in done.values()]
+        say(f"launch sizes {sorted(set(sizes))}")
+        assert max(sizes) <= 4
+        for t in tickets:
+            srv.release(t)
+    finally:
+        srv.close()
