@@ -586,12 +586,14 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
   const uint32_t base = b * kSortTile;
 
   uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
-  // Every load first (positions clamped to n - 1), so a thread waits for one round trip.
+  // Every load first (positions clamped to n - 1), so a thread waits for one round trip. The inputs
+  // are read once per pass, so they are loaded non-temporally: they then displace fewer of the
+  // scatter's partially written lines from the XCD's L2 before those lines are merged.
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) {
     const uint32_t pos = min(base + wave * (kSortItems * 64u) + j * 64u + lane, n - 1u);
-    key[j] = keys_in[pos];
-    val[j] = vals_in != nullptr ? vals_in[pos] : pos;
+    key[j] = __builtin_nontemporal_load(keys_in + pos);
+    val[j] = vals_in != nullptr ? __builtin_nontemporal_load(vals_in + pos) : pos;
   }
   uint32_t valid_rows = 0;  // bit j: row j's position holds a key
 #pragma unroll
