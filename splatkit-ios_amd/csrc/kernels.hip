@@ -255,8 +255,11 @@ __device__ __forceinline__ bool cull_lane(const Camera& cam, const World& w, uin
   if constexpr (DEG >= 1) {
     constexpr uint32_t kStride = ((DEG + 1) * (DEG + 1) - 1) * 3 / 2 + (((DEG + 1) * (DEG + 1) - 1) * 3) % 2;
     uint32_t words[kStride];
+    // Non-temporal: a frame reads each survivor's SH words once, and the world is larger than the
+    // last-level cache, so they should not displace the frame's sort and raster buffers from it.
 #pragma unroll
-    for (uint32_t k = 0; k < kStride; ++k) words[k] = w.sh[static_cast<size_t>(k) * w.capacity + index];
+    for (uint32_t k = 0; k < kStride; ++k)
+      words[k] = __builtin_nontemporal_load(w.sh + static_cast<size_t>(k) * w.capacity + index);
     const float dx = px - cam.camera_position[0];
     const float dy = py - cam.camera_position[1];
     const float dz = pz - cam.camera_position[2];
