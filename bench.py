@@ -274,6 +274,8 @@ def lod_flythrough(skm, frames, leaves, frames_in_flight=FRAMES_IN_FLIGHT):
         r.finish()
         elapsed = time.perf_counter() - t0
         tot = r.frame_totals(reset=True)
+        if tot["dropped"] != 0 or tot["frames"] != frames:
+            raise SystemExit(f"LOD fly-through: {tot['dropped']} frames dropped, {tot['frames']} drawn of {frames}")
         n = max(1, tot["frames"])
         out["keys"][str(bits)] = {"frames_per_s": round(frames / elapsed, 1),
                                   "stage_ms": {k: round(tot["stage_ms"][i] / n, 4) for i, k in enumerate(names)},
@@ -378,6 +380,10 @@ def main():
     elapsed = time.perf_counter() - t0
     tot = r.frame_totals(reset=True)
     fps = args.steps / elapsed
+    # Every timed frame was drawn whole: a frame whose tile pairs overflowed would have been dropped
+    # (skm_finish above raises then) and is counted apart.
+    if tot["dropped"] != 0 or tot["frames"] != args.steps:
+        raise SystemExit(f"timed region: {tot['dropped']} frames dropped, {tot['frames']} drawn of {args.steps}")
 
     # The same frames one at a time (each drawn and finished before the next): kernel durations with
     # the GPU to themselves, beside the overlapped ones of the timed region.

@@ -192,7 +192,9 @@ int skm_sorts_on_gpu(const skm_renderer* r);
 
 /* ---- frames (SplatRenderer.h:134-159) ---- */
 skm_status skm_draw(skm_renderer* r, const skm_frame* frame);
-/* Waits until every submitted frame completed (the host side of a completion handler). */
+/* Waits until every submitted frame completed (the host side of a completion handler). Returns
+ * SKM_ERR_CAPACITY when any frame or view batch since the previous skm_finish was dropped because
+ * its tile pairs overflowed (the capacity has grown: drawing it again fits), and clears that report. */
 skm_status skm_finish(skm_renderer* r);
 uint64_t skm_submitted_frames(const skm_renderer* r);
 uint64_t skm_completed_frames(skm_renderer* r);
@@ -203,7 +205,8 @@ uint32_t skm_last_draw_count(const skm_renderer* r);
 uint32_t skm_last_selected_count(const skm_renderer* r);
 double skm_last_select_millis(const skm_renderer* r);
 const char* skm_device_description(const skm_renderer* r);
-/* captureNextFrame (MetalRenderer.h:29): the last completed frame's pixels. */
+/* captureNextFrame (MetalRenderer.h:29): the last completed frame's pixels; SKM_ERR_CAPACITY when
+ * that frame was dropped. */
 skm_status skm_read_frame(skm_renderer* r, int format, void* dst, size_t dst_bytes);
 
 /* ---- inspection of the last completed frame (parity tests and tools) ---- */
@@ -225,12 +228,14 @@ skm_status skm_read_lod_cut(skm_renderer* r, uint32_t* count, uint32_t* indices,
                             uint32_t* stats6);
 
 /* Sums over the frames completed since the last reset: GPU milliseconds per stage (as above),
- * survivors and tile pairs. For throughput reports over many frames. */
+ * survivors and tile pairs of the frames drawn, and the frames and view batches dropped because
+ * their tile pairs overflowed (not in the other sums). For throughput reports over many frames. */
 typedef struct skm_frame_totals {
   uint64_t frames;
   double stage_ms[6];
   uint64_t visible;
   uint64_t pairs;
+  uint64_t dropped;
 } skm_frame_totals;
 skm_status skm_read_frame_totals(skm_renderer* r, skm_frame_totals* out, int reset);
 
@@ -257,6 +262,22 @@ uint32_t skm_morton_code(const float position[3], const float lo[3], const float
 skm_status skm_reorder_spatially(skm_renderer* r, float* positions, float* covariances, float* colors,
                                  float* alphas, float* sh, uint32_t sh_floats, uint32_t count, const float lo[3],
                                  const float hi[3], uint32_t* order);
+
+/* ---- the LOD threshold (LodThreshold, packages/splatkit-engine/include/splatkit/rendering/LodThreshold.h:9-45) ----
+ * The screen-space error a tree world's cut runs at: the policy's, raised 10% per completed frame
+ * whose refinements the budget denied (up to max(64 px, policy)), relaxed 3% per frame whose cut
+ * stays under 90% of its limit, reset by a policy change (MetalFrameCompute.mm:166-172). The
+ * renderer adapts one of these per frame; the value type is exported for callers and tests. */
+typedef struct skm_lod_threshold {
+  float policy_pixels;
+  float pixels;
+} skm_lod_threshold;
+void skm_lod_threshold_reset(skm_lod_threshold* t, float policy_pixels);
+/* Takes one completed selection (nodes selected, refinements denied, the cut's limit) and returns
+ * the threshold for the next. */
+float skm_lod_threshold_update(skm_lod_threshold* t, uint32_t selected, uint32_t limited, uint32_t limit);
+/* The renderer's threshold after the frames completed so far: what its next frame's cut runs at. */
+float skm_lod_threshold_pixels(skm_renderer* r);
 
 /* ---- the LOD cut on its own (MetalLodSelection, MetalLodSelection.h:13-48) ---- */
 /* Uploads the tree's clusters and leaf references, selects one cut for this camera and reads it

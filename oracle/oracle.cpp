@@ -1161,4 +1161,24 @@ int orc_render_frame(const orc_camera* cam, const uint32_t* splats, const uint32
   return 0;
 }
 
+// LodThreshold (LodThreshold.h:20-36) over a sequence of completed selections: starts at the policy's
+// threshold; each step multiplies it by 1.1 when refinements were denied (limited > 0), capped at
+// max(64, policy), else divides it by 1.03 when selected < limit / 10 * 9 (integer division), floored
+// at the policy; out[i] is the threshold after step i.
+void orc_lod_threshold(float policy, uint32_t steps, const uint32_t* selected, const uint32_t* limited,
+                       const uint32_t* limit, float* out) {
+  float pixels = policy;
+  for (uint32_t i = 0; i < steps; ++i) {
+    if (limited[i] > 0) {
+      const float raised = pixels * 1.1f;
+      const float ceiling = policy > 64.0f ? policy : 64.0f;
+      pixels = raised < ceiling ? raised : ceiling;
+    } else if (selected[i] < limit[i] / 10u * 9u) {
+      const float relaxed = pixels / 1.03f;
+      pixels = relaxed > policy ? relaxed : policy;
+    }
+    out[i] = pixels;
+  }
+}
+
 }  // extern "C"

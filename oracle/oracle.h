@@ -130,6 +130,11 @@ uint32_t orc_lod_select(const orc_camera* cam, const uint32_t* clusters, uint32_
 int orc_lod_cluster(const orc_camera* cam, const uint32_t* cluster16, float color_weight,
                     float* error_out);
 
+/* LodThreshold (LodThreshold.h:20-36) over `steps` completed selections: out[i] is the threshold
+ * after the i-th (selected, limited, limit), starting from the policy's. */
+void orc_lod_threshold(float policy, uint32_t steps, const uint32_t* selected, const uint32_t* limited,
+                       const uint32_t* limit, float* out);
+
 /* Transcendentals built from IEEE basic operations only (argument reduction plus series), so
  * any two IEEE machines compute identical bits. */
 double orc_log_d(double x);

@@ -61,6 +61,7 @@ _API = {
     "orc_region_keep": (C.c_int, [_u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "orc_region_misses": (C.c_uint64, [_u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                        C.POINTER(C.c_uint64)]),
+    "orc_lod_threshold": (None, [C.c_float, C.c_uint32, _u32p, _u32p, _u32p, _fp]),
     "orc_log_d": (C.c_double, [C.c_double]),
     "orc_log_f": (C.c_float, [C.c_float]),
     "orc_exp_d": (C.c_double, [C.c_double]),
@@ -299,3 +300,14 @@ def lod_select(cam, clusters, leaves, rounds, capacity, pixel_limit=1.0, color_w
     n = lib().orc_lod_select(C.byref(cam), up(clusters), len(clusters), up(leaves if len(leaves) else np.zeros(1, np.uint32)),
                              rounds, capacity, pixel_limit, color_weight, 1 if cull else 0, up(out), up(stats))
     return out[:n].copy(), stats
+
+
+def lod_threshold(policy, steps, f=None):
+    """LodThreshold over (selected, limited, limit) steps: the threshold after each. f: another library
+    with the same entry (the reference's ref_lod_threshold)."""
+    sel = np.ascontiguousarray([s[0] for s in steps], np.uint32)
+    lim = np.ascontiguousarray([s[1] for s in steps], np.uint32)
+    cap = np.ascontiguousarray([s[2] for s in steps], np.uint32)
+    out = np.zeros(max(1, len(steps)), np.float32)
+    (f or lib().orc_lod_threshold)(policy, len(steps), up(sel), up(lim), up(cap), fp(out))
+    return out[:len(steps)]

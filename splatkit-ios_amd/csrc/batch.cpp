@@ -345,6 +345,8 @@ hipError_t Batch::wait(bool* dropped) {
   return hipSuccess;
 }
 
+bool Batch::at_pair_limit() const { return pair_want_ >= kLaneLimit && pair_capacity_ >= kLaneLimit; }
+
 uint64_t Batch::device_bytes() const {
   uint64_t b = 0;
   for (const DeviceBuffer* d : {&ctrl_, &cams_, &views_, &tile_view_, &view_visible_, &cull_keys_,

@@ -130,19 +130,14 @@ struct Policy {
   double background[3] = {0.05, 0.05, 0.08};
 };
 
-// The selection threshold of LodThreshold (LodThreshold.h:9-45): raised 10% while refinements are
-// denied (up to max(64 px, policy)), relaxed 3% while the cut stays under 90% of its limit.
+// The selection threshold of LodThreshold (LodThreshold.h:9-45), on the exported value type.
 struct LodThreshold {
-  float policy = 1.0f, pixels = 1.0f;
-  void reset(float p) { policy = pixels = p; }
+  skm_lod_threshold t{1.0f, 1.0f};
+  void reset(float p) { skm_lod_threshold_reset(&t, p); }
   float update(uint32_t selected, uint32_t limited, uint32_t limit) {
-    if (limited > 0) {
-      pixels = std::min(pixels * 1.1f, std::max(64.0f, policy));
-    } else if (selected < limit / 10 * 9) {
-      pixels = std::max(pixels / 1.03f, policy);
-    }
-    return pixels;
+    return skm_lod_threshold_update(&t, selected, limited, limit);
   }
+  float pixels() const { return t.pixels; }
 };
 
 // Device buffers of one tree (MetalLodSelection::upload, MetalLodSelection.mm:60-108): clusters and
@@ -268,6 +263,7 @@ struct skm_renderer {
   uint32_t capacity = 0;
   uint64_t pair_target = 0;
   int sort_variant = skm::kSortReduceScan;
+  DeviceBuffer staging;  // upload staging, on `stream`
 
   // Standalone sort scratch, on `stream`.
   Scratch sort;
@@ -289,7 +285,11 @@ struct skm_renderer {
   bool completed_world_frame = false;
   bool grow_pairs = false;
   uint64_t pairs_needed = 0;
+  // A frame or batch whose tile pairs overflowed was dropped (its image is background only).
+  // last_dropped: the newest completed frame was; drops: frames and batches dropped since the last
+  // skm_finish, which reports them (however many frames completed after them).
   bool last_dropped = false;
+  uint64_t drops = 0;
   double stage_ms[6] = {};
   uint32_t last_draw_count = 0;
   skm_frame_totals totals{};
@@ -297,7 +297,7 @@ struct skm_renderer {
   // View batches (skm_draw_views).
   std::unique_ptr<skm::Batch> batch;
   skm_batch_tuning batch_tuning{16, 8, 1, 0, 1, 0};
-  bool batch_dropped = false;
+  bool batch_dropped = false;  // the newest completed batch was dropped
   bool batch_done = false;
 
   skm_status fail(skm_status s, const char* fmt, ...) {
@@ -352,13 +352,18 @@ struct skm_renderer {
     }
     last_draw_count = c->visible;
     last_dropped = c->overflow != 0;
-    totals.frames += 1;
-    for (int k = 0; k < 6; ++k) totals.stage_ms[k] += ms[k];
-    totals.visible += c->visible;
-    totals.pairs += c->pairs;
     if (last_dropped) {
+      // Counted apart: a dropped frame's work stopped after the binning, so its timings and pairs
+      // are not a drawn frame's.
+      totals.dropped += 1;
+      ++drops;
       grow_pairs = true;
       pairs_needed = std::max<uint64_t>(pairs_needed, c->pairs);
+    } else {
+      totals.frames += 1;
+      for (int k = 0; k < 6; ++k) totals.stage_ms[k] += ms[k];
+      totals.visible += c->visible;
+      totals.pairs += c->pairs;
     }
     ++completed;
     if (s.with_world) completed_world_frame = true;
@@ -387,9 +392,16 @@ struct skm_renderer {
     }
     if (batch) {
       bool dropped = false;
+      const bool batch_waited = batch->in_flight();
       const skm_status k = check(batch->wait(&dropped), "batch");
       if (k != SKM_OK) st = k;
-      if (dropped) batch_dropped = true;
+      if (k == SKM_OK && batch_waited) {
+        batch_dropped = dropped;
+        if (dropped) {
+          totals.dropped += 1;
+          ++drops;
+        }
+      }
     }
     return st;
   }
@@ -511,15 +523,17 @@ struct skm_renderer {
     return check(hipMemsetAsync(x.scan_status.ptr, 0, x.scan_status.bytes, st_), "clear");
   }
 
+  // Copies a cloud's floats through the upload staging buffer (kept between uploads: freeing device
+  // memory waits for the whole device, and tile uploads run while frames are in flight) and packs
+  // them into records [offset, offset + count) on the renderer's stream. Blocking.
   skm_status upload_range(WorldBuffers& w, uint32_t offset, const skm_cloud_view* c, int src_degree) {
     const uint32_t n = c->count;
     const size_t shf = src_degree > 0 ? static_cast<size_t>((src_degree + 1) * (src_degree + 1) - 1) * 3u : 0;
     const uint32_t chunk = std::min(n, kUploadChunk);
-    DeviceBuffer staging;
     const size_t per = 12 + 24 + 12 + 4 + shf * 4;
-    if (auto st = check(staging.allocate(size_t{std::max(chunk, 1u)} * per + 1024), "upload staging");
-        st != SKM_OK)
-      return st;
+    const size_t want = size_t{std::max(chunk, 1u)} * per + 1024;
+    if (want > staging.bytes)
+      if (auto st = check(staging.allocate(want), "upload staging"); st != SKM_OK) return st;
     float* d_pos = staging.as<float>(0);
     float* d_cov = d_pos + size_t{chunk} * 3;
     float* d_col = d_cov + size_t{chunk} * 6;
@@ -827,7 +841,9 @@ skm_status skm_create_slab(skm_renderer* r, uint32_t capacity, int sh_degree) {
   return r->check(hipStreamSynchronize(r->stream), "slab");
 }
 
-// MetalWorld::uploadTile (MetalWorld.mm:81-100).
+// MetalWorld::uploadTile (MetalWorld.mm:81-100). Blocking, and it does not wait for the frames in
+// flight: the caller rewrites a range only after every frame that drew it completed
+// (SplatRenderer.h:94-96), so frames drawing the slab's other ranges keep running.
 skm_status skm_upload_tile(skm_renderer* r, uint32_t offset, const skm_cloud_view* cloud) {
   if (r == nullptr) return SKM_ERR_INVALID;
   if (!r->has_world || !r->world.slab) return r->fail(SKM_ERR_STATE, "no slab");
@@ -835,7 +851,8 @@ skm_status skm_upload_tile(skm_renderer* r, uint32_t offset, const skm_cloud_vie
   if (offset > r->world.count || cloud->count > r->world.count - offset)
     return r->fail(SKM_ERR_INVALID, "tile [%u, +%u) outside the slab of %u", offset, cloud->count, r->world.count);
   if (cloud->count == 0) return SKM_OK;
-  if (auto st = r->drain(); st != SKM_OK) return st;
+  if (r->gpu_failed) return r->fail(SKM_ERR_DEVICE, "GPU failed earlier");
+  if (auto st = r->check(hipSetDevice(r->device), "device"); st != SKM_OK) return st;
   const int deg = r->world.sh_degree;
   const bool carries = deg > 0 && cloud->sh_degree >= deg && cloud->sh != nullptr;
   skm_cloud_view c = *cloud;
@@ -922,7 +939,7 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   if (lod_frame) {
     // The slot's own cut state: the cut overlaps the other slots' frames.
     s.lod_limit = r->policy.lod_splat_limit == 0 ? r->lod.capacity : std::min(r->policy.lod_splat_limit, r->lod.capacity);
-    lod = r->lod.args(cam_dev, s.cut, s.lod_limit, r->threshold.pixels, 4.0f, 1);
+    lod = r->lod.args(cam_dev, s.cut, s.lod_limit, r->threshold.pixels(), 4.0f, 1);
   }
   skm::FrameArgs a{};
   a.cam = cam_dev;
@@ -985,11 +1002,14 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
 skm_status skm_finish(skm_renderer* r) {
   if (r == nullptr) return SKM_ERR_INVALID;
   if (auto st = r->drain(); st != SKM_OK) return st;
-  if (r->last_dropped)
-    return r->fail(SKM_ERR_CAPACITY, "the last frame needed %llu tile pairs; capacity grows on the next draw",
-                   static_cast<unsigned long long>(r->pairs_needed));
-  if (r->batch_dropped)
-    return r->fail(SKM_ERR_CAPACITY, "the last batch's tile pairs overflowed; capacity grows on the next batch");
+  if (r->drops > 0) {
+    const unsigned long long n = r->drops;
+    r->drops = 0;
+    return r->fail(SKM_ERR_CAPACITY,
+                   "%llu frame(s) or batch(es) since the last finish were dropped: tile pairs overflowed (up to %llu "
+                   "needed); capacity grows on the next draw",
+                   n, static_cast<unsigned long long>(r->pairs_needed));
+  }
   return SKM_OK;
 }
 
@@ -1026,6 +1046,31 @@ skm_status skm_read_lod_cut(skm_renderer* r, uint32_t* count, uint32_t* indices,
   const LodCut& cut = r->slots[r->last_slot].cut;
   if (indices && n) return r->check(hipMemcpy(indices, cut.indices.ptr, size_t{n} * 4, hipMemcpyDeviceToHost), "read");
   return SKM_OK;
+}
+
+// LodThreshold::reset and ::update (LodThreshold.h:22-36): x1.1 while refinements are denied, up to
+// max(kCeilingPixels = 64, policy); /1.03 while the cut stays under limit / 10 * 9, down to the policy.
+void skm_lod_threshold_reset(skm_lod_threshold* t, float policy_pixels) {
+  if (t == nullptr) return;
+  t->policy_pixels = policy_pixels;
+  t->pixels = policy_pixels;
+}
+
+float skm_lod_threshold_update(skm_lod_threshold* t, uint32_t selected, uint32_t limited, uint32_t limit) {
+  if (t == nullptr) return 0.0f;
+  constexpr float kRaise = 1.1f, kRelax = 1.03f, kCeiling = 64.0f;
+  if (limited > 0) {
+    t->pixels = std::min(t->pixels * kRaise, std::max(kCeiling, t->policy_pixels));
+  } else if (selected < limit / 10 * 9) {
+    t->pixels = std::max(t->pixels / kRelax, t->policy_pixels);
+  }
+  return t->pixels;
+}
+
+float skm_lod_threshold_pixels(skm_renderer* r) {
+  if (r == nullptr) return 0.0f;
+  r->poll();
+  return r->threshold.pixels();
 }
 
 skm_status skm_lod_select(skm_renderer* r, const skm_lod_tree_view* tree, const float* view16, const float* proj16,
@@ -1079,6 +1124,7 @@ skm_status skm_read_frame(skm_renderer* r, int format, void* dst, size_t dst_byt
   if (r == nullptr || dst == nullptr) return SKM_ERR_INVALID;
   if (auto st = r->drain(); st != SKM_OK) return st;
   if (r->last_slot < 0) return r->fail(SKM_ERR_STATE, "no frame drawn");
+  if (r->last_dropped) return r->fail(SKM_ERR_CAPACITY, "the last frame was dropped (tile pairs overflowed)");
   Slot& s = r->slots[r->last_slot];
   const size_t px = size_t{s.width} * s.height;
   if (format == SKM_PIXELS_RGBA32F) {
@@ -1188,6 +1234,8 @@ skm_status skm_read_tiles(skm_renderer* r, uint64_t* pairs, uint32_t* pair_tiles
   if (auto st = r->check(hipMemcpy(&c, x.ctrl.ptr, sizeof(Ctrl), hipMemcpyDeviceToHost), "read"); st != SKM_OK)
     return st;
   *pairs = c.pairs;
+  if (c.overflow != 0 && (pair_tiles != nullptr || pair_values != nullptr || ranges != nullptr))
+    return r->fail(SKM_ERR_CAPACITY, "the last frame was dropped: %u tile pairs exceeded its capacity", c.pairs);
   const Slot& s = r->slots[r->last_slot];
   const int fin = s.tile_passes & 1;
   const uint64_t n = std::min<uint64_t>(c.sort_pairs, capacity);
@@ -1345,7 +1393,6 @@ skm_status skm_draw_views(skm_renderer* r, const skm_view* views, uint32_t count
   }
   if (auto st = r->check(hipSetDevice(r->device), "device"); st != SKM_OK) return st;
   if (auto st = r->drain(); st != SKM_OK) return st;
-  r->batch_dropped = false;
   r->batch_done = false;
   if (!r->batch) r->batch = std::make_unique<skm::Batch>();
   skm::BatchPolicy p;

@@ -37,6 +37,10 @@ using Clock = std::chrono::steady_clock;
 // 3840x2160 views; 512 MB of image).
 constexpr uint64_t kLaunchLanes = uint64_t{1} << 26;
 constexpr uint64_t kLaunchPixels = uint64_t{1} << 25;
+// Launches a request may take part in whose tile pairs overflowed (each grows the pair capacity to
+// what the last one needed, so the next fits unless the scene's pairs keep growing); then it
+// completes with SKM_ERR_CAPACITY.
+constexpr uint32_t kMaxAttempts = 3;
 
 double ms_between(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
@@ -76,6 +80,7 @@ struct Request {
   size_t bytes = 0;
   int32_t status = SKM_OK;
   uint32_t batch_views = 0;
+  uint32_t attempts = 0;  // launches that dropped it (tile pairs overflowed)
   bool done = false;
 };
 
@@ -178,33 +183,47 @@ struct skm_server {
   }
 
   // Launches `batch` on the scene's stream: the view batch, the BGRA8 conversion, one copy per
-  // request into its pinned block, and the completion event.
+  // request into its pinned block, and the completion event. Every host block and the output buffer
+  // are taken before any work is queued; if queueing fails part-way, the stream is drained before
+  // returning, so no copy is still writing a block its request gives back.
   skm_status launch(Scene& sc, std::vector<Request*>& batch) {
     std::vector<skm_view> views(batch.size());
-    for (size_t k = 0; k < batch.size(); ++k) views[k] = batch[k]->req.view;
+    uint64_t pixels = 0;
+    for (size_t k = 0; k < batch.size(); ++k) {
+      Request* q = batch[k];
+      views[k] = q->req.view;
+      const size_t bytes = size_t{q->req.view.width} * q->req.view.height * 4;
+      if (q->pixels != nullptr && q->bytes != bytes) {
+        pool.give(q->pixels, q->bytes);
+        q->pixels = nullptr;
+      }
+      q->bytes = bytes;
+      if (q->pixels == nullptr) q->pixels = static_cast<uint8_t*>(pool.take(q->bytes));
+      if (q->pixels == nullptr) return fail(SKM_ERR_MEMORY, "pinned output");
+      pixels += uint64_t{q->req.view.width} * q->req.view.height;
+    }
+    if (sc.bgra.bytes < pixels * 4 && sc.bgra.allocate(pixels * 4) != hipSuccess)
+      return fail(SKM_ERR_MEMORY, "output buffer");
     skm_batch_tuning t = tuning;
     const skm_range whole{0, sc.count};
     skm_status st = skm_draw_views(sc.r, views.data(), static_cast<uint32_t>(views.size()), &whole, 1, &t);
     if (st != SKM_OK) return fail(st, "%s", skm_last_error(sc.r));
     skm::Batch* b = skm::renderer_batch(sc.r);
     hipStream_t stream = skm::renderer_stream(sc.r);
-    const uint64_t pixels = b->pixels();
-    if (sc.bgra.bytes < pixels * 4 && sc.bgra.allocate(pixels * 4) != hipSuccess)
-      return fail(SKM_ERR_MEMORY, "output buffer");
-    hipError_t e = skm::launch_to_bgra8(b->image_device(), sc.bgra.as<uint32_t>(), pixels, stream);
+    hipError_t e = skm::launch_to_bgra8(b->image_device(), sc.bgra.as<uint32_t>(), b->pixels(), stream);
     const auto now = Clock::now();
     for (size_t k = 0; k < batch.size() && e == hipSuccess; ++k) {
       Request* q = batch[k];
       const skm::ViewInfo& vi = b->info(static_cast<uint32_t>(k));
-      q->bytes = size_t{vi.width} * vi.height * 4;
-      if (q->pixels == nullptr) q->pixels = static_cast<uint8_t*>(pool.take(q->bytes));
-      if (q->pixels == nullptr) return fail(SKM_ERR_MEMORY, "pinned output");
       e = hipMemcpyAsync(q->pixels, sc.bgra.as<uint32_t>() + vi.pixel_base, q->bytes, hipMemcpyDeviceToHost, stream);
       q->launched = now;
       q->batch_views = static_cast<uint32_t>(batch.size());
     }
     if (e == hipSuccess) e = hipEventRecord(sc.done, stream);
-    if (e != hipSuccess) return fail(SKM_ERR_DEVICE, "launch: %s", hipGetErrorString(e));
+    if (e != hipSuccess) {
+      (void)hipStreamSynchronize(stream);
+      return fail(SKM_ERR_DEVICE, "launch: %s", hipGetErrorString(e));
+    }
     sc.running = batch;
     sc.busy = true;
     ++launches;
@@ -212,8 +231,11 @@ struct skm_server {
     return SKM_OK;
   }
 
-  // Retires finished launches; relaunches one whose tile pairs overflowed (its scratch has grown).
-  void retire(std::vector<Scene*>& relaunch) {
+  // Retires finished launches. The requests of a launch whose tile pairs overflowed go back to the
+  // head of the queue in arrival order (the scene's pair capacity has grown), so admission prices
+  // the grown scratch against the budget again; after kMaxAttempts drops, or when the capacity is at
+  // its 32-bit limit, they complete with SKM_ERR_CAPACITY.
+  void retire() {
     for (auto& up : scenes) {
       Scene& sc = *up;
       if (!sc.busy || hipEventQuery(sc.done) != hipSuccess) continue;
@@ -221,10 +243,18 @@ struct skm_server {
       const skm_status st = skm_finish(sc.r);
       const auto now = Clock::now();
       if (st == SKM_ERR_CAPACITY) {
-        relaunch.push_back(&sc);
-        continue;
+        const bool limit = skm::renderer_batch(sc.r)->at_pair_limit();
+        std::vector<Request*> again;
+        for (Request* q : sc.running) {
+          if (++q->attempts >= kMaxAttempts || limit)
+            finish_request(q, SKM_ERR_CAPACITY, now);
+          else
+            again.push_back(q);
+        }
+        pending.insert(pending.begin(), again.begin(), again.end());
+      } else {
+        for (Request* q : sc.running) finish_request(q, st, now);
       }
-      for (Request* q : sc.running) finish_request(q, st, now);
       sc.running.clear();
       cv_done.notify_all();
     }
@@ -279,9 +309,9 @@ struct skm_server {
         continue;
       }
       for (Request* q : batch) pending.erase(std::find(pending.begin(), pending.end(), q));
-      if (launch(sc, batch) != SKM_OK) {
+      if (const skm_status st = launch(sc, batch); st != SKM_OK) {
         const auto now = Clock::now();
-        for (Request* q : batch) finish_request(q, SKM_ERR_DEVICE, now);
+        for (Request* q : batch) finish_request(q, st, now);
         cv_done.notify_all();
       }
     }
@@ -291,17 +321,7 @@ struct skm_server {
     (void)hipSetDevice(device);
     std::unique_lock<std::mutex> lock(mu);
     while (!stop) {
-      std::vector<Scene*> relaunch;
-      retire(relaunch);
-      for (Scene* sc : relaunch) {
-        std::vector<Request*> again = sc->running;
-        if (launch(*sc, again) != SKM_OK) {
-          const auto now = Clock::now();
-          for (Request* q : again) finish_request(q, SKM_ERR_DEVICE, now);
-          sc->running.clear();
-          cv_done.notify_all();
-        }
-      }
+      retire();
       admit();
       if (in_flight() > 0) {
         // Launches are in flight: poll their events at a fine grain.

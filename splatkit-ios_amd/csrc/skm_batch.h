@@ -41,6 +41,10 @@ class Batch {
   // Waits for the last launched batch. *dropped is set when its tile pairs overflowed the pair
   // capacity: the capacity has then been marked to grow and the batch must be launched again.
   hipError_t wait(bool* dropped);
+  bool in_flight() const { return in_flight_; }
+  // The pair capacity cannot grow further (32-bit pair indices): a batch that still overflows can
+  // never be drawn whole.
+  bool at_pair_limit() const;
 
   uint32_t view_count() const { return static_cast<uint32_t>(info_.size()); }
   const ViewInfo& info(uint32_t v) const { return info_[v]; }

@@ -83,9 +83,13 @@ class DeviceCapabilities(C.Structure):
                 ("max_texture_dimension", C.c_uint32), ("policy", PolicySupport)]
 
 
+class LodThreshold(C.Structure):
+    _fields_ = [("policy_pixels", C.c_float), ("pixels", C.c_float)]
+
+
 class FrameTotals(C.Structure):
     _fields_ = [("frames", C.c_uint64), ("stage_ms", C.c_double * 6), ("visible", C.c_uint64),
-                ("pairs", C.c_uint64)]
+                ("pairs", C.c_uint64), ("dropped", C.c_uint64)]
 
 
 class View(C.Structure):
@@ -200,6 +204,9 @@ API = {
     "skm_read_stage_millis": (C.c_int, [_R, C.POINTER(C.c_double)]),
     "skm_read_frame_totals": (C.c_int, [_R, C.POINTER(FrameTotals), C.c_int]),
     "skm_read_lod_cut": (C.c_int, [_R, _u32p, _u32p, C.c_uint32, _u32p]),
+    "skm_lod_threshold_reset": (None, [C.POINTER(LodThreshold), C.c_float]),
+    "skm_lod_threshold_update": (C.c_float, [C.POINTER(LodThreshold), C.c_uint32, C.c_uint32, C.c_uint32]),
+    "skm_lod_threshold_pixels": (C.c_float, [_R]),
     "skm_lod_select": (C.c_int, [_R, C.POINTER(LodTreeView), _fp, _fp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
                                  C.c_float, C.c_int, _u32p, _u32p]),
     "skm_draw_views": (C.c_int, [_R, C.POINTER(View), C.c_uint32, C.POINTER(Range), C.c_uint32,
@@ -681,6 +688,13 @@ class Renderer:
                                                   tiles))
         return pairs.value, pt[:cap].copy(), pv[:cap].copy(), ranges[:tiles].copy()
 
+    def set_linear_blending(self, linear):
+        self._lib.skm_set_linear_blending(self._h, 1 if linear else 0)
+
+    def lod_threshold(self):
+        """The screen-space error the next tree-world frame's cut runs at (LodThreshold::pixels)."""
+        return float(self._lib.skm_lod_threshold_pixels(self._h))
+
     def stage_millis(self):
         """GPU ms of the last frame: LOD cut, cull, depth sort, bin + tile sort + ranges, raster, total."""
         ms = (C.c_double * 6)()
@@ -690,7 +704,8 @@ class Renderer:
     def frame_totals(self, reset=True):
         t = FrameTotals()
         self._check(self._lib.skm_read_frame_totals(self._h, C.byref(t), 1 if reset else 0))
-        return {"frames": t.frames, "stage_ms": list(t.stage_ms), "visible": t.visible, "pairs": t.pairs}
+        return {"frames": t.frames, "stage_ms": list(t.stage_ms), "visible": t.visible, "pairs": t.pairs,
+                "dropped": t.dropped}
 
     def read_frame(self, fmt=PIXELS_RGBA32F):
         w, h = self.extent()

@@ -7,6 +7,7 @@ Every HIP source is compiled with -ffp-contract=off and without fast-math (each 
 one IEEE rounding, which the bit-exact parity of the cull relies on) and with packed-fp32
 instructions disabled.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -37,17 +38,22 @@ def _run(cmd):
 
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    jobs = []
     for src in ("kernels.hip", "renderer.cpp", "batch.cpp", "lod_build.cpp", "lod.hip", "server.cpp"):
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(BUILD, src + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        out = _run([HIPCC] + HIP_FLAGS + lang + ["-c", path, "-o", obj])
-        if verbose and out.strip():
-            print(out)
-        objs.append(obj)
+        jobs.append(([HIPCC] + HIP_FLAGS + lang + ["-c", path, "-o", obj], obj))
+    # One compiler per source, side by side.
+    with concurrent.futures.ThreadPoolExecutor(max_workers=max(1, min(len(jobs), os.cpu_count() or 1))) as pool:
+        outs = list(pool.map(lambda job: _run(job[0]), jobs))
+    if verbose:
+        for out in outs:
+            if out.strip():
+                print(out)
+    objs = [obj for _, obj in jobs]
     lib = os.path.join(HERE, "libskm.so")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib + ".tmp"] + objs)
     os.replace(lib + ".tmp", lib)

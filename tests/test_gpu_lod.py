@@ -71,8 +71,11 @@ def test_built_trees_at_many_cameras(renderer, skm_mod, oracle):
         both(renderer, oracle, tree, view, proj, np.asarray(eye, np.float32), 1920, 1080, budget, pixel)
 
 
-def test_lod_world_frame_matches_oracle(renderer, skm_mod, oracle):
-    say("case: LOD world frame (cut -> indexed cull -> sort -> tiles -> raster)")
+@pytest.mark.parametrize("key16", [False, True])
+def test_lod_world_frame_matches_oracle(renderer, skm_mod, oracle, key16):
+    """A tree world's frame with 32-bit and with 16-bit depth keys (the two variants of the LOD
+    fly-through; quantizedDepthKey, SplatVisibility.metal:21-28, on the cut's indexed cull)."""
+    say(f"case: LOD world frame (cut -> indexed cull -> sort -> tiles -> raster), key16={key16}")
     cloud = skm_mod.make_cloud(150_000, 1, seed=66, layout=1, spread=10.0, area=30.0, log_scale_mean=-3.5)
     tree = skm_mod.LodTree.build(cloud, 21, 256, threads=16)
     renderer.upload_lod_world(tree, budget=60_000)
@@ -81,16 +84,59 @@ def test_lod_world_frame_matches_oracle(renderer, skm_mod, oracle):
     view = skm_mod.look_at(eye)
     proj = skm_mod.perspective(np.pi / 3, 960 / 540, 0.2, 500.0)
     renderer.set_extent(960, 540)
-    renderer.apply_policy(sort_depth=32, lod_error=1.5)
+    renderer.apply_policy(sort_depth=16 if key16 else 32, lod_error=1.5)
     renderer.render(view, proj, eye)
     cut, stats = renderer.read_lod_cut()
     cam = oracle.make_camera(view, proj, 960, 540, eye)
     o_cut, o_stats = oracle.lod_select(cam, tree.clusters, tree.leaves, tree.rounds, 60_000, 1.5, 4.0, True)
     assert np.array_equal(cut, o_cut)
     assert renderer.stats()["selected"] == len(o_cut)
-    g = gpu_frame(renderer, skm_mod, None, view, proj, eye, 960, 540, upload=False, lod_error=1.5)
-    o = oracle.frame(tree.nodes, view, proj, eye, 960, 540, active=o_cut)
-    assert_parity(g, o, label="LOD frame")
+    g = gpu_frame(renderer, skm_mod, None, view, proj, eye, 960, 540, upload=False, lod_error=1.5, key16=key16)
+    o = oracle.frame(tree.nodes, view, proj, eye, 960, 540, active=o_cut, key16=key16)
+    assert_parity(g, o, label=f"LOD frame key16={key16}")
+
+
+def test_lod_budget_denials_raise_then_relax_the_threshold(skm_mod, oracle):
+    """adaptLodThreshold (MetalFrameCompute.mm:166-172, LodThreshold.h:27-36) on the GPU path: under a
+    budget the close-up cut cannot meet, every frame's cut equals the oracle's cut at the threshold
+    the renderer reported before it, denials raise the threshold 10% a frame, and once the camera
+    turns away (the cut sits under 90% of its limit) it relaxes 3% a frame back to the policy's."""
+    say("case: LOD budget 6000 at 1 px, denials then relaxation, one frame in flight")
+    cloud = skm_mod.make_cloud(150_000, 1, seed=68, layout=1, spread=10.0, area=30.0, log_scale_mean=-3.5)
+    tree = skm_mod.LodTree.build(cloud, 21, 32, threads=16)
+    budget = 6000
+    w, h = 960, 540
+    proj = skm_mod.perspective(np.pi / 3, w / h, 0.2, 500.0)
+    eye = np.array([0, 2.0, 14.0], np.float32)
+    r = skm_mod.Renderer(frames_in_flight=1)
+    try:
+        r.upload_lod_world(tree, budget=budget)
+        r.set_extent(w, h)
+        r.apply_policy(sort_depth=16, lod_error=1.0)
+        policy = 1.0
+        steps, seen = [], []
+        for k in range(14):
+            # Eight frames at the scene, then six looking away from it (most clusters culled).
+            view = skm_mod.look_at(eye) if k < 8 else skm_mod.look_at(eye, (0.0, 2.0, 100.0))
+            before = r.lod_threshold()
+            r.render(view, proj, eye)
+            cut, stats = r.read_lod_cut()
+            cam = oracle.make_camera(view, proj, w, h, eye)
+            o_cut, o_stats = oracle.lod_select(cam, tree.clusters, tree.leaves, tree.rounds, budget, before, 4.0, True)
+            assert np.array_equal(cut, o_cut), (k, len(cut), len(o_cut))
+            assert stats[4] == o_stats[4], (k, stats, o_stats)
+            steps.append((int(stats[0]), int(stats[4]), budget))
+            want = oracle.lod_threshold(policy, steps)[-1]
+            after = r.lod_threshold()
+            assert np.float32(after) == want, (k, after, want)
+            seen.append((before, int(stats[0]), int(stats[4]), after))
+        say(f"threshold trace (before, selected, denied, after): {seen}")
+        assert all(s[2] > 0 for s in seen[:8]) and seen[7][3] > 1.8  # raised by denials
+        assert all(s[2] == 0 and s[1] < budget // 10 * 9 for s in seen[8:])
+        assert all(seen[k][3] < seen[k][0] for k in range(8, 14))  # relaxing
+    finally:
+        r.close()
+        tree.close()
 
 
 def test_lod_frames_in_flight_match_one_at_a_time(skm_mod):

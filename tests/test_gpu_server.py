@@ -149,3 +149,95 @@ def test_launches_stay_within_the_pixel_cap(skm_mod, scenes):
             srv.release(t)
     finally:
         srv.close()
+
+
+def served_vs_oracle(img, o, label):
+    """A served BGRA8 image against the oracle's FP64 composite: each channel within 1/255 of FP64
+    (the bar of the single draw) plus the half step of the 8-bit rounding."""
+    want = np.clip(o["image"][..., [2, 1, 0]], 0.0, 1.0) * 255.0
+    err = np.abs(img[..., :3].astype(np.float64) - want).max()
+    say(f"{label}: visible {o['count']} pairs {o['pairs']} max |served - 255 fp64| = {err:.3f}")
+    assert err <= 1.5, label
+    assert np.all(img[..., 3] == 255), label
+
+
+def test_served_requests_match_oracle_across_sh_and_resolutions(skm_mod, oracle):
+    """Four resident scenes at SH0, SH1, SH2 and SH3, one request per scene and resolution (720p,
+    1080p, 2160p), all queued together so they are batched: every served image matches the oracle's
+    frame of that scene, camera and resolution (16-bit keys, the server's default)."""
+    say("case: 4 scenes SH0-3 x 720p/1080p/2160p against the oracle")
+    clouds = [skm_mod.make_cloud(n, deg, seed=400 + deg, layout=1, spread=3.0, area=6.0)
+              for deg, n in ((0, 120_000), (1, 150_000), (2, 180_000), (3, 200_000))]
+    resolutions = ((1280, 720), (1920, 1080), (3840, 2160))
+    srv = skm_mod.Server(max_batch_views=8)
+    try:
+        for c in clouds:
+            srv.add_scene(c)
+        reqs = []
+        for s in range(4):
+            views = skm_mod.views_c2(len(resolutions), seed=500 + s, radius=(3.0, 9.0), resolutions=resolutions)
+            for k, (view, proj, eye, _, _) in enumerate(views):
+                w, h = resolutions[k]
+                proj = skm_mod.perspective(np.pi / 3, w / h, 0.2, 100.0)
+                reqs.append((s, view, proj, eye, w, h))
+        tickets = [srv.submit(*q) for q in reqs]
+        done = srv.wait_all(tickets)
+        assert all(c["status"] == 0 for c in done.values())
+        say(f"launch sizes {sorted(c['batch_views'] for c in done.values())}")
+        for t, (s, view, proj, eye, w, h) in zip(tickets, reqs):
+            o = oracle.frame(clouds[s], view, proj, eye, w, h, key16=True)
+            served_vs_oracle(srv.image(t), o, f"scene SH{s} {w}x{h}")
+            srv.release(t)
+    finally:
+        srv.close()
+
+
+@pytest.mark.timeout(600)
+def test_c4_scale_alone_equals_in_the_mix(skm_mod, oracle):
+    """The serving workload's scale: eight resident scenes of 0.5, 1, 2, 3, 5, 6, 8 and 10M splats at
+    SH0-3 cyclic. One request per scene alone, then the same requests inside a Poisson-free burst of
+    48 mixed requests (720p to 2160p) with batching and several launches in flight: each result is
+    bit-identical alone and in the mix, and the two smallest scenes' match the oracle."""
+    say("case: 8 scenes of 0.5M-10M splats, probes alone and in a 48-request mix")
+    sizes = (500_000, 1_000_000, 2_000_000, 3_000_000, 5_000_000, 6_000_000, 8_000_000, 10_000_000)
+    srv = skm_mod.Server(max_batch_views=4, max_in_flight=3)
+    clouds = []
+    try:
+        for k, n in enumerate(sizes):
+            c = skm_mod.make_cloud(n, k % 4, seed=0x5EED0004 + k, layout=1, spread=5.0, area=10.0)
+            srv.add_scene(c)
+            clouds.append(c if k < 2 else None)  # the large scenes stay on the device only
+            say(f"scene {k}: {n} splats SH{k % 4} resident")
+        resolutions = ((1280, 720), (1920, 1080), (2560, 1440), (3840, 2160))
+        probes = []
+        for k in range(8):
+            view, proj, eye, w, h = skm_mod.views_c2(1, seed=600 + k, resolutions=(resolutions[k % 4],))[0]
+            probes.append((k, view, proj, eye, w, h))
+        alone = {}
+        for q in probes:
+            t = srv.submit(*q)
+            c = srv.wait_all([t], timeout_s=300)[t]
+            assert c["status"] == 0 and c["batch_views"] == 1
+            alone[q[0]] = srv.image(t)
+            srv.release(t)
+        rng = np.random.default_rng(601)
+        mix = [(int(rng.integers(8)),) + v for v in skm_mod.views_c2(40, seed=602, resolutions=resolutions)]
+        tagged = ([(None, q) for q in mix[:10]] + [(q[0], q) for q in probes[:4]] + [(None, q) for q in mix[10:30]] +
+                  [(q[0], q) for q in probes[4:]] + [(None, q) for q in mix[30:]])
+        tickets = [srv.submit(*q) for _, q in tagged]
+        done = srv.wait_all(tickets, timeout_s=300)
+        assert all(c["status"] == 0 for c in done.values())
+        st = srv.stats()
+        say(f"mix: launches {st['launches']} mean views per launch {st['mean_batch_views']:.2f}")
+        assert st["mean_batch_views"] > 1.0
+        for t, (probe, _) in zip(tickets, tagged):
+            if probe is not None:
+                assert np.array_equal(srv.image(t), alone[probe]), probe
+        for k in (0, 1):
+            _, view, proj, eye, w, h = probes[k]
+            served_vs_oracle(alone[k], oracle.frame(clouds[k], view, proj, eye, w, h, key16=True),
+                             f"scene {k} ({sizes[k]} splats) {w}x{h}")
+        for t in tickets:
+            srv.release(t)
+    finally:
+        srv.close()
