@@ -250,6 +250,15 @@ skm_status skm_argsort_u32(skm_renderer* r, const uint32_t* keys, uint32_t n, in
 skm_status skm_argsort_u32_device(skm_renderer* r, void* keys_dev, uint32_t n, int key_bits,
                                   void* order_dev, double* ms);
 
+/* The same sort with no allocation of any kind, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in the
+ * reference. Its scratch is one static device array of the library (8.7 MB per GPU, held until the
+ * process ends), so n may be at most 1048576 (SKM_ERR_INVALID beyond); the caller passes the keys
+ * (overwritten, as above) and the order. Enqueued on the renderer's stream and waited for. Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU, from any renderer, race. */
+skm_status skm_argsort_u32_device_static(skm_renderer* r, void* keys_dev, uint32_t n, int key_bits,
+                                         void* order_dev);
+
 /* ---- the loader's spatial order (reorderSpatially, packages/splat-core/src/sorting/SpatialOrder.cpp:48-92) ---- */
 /* mortonCode (SpatialOrder.cpp:39-44): 30-bit Morton code of a point quantised to 1024 cells per
  * axis of the box [lo, hi] (an axis where the box is flat quantises to 0). */

@@ -567,6 +567,9 @@ Batch* renderer_batch(skm_renderer* r) {
 }
 hipStream_t renderer_stream(skm_renderer* r) { return r->stream; }
 uint64_t renderer_world_bytes(const skm_renderer* r) { return r->world.mem.bytes; }
+int renderer_device(const skm_renderer* r) { return r->device; }
+skm_status renderer_fail(skm_renderer* r, skm_status s, const char* message) { return r->fail(s, "%s", message); }
+void static_scratch_setup(int device);  // noalloc.hip
 }  // namespace skm
 
 extern "C" {
@@ -600,6 +603,8 @@ skm_status skm_renderer_create(const skm_config* config, skm_renderer** out) {
   std::snprintf(desc, sizeof(desc), "%s (%s, %d CUs), HIP %d.%d", prop.name[0] ? prop.name : "AMD GPU", prop.gcnArchName,
                 prop.multiProcessorCount, runtime / 10000000, (runtime / 100000) % 100);
   r->description = desc;
+  // The static sort's scratch on this device (its address, resolved once per device).
+  skm::static_scratch_setup(device);
   bool ok = hipEventCreate(&r->sort_marks[0]) == hipSuccess && hipEventCreate(&r->sort_marks[1]) == hipSuccess;
   for (Slot& s : r->active()) {
     ok = ok && hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) == hipSuccess &&

@@ -225,6 +225,7 @@ API = {
     "skm_reorder_spatially": (C.c_int, [_R, _fp, _fp, _fp, _fp, _fp, C.c_uint32, C.c_uint32, _fp, _fp, _u32p]),
     "skm_argsort_u32_device": (C.c_int, [_R, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p,
                                          C.POINTER(C.c_double)]),
+    "skm_argsort_u32_device_static": (C.c_int, [_R, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]),
 }
 
 _S = C.c_void_p
@@ -780,6 +781,12 @@ class Renderer:
         self._check(self._lib.skm_argsort_u32_device(self._h, C.c_void_p(keys_ptr), n, key_bits,
                                                      C.c_void_p(order_ptr), C.byref(ms)))
         return ms.value
+
+    def argsort_device_static(self, keys_ptr, n, key_bits, order_ptr):
+        """skm_argsort_u32_device_static: the same sort with its scratch in the library's static device
+        array (no allocation); returns the status rather than raising, for the tests of its limits."""
+        return self._lib.skm_argsort_u32_device_static(self._h, C.c_void_p(keys_ptr), n, key_bits,
+                                                       C.c_void_p(order_ptr))
 
     def stats(self):
         L, h = self._lib, self._h
