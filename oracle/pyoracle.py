@@ -78,6 +78,15 @@ _REF_API = {
     "ref_covariance": (None, [_fp, _fp, _fp]),
     "ref_perspective": (None, [C.c_float, C.c_float, C.c_float, C.c_float, _fp]),
     "ref_lod_threshold": (None, [C.c_float, C.c_uint32, _u32p, _u32p, _u32p, _fp]),
+    "ref_lod_build": (C.c_void_p, [C.c_uint32, C.c_int, _fp, _fp, _fp, _fp, _fp, C.c_uint32, C.c_uint32]),
+    "ref_lod_free": (None, [C.c_void_p]),
+    "ref_lod_counts": (None, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "ref_lod_read": (None, [C.c_void_p, _fp, _fp, _fp, _fp, _fp, _u32p, _u32p, _u32p]),
+    "ref_lod_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_size_t]),
+    "ref_lod_decode": (C.c_void_p, [C.c_void_p, C.c_size_t, C.c_int, C.c_char_p, C.c_size_t]),
+    "ref_lod_header": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_uint64), C.c_char_p, C.c_size_t]),
+    "ref_lod_validate": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_char_p, C.c_size_t]),
+    "ref_lod_bounds": (None, [C.c_void_p, _fp]),
 }
 
 _lib = None

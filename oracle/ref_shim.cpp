@@ -7,6 +7,10 @@
 #include <cstring>
 #include <vector>
 
+#include <cstdio>
+#include <string>
+
+#include "splat/lod/LodFile.h"
 #include "splat/lod/LodTree.h"
 #include "splat/math/Covariance.h"
 #include "splat/math/Half.h"
@@ -147,6 +151,55 @@ void ref_lod_read(const void* t, float* positions, float* covariances, float* co
     std::memcpy(clusters, tree->selection.clusters.data(),
                 tree->selection.clusters.size() * sizeof(splat::LodCluster));
   if (leaves) std::memcpy(leaves, tree->selection.leaves.data(), tree->selection.leaves.size() * 4);
+}
+
+// writeLodSplat (LodFile.cpp:430-460): 1 when written; the reason otherwise.
+int ref_lod_write(const void* t, const char* path, char* error, size_t error_size) {
+  const auto r = splat::writeLodSplat(*static_cast<const splat::LodTree*>(t), path);
+  if (!r) std::snprintf(error, error_size, "%s", r.error().message.c_str());
+  return r ? 1 : 0;
+}
+// decodeLodSplat (LodFile.cpp:382-427): a tree to read back with the accessors above, or null and
+// the reason.
+void* ref_lod_decode(const uint8_t* data, size_t size, int max_sh_degree, char* error, size_t error_size) {
+  auto r = splat::decodeLodSplat(data, size, max_sh_degree);
+  if (!r) {
+    std::snprintf(error, error_size, "%s", r.error().message.c_str());
+    return nullptr;
+  }
+  return new splat::LodTree(std::move(r.value()));
+}
+// readLodSplatHeader (LodFile.cpp:355-380): 1 and out[0..6] = version, nodes, leaves, SH degree,
+// depth, clusters, leaf references; 0 and the reason otherwise.
+int ref_lod_header(const uint8_t* data, size_t size, uint64_t* out, char* error, size_t error_size) {
+  const auto r = splat::readLodSplatHeader(data, size);
+  if (!r) {
+    std::snprintf(error, error_size, "%s", r.error().message.c_str());
+    return 0;
+  }
+  const auto& h = r.value();
+  const uint64_t v[7] = {h.version, h.nodeCount, h.leafCount, static_cast<uint64_t>(h.shDegree), h.depth,
+                         h.clusterCount, h.leafRefCount};
+  std::memcpy(out, v, sizeof v);
+  return 1;
+}
+// validateLodTree (LodFile.cpp:336-344): 1 and the depth, or 0 and the reason.
+int ref_lod_validate(const void* t, uint32_t* depth, char* error, size_t error_size) {
+  const auto r = splat::validateLodTree(*static_cast<const splat::LodTree*>(t));
+  if (!r) {
+    std::snprintf(error, error_size, "%s", r.error().message.c_str());
+    return 0;
+  }
+  *depth = r.value();
+  return 1;
+}
+// The tree's bounds: min xyz, max xyz.
+void ref_lod_bounds(const void* t, float* out6) {
+  const auto& b = static_cast<const splat::LodTree*>(t)->nodes.bounds;
+  for (int j = 0; j < 3; ++j) {
+    out6[j] = b.min[j];
+    out6[3 + j] = b.max[j];
+  }
 }
 
 }  // extern "C"

@@ -14,16 +14,7 @@
 #include <vector>
 
 #include "skm_lod.h"
-
-struct skm_lod_tree {
-  std::vector<float> positions, covariances, colors, alphas, sh;
-  int sh_degree = 0;
-  size_t sh_stride = 0;
-  std::vector<skm_lod_node> layout;
-  uint64_t leaf_count = 0;
-  std::vector<skm_lod_cluster> clusters;
-  std::vector<uint32_t> leaves;
-};
+#include "skm_lod_host.h"
 
 namespace {
 
@@ -44,9 +35,12 @@ void parallel_for(size_t n, uint32_t threads, F&& body) {
   for (auto& th : pool) th.join();
 }
 
+}  // namespace
+
+namespace skm {
 // Eigenvalues of a symmetric 3x3 matrix (upper triangle xx xy xz yy yz zz), descending, by the
 // trigonometric closed form (Smith 1961), in double (SymmetricEigen.cpp:8-40).
-std::array<float, 3> eigenvalues(const float* u) {
+std::array<float, 3> symmetric_eigenvalues(const float* u) {
   const double xx = u[0], xy = u[1], xz = u[2], yy = u[3], yz = u[4], zz = u[5];
   const double p1 = xy * xy + xz * xz + yz * yz;
   std::array<double, 3> e;
@@ -69,9 +63,13 @@ std::array<float, 3> eigenvalues(const float* u) {
   std::sort(e.begin(), e.end(), [](double a, double b) { return a > b; });
   return {static_cast<float>(e[0]), static_cast<float>(e[1]), static_cast<float>(e[2])};
 }
+}  // namespace skm
+
+namespace {
+using skm::symmetric_eigenvalues;
 
 std::array<float, 3> semi_axes(const float* cov) {
-  const auto e = eigenvalues(cov);
+  const auto e = symmetric_eigenvalues(cov);
   return {std::sqrt(std::max(e[0], 0.0f)), std::sqrt(std::max(e[1], 0.0f)), std::sqrt(std::max(e[2], 0.0f))};
 }
 
@@ -373,6 +371,10 @@ skm_lod_tree* skm_lod_build(const skm_cloud_view* cloud, uint32_t octree_depth, 
       lo[c] = std::min(lo[c], arena.pos[i * 3 + c]);
       hi[c] = std::max(hi[c], arena.pos[i * 3 + c]);
     }
+  for (int c = 0; c < 3; ++c) {
+    tree->bounds_min[c] = lo[c];
+    tree->bounds_max[c] = hi[c];
+  }
   const uint32_t depth = std::clamp(octree_depth, 1u, kMaxOctreeDepth);
   float extent = 1e-6f;
   for (int c = 0; c < 3; ++c) extent = std::max(extent, hi[c] - lo[c]);
@@ -454,6 +456,7 @@ skm_lod_tree* skm_lod_from_layout(const skm_cloud_view* nodes, const skm_lod_nod
       return nullptr;
   auto tree = std::make_unique<skm_lod_tree>();
   copy_cloud(*tree, nodes);
+  tree->set_bounds_from_positions();
   tree->layout.assign(layout, layout + nodes->count);
   tree->leaf_count = leaf_count;
   build_selection(*tree, 1);

@@ -152,6 +152,13 @@ class LodCluster(C.Structure):
                 ("leaf_count", C.c_uint32), ("subtree_leaves", C.c_uint32)]
 
 
+class LodFileHeader(C.Structure):
+    _fields_ = [("version", C.c_uint32), ("node_count", C.c_uint32), ("leaf_count", C.c_uint32),
+                ("sh_degree", C.c_int32), ("depth", C.c_uint32), ("cluster_count", C.c_uint32),
+                ("leaf_ref_count", C.c_uint32), ("reserved", C.c_uint32), ("bounds_min", C.c_float * 3),
+                ("bounds_max", C.c_float * 3)]
+
+
 class LodTreeView(C.Structure):
     _fields_ = [("nodes", CloudView), ("layout", C.POINTER(LodNode)), ("leaf_count", C.c_uint64),
                 ("clusters", C.POINTER(LodCluster)), ("cluster_count", C.c_uint64),
@@ -249,6 +256,12 @@ LOD_API = {
     "skm_lod_free": (None, [C.c_void_p]),
     "skm_lod_view": (None, [C.c_void_p, C.POINTER(LodTreeView)]),
     "skm_lod_rounds": (C.c_uint32, [C.POINTER(LodTreeView)]),
+    "skm_lod_bounds": (None, [C.c_void_p, _fp]),
+    "skm_lod_read_header": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(LodFileHeader), C.c_char_p, C.c_size_t]),
+    "skm_lod_decode": (C.c_void_p, [C.c_void_p, C.c_size_t, C.c_int, C.c_char_p, C.c_size_t]),
+    "skm_lod_read": (C.c_void_p, [C.c_char_p, C.c_int, C.c_char_p, C.c_size_t]),
+    "skm_lod_validate": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_char_p, C.c_size_t]),
+    "skm_lod_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_size_t]),
 }
 
 _lib = None
@@ -294,6 +307,21 @@ def uptr(a):
 
 # ---------------------------------------------------------------------------------------------
 # Scenes and cameras.
+
+def read_lod_header(data):
+    """readLodSplatHeader: (header dict, None) or (None, reason); `data` is the file's first bytes and
+    its length the file's (pass a bytes-like of the file, or (prefix, length))."""
+    if isinstance(data, tuple):
+        prefix, size = data
+    else:
+        prefix, size = data, len(data)
+    buf = np.frombuffer(bytes(prefix), np.uint8)
+    h = LodFileHeader()
+    err = C.create_string_buffer(256)
+    if lib().skm_lod_read_header(buf.ctypes.data_as(C.c_void_p), size, C.byref(h), err, 256) != 0:
+        return None, err.value.decode()
+    return {k: (list(getattr(h, k)) if k.startswith("bounds") else getattr(h, k)) for k, _ in LodFileHeader._fields_}, None
+
 
 def sh_floats(degree):
     return 3 * ((degree + 1) ** 2 - 1)
@@ -917,6 +945,44 @@ class LodTree:
         h = lib().skm_lod_from_layout(C.byref(view), lay.ctypes.data_as(C.POINTER(LodNode)), leaf_count)
         del keep
         return cls(h)
+
+    @classmethod
+    def read(cls, path, max_sh_degree=3):
+        """A tree from a LODSPLAT file (skm_lod_read); raises ValueError with the decoder's reason."""
+        err = C.create_string_buffer(256)
+        h = lib().skm_lod_read(os.fsencode(path), max_sh_degree, err, 256)
+        if not h:
+            raise ValueError(err.value.decode())
+        return cls(h)
+
+    @classmethod
+    def decode(cls, data, max_sh_degree=3):
+        """A tree from LODSPLAT bytes (skm_lod_decode)."""
+        buf = np.frombuffer(data, np.uint8)
+        err = C.create_string_buffer(256)
+        h = lib().skm_lod_decode(buf.ctypes.data_as(C.c_void_p), buf.nbytes, max_sh_degree, err, 256)
+        if not h:
+            raise ValueError(err.value.decode())
+        return cls(h)
+
+    def write(self, path):
+        err = C.create_string_buffer(256)
+        rc = lib().skm_lod_write(self._h, os.fsencode(path), err, 256)
+        if rc != 0:
+            raise SkmError(rc, err.value.decode())
+
+    def validate(self):
+        """validateLodTree: the refinement rounds, or ValueError with the first failure."""
+        err = C.create_string_buffer(256)
+        depth = C.c_uint32()
+        if lib().skm_lod_validate(self._h, C.byref(depth), err, 256) != 0:
+            raise ValueError(err.value.decode())
+        return depth.value
+
+    def bounds(self):
+        b = np.zeros(6, np.float32)
+        lib().skm_lod_bounds(self._h, fptr(b))
+        return b[:3].copy(), b[3:].copy()
 
     def close(self):
         if self._h:

@@ -39,7 +39,7 @@ def _run(cmd):
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     jobs = []
-    for src in ("kernels.hip", "renderer.cpp", "batch.cpp", "lod_build.cpp", "lod.hip", "server.cpp", "noalloc.hip"):
+    for src in ("kernels.hip", "renderer.cpp", "batch.cpp", "lod_build.cpp", "lod.hip", "server.cpp", "noalloc.hip", "lod_file.cpp"):
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue

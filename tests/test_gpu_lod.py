@@ -168,3 +168,32 @@ def test_lod_frames_in_flight_match_one_at_a_time(skm_mod):
     assert np.array_equal(results[0][3], results[1][3])
     assert np.array_equal(results[0][4], results[1][4])
     assert results[0][0] == 6
+
+
+def test_lod_world_from_a_lodsplat_file_matches_oracle(renderer, skm_mod, oracle, tmp_path):
+    """The C3 input path from a file: a built tree written as LODSPLAT (version 2), read back by the
+    decoder (decodeLodSplat's checks, node cap lifted), uploaded (uploadLodWorld) and drawn: the cut and
+    the frame match the oracle's on the decoded tree."""
+    say("case: LOD world read from a LODSPLAT file")
+    cloud = skm_mod.make_cloud(120_000, 1, seed=69, layout=1, spread=10.0, area=30.0, log_scale_mean=-3.5)
+    built = skm_mod.LodTree.build(cloud, 21, 256, threads=16)
+    path = tmp_path / "world.lodsplat"
+    built.write(str(path))
+    tree = skm_mod.LodTree.read(str(path))
+    assert tree.node_count == built.node_count and np.array_equal(tree.clusters, built.clusters)
+    renderer.upload_lod_world(tree, budget=50_000)
+    eye = np.array([3.0, 2.5, 16.0], np.float32)
+    view = skm_mod.look_at(eye)
+    proj = skm_mod.perspective(np.pi / 3, 960 / 540, 0.2, 500.0)
+    renderer.set_extent(960, 540)
+    renderer.apply_policy(sort_depth=16, lod_error=1.5)
+    renderer.render(view, proj, eye)
+    cut, _ = renderer.read_lod_cut()
+    cam = oracle.make_camera(view, proj, 960, 540, eye)
+    o_cut, _ = oracle.lod_select(cam, tree.clusters, tree.leaves, tree.rounds, 50_000, 1.5, 4.0, True)
+    assert np.array_equal(cut, o_cut)
+    g = gpu_frame(renderer, skm_mod, None, view, proj, eye, 960, 540, upload=False, lod_error=1.5, key16=True)
+    o = oracle.frame(tree.nodes, view, proj, eye, 960, 540, active=o_cut, key16=True)
+    assert_parity(g, o, label="LOD world from file")
+    tree.close()
+    built.close()
