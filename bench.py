@@ -23,7 +23,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "splatkit-ios_amd"))
 
 HBM_PEAK = 8.0e12          # bytes/s, MI355X HBM3E
-VALU_PEAK_FP32 = 157.3e12  # flop/s, fp32 vector (packed)
+# VALU issue: 256 CUs x 4 SIMDs x 16 lanes per clock at 2.4 GHz, one lane operation per lane per
+# instruction (a wave64 instruction takes 4 clocks on a 16-lane SIMD). The packed-fp32 figure
+# (157.3 TFLOP/s) counts 2 per FMA and 2 per packed pair; this build disables packed fp32.
+VALU_LANE_OPS_PEAK = 256 * 4 * 16 * 2.4e9
 WIDTH, HEIGHT = 1920, 1080
 N_SPLATS = 3_000_000
 SH_DEGREE = 3
@@ -31,9 +34,20 @@ SH_DEGREE = 3
 # frames overlap on the GPU); the reference keeps two.
 FRAMES_IN_FLIGHT = 3
 SH_BYTES = {0: 0, 1: 20, 2: 48, 3: 92}
-# fp32 operations of the per-(pixel, splat) coverage test in the raster's inner loop: offset (2),
-# quad coordinates (4 mul + 2 add), Gaussian exponent (2 + 2 mul, 1 add, 1 sub).
-RASTER_FLOPS_PER_EVAL = 14
+ORBIT_FRAMES = 60
+
+
+def measured_valu(kernel, steps):
+    """Wave64 VALU instructions per launch of `kernel` over the orbit frames a timed region of `steps`
+    draws (frame i mod 60), from the committed SQ_INSTS_VALU counter run of this workload
+    (profiles/sq_counters_3M_sh3_1080p.json, one value per orbit frame), or None when absent."""
+    path = os.path.join(ROOT, "profiles", "sq_counters_3M_sh3_1080p.json")
+    if not os.path.exists(path):
+        return None
+    per_frame = json.load(open(path)).get("kernels", {}).get(kernel, {}).get("valu_by_orbit_frame")
+    if not per_frame or len(per_frame) != ORBIT_FRAMES:
+        return None
+    return sum(per_frame[i % ORBIT_FRAMES] for i in range(steps)) / steps
 
 
 def measured_traffic(kernel):
@@ -47,6 +61,16 @@ def measured_traffic(kernel):
         if name == kernel:
             return v["fetch_bytes"] + v["write_bytes"]
     return None
+
+
+def measured_frame_valu():
+    """Wave64 VALU instructions of one frame, summed over its kernels (launches per frame x instructions
+    per launch, from the committed counter run), or None when absent."""
+    path = os.path.join(ROOT, "profiles", "sq_counters_3M_sh3_1080p.json")
+    if not os.path.exists(path):
+        return None
+    kernels = json.load(open(path)).get("kernels", {})
+    return sum(e["launches_per_frame"] * e["per_launch_mean"].get("SQ_INSTS_VALU", 0.0) for e in kernels.values()) or None
 
 
 def stage_bytes(N, V, P, T, depth_passes, tile_passes):
@@ -133,10 +157,19 @@ def batched_vs_loop(r, skm, batches, loops, single_tuning):
     st = r.batch_stats()
     frames = [(r.make_frame(v, p, e), w, h) for v, p, e, w, h in views]
     r.set_tuning(**single_tuning)
-    for f, w, h in frames[:4]:
-        r.set_extent(w, h)
-        r.draw(f)
-    r.finish()
+    # One untimed pass over every view first: a view whose tile pairs outgrow the capacity is dropped
+    # (skm_finish reports it) and the capacity grows, so the timed passes draw every view whole.
+    for attempt in range(3):
+        for f, w, h in frames:
+            r.set_extent(w, h)
+            r.draw(f)
+        try:
+            r.finish()
+            break
+        except skm.SkmError as e:
+            if e.status != skm.SKM_ERR_CAPACITY or attempt == 2:
+                raise
+    r.frame_totals(reset=True)
     t0 = time.perf_counter()
     for _ in range(loops):
         for f, w, h in frames:
@@ -144,6 +177,9 @@ def batched_vs_loop(r, skm, batches, loops, single_tuning):
             r.draw(f)
     r.finish()
     loop_s = (time.perf_counter() - t0) / loops
+    drawn = r.frame_totals(reset=True)
+    if drawn["dropped"] != 0 or drawn["frames"] != loops * len(frames):
+        raise SystemExit(f"per-view loop: {drawn['dropped']} views dropped, {drawn['frames']} drawn")
     return {"workload": "64 views of the 3M SH3 scene, resolutions uniform over 720p/1080p/1440p, eyes on "
                         "spheres of radius U(4,12) m", "views": 64,
             "batched_frames_per_s": round(64 / batch_s, 1), "batched_ms_per_batch": round(batch_s * 1000, 3),
@@ -328,6 +364,27 @@ def cpu_baseline(frames, threads):
     return frames / total, (stages / frames * 1000).round(1).tolist()
 
 
+def pixel_error_fp64(images):
+    """max |GPU - FP64| per channel of the two frames drawn on the GPU above, against the oracle's FP64
+    composite of the same world, camera and order (the checker; in the CPU leg)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+    import skm
+
+    cloud = skm.scene_c1(N_SPLATS, SH_DEGREE)
+    order = pyoracle.spatial_order(cloud["positions"], cloud["positions"].min(0), cloud["positions"].max(0))
+    cloud = {k: (v[order] if isinstance(v, np.ndarray) else v) for k, v in cloud.items()}
+    o1 = pyoracle.frame(cloud, *skm.orbit_camera(0, WIDTH, HEIGHT), WIDTH, HEIGHT)
+    del cloud
+    o0 = pyoracle.frame(skm.scene_c0(), *skm.camera_c0(), 1280, 720)
+    return {"C0_100k_sh0_720p": float(np.abs(images["C0"] - o0["image"]).max()),
+            "C1_3M_sh3_1080p_frame0": float(np.abs(images["C1"] - o1["image"]).max()),
+            "bar": 1.0 / 255.0,
+            "note": "max over pixels and RGB of |fp32 GPU composite - FP64 oracle composite| on the same "
+                    "survivors, order and tile lists"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -426,30 +483,81 @@ def main():
         stages[n] = {"ms": round(ms[n], 4), "bytes": int(by[n]), "GB_s": round(gbs, 1),
                      "hbm_frac": round(gbs * 1e9 / HBM_PEAK, 4)}
     tile = plain_best["tile_size"]
-    evals = float(tile * tile) * P
-    raster_tflops = evals * RASTER_FLOPS_PER_EVAL / (ms["raster"] * 1e-3) / 1e12 if ms["raster"] > 0 else 0.0
-    stages["raster"]["pair_evals"] = int(evals)
-    stages["raster"]["coverage_TFLOP_s"] = round(raster_tflops, 2)
-    # The roofline's kernel is the raster: the longest single kernel of the frame in the kernel trace
-    # (profiles/k_raster_3M_sh3_1080p_kernel_stats.md). Stage spans are not kernels: the binning
-    # stage holds eleven launches, and with frames overlapping each span also holds other frames' work.
-    traffic = measured_traffic(f"k_raster<{tile}, {plain_best['raster_px']}>")
+    # Raster work: the pair evaluations of the tile lists (E = tile^2 x P, an upper bound), and the
+    # (pixel, record) tests the kernel performed (counted on the GPU: records each wave walked before
+    # its pixels saturated, times its pixels).
+    evals_upper = float(tile * tile) * P
+    tests = tot["raster_tests"] / n_frames
     raster_alone_ms = alone["stage_ms"][4] / max(1, alone["frames"])
-    alone_tflops = evals * RASTER_FLOPS_PER_EVAL / (raster_alone_ms * 1e-3) / 1e12 if raster_alone_ms > 0 else 0.0
-    roofline = {"bound": "valu", "kernel": f"k_raster<{tile}, {plain_best['raster_px']}>",
-                "achieved": round(raster_tflops, 3),
-                "peak": VALU_PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(raster_tflops * 1e12 / VALU_PEAK_FP32, 4),
-                "traffic": traffic,
-                "alone": {"ms": round(raster_alone_ms, 4), "achieved": round(alone_tflops, 3),
-                          "frac": round(alone_tflops * 1e12 / VALU_PEAK_FP32, 4), "frames": alone["frames"]},
-                "note": "fp32 VALU work of the coverage test over tile^2 x pairs evaluations, over the raster's "
-                        f"average duration in the timed region, where {args.frames_in_flight} frames overlap and share "
-                        "the GPU; 'alone' times the same frames drawn one at a time. traffic = HBM bytes per launch "
-                        "from the FETCH_SIZE/WRITE_SIZE counter runs in profiles/hbm_traffic_3M_sh3_1080p.md"}
+    stages["raster"]["pair_evals_upper"] = int(evals_upper)
+    stages["raster"]["coverage_tests"] = int(tests)
+    stages["raster"]["Gtests_per_s"] = round(tests / (ms["raster"] * 1e-3) / 1e9, 1) if ms["raster"] > 0 else 0.0
+    # The roofline's kernel is the raster, the longest single kernel of the frame in the kernel trace
+    # (profiles/k_raster_3M_sh3_1080p_kernel_stats.md); it is bound by VALU issue. Its VALU instructions
+    # per launch for these orbit frames come from the counter run; its duration is measured here (HIP
+    # events on the frame slot's stream, averaged over the timed frames, where frames overlap and share
+    # the GPU; 'alone' over the same frames drawn one at a time). frac = SQ_INSTS_VALU x 4 clocks /
+    # (1024 SIMDs x 2.4 GHz x duration): the share of the chip's VALU issue slots the raster fills.
+    kernel = f"k_raster<{tile}, {plain_best['raster_px']}>"
+    valu = measured_valu(kernel, args.steps)
+    traffic = measured_traffic(kernel)
 
+    def valu_rate(t_ms):
+        return valu * 64 / (t_ms * 1e-3) if valu is not None and t_ms > 0 else None
+
+    rate, rate_alone = valu_rate(ms["raster"]), valu_rate(raster_alone_ms)
+    roofline = {"bound": "valu", "kernel": kernel,
+                "achieved": round(rate / 1e12, 3) if rate else None,
+                "peak": round(VALU_LANE_OPS_PEAK / 1e12, 2), "unit": "T VALU lane-op/s",
+                "frac": round(rate / VALU_LANE_OPS_PEAK, 4) if rate else None,
+                "traffic": traffic,
+                "valu_instructions_per_launch": round(valu) if valu is not None else None,
+                "launch_ms": round(ms["raster"], 4),
+                "alone": {"launch_ms": round(raster_alone_ms, 4), "frames": alone["frames"],
+                          "frac": round(rate_alone / VALU_LANE_OPS_PEAK, 4) if rate_alone else None},
+                "coverage_tests_per_launch": int(tests), "pair_evals_upper_per_launch": int(evals_upper),
+                "note": "VALU lane operations (SQ_INSTS_VALU x 64, from the counter run of these orbit frames in "
+                        "profiles/sq_counters_3M_sh3_1080p.json) over the raster's average duration measured in this "
+                        f"run with {args.frames_in_flight} frames in flight; peak = 1024 SIMDs x 16 lanes x 2.4 GHz "
+                        "(packed fp32 disabled). traffic = HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE "
+                        "counter runs in profiles/hbm_traffic_3M_sh3_1080p.md"}
+
+    # The sorts against HBM: the depth sort of the frames drawn alone (4 passes over V survivors,
+    # B = V (2 (k + 4) 4 + k)), and the standalone sort at the frame's pair count below.
+    depth_alone_ms = alone["stage_ms"][2] / max(1, alone["frames"])
+    sort_in_frame = {"depth_sort_alone_ms": round(depth_alone_ms, 4),
+                     "depth_sort_bytes": int(stage_bytes(N_SPLATS, V, P, T, 4, 0)["depth_sort"]),
+                     "hbm_frac": round(stage_bytes(N_SPLATS, V, P, T, 4, 0)["depth_sort"] / (depth_alone_ms * 1e-3)
+                                       / HBM_PEAK, 4) if depth_alone_ms > 0 else None}
+    # The frame as a whole against both rooflines: every kernel's VALU instructions per frame (the
+    # counter run) over the measured time per frame, and the stages' algorithmic bytes over it.
+    frame_valu = measured_frame_valu()
+    frame_s = elapsed / args.steps
+    frame_roofline = {"ms_per_frame": round(frame_s * 1e3, 4),
+                      "valu_instructions_per_frame": round(frame_valu) if frame_valu else None,
+                      "valu_issue_frac": round(frame_valu * 4 / (1024 * 2.4e9 * frame_s), 4) if frame_valu else None,
+                      "algorithmic_bytes_per_frame": int(sum(by.values())),
+                      "hbm_frac": round(sum(by.values()) / frame_s / HBM_PEAK, 4),
+                      "note": "whole-chip VALU issue slots filled by the frame's kernels (SQ_INSTS_VALU x 4 / (1024 SIMDs x "
+                              "2.4 GHz x time per frame)), and the stages' algorithmic bytes per frame over 8 TB/s"}
     sizes = [int(x) for x in args.sort_sizes.split(",") if x]
     runs = [(int(P), tile_bits, "tile sort at the frame's P")] + [(n, 32, "depth-key sort") for n in sizes]
     sort_leg = radix_sort(r, skm, runs) if sizes else None
+    # Images for the FP64 pixel check of the CPU leg: orbit frame 0 of this world, drawn alone, and the
+    # 100k SH0 scene at 1280x720.
+    check_images = None
+    if not args.no_cpu:
+        r.set_tuning(**plain_best)
+        r.render(*frames[0])
+        c0_cloud = skm.scene_c0()
+        c0 = skm.Renderer(0)
+        c0.upload_world(c0_cloud)
+        c0.set_extent(1280, 720)
+        c0.apply_policy(sort_depth=32)
+        c0.render(*skm.camera_c0())
+        check_images = {"C1": r.read_frame(), "C0": c0.read_frame()}
+        c0.close()
+        del c0_cloud
     batched = batched_vs_loop(r, skm, args.batches, 2, plain_best) if args.batches > 0 else None
     r.close()
     lod = (lod_flythrough(skm, args.lod_frames, args.lod_leaves, args.frames_in_flight) if args.lod_leaves > 0
@@ -459,7 +567,9 @@ def main():
         served = serving(skm, [float(x) for x in args.serving_rates.split(",")], args.serving_requests)
 
     cpu = None
+    pixel_err = None
     if not args.no_cpu:
+        pixel_err = pixel_error_fp64(check_images)
         cpu_fps, cpu_stages = cpu_baseline(args.cpu_frames, 1)
         threads = min(16, os.cpu_count() or 1)  # the host share of one GPU
         mt_fps, mt_stages = cpu_baseline(args.cpu_frames, threads)
@@ -479,6 +589,9 @@ def main():
         "data": "synthetic (seeded 3M-Gaussian SH3 scene: 70% near a ground square and boxes, 30% iid N(0,5^2)), "
                 "in the loader's Morton order (reorderSpatially, applied on the GPU at load)",
         "config": {"workload": "plain single-view render, 3M Gaussians SH3, 1920x1080, 60-frame orbit, 32-bit keys",
+                   "frames_timed": (f"orbit frames 0-{min(args.steps, ORBIT_FRAMES) - 1}" +
+                                    (f", then frame i mod 60 for steps {ORBIT_FRAMES}-{args.steps - 1}"
+                                     if args.steps > ORBIT_FRAMES else "")),
                    "splats": N_SPLATS, "sh_degree": SH_DEGREE, "width": WIDTH, "height": HEIGHT,
                    "tile": plain_best["tile_size"], "raster_px": plain_best["raster_px"],
                    "sort_kernel": ["reduce_then_scan", "onesweep_lookback"][plain_best["sort_kernel"]],
@@ -489,7 +602,9 @@ def main():
         "msplats_per_s": round(N_SPLATS * fps / 1e6, 1),
         "visible_per_frame": int(V), "pairs_per_frame": int(P),
         "gpu_ms_per_frame": round(tot["stage_ms"][5] / n_frames, 4),
-        "stages": stages, "roofline": roofline, "cpu_baseline": cpu, "radix_sort": sort_leg, "batched": batched,
+        "stages": stages, "roofline": roofline, "frame_roofline": frame_roofline, "pixel_err_fp64": pixel_err,
+        "depth_sort_in_frame": sort_in_frame,
+        "cpu_baseline": cpu, "radix_sort": sort_leg, "batched": batched,
         "serving": served, "lod": lod,
         "device": device, "scene_generation_s": round(gen_s, 2), "load_reorder_s": round(reorder_s, 3),
     }

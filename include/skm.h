@@ -228,14 +228,17 @@ skm_status skm_read_lod_cut(skm_renderer* r, uint32_t* count, uint32_t* indices,
                             uint32_t* stats6);
 
 /* Sums over the frames completed since the last reset: GPU milliseconds per stage (as above),
- * survivors and tile pairs of the frames drawn, and the frames and view batches dropped because
- * their tile pairs overflowed (not in the other sums). For throughput reports over many frames. */
+ * survivors, tile pairs and the raster's (pixel, record) coverage tests of the frames drawn (every
+ * record a wave walked before its pixels saturated, times the wave's pixels), and the frames and view
+ * batches dropped because their tile pairs overflowed (not in the other sums). For throughput
+ * reports over many frames. */
 typedef struct skm_frame_totals {
   uint64_t frames;
   double stage_ms[6];
   uint64_t visible;
   uint64_t pairs;
   uint64_t dropped;
+  uint64_t raster_tests;
 } skm_frame_totals;
 skm_status skm_read_frame_totals(skm_renderer* r, skm_frame_totals* out, int reset);
 

@@ -346,6 +346,7 @@ __global__ __launch_bounds__(256) void k_prologue(Ctrl* ctrl, uint32_t* hist, ui
     ctrl->overflow = 0;
     ctrl->sort_pairs = 0;
     ctrl->cull_lanes = 0;
+    for (int k = 0; k < 8; ++k) ctrl->raster_tests[k] = 0;
     for (int k = 0; k < 16; ++k) ctrl->ticket[k] = 0;
     // The frame's LOD cut, kept with the frame's counters: the selection buffers are free for the
     // next frame's cut once this frame's cull has read the indices.
@@ -1199,6 +1200,7 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
     w0 = src[0];
     w1 = src[1];
   };
+  uint32_t walked = 0;  // records this wave tested (wave-uniform)
   uint4 next0 = make_uint4(0u, 0u, 0u, 0u), next1 = next0;
   if (start + tid < end) record(pair_vals[start + tid], next0, next1);
   uint32_t next_idx = start + kB + tid < end ? pair_vals[start + kB + tid] : 0u;
@@ -1275,6 +1277,7 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
     if (lane < kGroup) s_list[wave][n + lane] = kB * static_cast<uint32_t>(sizeof(float4));  // the null record
     if (__ballot(live()) != 0) {
       for (uint32_t i = 0; i < n; i += kGroup) {
+        walked += kGroup;
         // A group of kGroup records: their geometry does not depend on the transmittance, so all of
         // them are read and tested together (one LDS latency per group); the blends then run in order.
         uint32_t addr[kGroup];
@@ -1311,6 +1314,16 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
       }
     }
     if (__syncthreads_count(live() ? 1 : 0) == 0) break;
+  }
+  // The block's coverage tests: one add per block, to one of 8 slots.
+  __shared__ uint32_t s_walked[kW];
+  if (lane == 0) s_walked[wave] = walked;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kW; ++w) sum += s_walked[w];
+    if (sum != 0u) atomicAdd(&a.ctrl->raster_tests[blockIdx.x & 7u], static_cast<unsigned long long>(sum) * (64u * kP));
   }
 #pragma unroll
   for (uint32_t q = 0; q < kP; ++q) {

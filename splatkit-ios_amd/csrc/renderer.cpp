@@ -364,6 +364,7 @@ struct skm_renderer {
       for (int k = 0; k < 6; ++k) totals.stage_ms[k] += ms[k];
       totals.visible += c->visible;
       totals.pairs += c->pairs;
+      for (unsigned long long t : c->raster_tests) totals.raster_tests += t;
     }
     ++completed;
     if (s.with_world) completed_world_frame = true;

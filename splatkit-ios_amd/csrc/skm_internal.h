@@ -65,6 +65,10 @@ struct Ctrl {
   uint32_t sort_pairs;  // pairs the tile sort orders: `pairs`, or 0 after an overflow
   uint32_t cull_lanes;  // cull tiles x kCullTile: the extent of the cull's gapped survivor layout
   uint32_t reserved[2];
+  // (pixel, record) coverage tests the raster performed: every record a wave walked, times its
+  // pixels (records past a wave's or tile's termination are not walked). Block b adds its count to
+  // slot b % kTestSlots, so the adds of concurrent blocks spread over several addresses.
+  unsigned long long raster_tests[8];
   uint32_t ticket[16];
   uint32_t lod[16];   // LOD selection counters
 };

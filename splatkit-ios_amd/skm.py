@@ -89,7 +89,7 @@ class LodThreshold(C.Structure):
 
 class FrameTotals(C.Structure):
     _fields_ = [("frames", C.c_uint64), ("stage_ms", C.c_double * 6), ("visible", C.c_uint64),
-                ("pairs", C.c_uint64), ("dropped", C.c_uint64)]
+                ("pairs", C.c_uint64), ("dropped", C.c_uint64), ("raster_tests", C.c_uint64)]
 
 
 class View(C.Structure):
@@ -734,7 +734,7 @@ class Renderer:
         t = FrameTotals()
         self._check(self._lib.skm_read_frame_totals(self._h, C.byref(t), 1 if reset else 0))
         return {"frames": t.frames, "stage_ms": list(t.stage_ms), "visible": t.visible, "pairs": t.pairs,
-                "dropped": t.dropped}
+                "dropped": t.dropped, "raster_tests": t.raster_tests}
 
     def read_frame(self, fmt=PIXELS_RGBA32F):
         w, h = self.extent()
