@@ -414,10 +414,13 @@ __device__ __forceinline__ uint32_t tile_rect(const uint32_t* p, uint32_t width,
   px1 = min(px1, static_cast<int32_t>(width) - 1);
   py1 = min(py1, static_cast<int32_t>(height) - 1);
   if (px0 > px1 || py0 > py1) return 0;
-  r.tx0 = px0 / static_cast<int32_t>(tile);
-  r.ty0 = py0 / static_cast<int32_t>(tile);
-  r.tx1 = px1 / static_cast<int32_t>(tile);
-  r.ty1 = py1 / static_cast<int32_t>(tile);
+  // 0 <= px0 <= px1 and 0 <= py0 <= py1 here and the tile is a power of two (8, 16 or 32), so the
+  // quotient by the tile is a shift by its (wave-uniform) logarithm.
+  const uint32_t shift = static_cast<uint32_t>(__builtin_ctz(tile));
+  r.tx0 = px0 >> shift;
+  r.ty0 = py0 >> shift;
+  r.tx1 = px1 >> shift;
+  r.ty1 = py1 >> shift;
   return static_cast<uint32_t>((r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1));
 }
 
