@@ -267,6 +267,69 @@ def serving(skm, rates, per_rate, modes=SERVING_MODES):
     return out
 
 
+def counter_leg(skm, leg):
+    """One leg's GPU work once, untimed and with nothing printed but a summary line, for a profiler's
+    counter run (which serialises dispatches). batch: two 64-view launches of the 3M scene (the second is
+    the measured one); loop: the same 64 views drawn one per launch twice (the first pass lets the pair
+    buffers grow); serve / serve_single: 288 requests of the 8-scene serving mix, the first 32 a warm-up,
+    with continuous batching (4 per launch, 3 in flight) or one request per launch."""
+    if leg in ("batch", "loop"):
+        cloud = skm.scene_c1(N_SPLATS, SH_DEGREE)
+        r = skm.Renderer(0, frames_in_flight=FRAMES_IN_FLIGHT)
+        spatially_ordered(r, cloud)
+        r.upload_world(cloud)
+        r.apply_policy(sort_depth=32)
+        views = skm.views_c2(64)
+        tuning = dict(tile_size=16, raster_px=1, sort_kernel=0, radix_bits=8)
+        if leg == "batch":
+            arr = r.make_views(views)
+            for _ in range(2):
+                r.draw_views(arr, **tuning)
+                r.finish()
+            return {"leg": leg, **r.batch_stats()}
+        r.set_tuning(**tuning)
+        frames = [(r.make_frame(v, p, e), w, h) for v, p, e, w, h in views]
+        for attempt in range(3):
+            for f, w, h in frames:
+                r.set_extent(w, h)
+                r.draw(f)
+            try:
+                r.finish()
+                break
+            except skm.SkmError as e:
+                if e.status != skm.SKM_ERR_CAPACITY or attempt == 2:
+                    raise
+        r.frame_totals(reset=True)
+        for f, w, h in frames:
+            r.set_extent(w, h)
+            r.draw(f)
+        r.finish()
+        return {"leg": leg, **r.frame_totals(reset=True)}
+    clouds = [skm.make_cloud(n, k % 4, seed=0x5EED0004 + k, layout=1, spread=5.0, area=10.0)
+              for k, n in enumerate(SERVING_SIZES)]
+    loader = skm.Renderer(0)
+    for c in clouds:
+        spatially_ordered(loader, c)
+    loader.close()
+    _, batch, flight = SERVING_MODES[0 if leg == "serve" else 1]
+    srv = skm.Server(max_batch_views=batch, max_in_flight=flight)
+    for c in clouds:
+        srv.add_scene(c)
+    rng = np.random.default_rng(1003)
+    views = skm.views_c2(288, seed=2003, resolutions=SERVING_RESOLUTIONS)
+    scenes = rng.integers(0, len(clouds), 288)
+    failed = []
+    for lo, hi in ((0, 32), (32, 288)):
+        tickets = [srv.submit(int(s), *v) for s, v in zip(scenes[lo:hi], views[lo:hi])]
+        done = srv.wait_all(tickets, timeout_s=300)
+        for k, t in enumerate(tickets):
+            if done[t]["status"] != 0:
+                failed.append({"request": lo + k, "scene": int(scenes[lo + k]), "size": list(views[lo + k][3:5]),
+                               "status": done[t]["status"], "batch_views": done[t]["batch_views"]})
+            srv.release(t)
+    return {"leg": leg, "requests": 288, **srv.stats(reset=True), "failed": failed}
+
+
 def fly_camera(frame, frames, width=1920, height=1080):
     """A straight pass over the 200 m square at 4 m height, weaving 30 m either side, looking ahead
     and slightly down."""
@@ -408,6 +471,8 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a known-size kernel for the HBM "
                          "byte counters of a profiler)")
+    ap.add_argument("--counter-leg", choices=("batch", "loop", "serve", "serve_single"), default="",
+                    help="run one leg's work once for a profiler's counter run and exit (no JSON line)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the render path on a single MI355X "
@@ -416,6 +481,9 @@ def main():
 
     import skm
 
+    if args.counter_leg:
+        print(json.dumps(counter_leg(skm, args.counter_leg)))
+        return
     if args.copy_calibration_mib > 0:
         import torch
 

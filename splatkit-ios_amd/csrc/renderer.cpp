@@ -285,6 +285,7 @@ struct skm_renderer {
   bool completed_world_frame = false;
   bool grow_pairs = false;
   uint64_t pairs_needed = 0;
+  uint64_t batch_pairs_needed = 0;  // the same for view batches (their pair buffers are the batch's)
   // A frame or batch whose tile pairs overflowed was dropped (its image is background only).
   // last_dropped: the newest completed frame was; drops: frames and batches dropped since the last
   // skm_finish, which reports them (however many frames completed after them).
@@ -401,6 +402,7 @@ struct skm_renderer {
         if (dropped) {
           totals.dropped += 1;
           ++drops;
+          batch_pairs_needed = std::max<uint64_t>(batch_pairs_needed, batch->pairs());
         }
       }
     }
@@ -1014,7 +1016,7 @@ skm_status skm_finish(skm_renderer* r) {
     return r->fail(SKM_ERR_CAPACITY,
                    "%llu frame(s) or batch(es) since the last finish were dropped: tile pairs overflowed (up to %llu "
                    "needed); capacity grows on the next draw",
-                   n, static_cast<unsigned long long>(r->pairs_needed));
+                   n, static_cast<unsigned long long>(std::max(r->pairs_needed, r->batch_pairs_needed)));
   }
   return SKM_OK;
 }

@@ -81,6 +81,7 @@ struct Request {
   int32_t status = SKM_OK;
   uint32_t batch_views = 0;
   uint32_t attempts = 0;  // launches that dropped it (tile pairs overflowed)
+  uint64_t retry_of = 0;  // the dropped launch it is to be launched again with (0: none)
   bool done = false;
 };
 
@@ -118,6 +119,7 @@ struct skm_server {
 
   // Stats window.
   uint64_t submitted = 0, completed = 0, rejected = 0, launches = 0, launched_views = 0;
+  uint64_t retry_groups = 0;  // dropped launches so far (the marks of their requests)
   std::vector<double> latencies;
   Clock::time_point window_start = Clock::now();
 
@@ -232,9 +234,11 @@ struct skm_server {
   }
 
   // Retires finished launches. The requests of a launch whose tile pairs overflowed go back to the
-  // head of the queue in arrival order (the scene's pair capacity has grown), so admission prices
-  // the grown scratch against the budget again; after kMaxAttempts drops, or when the capacity is at
-  // its 32-bit limit, they complete with SKM_ERR_CAPACITY.
+  // head of the queue in arrival order, marked with that launch: admission launches them again as
+  // the same group and nothing else (the scene's pair capacity has grown to fit exactly those views;
+  // requests that arrived meanwhile would make the launch bigger than that) and prices the grown
+  // scratch against the budget again. After kMaxAttempts drops, or when the capacity is at its 32-bit
+  // limit, they complete with SKM_ERR_CAPACITY.
   void retire() {
     for (auto& up : scenes) {
       Scene& sc = *up;
@@ -244,12 +248,15 @@ struct skm_server {
       const auto now = Clock::now();
       if (st == SKM_ERR_CAPACITY) {
         const bool limit = skm::renderer_batch(sc.r)->at_pair_limit();
+        const uint64_t group = ++retry_groups;
         std::vector<Request*> again;
         for (Request* q : sc.running) {
-          if (++q->attempts >= kMaxAttempts || limit)
+          if (++q->attempts >= kMaxAttempts || limit) {
             finish_request(q, SKM_ERR_CAPACITY, now);
-          else
+          } else {
+            q->retry_of = group;
             again.push_back(q);
+          }
         }
         pending.insert(pending.begin(), again.begin(), again.end());
       } else {
@@ -282,7 +289,9 @@ struct skm_server {
       std::vector<Request*> batch;
       uint64_t pixels = 0;
       for (Request* q : pending) {
-        if (q->req.scene != head->req.scene || batch.size() >= std::max(1u, cfg.max_batch_views)) continue;
+        if (q->req.scene != head->req.scene || q->retry_of != head->retry_of ||
+            batch.size() >= std::max(1u, cfg.max_batch_views))
+          continue;
         const uint64_t px = uint64_t{q->req.view.width} * q->req.view.height;
         if (!batch.empty() &&
             ((batch.size() + 1) * lanes_per_view > kLaunchLanes || pixels + px > kLaunchPixels))

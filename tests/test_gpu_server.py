@@ -161,6 +161,49 @@ def served_vs_oracle(img, o, label):
     assert np.all(img[..., 3] == 255), label
 
 
+def test_a_dropped_launch_is_relaunched_as_the_same_group(skm_mod):
+    """Requests of one scene (1M splats SH1, 720p to 2160p views) arriving one by one while launches
+    run: a launch whose tile pairs overflow the scene's pair buffers is dropped and its requests are
+    launched again as exactly that group, which then fits (requests that arrived meanwhile wait for
+    the next launch, whose overflow is handled the same way). Every request completes, each with the
+    pixels of the same request served alone."""
+    say("case: 1M SH1 scene, 5 requests 720p-2160p trickled in, fresh server x3")
+    cloud = skm_mod.make_cloud(1_000_000, 1, seed=0x5EED0005, layout=1, spread=5.0, area=10.0)
+    rng = np.random.default_rng(1003)
+    views = skm_mod.views_c2(32, seed=2003, resolutions=((1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)))
+    picks = [v for v, s in zip(views, rng.integers(0, 8, 32)) if s == 1]
+    assert len(picks) >= 4
+    alone = []
+    srv = skm_mod.Server(max_batch_views=1, max_in_flight=1)
+    try:
+        srv.add_scene(cloud)
+        for v in picks:
+            t = srv.submit(0, *v)
+            assert srv.wait_all([t], timeout_s=120)[t]["status"] == 0
+            alone.append(srv.image(t))
+            srv.release(t)
+    finally:
+        srv.close()
+    for gap_s in (0.0, 2e-4, 1e-3):
+        srv = skm_mod.Server(max_batch_views=4, max_in_flight=3)
+        try:
+            srv.add_scene(cloud)
+            tickets = []
+            for v in picks:
+                tickets.append(srv.submit(0, *v))
+                time.sleep(gap_s)
+            done = srv.wait_all(tickets, timeout_s=120)
+            st = srv.stats()
+            say(f"gap {gap_s * 1e3:.1f} ms: launches {st['launches']} rejected {st['rejected']} "
+                f"statuses {[done[t]['status'] for t in tickets]}")
+            assert all(done[t]["status"] == 0 for t in tickets) and st["rejected"] == 0
+            for t, a in zip(tickets, alone):
+                assert np.array_equal(srv.image(t), a)
+                srv.release(t)
+        finally:
+            srv.close()
+
+
 def test_served_requests_match_oracle_across_sh_and_resolutions(skm_mod, oracle):
     """Four resident scenes at SH0, SH1, SH2 and SH3, one request per scene and resolution (720p,
     1080p, 2160p), all queued together so they are batched: every served image matches the oracle's
