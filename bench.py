@@ -530,11 +530,14 @@ def main():
         raise SystemExit(f"timed region: {tot['dropped']} frames dropped, {tot['frames']} drawn of {args.steps}")
 
     # The same frames one at a time (each drawn and finished before the next): kernel durations with
-    # the GPU to themselves, beside the overlapped ones of the timed region.
+    # the GPU to themselves, beside the overlapped ones of the timed region. These frames also count
+    # the raster's coverage tests (off in the timed region: the count costs an atomic per raster block).
+    r.count_raster_tests(True)
     for i in range(20):
         r.draw(r.make_frame(*frames[i % 60]))
         r.finish()
     alone = r.frame_totals(reset=True)
+    r.count_raster_tests(False)
 
     n_frames = max(1, tot["frames"])
     V = tot["visible"] / n_frames
@@ -552,10 +555,10 @@ def main():
                      "hbm_frac": round(gbs * 1e9 / HBM_PEAK, 4)}
     tile = plain_best["tile_size"]
     # Raster work: the pair evaluations of the tile lists (E = tile^2 x P, an upper bound), and the
-    # (pixel, record) tests the kernel performed (counted on the GPU: records each wave walked before
-    # its pixels saturated, times its pixels).
+    # (pixel, record) tests the kernel performed (counted on the GPU over the frames drawn alone, orbit
+    # frames 0-19: records each wave walked before its pixels saturated, times its pixels).
     evals_upper = float(tile * tile) * P
-    tests = tot["raster_tests"] / n_frames
+    tests = alone["raster_tests"] / max(1, alone["frames"])
     raster_alone_ms = alone["stage_ms"][4] / max(1, alone["frames"])
     stages["raster"]["pair_evals_upper"] = int(evals_upper)
     stages["raster"]["coverage_tests"] = int(tests)
@@ -588,7 +591,10 @@ def main():
                         "profiles/sq_counters_3M_sh3_1080p.json) over the raster's average duration measured in this "
                         f"run with {args.frames_in_flight} frames in flight; peak = 1024 SIMDs x 16 lanes x 2.4 GHz "
                         "(packed fp32 disabled). traffic = HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE "
-                        "counter runs in profiles/hbm_traffic_3M_sh3_1080p.md"}
+                        "counter runs in profiles/hbm_traffic_3M_sh3_1080p.md. coverage_tests_per_launch: the "
+                        "(pixel, record) tests the kernel performed, counted on the GPU over orbit frames 0-19 drawn "
+                        "alone (the timed frames do not count them); pair_evals_upper_per_launch = tile^2 x pairs of "
+                        "the timed frames"}
 
     # The sorts against HBM: the depth sort of the frames drawn alone (4 passes over V survivors,
     # B = V (2 (k + 4) 4 + k)), and the standalone sort at the frame's pair count below.

@@ -228,10 +228,10 @@ skm_status skm_read_lod_cut(skm_renderer* r, uint32_t* count, uint32_t* indices,
                             uint32_t* stats6);
 
 /* Sums over the frames completed since the last reset: GPU milliseconds per stage (as above),
- * survivors, tile pairs and the raster's (pixel, record) coverage tests of the frames drawn (every
- * record a wave walked before its pixels saturated, times the wave's pixels), and the frames and view
- * batches dropped because their tile pairs overflowed (not in the other sums). For throughput
- * reports over many frames. */
+ * survivors, tile pairs and, while skm_count_raster_tests is on, the raster's (pixel, record)
+ * coverage tests of the frames drawn (every record a wave walked before its pixels saturated, times
+ * the wave's pixels; 0 otherwise), and the frames and view batches dropped because their tile pairs
+ * overflowed (not in the other sums). For throughput reports over many frames. */
 typedef struct skm_frame_totals {
   uint64_t frames;
   double stage_ms[6];
@@ -241,6 +241,10 @@ typedef struct skm_frame_totals {
   uint64_t raster_tests;
 } skm_frame_totals;
 skm_status skm_read_frame_totals(skm_renderer* r, skm_frame_totals* out, int reset);
+/* Counting of the raster's coverage tests for the frames drawn from now on (off by default: the count
+ * costs each raster block one device-wide atomic add, about 5% of a light frame's raster). No
+ * counterpart in the reference: a profiling statistic of this backend. */
+void skm_count_raster_tests(skm_renderer* r, int enable);
 
 /* ---- the sort on its own (MetalRadixSort, MetalRadixSort.h:13-19) ---- */
 /* Stable ascending argsort of host keys below 2^key_bits, key_bits 1 to 32: ceil(key_bits / 8) LSD

@@ -1318,15 +1318,18 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
     }
     if (__syncthreads_count(live() ? 1 : 0) == 0) break;
   }
-  // The block's coverage tests: one add per block, to one of 8 slots.
-  __shared__ uint32_t s_walked[kW];
-  if (lane == 0) s_walked[wave] = walked;
-  __syncthreads();
-  if (tid == 0) {
-    uint32_t sum = 0;
+  // The block's coverage tests, when the renderer counts them: one add per block, to one of 8 slots.
+  if (a.count_tests != 0u) {
+    __shared__ uint32_t s_walked[kW];
+    if (lane == 0) s_walked[wave] = walked;
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t sum = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < kW; ++w) sum += s_walked[w];
-    if (sum != 0u) atomicAdd(&a.ctrl->raster_tests[blockIdx.x & 7u], static_cast<unsigned long long>(sum) * (64u * kP));
+      for (uint32_t w = 0; w < kW; ++w) sum += s_walked[w];
+      if (sum != 0u)
+        atomicAdd(&a.ctrl->raster_tests[blockIdx.x & 7u], static_cast<unsigned long long>(sum) * (64u * kP));
+    }
   }
 #pragma unroll
   for (uint32_t q = 0; q < kP; ++q) {

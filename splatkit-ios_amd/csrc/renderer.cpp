@@ -245,6 +245,7 @@ struct skm_renderer {
 
   float render_scale = 1.0f;
   bool linear = false;
+  bool count_tests = false;  // skm_count_raster_tests
   bool vsync = true;
   uint32_t surface_w = 0, surface_h = 0;
   uint32_t generation = 0;
@@ -661,6 +662,9 @@ void skm_set_linear_blending(skm_renderer* r, int linear) {
   if (r) r->linear = linear != 0;
 }
 int skm_linear_blending(const skm_renderer* r) { return r && r->linear ? 1 : 0; }
+void skm_count_raster_tests(skm_renderer* r, int enable) {
+  if (r) r->count_tests = enable != 0;
+}
 void skm_set_vsync(skm_renderer* r, int vsync) {
   if (r) r->vsync = vsync != 0;
 }
@@ -962,6 +966,7 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   a.width = W;
   a.height = H;
   a.tile = r->tile;
+  a.count_tests = r->count_tests ? 1u : 0u;
   a.tiles_x = tiles_x;
   a.tiles_y = tiles_y;
   for (int c = 0; c < 3; ++c) a.background[c] = static_cast<float>(r->policy.background[c]);

@@ -65,12 +65,12 @@ struct Ctrl {
   uint32_t sort_pairs;  // pairs the tile sort orders: `pairs`, or 0 after an overflow
   uint32_t cull_lanes;  // cull tiles x kCullTile: the extent of the cull's gapped survivor layout
   uint32_t reserved[2];
-  // (pixel, record) coverage tests the raster performed: every record a wave walked, times its
-  // pixels (records past a wave's or tile's termination are not walked). Block b adds its count to
-  // slot b % kTestSlots, so the adds of concurrent blocks spread over several addresses.
-  unsigned long long raster_tests[8];
   uint32_t ticket[16];
   uint32_t lod[16];   // LOD selection counters
+  // (pixel, record) coverage tests the raster performed: every record a wave walked, times its
+  // pixels (records past a wave's or tile's termination are not walked). Block b adds its count to
+  // slot b % 8, so the adds of concurrent blocks spread over several addresses.
+  unsigned long long raster_tests[8];
 };
 
 enum Ticket : uint32_t {
@@ -200,6 +200,9 @@ struct FrameArgs {
   // Raster block shape: pixels per lane side (1 or 2) at tile size `tile` (8, 16 or 32); the
   // supported pairs are (8, 1), (16, 1), (16, 2), (32, 2).
   uint32_t raster_px;
+  // Nonzero: the raster adds its (pixel, record) coverage tests to ctrl->raster_tests (one atomic
+  // per block, a few percent of a light frame's raster time).
+  uint32_t count_tests;
 };
 
 // Whether the raster has a block shape for (tile, pixels per lane side).

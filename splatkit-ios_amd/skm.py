@@ -174,6 +174,7 @@ API = {
     "skm_set_render_scale": (None, [_R, C.c_float]),
     "skm_render_scale": (C.c_float, [_R]),
     "skm_set_linear_blending": (None, [_R, C.c_int]),
+    "skm_count_raster_tests": (None, [_R, C.c_int]),
     "skm_linear_blending": (C.c_int, [_R]),
     "skm_set_vsync": (None, [_R, C.c_int]),
     "skm_set_draw_extent": (C.c_int, [_R, C.c_uint32, C.c_uint32]),
@@ -719,6 +720,10 @@ class Renderer:
 
     def set_linear_blending(self, linear):
         self._lib.skm_set_linear_blending(self._h, 1 if linear else 0)
+
+    def count_raster_tests(self, enable=True):
+        """Counts the raster's coverage tests into frame_totals()['raster_tests'] (off by default)."""
+        self._lib.skm_count_raster_tests(self._h, 1 if enable else 0)
 
     def lod_threshold(self):
         """The screen-space error the next tree-world frame's cut runs at (LodThreshold::pixels)."""

@@ -179,3 +179,36 @@ def test_tile_upload_while_frames_draw_other_ranges(skm_mod, oracle):
         assert_parity(g, o, label="A + B + C")
     finally:
         r.close()
+
+
+def test_raster_test_counting_is_opt_in(skm_mod):
+    """skm_count_raster_tests: off by default (frame totals report 0 coverage tests); on, the count is
+    the same for the same frame drawn twice, positive, at most 256 (P + 3 T) (each of a 16 x 16 tile's
+    four 64-pixel waves walks at most its tile's P_t records, padded to groups of 4), and the pixels are
+    those of the frames drawn without counting."""
+    say("case: 100k SH1 720p, raster tests off then on")
+    cloud = skm_mod.make_cloud(100_000, 1, seed=23, layout=1, spread=3.0, area=6.0)
+    r = skm_mod.Renderer(0)
+    try:
+        r.upload_world(cloud)
+        r.set_extent(1280, 720)
+        cam = skm_mod.orbit_camera(4, 1280, 720, radius=5.0, height_m=1.0)
+        r.render(*cam)
+        img_off = r.read_frame()
+        off = r.frame_totals(reset=True)
+        assert off["frames"] == 1 and off["raster_tests"] == 0
+        r.count_raster_tests(True)
+        counts = []
+        for _ in range(2):
+            r.render(*cam)
+            assert np.array_equal(r.read_frame(), img_off)
+            t = r.frame_totals(reset=True)
+            counts.append(t["raster_tests"])
+        tiles = ((1280 + 15) // 16) * ((720 + 15) // 16)
+        say(f"tests {counts}, pairs {t['pairs']}")
+        assert counts[0] == counts[1] and 0 < counts[0] <= 256 * (t["pairs"] + 3 * tiles)
+        r.count_raster_tests(False)
+        r.render(*cam)
+        assert r.frame_totals(reset=True)["raster_tests"] == 0
+    finally:
+        r.close()
