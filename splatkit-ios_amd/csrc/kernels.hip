@@ -1064,7 +1064,8 @@ __global__ __launch_bounds__(256) void k_order_count(const uint32_t* __restrict_
   if (tid < 32u && s_count[tid] != 0u) atomicAdd(&counters[tid], s_count[tid]);
 }
 __global__ __launch_bounds__(256) void k_order_scatter(const uint32_t* __restrict__ ranges, uint32_t tiles,
-                                                       uint32_t* __restrict__ counters, uint32_t* __restrict__ order) {
+                                                       uint32_t* __restrict__ counters, uint32_t* __restrict__ order,
+                                                       uint32_t split_cap) {
   __shared__ uint32_t s_start[32], s_count[32], s_base[32];
   const uint32_t tid = threadIdx.x, t = blockIdx.x * 256u + tid;
   if (tid < 64u) {
@@ -1074,6 +1075,12 @@ __global__ __launch_bounds__(256) void k_order_scatter(const uint32_t* __restric
       s_start[tid] = incl - v;
       s_count[tid] = 0u;
     }
+    // The raster's split tiles: the heaviest whole buckets holding at most split_cap tiles (the scan
+    // is non-decreasing, so its largest value within the cap).
+    const uint32_t within = __builtin_amdgcn_readfirstlane(
+        static_cast<uint32_t>(__popcll(__ballot(tid < 32u && incl <= split_cap))));
+    if (blockIdx.x == 0 && tid == 0)
+      counters[64] = within == 0u ? 0u : __builtin_amdgcn_readlane(incl, within - 1u);
   }
   __syncthreads();
   const uint32_t bucket = t < tiles ? order_bucket(ranges, t) : 0u;
@@ -1136,6 +1143,9 @@ struct RasterShape {
   static constexpr uint32_t kWaves = kSide * kSide;
   static constexpr uint32_t kThreads = 64u * kWaves;   // also the records prepared per batch
   static constexpr uint32_t kPixels = PX * PX;
+  // Waves per SIMD the register budget is held to: six (80 VGPRs) for one pixel per lane, five for
+  // four.
+  static constexpr uint32_t kMinWaves = PX == 1 ? 6u : 5u;
   static_assert(TILE % kRegion == 0 && kWaves <= 4, "shape");
 };
 
@@ -1144,22 +1154,83 @@ struct RasterShape {
 // colour), and every region gets the list, in batch order, of the records whose footprint box
 // reaches one of its pixel centres. A wave walks its region's list front to back; it leaves the list
 // once all of its pixels are below kTransmittanceStop, and the tile stops when every pixel is.
+//
+// Split tiles (16 x 16 tiles, one pixel per lane): the heaviest whole buckets of the tile order, at
+// most `split_cap` tiles (a choice that does not depend on the order within a bucket), are each
+// drawn by four blocks, one per 8 x 8 quadrant, whose waves take 4 x 4 pixels with four lanes per
+// pixel: lane 4 p + r tests record r of each group of four against pixel p, then the group's blends
+// run in order, each with record r's result broadcast across the pixel's four lanes. A wave thus
+// walks a 4 x 4 region's list, four records per test, instead of an 8 x 8 region's list one record
+// per test: the heaviest tiles, whose walks run long after the others have finished, finish in about
+// a third of the time. Every blend is the same operation on the same operands, so the pixels are those
+// of an unsplit tile bit for bit.
 template <int TILE, int PX>
-__global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(FrameArgs a,
-                                                                           const uint32_t* __restrict__ pair_vals,
-                                                                           const uint32_t* __restrict__ ranges) {
+__device__ __forceinline__ uint32_t region_mask(uint32_t size, float bx, float by, float cx, float cy, float ex,
+                                                float ey, float g00, float g01, float g10, float g11, float rk) {
+  const float x0 = (cx - ex) - kBinPad, x1 = (cx + ex) + kBinPad;
+  const float y0 = (cy - ey) - kBinPad, y1 = (cy + ey) + kBinPad;
+  // The quad's two slabs |gu|, |gv| <= K r against the region: gu varies by at most
+  // (|g00| + |g01|) h around the region's centre, h its half span plus the binning pad. The
+  // pad's share, (|g00| + |g01|) / 16, and the slack cover the fp32 rounding of the per-pixel
+  // rule, at most about 3 W 2^-24 (|g00| + |g01|) at image width W.
+  const float span = static_cast<float>(size - 1u);
+  const float half = 0.5f * span + kBinPad;
+  const float hu = (fabsf(g00) + fabsf(g01)) * half, hv = (fabsf(g10) + fabsf(g11)) * half;
+  const float lim_u = (rk + hu) * kFootprintSlack, lim_v = (rk + hv) * kFootprintSlack;
+  uint32_t regions = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < RasterShape<TILE, PX>::kWaves; ++q) {
+    const float qx = bx + static_cast<float>((q % RasterShape<TILE, PX>::kSide) * size) + 0.5f;
+    const float qy = by + static_cast<float>((q / RasterShape<TILE, PX>::kSide) * size) + 0.5f;
+    const float dxc = (qx + 0.5f * span) - cx, dyc = cy - (qy + 0.5f * span);
+    const float uc = fmaf(g00, dxc, g01 * dyc), vc = fmaf(g10, dxc, g11 * dyc);
+    if (x1 >= qx && x0 <= qx + span && y1 >= qy && y0 <= qy + span && fabsf(uc) <= lim_u && fabsf(vc) <= lim_v)
+      regions |= 1u << q;
+  }
+  return regions;
+}
+
+// Step U of a split wave's group: record U's result, held by lane 4 p + U, broadcast to pixel p's four
+// lanes (a quad permutation), then blended as in the unsplit walk.
+template <int U>
+__device__ __forceinline__ void quad_blend(float x, const float4& Cc, float& T, float& C0, float& C1, float& C2) {
+  const float xb = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), U * 0x55, 0xf, 0xf, false));
+  const float alpha = exp2_clamped(T >= kTransmittanceStop ? xb : -__builtin_inff());
+  const float w = T * alpha;
+  C0 = fmaf(w, Cc.x, C0);
+  C1 = fmaf(w, Cc.y, C1);
+  C2 = fmaf(w, Cc.z, C2);
+  T = T - w;
+}
+
+// The LDS of a raster block. s_rec[0..2][k]: record k's A, B (coverage) and colour. The region lists
+// hold the byte offset of s_rec[0][k] (16 k), which the LDS reads take as their address with no
+// arithmetic. Slot kB of each record array is a null record that no pixel takes (radius -1): region
+// lists are padded with it to whole groups.
+template <int TILE, int PX>
+struct RasterLds {
+  static constexpr uint32_t kB = RasterShape<TILE, PX>::kThreads, kW = RasterShape<TILE, PX>::kWaves;
+  static constexpr uint32_t kS = kB + 1u;
+  float4 rec[3][kS];
+  __align__(16) uint32_t list[kW][kB + kGroup];
+  uint32_t count[kW][kW];  // [region][wave]
+  uint32_t walked[kW];
+};
+
+// One block's work: a whole tile, or (QUAD) one quadrant of a split tile.
+template <int TILE, int PX, bool QUAD>
+__device__ __forceinline__ void raster_block(const FrameArgs& a, const uint32_t* __restrict__ pair_vals,
+                                             const uint32_t* __restrict__ ranges, uint32_t pos,
+                                             RasterLds<TILE, PX>& L) {
   using S = RasterShape<TILE, PX>;
   constexpr uint32_t kB = S::kThreads, kW = S::kWaves, kP = S::kPixels;
-  // s_rec[0..2][k]: record k's A, B (coverage) and colour. The region lists hold the byte offset of
-  // s_rec[0][k] (16 k), which the LDS reads take as their address with no arithmetic.
-  // Slot kB of each record array is a null record that no pixel takes (radius -1): region lists are
-  // padded with it to whole groups.
   constexpr uint32_t kS = kB + 1u;
-  __shared__ float4 s_rec[3][kS];
-  __shared__ __align__(16) uint32_t s_list[kW][kB + kGroup];
-  __shared__ uint32_t s_count[kW][kW];  // [region][wave]
+  constexpr bool quad = QUAD;
+  auto& s_rec = L.rec;
+  auto& s_list = L.list;
+  auto& s_count = L.count;
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
-  const uint32_t tile_id = a.tile_order != nullptr ? a.tile_order[blockIdx.x] : blockIdx.x;  // global tile id
+  const uint32_t tile_id = a.tile_order != nullptr ? a.tile_order[pos] : pos;  // global tile id
   uint32_t width = a.width, height = a.height, tiles_x = a.tiles_x, local = tile_id;
   uint64_t pixel_base = 0;
   if (a.tile_view != nullptr) {
@@ -1171,14 +1242,20 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
     pixel_base = vi.pixel_base;
   }
   const uint32_t ox = (local % tiles_x) * TILE, oy = (local / tiles_x) * TILE;
-  const uint32_t rx = ox + (wave % S::kSide) * S::kRegion, ry = oy + (wave / S::kSide) * S::kRegion;
+  // The block's regions: the tile's 8 x 8 quadrants, or a split block's quadrant's 4 x 4 quarters.
+  const uint32_t rsize = quad ? S::kRegion / 2u : S::kRegion;
+  const uint32_t bx = quad ? ox + (blockIdx.x & 1u) * S::kRegion : ox;
+  const uint32_t by = quad ? oy + ((blockIdx.x >> 1) & 1u) * S::kRegion : oy;
+  const uint32_t rx = bx + (wave % S::kSide) * rsize, ry = by + (wave / S::kSide) * rsize;
+  // The lane's pixel: (lane % 8, lane / 8) of an 8 x 8 region, or pixel lane / 4 of a 4 x 4 one.
+  const uint32_t lx = quad ? (lane >> 2) & 3u : lane & 7u, ly = quad ? lane >> 4 : lane >> 3;
   const uint32_t start = ranges[2 * tile_id], end = ranges[2 * tile_id + 1];
   const float fw = static_cast<float>(width), fh = static_cast<float>(height);
   float fx[kP], fy[kP], T[kP], C0[kP], C1[kP], C2[kP];
   bool in_image[kP];
 #pragma unroll
   for (uint32_t q = 0; q < kP; ++q) {
-    const uint32_t px = rx + (lane & 7u) + 8u * (q % PX), py = ry + (lane >> 3) + 8u * (q / PX);
+    const uint32_t px = rx + lx + 8u * (q % PX), py = ry + ly + 8u * (q / PX);
     in_image[q] = px < width && py < height;
     fx[q] = static_cast<float>(px) + 0.5f;
     fy[q] = static_cast<float>(py) + 0.5f;
@@ -1225,28 +1302,9 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
       const float g00 = a2y * sc, g01 = -(a2x * sc), g10 = -(a1y * sc), g11 = a1x * sc;
       const float rk = kGaussianScale * f32(w1.z);
       float ex = 0.0f, ey = 0.0f;
-      if (footprint(p, log2a0, ex, ey)) {
-        const float x0 = (cx - ex) - kBinPad, x1 = (cx + ex) + kBinPad;
-        const float y0 = (cy - ey) - kBinPad, y1 = (cy + ey) + kBinPad;
-        // The quad's two slabs |gu|, |gv| <= K r against the region: gu varies by at most
-        // (|g00| + |g01|) h around the region's centre, h its half span plus the binning pad. The
-        // pad's share, (|g00| + |g01|) / 16, and the slack cover the fp32 rounding of the per-pixel
-        // rule, at most about 3 W 2^-24 (|g00| + |g01|) at image width W.
-        constexpr float kHalf = 0.5f * static_cast<float>(S::kRegion - 1u) + kBinPad;
-        const float hu = (fabsf(g00) + fabsf(g01)) * kHalf, hv = (fabsf(g10) + fabsf(g11)) * kHalf;
-        const float lim_u = (rk + hu) * kFootprintSlack, lim_v = (rk + hv) * kFootprintSlack;
-#pragma unroll
-        for (uint32_t q = 0; q < kW; ++q) {
-          const float qx = static_cast<float>(ox + (q % S::kSide) * S::kRegion) + 0.5f;
-          const float qy = static_cast<float>(oy + (q / S::kSide) * S::kRegion) + 0.5f;
-          constexpr float kSpan = static_cast<float>(S::kRegion - 1u);
-          const float dxc = (qx + 0.5f * kSpan) - cx, dyc = cy - (qy + 0.5f * kSpan);
-          const float uc = fmaf(g00, dxc, g01 * dyc), vc = fmaf(g10, dxc, g11 * dyc);
-          if (x1 >= qx && x0 <= qx + kSpan && y1 >= qy && y0 <= qy + kSpan && fabsf(uc) <= lim_u &&
-              fabsf(vc) <= lim_v)
-            regions |= 1u << q;
-        }
-      }
+      if (footprint(p, log2a0, ex, ey))
+        regions = region_mask<TILE, PX>(rsize, static_cast<float>(bx), static_cast<float>(by), cx, cy, ex, ey, g00,
+                                        g01, g10, g11, rk);
       // gu = g00 (fx - cx) + g01 (cy - fy) = g00 fx + h01 fy + u0 with h01 = -g01 and
       // u0 = g01 cy - g00 cx; likewise gv.
       const float u0 = fmaf(g01, cy, -(g00 * cx)), v0 = fmaf(g11, cy, -(g10 * cx));
@@ -1278,7 +1336,33 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
     n = __builtin_amdgcn_readfirstlane(n);
     __syncthreads();
     if (lane < kGroup) s_list[wave][n + lane] = kB * static_cast<uint32_t>(sizeof(float4));  // the null record
-    if (__ballot(live()) != 0) {
+    if (quad && __ballot(live()) != 0) {
+      if constexpr (QUAD) {
+        static_assert(kGroup == 4 && kP == 1, "a split wave tests one group of four records per lane quad");
+        const char* base = reinterpret_cast<const char*>(&s_rec[0][0]);
+        const uint32_t r = lane & 3u;
+        for (uint32_t i = 0; i < n; i += kGroup) {
+          walked += kGroup;
+          // Lane 4 p + r tests record i + r (past n: the null record padding) against pixel p.
+          const uint32_t mine = s_list[wave][i + r];
+          const float4* rec = reinterpret_cast<const float4*>(base + mine);
+          float e;
+          const bool hit = shape_test(rec[0], rec[kS], fx[0], fy[0], e);
+          const float x = hit ? e : -__builtin_inff();
+          // Records that no live pixel of the wave hits leave every pixel as it is: skipped.
+          const uint64_t takes = __ballot(hit && T[0] >= kTransmittanceStop);
+          auto colour = [&](uint32_t u) {
+            const uint32_t at = __builtin_amdgcn_readlane(mine, u);
+            return reinterpret_cast<const float4*>(base + at)[2 * kS];
+          };
+          if (takes & 0x1111111111111111ull) quad_blend<0>(x, colour(0), T[0], C0[0], C1[0], C2[0]);
+          if (takes & 0x2222222222222222ull) quad_blend<1>(x, colour(1), T[0], C0[0], C1[0], C2[0]);
+          if (takes & 0x4444444444444444ull) quad_blend<2>(x, colour(2), T[0], C0[0], C1[0], C2[0]);
+          if (takes & 0x8888888888888888ull) quad_blend<3>(x, colour(3), T[0], C0[0], C1[0], C2[0]);
+          if (__ballot(live()) == 0) break;
+        }
+      }
+    } else if (__ballot(live()) != 0) {
       for (uint32_t i = 0; i < n; i += kGroup) {
         walked += kGroup;
         // A group of kGroup records: their geometry does not depend on the transmittance, so all of
@@ -1320,25 +1404,45 @@ __global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads)) void k_raster(Fr
   }
   // The block's coverage tests, when the renderer counts them: one add per block, to one of 8 slots.
   if (a.count_tests != 0u) {
-    __shared__ uint32_t s_walked[kW];
+    auto& s_walked = L.walked;
     if (lane == 0) s_walked[wave] = walked;
     __syncthreads();
     if (tid == 0) {
       uint32_t sum = 0;
 #pragma unroll
       for (uint32_t w = 0; w < kW; ++w) sum += s_walked[w];
+      // A split wave tests 16 pixels per record, an unsplit one 64 kP.
       if (sum != 0u)
-        atomicAdd(&a.ctrl->raster_tests[blockIdx.x & 7u], static_cast<unsigned long long>(sum) * (64u * kP));
+        atomicAdd(&a.ctrl->raster_tests[blockIdx.x & 7u], static_cast<unsigned long long>(sum) * (quad ? 16u : 64u * kP));
     }
   }
 #pragma unroll
   for (uint32_t q = 0; q < kP; ++q) {
-    if (!in_image[q]) continue;
-    const uint32_t px = rx + (lane & 7u) + 8u * (q % PX), py = ry + (lane >> 3) + 8u * (q / PX);
+    if (!in_image[q] || (quad && (lane & 3u) != 0u)) continue;  // a split pixel's first lane writes it
+    const uint32_t px = rx + lx + 8u * (q % PX), py = ry + ly + 8u * (q / PX);
     float4* out = reinterpret_cast<float4*>(a.image) + pixel_base + static_cast<size_t>(py) * width + px;
     *out = make_float4(C0[q] + T[q] * a.background[0], C1[q] + T[q] * a.background[1],
                        C2[q] + T[q] * a.background[2], 1.0f);
   }
+}
+
+template <int TILE, int PX>
+__global__ __launch_bounds__((RasterShape<TILE, PX>::kThreads), (RasterShape<TILE, PX>::kMinWaves)) void k_raster(
+    FrameArgs a, const uint32_t* __restrict__ pair_vals, const uint32_t* __restrict__ ranges, uint32_t tiles,
+    uint32_t split_cap) {
+  constexpr bool kSplit = TILE == 16 && PX == 1;
+  __shared__ RasterLds<TILE, PX> L;
+  // Blocks [0, 4 split) draw the quadrants of the first `split` tiles of the order, the next ones a
+  // tile each; the grid has 3 split_cap spare blocks, which exit.
+  const uint32_t split = kSplit && split_cap != 0u ? a.hist[kOrderCounters + 64u] : 0u;
+  if (blockIdx.x >= tiles + 3u * split) return;
+  if constexpr (kSplit) {
+    if (blockIdx.x < 4u * split) {
+      raster_block<TILE, PX, true>(a, pair_vals, ranges, blockIdx.x >> 2, L);
+      return;
+    }
+  }
+  raster_block<TILE, PX, false>(a, pair_vals, ranges, blockIdx.x - 3u * split, L);
 }
 
 // The raster block shapes on offer: (tile, pixels per lane side).
@@ -1346,17 +1450,21 @@ hipError_t launch_raster(const FrameArgs& a, const uint32_t* pair_vals, uint32_t
   const int shape = a.tile * 4 + a.raster_px;
   switch (shape) {
     case 8 * 4 + 1:
-      hipLaunchKernelGGL((k_raster<8, 1>), dim3(tiles), dim3(64), 0, s, a, pair_vals, a.tile_ranges);
+      hipLaunchKernelGGL((k_raster<8, 1>), dim3(tiles), dim3(64), 0, s, a, pair_vals, a.tile_ranges, tiles, 0u);
       break;
     case 16 * 4 + 2:
-      hipLaunchKernelGGL((k_raster<16, 2>), dim3(tiles), dim3(64), 0, s, a, pair_vals, a.tile_ranges);
+      hipLaunchKernelGGL((k_raster<16, 2>), dim3(tiles), dim3(64), 0, s, a, pair_vals, a.tile_ranges, tiles, 0u);
       break;
     case 32 * 4 + 2:
-      hipLaunchKernelGGL((k_raster<32, 2>), dim3(tiles), dim3(256), 0, s, a, pair_vals, a.tile_ranges);
+      hipLaunchKernelGGL((k_raster<32, 2>), dim3(tiles), dim3(256), 0, s, a, pair_vals, a.tile_ranges, tiles, 0u);
       break;
-    case 16 * 4 + 1:
-      hipLaunchKernelGGL((k_raster<16, 1>), dim3(tiles), dim3(256), 0, s, a, pair_vals, a.tile_ranges);
+    case 16 * 4 + 1: {
+      // The heaviest tiles of the order are split (the order is heaviest first).
+      const uint32_t cap = a.tile_order != nullptr ? std::min(tiles, a.split_cap) : 0u;
+      hipLaunchKernelGGL((k_raster<16, 1>), dim3(tiles + 3u * cap), dim3(256), 0, s, a, pair_vals, a.tile_ranges,
+                         tiles, cap);
       break;
+    }
     default:
       return hipErrorInvalidValue;
   }
@@ -1595,7 +1703,7 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
     uint32_t* counters = a.hist + kOrderCounters;
     hipLaunchKernelGGL(k_order_count, dim3(grid_for(tiles, 256)), dim3(256), 0, s, a.tile_ranges, tiles, counters);
     hipLaunchKernelGGL(k_order_scatter, dim3(grid_for(tiles, 256)), dim3(256), 0, s, a.tile_ranges, tiles, counters,
-                       a.tile_order);
+                       a.tile_order, a.split_cap);
   }
   if (marks) (void)hipEventRecord(marks[4], s);
   if (const hipError_t e = launch_raster(a, a.pair_vals[fin], tiles, s); e != hipSuccess) return e;

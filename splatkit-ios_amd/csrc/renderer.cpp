@@ -967,6 +967,7 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   a.height = H;
   a.tile = r->tile;
   a.count_tests = r->count_tests ? 1u : 0u;
+  a.split_cap = r->slot_count == 1 ? skm::kSplitTiles : 0u;
   a.tiles_x = tiles_x;
   a.tiles_y = tiles_y;
   for (int c = 0; c < 3; ++c) a.background[c] = static_cast<float>(r->policy.background[c]);

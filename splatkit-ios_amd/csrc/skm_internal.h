@@ -134,7 +134,13 @@ constexpr uint32_t kSortTile = kThreads * kSortItems;  // keys per radix-pass bl
 constexpr uint32_t kRadix = 256;
 constexpr uint32_t kMaxRanges = 65536;  // MetalVisibilityPass.h:72
 constexpr uint32_t kOrderCounters = 2u * 4u * 256u;  // offset of the tile-order counters in hist
-constexpr uint32_t kHistWords = kOrderCounters + 64u;
+// The tile order's counters: tiles per bucket [0, 32), cursors [32, 64), and the raster's split
+// tiles [64].
+constexpr uint32_t kHistWords = kOrderCounters + 65u;
+// The raster splits the heaviest whole buckets of its tile order, up to this many tiles, when frames
+// do not overlap (one frame slot): a split tile's four quadrant blocks finish its long walk in about a
+// third of the time, at about three times the VALU work, which overlapped frames would rather have.
+constexpr uint32_t kSplitTiles = 1024;
 constexpr float kTransmittanceStop = 1.0f / 4096.0f;
 
 // One view of a batch: its extent, its tiles' place in the batch's global tile ids, and its first
@@ -203,6 +209,8 @@ struct FrameArgs {
   // Nonzero: the raster adds its (pixel, record) coverage tests to ctrl->raster_tests (one atomic
   // per block, a few percent of a light frame's raster time).
   uint32_t count_tests;
+  // At most this many of the heaviest tiles are drawn split (0: none).
+  uint32_t split_cap;
 };
 
 // Whether the raster has a block shape for (tile, pixels per lane side).

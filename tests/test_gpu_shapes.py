@@ -57,6 +57,39 @@ def test_shapes_and_sort_variants_give_identical_pixels(skm_mod, oracle, scene):
                 r.close()
 
 
+def test_split_heavy_tiles_give_identical_pixels(skm_mod, oracle, scene):
+    """With one frame slot the raster draws the heaviest tiles as four quadrant blocks (four lanes per
+    pixel, record-parallel tests): every pixel is the unsplit raster's bit for bit, over several views
+    (close ones put thousands of pairs in a tile), and within 1/255 of the oracle's FP64 image."""
+    cloud, view, proj, eye, w, h = scene
+    cams = [(view, proj, eye)] + [skm_mod.orbit_camera(k, w, h, radius=rad, height_m=0.4)
+                                  for k, rad in ((3, 2.0), (11, 1.2), (20, 3.0))]
+    images = {}
+    for fif in (1, 2):
+        r = skm_mod.Renderer(0, frames_in_flight=fif)
+        try:
+            r.upload_world(cloud)
+            r.set_extent(w, h)
+            r.apply_policy(sort_depth=32)
+            images[fif] = []
+            for cam in cams:
+                r.render(*cam)
+                images[fif].append(r.read_frame())
+                if fif == 1:
+                    pairs, _, _, ranges = r.read_tiles()
+                    per = ranges[:, 1].astype(np.int64) - ranges[:, 0]
+                    say(f"case: split, {pairs} pairs, heaviest tile {per.max()}, tiles over 1024 pairs {(per > 1024).sum()}")
+        finally:
+            r.close()
+    for k, cam in enumerate(cams):
+        assert np.array_equal(images[1][k].view(np.uint32), images[2][k].view(np.uint32)), k
+    for k in (0, 2):
+        o = oracle.frame(cloud, *cams[k], w, h)
+        err = np.abs(images[1][k] - o["image"]).max()
+        say(f"view {k}: max |GPU - fp64| {err:.3e}")
+        assert err <= 1 / 255
+
+
 def test_autotuner_times_every_candidate(renderer, skm_mod, scene):
     cloud = scene[0]
     say("case: autotune a 6-view batch")
