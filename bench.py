@@ -539,6 +539,30 @@ def main():
     alone = r.frame_totals(reset=True)
     r.count_raster_tests(False)
 
+    # One frame slot (frames do not overlap, the latency form): the raster then draws its heaviest
+    # tiles as four quadrant blocks each. The 60 orbit frames, after 5 untimed ones.
+    one = skm.Renderer(0, frames_in_flight=1)
+    one.upload_world(cloud)
+    one.set_extent(WIDTH, HEIGHT)
+    one.apply_policy(sort_depth=32)
+    one.set_tuning(**plain_best)
+    for i in range(5):
+        one.draw(one.make_frame(*frames[i]))
+    one.finish()
+    one.frame_totals(reset=True)
+    t0 = time.perf_counter()
+    for i in range(ORBIT_FRAMES):
+        one.draw(one.make_frame(*frames[i]))
+    one.finish()
+    one_s = time.perf_counter() - t0
+    one_tot = one.frame_totals(reset=True)
+    one.close()
+    one_slot = {"frames_per_s": round(ORBIT_FRAMES / one_s, 1), "frames": one_tot["frames"],
+                "raster_ms": round(one_tot["stage_ms"][4] / max(1, one_tot["frames"]), 4),
+                "gpu_ms_per_frame": round(one_tot["stage_ms"][5] / max(1, one_tot["frames"]), 4),
+                "note": "frames_in_flight 1, orbit frames 0-59: each frame waits for the previous; the raster "
+                        "splits its heaviest tiles (whole order buckets, up to 1024 tiles) into quadrant blocks"}
+
     n_frames = max(1, tot["frames"])
     V = tot["visible"] / n_frames
     P = tot["pairs"] / n_frames
@@ -677,7 +701,7 @@ def main():
         "visible_per_frame": int(V), "pairs_per_frame": int(P),
         "gpu_ms_per_frame": round(tot["stage_ms"][5] / n_frames, 4),
         "stages": stages, "roofline": roofline, "frame_roofline": frame_roofline, "pixel_err_fp64": pixel_err,
-        "depth_sort_in_frame": sort_in_frame,
+        "depth_sort_in_frame": sort_in_frame, "one_slot": one_slot,
         "cpu_baseline": cpu, "radix_sort": sort_leg, "batched": batched,
         "serving": served, "lod": lod,
         "device": device, "scene_generation_s": round(gen_s, 2), "load_reorder_s": round(reorder_s, 3),
