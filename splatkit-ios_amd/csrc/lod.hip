@@ -306,12 +306,27 @@ __global__ __launch_bounds__(256) void k_lod_compact(LodArgs a, uint32_t round) 
 }
 
 // emitSplatLOD (SplatLOD.metal:210-220): one packet per block iteration.
+// One wave per packet (a packet holds 33 to a cluster's worth of leaves, 256 at most in the trees the
+// builder makes), four lanes' worth of leaf references loaded before any index is stored, so each wave
+// waits for about one round trip per packet and thousands of packets are in flight at once.
 __global__ __launch_bounds__(256) void k_lod_emit(LodArgs a) {
   const LodState s = a.state[a.rounds];
-  for (uint32_t p = blockIdx.x; p < s.packets; p += gridDim.x) {
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6), lane = lane_id();
+  for (uint32_t p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); p < s.packets; p += waves) {
     const uint4 packet = a.packets[p];
-    for (uint32_t k = threadIdx.x; k < packet.y; k += blockDim.x)
-      a.indices[packet.z + k] = packet.w ? a.leaves[packet.x + k] : packet.x;
+    for (uint32_t k0 = 0; k0 < packet.y; k0 += 256u) {
+      uint32_t v[4];
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t k = k0 + j * 64u + lane;
+        v[j] = k < packet.y ? (packet.w ? a.leaves[packet.x + k] : packet.x) : 0u;
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t k = k0 + j * 64u + lane;
+        if (k < packet.y) a.indices[packet.z + k] = v[j];
+      }
+    }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl->final_limited = a.ctrl->limited;
 }
@@ -326,7 +341,7 @@ hipError_t launch_lod(const LodArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_lod_budget, dim3(1), dim3(64), 0, s, a, r);
     hipLaunchKernelGGL(k_lod_compact, dim3(blocks ? blocks : 1u), dim3(256), 0, s, a, r);
   }
-  hipLaunchKernelGGL(k_lod_emit, dim3(256), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_lod_emit, dim3(1024), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
