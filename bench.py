@@ -50,6 +50,17 @@ def measured_valu(kernel, steps):
     return sum(per_frame[i % ORBIT_FRAMES] for i in range(steps)) / steps
 
 
+def measured_one_slot_valu():
+    """Wave64 VALU instructions per raster launch of the 60 orbit frames drawn with one frame slot (the
+    split raster), from the committed counter run (profiles/sq_counters_3M_sh3_1080p.json, "one_slot"),
+    or None when absent."""
+    path = os.path.join(ROOT, "profiles", "sq_counters_3M_sh3_1080p.json")
+    if not os.path.exists(path):
+        return None
+    per_frame = json.load(open(path)).get("one_slot", {}).get("valu_by_orbit_frame")
+    return sum(per_frame) / len(per_frame) if per_frame else None
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` at this workload, from the committed counter runs (fetched
     and written bytes, each scaled by a 1 GiB copy's reading), or None when they are absent."""
@@ -272,7 +283,24 @@ def counter_leg(skm, leg):
     counter run (which serialises dispatches). batch: two 64-view launches of the 3M scene (the second is
     the measured one); loop: the same 64 views drawn one per launch twice (the first pass lets the pair
     buffers grow); serve / serve_single: 288 requests of the 8-scene serving mix, the first 32 a warm-up,
-    with continuous batching (4 per launch, 3 in flight) or one request per launch."""
+    with continuous batching (4 per launch, 3 in flight) or one request per launch; one_slot: the 60 orbit
+    frames with one frame slot (the split raster), after 5 untimed ones."""
+    if leg == "one_slot":
+        cloud = skm.scene_c1(N_SPLATS, SH_DEGREE)
+        r = skm.Renderer(0, frames_in_flight=1)
+        spatially_ordered(r, cloud)
+        r.upload_world(cloud)
+        r.set_extent(WIDTH, HEIGHT)
+        r.apply_policy(sort_depth=32)
+        r.set_tuning(tile_size=16, raster_px=1, sort_kernel=0, radix_bits=7)
+        frames = [skm.orbit_camera(f, WIDTH, HEIGHT) for f in range(ORBIT_FRAMES)]
+        for i in range(5):
+            r.render(*frames[i])
+        r.frame_totals(reset=True)
+        for f in frames:
+            r.draw(r.make_frame(*f))
+        r.finish()
+        return {"leg": leg, **r.frame_totals(reset=True)}
     if leg in ("batch", "loop"):
         cloud = skm.scene_c1(N_SPLATS, SH_DEGREE)
         r = skm.Renderer(0, frames_in_flight=FRAMES_IN_FLIGHT)
@@ -477,7 +505,7 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a known-size kernel for the HBM "
                          "byte counters of a profiler)")
-    ap.add_argument("--counter-leg", choices=("batch", "loop", "serve", "serve_single"), default="",
+    ap.add_argument("--counter-leg", choices=("batch", "loop", "serve", "serve_single", "one_slot"), default="",
                     help="run one leg's work once for a profiler's counter run and exit (no JSON line)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -563,11 +591,18 @@ def main():
     one_s = time.perf_counter() - t0
     one_tot = one.frame_totals(reset=True)
     one.close()
+    one_raster_ms = one_tot["stage_ms"][4] / max(1, one_tot["frames"])
+    one_valu = measured_one_slot_valu()
     one_slot = {"frames_per_s": round(ORBIT_FRAMES / one_s, 1), "frames": one_tot["frames"],
-                "raster_ms": round(one_tot["stage_ms"][4] / max(1, one_tot["frames"]), 4),
+                "raster_ms": round(one_raster_ms, 4),
+                "raster_valu_instructions_per_launch": round(one_valu) if one_valu else None,
+                "raster_valu_issue_frac": (round(one_valu * 64 / (one_raster_ms * 1e-3) / VALU_LANE_OPS_PEAK, 4)
+                                           if one_valu and one_raster_ms > 0 else None),
                 "gpu_ms_per_frame": round(one_tot["stage_ms"][5] / max(1, one_tot["frames"]), 4),
                 "note": "frames_in_flight 1, orbit frames 0-59: each frame waits for the previous; the raster "
-                        "splits its heaviest tiles (whole order buckets, up to 1024 tiles) into quadrant blocks"}
+                        "splits its heaviest tiles (whole order buckets, up to 1024 tiles) into quadrant blocks. "
+                        "raster_valu_issue_frac: SQ_INSTS_VALU x 4 / (1024 SIMDs x 2.4 GHz x raster_ms), the "
+                        "instruction count from the counter run of these frames (profiles/sq_counters_3M_sh3_1080p.json)"}
 
     n_frames = max(1, tot["frames"])
     V = tot["visible"] / n_frames
