@@ -282,3 +282,28 @@ def test_c1_frame_matches_oracle(renderer, skm_mod, oracle):
     g = gpu_frame(renderer, skm_mod, cloud, view, proj, eye, 1920, 1080)
     o = oracle.frame(cloud, view, proj, eye, 1920, 1080)
     assert_parity(g, o, label="C1")
+
+
+def test_c1_orbit_frames_with_one_slot_match_oracle(renderer, skm_mod, oracle):
+    """Three more orbit frames of the 3M-splat SH3 scene at 1920x1080, drawn with one frame slot, where
+    the raster draws the heaviest tiles as four quadrant blocks: records, order, pairs and ranges
+    bit-exact against the oracle, pixels within 1/255 of FP64 and bit for bit the image that the
+    several-slot renderer (whole-tile raster) draws of the same frame."""
+    cloud = skm_mod.scene_c1()
+    renderer.reorder_spatially(cloud)
+    renderer.upload_world(cloud)
+    one = skm_mod.Renderer(frames_in_flight=1)
+    try:
+        one.upload_world(cloud)
+        for k in (15, 30, 45):
+            say(f"case: C1 3M SH3 1920x1080 orbit frame {k}, one frame slot")
+            view, proj, eye = skm_mod.orbit_camera(k, 1920, 1080)
+            g = gpu_frame(one, skm_mod, cloud, view, proj, eye, 1920, 1080, upload=False)
+            per = g["ranges"][:, 1].astype(np.int64) - g["ranges"][:, 0]
+            say(f"heaviest tile {per.max()} pairs, tiles over 1024 pairs {(per > 1024).sum()}")
+            o = oracle.frame(cloud, view, proj, eye, 1920, 1080)
+            assert_parity(g, o, label=f"C1 frame {k}")
+            whole = gpu_frame(renderer, skm_mod, cloud, view, proj, eye, 1920, 1080, upload=False)
+            assert np.array_equal(whole["image"].view(np.uint32), g["image"].view(np.uint32)), k
+    finally:
+        one.close()
