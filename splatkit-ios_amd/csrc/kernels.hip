@@ -58,6 +58,25 @@ __device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
+// Lanes of `live` whose BITS-bit digit equals this lane's d. Bit slice b: the lanes whose bit b is
+// set (one ballot); a lane differs from lane L in bit b exactly where (ballot XOR its own bit
+// spread over 32 bits) is set, so the lanes that differ in any bit are the OR of those words and the
+// peers are the rest. Each slice is a sign-extending bit extract, the ballot's compare and one
+// three-input logic op per half-word.
+template <int BITS>
+__device__ __forceinline__ uint64_t digit_peers(uint32_t d, uint64_t live) {
+  uint32_t diff_lo = 0u, diff_hi = 0u;
+#pragma unroll
+  for (int bit = 0; bit < BITS; ++bit) {
+    uint32_t spread = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - bit)) >> 31);
+    asm volatile("" : "+v"(spread));  // one extract serves the ballot and the XOR
+    const uint64_t vote = __ballot(spread != 0u);
+    diff_lo |= static_cast<uint32_t>(vote) ^ spread;
+    diff_hi |= static_cast<uint32_t>(vote >> 32) ^ spread;
+  }
+  return live & ~((static_cast<uint64_t>(diff_hi) << 32) | diff_lo);
+}
+
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
   const uint32_t lane = lane_id();
 #pragma unroll
@@ -615,17 +634,12 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
   for (uint32_t j = 0; j < kSortItems; ++j) {
     const bool valid = (valid_rows >> j) & 1u;
     const uint32_t d = (key[j] >> shift) & mask;  // mask < 2^BITS
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (uint32_t bit = 0; bit < static_cast<uint32_t>(BITS); ++bit) {
-      const bool set = (d >> bit) & 1u;
-      const uint64_t vote = __ballot(set);
-      peers &= set ? vote : ~vote;
-    }
+    const uint64_t peers = digit_peers<BITS>(d, __ballot(valid));
+    const uint32_t below = rank_below(peers);
     const uint32_t before = s_wave_count[wave][d];
-    rank[j] = before + rank_below(peers);
-    const uint32_t leader = peers != 0ull ? static_cast<uint32_t>(__builtin_ctzll(peers)) : 64u;
-    if (valid && lane == leader) s_wave_count[wave][d] = before + static_cast<uint32_t>(__popcll(peers));
+    rank[j] = before + below;
+    // The lowest lane of each digit (no peer below it) stores the digit's new count.
+    if (valid && below == 0u) s_wave_count[wave][d] = before + static_cast<uint32_t>(__popcll(peers));
   }
   __syncthreads();
   // Per digit (thread = digit): offsets of each wave, the tile's total, and its start in the tile.
@@ -694,11 +708,31 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
     }
   }
   __syncthreads();
-  for (uint32_t i = tid; i < tile_total; i += 256u) {
-    const uint32_t k = s_keys[i];
-    const uint32_t dst = s_base[(k >> shift) & mask] + i;
-    if (keys_out != nullptr) keys_out[dst] = k;
-    vals_out[dst] = s_vals[i];
+  // Write the staged tile: every LDS read of the thread's keys first, then their bases, then the
+  // stores (one round trip of each instead of two per key).
+#pragma unroll
+  for (uint32_t j = 0; j < kSortItems; ++j) {
+    key[j] = s_keys[j * 256u + tid];
+    val[j] = s_vals[j * 256u + tid];
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kSortItems; ++j) rank[j] = s_base[(key[j] >> shift) & mask] + j * 256u + tid;
+  if (tile_total == kSortTile) {  // a full tile (every tile of a packed input but the last): no masks
+    if (keys_out != nullptr) {
+#pragma unroll
+      for (uint32_t j = 0; j < kSortItems; ++j) keys_out[rank[j]] = key[j];
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kSortItems; ++j) vals_out[rank[j]] = val[j];
+  } else {
+    if (keys_out != nullptr) {
+#pragma unroll
+      for (uint32_t j = 0; j < kSortItems; ++j)
+        if (j * 256u + tid < tile_total) keys_out[rank[j]] = key[j];
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kSortItems; ++j)
+      if (j * 256u + tid < tile_total) vals_out[rank[j]] = val[j];
   }
 }
 
