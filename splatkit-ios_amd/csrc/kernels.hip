@@ -77,13 +77,23 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, uint64_t live) {
   return live & ~((static_cast<uint64_t>(diff_hi) << 32) | diff_lo);
 }
 
+// Inclusive sum over the wave in lane order, by data-parallel moves between lanes (no LDS round
+// trips): shifts by 1, 2, 4 and 8 lanes within each row of 16, then row 0's last lane added to row 1
+// and row 2's to row 3 (row_bcast:15), then lane 31 to rows 2 and 3 (row_bcast:31).
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (uint32_t d = 1; d < 64; d <<= 1) {
-    const uint32_t t = __shfl_up(v, d, 64);
-    if (lane >= d) v += t;
-  }
+  const uint32_t lane = lane_id(), row_lane = lane & 15u;
+  uint32_t t = __builtin_amdgcn_mov_dpp(v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  if (row_lane >= 1u) v += t;
+  t = __builtin_amdgcn_mov_dpp(v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  if (row_lane >= 2u) v += t;
+  t = __builtin_amdgcn_mov_dpp(v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  if (row_lane >= 4u) v += t;
+  t = __builtin_amdgcn_mov_dpp(v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  if (row_lane >= 8u) v += t;
+  t = __builtin_amdgcn_mov_dpp(v, 0x142, 0xf, 0xf, false);  // row_bcast:15
+  if ((lane & 31u) >= 16u) v += t;
+  t = __builtin_amdgcn_mov_dpp(v, 0x143, 0xf, 0xf, false);  // row_bcast:31
+  if (lane >= 32u) v += t;
   return v;
 }
 
@@ -737,12 +747,19 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
 }
 
 // Reduce then scan, step 1: the digit counts of each kSortTile-key tile, digit-major
-// (counts[d * stride + tile]), from per-wave LDS histograms.
+// (counts[d * stride + tile]), from per-wave LDS histograms. Low digits are close to random in a
+// pass's input order (the pairs of one survivor have distinct tiles, and every later pass reads keys
+// in the previous digit's order), so each key adds itself. A key's top digit takes few values (the
+// exponent of a depth, the row band of a tile), and many lanes of a row adding to one LDS word are
+// serialised, so with RUNS a row adds once per run of equal digits instead: a lane starts a run when
+// its digit differs from the previous lane's, and the run ends at the next start or at the row's
+// last key (a row's keys are a prefix of its lanes, in both the packed and the gapped layout).
+template <bool RUNS>
 __global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
                                                      uint32_t shift, uint32_t mask, uint32_t* __restrict__ counts,
                                                      uint32_t stride, const uint32_t* __restrict__ seg) {
   __shared__ uint32_t s_h[4][kRadix];
-  const uint32_t tid = threadIdx.x, wave = tid >> 6;
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
   const uint32_t n = *n_ptr;
   const uint32_t blocks = (n + kSortTile - 1u) / kSortTile;
   if (blockIdx.x >= blocks) return;
@@ -754,72 +771,58 @@ __global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict_
   uint32_t key[kSortItems];
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) key[j] = keys[min(base + j * 256u + tid, n - 1u)];
-  // Runs of equal digits in a wave row (common: pairs come out a survivor at a time, and high digits
-  // change slowly) add once per run: a lane starts a run when its digit differs from the previous
-  // lane's, and the run ends at the next start or at the row's last key (a row's keys are a prefix
-  // of its lanes, in both the packed and the gapped layout).
-  const uint32_t lane = tid & 63u;
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) {
     const uint32_t row = base + j * 256u + wave * 64u;
     const bool valid = sort_valid(row + lane, n, row_limit(seg, row, n));
     const uint32_t d = (key[j] >> shift) & mask;
-    const uint32_t prev = __shfl_up(d, 1u, 64);
-    const bool start = valid && (lane == 0u || prev != d);
-    const uint64_t starts = __ballot(start);
-    const uint32_t keys_in_row = static_cast<uint32_t>(__popcll(__ballot(valid)));
-    const uint64_t later = starts & ~((2ull << lane) - 1ull);  // starts after this lane
-    const uint32_t end = later != 0ull ? static_cast<uint32_t>(__builtin_ctzll(later)) : keys_in_row;
-    if (start) atomicAdd(&s_h[wave][d], end - lane);
+    if constexpr (RUNS) {
+      const uint32_t prev = __shfl_up(d, 1u, 64);
+      const bool start = valid && (lane == 0u || prev != d);
+      const uint64_t starts = __ballot(start);
+      const uint32_t keys_in_row = static_cast<uint32_t>(__popcll(__ballot(valid)));
+      const uint64_t later = starts & ~((2ull << lane) - 1ull);  // starts after this lane
+      const uint32_t end = later != 0ull ? static_cast<uint32_t>(__builtin_ctzll(later)) : keys_in_row;
+      if (start) atomicAdd(&s_h[wave][d], end - lane);
+    } else {
+      if (valid) atomicAdd(&s_h[wave][d], 1u);
+    }
   }
   __syncthreads();
   counts[static_cast<size_t>(tid) * stride + b] = s_h[0][tid] + s_h[1][tid] + s_h[2][tid] + s_h[3][tid];
 }
 
 // Reduce then scan, step 2: block d scans digit d's row of tile counts in place (exclusive, from 0)
-// and stores the row's total in totals[d]. Per round, wave w takes kPer rows of 64 consecutive tiles
-// (lanes read consecutive words, every load in flight together) and scans them in order with one
-// wave-wide scan per row; the waves' runs are then offset by the runs before them.
+// and stores the row's total in totals[d]. Per round, thread i takes kPer consecutive tiles: it sums
+// them in registers, one block scan offsets the threads' sums, and each thread writes its tiles'
+// prefixes (one scan per round instead of one per 64 tiles).
 __global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ counts, const uint32_t* n_ptr,
                                                     uint32_t* __restrict__ totals, uint32_t stride) {
   constexpr uint32_t kPer = 16;
   constexpr uint32_t kRound = 256u * kPer;
-  __shared__ uint32_t s_run[4];
-  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id(), d = blockIdx.x;
+  __shared__ uint32_t s_scan[4];
+  const uint32_t tid = threadIdx.x, d = blockIdx.x;
   const uint32_t n = *n_ptr;
   const uint32_t tiles = (n + kSortTile - 1u) / kSortTile;
   uint32_t carry = 0;
   uint32_t* row = counts + static_cast<size_t>(d) * stride;
   for (uint32_t t0 = 0; t0 < tiles; t0 += kRound) {
-    const uint32_t w0 = t0 + wave * (kPer * 64u);
+    const uint32_t first = t0 + tid * kPer;
     uint32_t v[kPer];
 #pragma unroll
-    for (uint32_t i = 0; i < kPer; ++i) {
-      const uint32_t t = w0 + i * 64u + lane;
-      v[i] = t < tiles ? row[t] : 0u;
-    }
-    uint32_t run = 0;  // this wave's sum over its rows so far
+    for (uint32_t i = 0; i < kPer; ++i) v[i] = first + i < tiles ? row[first + i] : 0u;
+    uint32_t sum = 0;
 #pragma unroll
     for (uint32_t i = 0; i < kPer; ++i) {
-      const uint32_t incl = wave_inclusive_scan(v[i]);
-      const uint32_t excl = run + incl - v[i];
-      run += static_cast<uint32_t>(__shfl(incl, 63, 64));
-      v[i] = excl;
+      const uint32_t c = v[i];
+      v[i] = sum;
+      sum += c;
     }
-    if (lane == 0u) s_run[wave] = run;
-    __syncthreads();
-    uint32_t before = carry, all = 0;
+    uint32_t all;
+    const uint32_t before = carry + block_exclusive_scan(sum, s_scan, &all);
 #pragma unroll
-    for (uint32_t w = 0; w < 4; ++w) {
-      before += w < wave ? s_run[w] : 0u;
-      all += s_run[w];
-    }
-    __syncthreads();
-#pragma unroll
-    for (uint32_t i = 0; i < kPer; ++i) {
-      const uint32_t t = w0 + i * 64u + lane;
-      if (t < tiles) row[t] = before + v[i];
-    }
+    for (uint32_t i = 0; i < kPer; ++i)
+      if (first + i < tiles) row[first + i] = before + v[i];
     carry += all;
   }
   if (tid == 0) totals[d] = carry;
@@ -1547,13 +1550,18 @@ template <bool LOOKBACK, int BITS>
 void sort_pass_bits(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
                     const uint32_t* n_ptr, uint32_t tiles, uint32_t* hist, uint32_t shift, uint32_t bits,
                     uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass, uint32_t* counts,
-                    const uint32_t* seg, hipStream_t s) {
+                    const uint32_t* seg, bool runs, hipStream_t s) {
   const uint32_t mask = (1u << bits) - 1u;
   if constexpr (LOOKBACK) {
     hipLaunchKernelGGL((k_onesweep<true, BITS>), dim3(tiles), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist,
                        shift, mask, status, ctrl, ticket, tag_pass, nullptr, 0u, seg);
   } else {
-    hipLaunchKernelGGL(k_tile_counts, dim3(tiles), dim3(256), 0, s, kin, n_ptr, shift, mask, counts, tiles, seg);
+    if (runs)
+      hipLaunchKernelGGL(k_tile_counts<true>, dim3(tiles), dim3(256), 0, s, kin, n_ptr, shift, mask, counts, tiles,
+                         seg);
+    else
+      hipLaunchKernelGGL(k_tile_counts<false>, dim3(tiles), dim3(256), 0, s, kin, n_ptr, shift, mask, counts, tiles,
+                         seg);
     hipLaunchKernelGGL(k_digit_scan, dim3(1u << bits), dim3(256), 0, s, counts, n_ptr, hist, tiles);
     hipLaunchKernelGGL((k_onesweep<false, BITS>), dim3(tiles), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist,
                        shift, mask, status, ctrl, ticket, tag_pass, counts, tiles, seg);
@@ -1564,20 +1572,20 @@ template <bool LOOKBACK>
 void sort_pass_variant(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
                        const uint32_t* n_ptr, uint32_t tiles, uint32_t* hist, uint32_t shift, uint32_t bits,
                        uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass, uint32_t* counts,
-                       const uint32_t* seg, hipStream_t s) {
+                       const uint32_t* seg, bool runs, hipStream_t s) {
   if (bits <= 6)
     sort_pass_bits<LOOKBACK, 6>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
-                                tag_pass, counts, seg, s);
+                                tag_pass, counts, seg, runs, s);
   else if (bits == 7)
     sort_pass_bits<LOOKBACK, 7>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
-                                tag_pass, counts, seg, s);
+                                tag_pass, counts, seg, runs, s);
   else
     sort_pass_bits<LOOKBACK, 8>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
-                                tag_pass, counts, seg, s);
+                                tag_pass, counts, seg, runs, s);
 }
 
 // Pass p of a sort under its digit plan; `seg` is the cull's gapped layout (first depth pass) or
-// null.
+// null. The last pass, on the key's top digit, counts runs of equal digits.
 void sort_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, const uint32_t* n_ptr,
                uint32_t capacity, uint32_t* hist, const DigitPlan& plan, int p, uint64_t* status, Ctrl* ctrl,
                uint32_t ticket, uint32_t tag_pass, uint32_t* counts, int variant, const uint32_t* seg,
@@ -1585,10 +1593,10 @@ void sort_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_
   const uint32_t tiles = grid_for(capacity, kSortTile);
   if (variant == kSortOnesweep)
     sort_pass_variant<true>(kin, vin, kout, vout, n_ptr, tiles, hist, plan.shift[p], plan.bits[p], status, ctrl,
-                            ticket, tag_pass, counts, seg, s);
+                            ticket, tag_pass, counts, seg, p + 1 == plan.passes, s);
   else
     sort_pass_variant<false>(kin, vin, kout, vout, n_ptr, tiles, hist, plan.shift[p], plan.bits[p], status, ctrl,
-                             ticket, tag_pass, counts, seg, s);
+                             ticket, tag_pass, counts, seg, p + 1 == plan.passes, s);
 }
 
 }  // namespace
@@ -1657,13 +1665,15 @@ hipError_t launch_argsort(Ctrl* ctrl, uint32_t* keys[2], uint32_t* vals[2], cons
   const int passes = plan.passes;
   // keys[0] holds the input; passes alternate keys[0] -> keys[1] -> keys[0] ...; the order ends in
   // vals[(passes - 1) & 1].
-  hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, stream, ctrl, hist, 4u * 256u, nullptr, 0u, nullptr,
-                     0u, nullptr, nullptr);
-  // The reduce-then-scan passes take each digit's total from their own tile counts; the onesweep
-  // passes need every pass's histogram first.
-  if (variant == kSortOnesweep)
+  // The reduce-then-scan passes take each digit's total from their own tile counts and use neither
+  // the control block nor the histograms; the onesweep passes need a fresh epoch, tickets and every
+  // pass's histogram first.
+  if (variant == kSortOnesweep) {
+    hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, stream, ctrl, hist, 4u * 256u, nullptr, 0u, nullptr,
+                       0u, nullptr, nullptr);
     hipLaunchKernelGGL(k_hist, dim3(hist_grid(capacity)), dim3(256), 0, stream, keys[0], n_dev, nullptr, hist,
                        plan);
+  }
   for (int p = 0; p < passes; ++p) {
     const uint32_t* kin = keys[p & 1];
     uint32_t* kout = (p + 1 < passes) ? keys[(p + 1) & 1] : nullptr;

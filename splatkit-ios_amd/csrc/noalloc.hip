@@ -99,8 +99,8 @@ extern "C" skm_status skm_argsort_u32_device_static(skm_renderer* r, void* keys_
     hipLaunchKernelGGL(k_store_word, dim3(1), dim3(1), 0, stream, n_dev, n);
     e = hipGetLastError();
   }
-  // The reduce-then-scan passes: every word they read is written earlier in the same call (the
-  // prologue clears the control block and the digit totals, each pass writes all its tile counts),
+  // The reduce-then-scan passes: every word they read is written earlier in the same call (each
+  // pass writes all its tile counts, and its digit scan the totals of every digit the pass reads),
   // so nothing the previous call left matters.
   if (e == hipSuccess)
     e = launch_argsort(ctrl, keys, vals, n_dev, n, plan, hist, nullptr, counts, kSortReduceScan, stream);
