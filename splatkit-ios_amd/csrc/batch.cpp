@@ -107,7 +107,8 @@ hipError_t Batch::ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_
   const size_t sort = static_cast<size_t>(std::max(lane_capacity_, pair_capacity_) / kSortTile + 2) * kRadix;
   if (e == hipSuccess && sort > sort_words_) {
     need(sort_status_, sort * 8);
-    need(sort_counts_, sort * 4);
+    // The tile counts, then the binning's block starts (3 per sort tile of survivors, and 2 spare).
+    need(sort_counts_, (sort + sort / kRadix * 3 + 2) * 4);
     if (e == hipSuccess) e = hipMemsetAsync(sort_status_.ptr, 0, sort_status_.bytes, stream);
     if (e == hipSuccess) sort_words_ = sort;
   }
@@ -269,6 +270,7 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
   a.sort_variant = tuning.sort_kernel == 1 ? kSortOnesweep : kSortReduceScan;
   a.raster_px = px;
   a.sort_counts = sort_counts_.as<uint32_t>();
+  a.bin_starts = a.sort_counts + sort_words_;
   const DigitPlan depth_plan = make_digit_plan(policy.key16 ? 16u : 32u, 8u);
   const DigitPlan tile_plan = make_digit_plan(key_bits_for(static_cast<uint32_t>(tiles - 1)), radix_bits);
   depth_passes_ = depth_plan.passes;

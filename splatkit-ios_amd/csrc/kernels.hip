@@ -16,6 +16,7 @@
 #include "skm_internal.h"
 #include "skm_lod_internal.h"
 #include "skm_math.h"
+#include "skm_wave.h"
 
 namespace skm {
 namespace {
@@ -39,7 +40,6 @@ __device__ __forceinline__ uint64_t status_load(uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
 // Workgroups reach the 8 XCDs of an MI355X round-robin in launch order (block i on XCD i % 8), and
 // each XCD has its own L2. Block i of g takes logical block xcd_tile(i, g): XCD x works through one
@@ -52,12 +52,6 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t i, uint32_t g) {
   const uint32_t x = i % kXcds, q = g / kXcds, r = g % kXcds;
   return x * q + min(x, r) + i / kXcds;
 }
-// Number of set bits of `mask` below this lane.
-__device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
-  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
-                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-}
-
 // Lanes of `live` whose BITS-bit digit equals this lane's d. Bit slice b: the lanes whose bit b is
 // set (one ballot); a lane differs from lane L in bit b exactly where (ballot XOR its own bit
 // spread over 32 bits) is set, so the lanes that differ in any bit are the OR of those words and the
@@ -75,44 +69,6 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, uint64_t live) {
     diff_hi |= static_cast<uint32_t>(vote >> 32) ^ spread;
   }
   return live & ~((static_cast<uint64_t>(diff_hi) << 32) | diff_lo);
-}
-
-// Inclusive sum over the wave in lane order, by data-parallel moves between lanes (no LDS round
-// trips): shifts by 1, 2, 4 and 8 lanes within each row of 16, then row 0's last lane added to row 1
-// and row 2's to row 3 (row_bcast:15), then lane 31 to rows 2 and 3 (row_bcast:31).
-__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
-  const uint32_t lane = lane_id(), row_lane = lane & 15u;
-  uint32_t t = __builtin_amdgcn_mov_dpp(v, 0x111, 0xf, 0xf, false);  // row_shr:1
-  if (row_lane >= 1u) v += t;
-  t = __builtin_amdgcn_mov_dpp(v, 0x112, 0xf, 0xf, false);  // row_shr:2
-  if (row_lane >= 2u) v += t;
-  t = __builtin_amdgcn_mov_dpp(v, 0x114, 0xf, 0xf, false);  // row_shr:4
-  if (row_lane >= 4u) v += t;
-  t = __builtin_amdgcn_mov_dpp(v, 0x118, 0xf, 0xf, false);  // row_shr:8
-  if (row_lane >= 8u) v += t;
-  t = __builtin_amdgcn_mov_dpp(v, 0x142, 0xf, 0xf, false);  // row_bcast:15
-  if ((lane & 31u) >= 16u) v += t;
-  t = __builtin_amdgcn_mov_dpp(v, 0x143, 0xf, 0xf, false);  // row_bcast:31
-  if (lane >= 32u) v += t;
-  return v;
-}
-
-// Exclusive scan of one value per thread over a 256-thread block; `scratch` holds 4 words.
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* scratch, uint32_t* total) {
-  const uint32_t incl = wave_inclusive_scan(v);
-  const uint32_t wave = threadIdx.x >> 6;
-  if (lane_id() == 63) scratch[wave] = incl;
-  __syncthreads();
-  uint32_t before = 0, all = 0;
-#pragma unroll
-  for (uint32_t w = 0; w < 4; ++w) {
-    const uint32_t c = scratch[w];
-    before += (w < wave) ? c : 0u;
-    all += c;
-  }
-  __syncthreads();
-  *total = all;
-  return before + incl - v;
 }
 
 __device__ __forceinline__ void mat_vec(const float* m, float x, float y, float z, float w, float out[4]) {
@@ -594,8 +550,11 @@ constexpr uint32_t kLookback = 16;
 //     status words (decoupled look-back), one kernel per pass;
 //   !LOOKBACK (reduce then scan): k_tile_counts and k_digit_scan have already written the offset of
 //     digit d of tile t within digit d at offsets[d * stride + t] and digit d's total at hist[d].
+#ifndef SKM_SCATTER_WAVES
+#define SKM_SCATTER_WAVES 4  // waves per SIMD the scatter's register budget is held to
+#endif
 template <bool LOOKBACK, int BITS>
-__global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ keys_in,
+__global__ __launch_bounds__(256, SKM_SCATTER_WAVES) void k_onesweep(const uint32_t* __restrict__ keys_in,
                                                   const uint32_t* __restrict__ vals_in,
                                                   uint32_t* __restrict__ keys_out,
                                                   uint32_t* __restrict__ vals_out, const uint32_t* n_ptr,
@@ -849,7 +808,7 @@ __device__ __forceinline__ uint64_t wave_inclusive_scan64(uint64_t v) {
 
 // Binning, first pass (reduce then scan) over the survivors in depth order, kBinItems * 256 per
 // block: a block gathers its survivors' tile rectangles into depth order (low words in keys[0], high
-// words in keys[1]: the depth sort is done with both) and stores its pair total in sort_counts[b].
+// words in keys[1]: the depth sort is done with both) and stores its pair total in bin_starts[b].
 __global__ __launch_bounds__(256) void k_bin_count(FrameArgs a, const uint32_t* __restrict__ order) {
   constexpr uint32_t kTile = 256u * kBinItems;
   __shared__ uint32_t s_scan[4];
@@ -877,7 +836,7 @@ __global__ __launch_bounds__(256) void k_bin_count(FrameArgs a, const uint32_t* 
   }
   uint32_t total;
   (void)block_exclusive_scan(sum, s_scan, &total);
-  if (tid == 0) a.sort_counts[b] = total;
+  if (tid == 0) a.bin_starts[b] = total;
 }
 
 // Binning, the scan: one block turns the block totals into block starts in place (exclusive,
@@ -895,7 +854,7 @@ __global__ __launch_bounds__(256) void k_bin_scan(FrameArgs a) {
 #pragma unroll
     for (uint32_t i = 0; i < kPer; ++i) {
       const uint32_t t = t0 + tid * kPer + i;
-      v[i] = t < blocks ? a.sort_counts[t] : 0u;
+      v[i] = t < blocks ? a.bin_starts[t] : 0u;
       sum += v[i];
     }
     const uint64_t incl = wave_inclusive_scan64(sum);
@@ -912,7 +871,7 @@ __global__ __launch_bounds__(256) void k_bin_scan(FrameArgs a) {
 #pragma unroll
     for (uint32_t i = 0; i < kPer; ++i) {
       const uint32_t t = t0 + tid * kPer + i;
-      if (t < blocks) a.sort_counts[t] = run > 0xffffffffull ? 0xffffffffu : static_cast<uint32_t>(run);
+      if (t < blocks) a.bin_starts[t] = run > 0xffffffffull ? 0xffffffffu : static_cast<uint32_t>(run);
       run += v[i];
     }
     carry += all;
@@ -948,13 +907,18 @@ __device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t v) {
 // chunk are a window of 64 slots from the previous chunk's last owner, whose starts are marked in
 // LDS and spread by a wave max-scan (a binary search covers the first chunk and any pair the window
 // misses).
-constexpr uint32_t kEmitChunk = 4096;
-__global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __restrict__ order) {
+//
+// An emission block's pairs are exactly one tile of the tile sort's first radix pass, so with the
+// reduce-then-scan passes (`digit_mask` != 0) it also counts that pass's digits (the tile id's low
+// bits) in LDS and writes the tile's counts, which the pass would otherwise read every key again for.
+constexpr uint32_t kEmitChunk = kSortTile;
+__global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __restrict__ order, uint32_t digit_mask) {
   constexpr uint32_t kItems = kBinItems;
   constexpr uint32_t kTile = 256u * kItems;
   __shared__ uint32_t s_scan[4];
   __shared__ uint32_t s_excl[kTile + 1u], s_val[kTile], s_origin[kTile], s_width[kTile], s_base[kTile];
   __shared__ uint32_t s_mark[4][64];
+  __shared__ uint32_t s_h[4][kRadix];
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
   const uint32_t total = a.ctrl->sort_pairs;
   const uint32_t e0 = blockIdx.x * kEmitChunk;
@@ -962,7 +926,9 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
   const uint32_t e1 = min(e0 + kEmitChunk, total);
   const uint32_t n = a.ctrl->visible;
   const uint32_t blocks = (n + kTile - 1u) / kTile;
-  const uint32_t* starts = a.sort_counts;  // exclusive pair start of each survivor block
+  const uint32_t* starts = a.bin_starts;  // exclusive pair start of each survivor block
+#pragma unroll
+  for (uint32_t w = 0; w < 4; ++w) s_h[w][tid] = 0u;  // published by the first __syncthreads below
   // The survivor block of pair e0: the last block whose start is <= e0.
   uint32_t lo = 0, hi = blocks;
   while (hi - lo > 1u) {
@@ -1064,11 +1030,19 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
         const uint32_t tx = (s_origin[owner] & 0xffffu) + static_cast<uint32_t>(rem);
         const uint32_t ty = (s_origin[owner] >> 16) + q;
         const uint32_t idx = block_start + e;  // < total <= pair capacity
-        a.pair_keys[0][idx] = s_base[owner] + ty * stride + tx;
+        const uint32_t tile = s_base[owner] + ty * stride + tx;
+        a.pair_keys[0][idx] = tile;
         a.pair_vals[0][idx] = s_val[owner];
+        if (digit_mask != 0u) atomicAdd(&s_h[wave][tile & digit_mask], 1u);
       }
       s0 = __shfl(owner, 63, 64);
     }
+  }
+  if (digit_mask != 0u) {
+    __syncthreads();
+    // counts[d * stride + tile] as k_tile_counts lays them out (stride: the pair capacity's tiles).
+    if (tid <= digit_mask)
+      a.sort_counts[static_cast<size_t>(tid) * gridDim.x + blockIdx.x] = s_h[0][tid] + s_h[1][tid] + s_h[2][tid] + s_h[3][tid];
   }
 }
 
@@ -1550,13 +1524,15 @@ template <bool LOOKBACK, int BITS>
 void sort_pass_bits(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
                     const uint32_t* n_ptr, uint32_t tiles, uint32_t* hist, uint32_t shift, uint32_t bits,
                     uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass, uint32_t* counts,
-                    const uint32_t* seg, bool runs, hipStream_t s) {
+                    const uint32_t* seg, bool runs, bool counted, hipStream_t s) {
   const uint32_t mask = (1u << bits) - 1u;
   if constexpr (LOOKBACK) {
     hipLaunchKernelGGL((k_onesweep<true, BITS>), dim3(tiles), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist,
                        shift, mask, status, ctrl, ticket, tag_pass, nullptr, 0u, seg);
   } else {
-    if (runs)
+    if (counted) {
+      // The producer of the keys has written this pass's tile counts.
+    } else if (runs)
       hipLaunchKernelGGL(k_tile_counts<true>, dim3(tiles), dim3(256), 0, s, kin, n_ptr, shift, mask, counts, tiles,
                          seg);
     else
@@ -1572,31 +1548,32 @@ template <bool LOOKBACK>
 void sort_pass_variant(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
                        const uint32_t* n_ptr, uint32_t tiles, uint32_t* hist, uint32_t shift, uint32_t bits,
                        uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass, uint32_t* counts,
-                       const uint32_t* seg, bool runs, hipStream_t s) {
+                       const uint32_t* seg, bool runs, bool counted, hipStream_t s) {
   if (bits <= 6)
     sort_pass_bits<LOOKBACK, 6>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
-                                tag_pass, counts, seg, runs, s);
+                                tag_pass, counts, seg, runs, counted, s);
   else if (bits == 7)
     sort_pass_bits<LOOKBACK, 7>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
-                                tag_pass, counts, seg, runs, s);
+                                tag_pass, counts, seg, runs, counted, s);
   else
     sort_pass_bits<LOOKBACK, 8>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
-                                tag_pass, counts, seg, runs, s);
+                                tag_pass, counts, seg, runs, counted, s);
 }
 
 // Pass p of a sort under its digit plan; `seg` is the cull's gapped layout (first depth pass) or
-// null. The last pass, on the key's top digit, counts runs of equal digits.
+// null. The last pass, on the key's top digit, counts runs of equal digits; `counted`: the keys'
+// producer has written the pass's tile counts.
 void sort_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, const uint32_t* n_ptr,
                uint32_t capacity, uint32_t* hist, const DigitPlan& plan, int p, uint64_t* status, Ctrl* ctrl,
                uint32_t ticket, uint32_t tag_pass, uint32_t* counts, int variant, const uint32_t* seg,
-               hipStream_t s) {
+               hipStream_t s, bool counted = false) {
   const uint32_t tiles = grid_for(capacity, kSortTile);
   if (variant == kSortOnesweep)
     sort_pass_variant<true>(kin, vin, kout, vout, n_ptr, tiles, hist, plan.shift[p], plan.bits[p], status, ctrl,
-                            ticket, tag_pass, counts, seg, p + 1 == plan.passes, s);
+                            ticket, tag_pass, counts, seg, p + 1 == plan.passes, counted, s);
   else
     sort_pass_variant<false>(kin, vin, kout, vout, n_ptr, tiles, hist, plan.shift[p], plan.bits[p], status, ctrl,
-                             ticket, tag_pass, counts, seg, p + 1 == plan.passes, s);
+                             ticket, tag_pass, counts, seg, p + 1 == plan.passes, counted, s);
 }
 
 }  // namespace
@@ -1729,7 +1706,10 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
   if (marks) (void)hipEventRecord(marks[3], s);
   hipLaunchKernelGGL(k_bin_count, dim3(grid_for(a.capacity, 256u * kBinItems)), dim3(256), 0, s, a, order);
   hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_bin, dim3(grid_for(a.pair_capacity, kEmitChunk)), dim3(256), 0, s, a, order);
+  // With reduce-then-scan passes the emission counts the tile sort's first digits itself.
+  const bool bin_counts = a.sort_variant != kSortOnesweep && tile_passes > 0;
+  hipLaunchKernelGGL(k_bin, dim3(grid_for(a.pair_capacity, kEmitChunk)), dim3(256), 0, s, a, order,
+                     bin_counts ? (1u << tile_plan.bits[0]) - 1u : 0u);
   const uint32_t* sort_n = &a.ctrl->sort_pairs;
   uint32_t* tile_hist = a.hist + 4 * 256;
   if (a.sort_variant == kSortOnesweep)
@@ -1738,7 +1718,7 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
   for (int p = 0; p < tile_passes; ++p) {
     sort_pass(a.pair_keys[p & 1], a.pair_vals[p & 1], a.pair_keys[(p + 1) & 1], a.pair_vals[(p + 1) & 1],
               sort_n, a.pair_capacity, tile_hist + p * 256, tile_plan, p, a.sort_status, a.ctrl, kTicketTileSort + p,
-              6u + p, a.sort_counts, a.sort_variant, nullptr, s);
+              6u + p, a.sort_counts, a.sort_variant, nullptr, s, p == 0 && bin_counts);
   }
   const int fin = tile_passes & 1;
   hipLaunchKernelGGL(k_ranges, dim3(min(grid_for(a.pair_capacity, 256), 2048u)), dim3(256), 0, s,

@@ -16,6 +16,7 @@
 // concurrent block still reads.
 #include "skm_lod_internal.h"
 #include "skm_math.h"
+#include "skm_wave.h"
 
 namespace skm {
 namespace {
@@ -26,7 +27,6 @@ constexpr uint32_t kCost = 0x3fffffffu;
 constexpr uint32_t kFlagAggregate = 1;
 constexpr uint32_t kFlagPrefix = 2;
 
-__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 __device__ __forceinline__ uint32_t make_tag(uint32_t epoch, uint32_t launch) {
   return (epoch * 128u + launch) & 0x3fffffffu;
 }
@@ -44,38 +44,10 @@ __device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
   return s < a ? 0xffffffffu : s;
 }
 
-__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (uint32_t d = 1; d < 64; d <<= 1) {
-    const uint32_t t = __shfl_up(v, d, 64);
-    if (lane >= d) v += t;
-  }
-  return v;
-}
-
 __device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
   return v;
-}
-
-// Exclusive scan over the 256-thread block; scratch holds 4 words.
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* scratch, uint32_t* total) {
-  const uint32_t incl = wave_inclusive_scan(v);
-  const uint32_t wave = threadIdx.x >> 6;
-  if (lane_id() == 63) scratch[wave] = incl;
-  __syncthreads();
-  uint32_t before = 0, all = 0;
-#pragma unroll
-  for (uint32_t w = 0; w < 4; ++w) {
-    const uint32_t c = scratch[w];
-    before += (w < wave) ? c : 0u;
-    all += c;
-  }
-  __syncthreads();
-  *total = all;
-  return before + incl - v;
 }
 
 // Decoupled look-back by one wave (see kernels.hip): the exclusive prefix of block `block`.

@@ -415,7 +415,8 @@ struct skm_renderer {
     if (blocks <= x.sort_status_blocks) return SKM_OK;
     DeviceBuffer b, c;
     if (auto st = check(b.allocate(blocks * skm::kRadix * 8u), "sort status"); st != SKM_OK) return st;
-    if (auto st = check(c.allocate(blocks * skm::kRadix * 4u), "sort counts"); st != SKM_OK) return st;
+    // The tile counts, then the binning's block starts (3 per sort tile of survivors, and 2 spare).
+    if (auto st = check(c.allocate((blocks * (skm::kRadix + 3u) + 2u) * 4u), "sort counts"); st != SKM_OK) return st;
     if (auto st = check(hipMemsetAsync(b.ptr, 0, b.bytes, st_), "clear"); st != SKM_OK) return st;
     x.sort_status = std::move(b);
     x.sort_counts = std::move(c);
@@ -994,6 +995,7 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   a.total_tiles = tiles_x * tiles_y;
   a.sort_variant = r->sort_variant;
   a.sort_counts = x.sort_counts.as<uint32_t>();
+  a.bin_starts = a.sort_counts + x.sort_status_blocks * skm::kRadix;
   a.raster_px = r->raster_px;
   s.tile = r->tile;
   s.depth_plan = skm::make_digit_plan(r->policy.key16 ? 16u : 32u, 8u);

@@ -203,6 +203,9 @@ struct FrameArgs {
   // survivor and pair capacities).
   int sort_variant;
   uint32_t* sort_counts;
+  // The binning's survivor-block totals, then starts (one word per 256 kBinItems survivors), kept
+  // apart from sort_counts: the emission pass reads them while it writes the tile sort's first counts.
+  uint32_t* bin_starts;
   // Raster block shape: pixels per lane side (1 or 2) at tile size `tile` (8, 16 or 32); the
   // supported pairs are (8, 1), (16, 1), (16, 2), (32, 2).
   uint32_t raster_px;
