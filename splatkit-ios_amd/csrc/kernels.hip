@@ -752,36 +752,49 @@ __global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict_
 }
 
 // Reduce then scan, step 2: block d scans digit d's row of tile counts in place (exclusive, from 0)
-// and stores the row's total in totals[d]. Per round, thread i takes kPer consecutive tiles: it sums
-// them in registers, one block scan offsets the threads' sums, and each thread writes its tiles'
-// prefixes (one scan per round instead of one per 64 tiles).
+// and stores the row's total in totals[d]. Per round, wave w takes kPer rows of 64 consecutive tiles
+// (lanes read consecutive words, every load in flight together) and scans them in order, one wave scan
+// (data-parallel lane moves, no LDS) per row; the waves' runs are then offset by the runs before them.
 __global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ counts, const uint32_t* n_ptr,
                                                     uint32_t* __restrict__ totals, uint32_t stride) {
   constexpr uint32_t kPer = 16;
   constexpr uint32_t kRound = 256u * kPer;
-  __shared__ uint32_t s_scan[4];
-  const uint32_t tid = threadIdx.x, d = blockIdx.x;
+  __shared__ uint32_t s_run[4];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id(), d = blockIdx.x;
   const uint32_t n = *n_ptr;
   const uint32_t tiles = (n + kSortTile - 1u) / kSortTile;
   uint32_t carry = 0;
   uint32_t* row = counts + static_cast<size_t>(d) * stride;
   for (uint32_t t0 = 0; t0 < tiles; t0 += kRound) {
-    const uint32_t first = t0 + tid * kPer;
+    const uint32_t w0 = t0 + wave * (kPer * 64u);
     uint32_t v[kPer];
 #pragma unroll
-    for (uint32_t i = 0; i < kPer; ++i) v[i] = first + i < tiles ? row[first + i] : 0u;
-    uint32_t sum = 0;
+    for (uint32_t i = 0; i < kPer; ++i) {
+      const uint32_t t = w0 + i * 64u + lane;
+      v[i] = t < tiles ? row[t] : 0u;
+    }
+    uint32_t run = 0;  // this wave's sum over its rows so far
 #pragma unroll
     for (uint32_t i = 0; i < kPer; ++i) {
-      const uint32_t c = v[i];
-      v[i] = sum;
-      sum += c;
+      const uint32_t incl = wave_inclusive_scan(v[i]);
+      const uint32_t excl = run + incl - v[i];
+      run += static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+      v[i] = excl;
     }
-    uint32_t all;
-    const uint32_t before = carry + block_exclusive_scan(sum, s_scan, &all);
+    if (lane == 0u) s_run[wave] = run;
+    __syncthreads();
+    uint32_t before = carry, all = 0;
 #pragma unroll
-    for (uint32_t i = 0; i < kPer; ++i)
-      if (first + i < tiles) row[first + i] = before + v[i];
+    for (uint32_t w = 0; w < 4; ++w) {
+      before += w < wave ? s_run[w] : 0u;
+      all += s_run[w];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+      const uint32_t t = w0 + i * 64u + lane;
+      if (t < tiles) row[t] = before + v[i];
+    }
     carry += all;
   }
   if (tid == 0) totals[d] = carry;
