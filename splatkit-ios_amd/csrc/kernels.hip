@@ -564,7 +564,6 @@ __global__ __launch_bounds__(256, SKM_SCATTER_WAVES) void k_onesweep(const uint3
                                                   uint32_t stride, const uint32_t* __restrict__ seg) {
   __shared__ uint32_t s_ticket;
   __shared__ uint32_t s_wave_count[4][kRadix];
-  __shared__ uint32_t s_digit_start[kRadix];
   __shared__ uint32_t s_base[kRadix];
   __shared__ uint32_t s_scan[4];
   __shared__ uint32_t s_keys[kSortTile];
@@ -617,11 +616,11 @@ __global__ __launch_bounds__(256, SKM_SCATTER_WAVES) void k_onesweep(const uint3
   const uint32_t count = c0 + c1 + c2 + c3;
   uint32_t tile_total;
   const uint32_t start = block_exclusive_scan(count, s_scan, &tile_total);
-  s_wave_count[0][tid] = 0;
-  s_wave_count[1][tid] = c0;
-  s_wave_count[2][tid] = c0 + c1;
-  s_wave_count[3][tid] = c0 + c1 + c2;
-  s_digit_start[tid] = start;
+  // Where wave w's keys of digit `tid` start in the staged tile.
+  s_wave_count[0][tid] = start;
+  s_wave_count[1][tid] = start + c0;
+  s_wave_count[2][tid] = start + c0 + c1;
+  s_wave_count[3][tid] = start + c0 + c1 + c2;
   if constexpr (LOOKBACK) {
     // Global start of each digit: exclusive scan of the pass histogram.
     uint32_t all;
@@ -671,7 +670,7 @@ __global__ __launch_bounds__(256, SKM_SCATTER_WAVES) void k_onesweep(const uint3
   for (uint32_t j = 0; j < kSortItems; ++j) {
     if ((valid_rows >> j) & 1u) {
       const uint32_t d = (key[j] >> shift) & mask;
-      const uint32_t local = s_digit_start[d] + s_wave_count[wave][d] + rank[j];
+      const uint32_t local = s_wave_count[wave][d] + rank[j];
       s_keys[local] = key[j];
       s_vals[local] = val[j];
     }
