@@ -1058,13 +1058,33 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
   }
 }
 
+// [start, end) of each tile's pairs by boundary detection: thread t takes keys [4t, 4t + 4) as one
+// 16-byte load (the key arrays are 256-byte aligned) and its two neighbours from the cache.
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
                                                 uint32_t* ranges) {
   const uint32_t n = *n_ptr;
-  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
-    const uint32_t t = keys[i];
-    if (i == 0 || keys[i - 1] != t) ranges[2 * t] = i;
-    if (i + 1 == n || keys[i + 1] != t) ranges[2 * t + 1] = i + 1;
+  const uint32_t i0 = (blockIdx.x * 256u + threadIdx.x) * 4u;
+  if (i0 >= n) return;
+  uint32_t k[6];
+  if (i0 + 4u <= n) {
+    const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
+    k[1] = v.x;
+    k[2] = v.y;
+    k[3] = v.z;
+    k[4] = v.w;
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) k[1 + j] = i0 + j < n ? keys[i0 + j] : ~0u;
+  }
+  k[0] = i0 > 0u ? keys[i0 - 1u] : ~0u;
+  k[5] = i0 + 4u < n ? keys[i0 + 4u] : ~0u;
+#pragma unroll
+  for (uint32_t j = 0; j < 4; ++j) {
+    const uint32_t i = i0 + j;
+    if (i >= n) break;
+    const uint32_t t = k[1 + j];
+    if (i == 0u || k[j] != t) ranges[2 * t] = i;
+    if (i + 1u == n || k[2 + j] != t) ranges[2 * t + 1] = i + 1u;
   }
 }
 
@@ -1733,7 +1753,7 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
               6u + p, a.sort_counts, a.sort_variant, nullptr, s, p == 0 && bin_counts);
   }
   const int fin = tile_passes & 1;
-  hipLaunchKernelGGL(k_ranges, dim3(min(grid_for(a.pair_capacity, 256), 2048u)), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_ranges, dim3(grid_for(a.pair_capacity, 1024u)), dim3(256), 0, s,
                      a.pair_keys[fin], sort_n, a.tile_ranges);
   if (a.tile_order != nullptr) {
     uint32_t* counters = a.hist + kOrderCounters;
