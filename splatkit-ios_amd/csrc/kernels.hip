@@ -898,15 +898,6 @@ __global__ __launch_bounds__(256) void k_bin_scan(FrameArgs a) {
   }
 }
 
-__device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t v) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (uint32_t d = 1; d < 64; d <<= 1) {
-    const uint32_t t = __shfl_up(v, d, 64);
-    if (lane >= d) v = max(v, t);
-  }
-  return v;
-}
 
 // Binning, second pass, balanced by output: emission block k writes pairs [k C, (k + 1) C) of the
 // frame (C = kEmitChunk), whatever survivors they come from (the nearest survivors come first in
@@ -1047,7 +1038,7 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
         a.pair_vals[0][idx] = s_val[owner];
         if (digit_mask != 0u) atomicAdd(&s_h[wave][tile & digit_mask], 1u);
       }
-      s0 = __shfl(owner, 63, 64);
+      s0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(owner), 63));
     }
   }
   if (digit_mask != 0u) {

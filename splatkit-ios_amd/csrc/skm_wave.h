@@ -35,6 +35,24 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
   return v;
 }
 
+// Inclusive maximum over the wave in lane order, by the same lane moves as wave_inclusive_scan.
+__device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t v) {
+  const uint32_t lane = lane_id(), row_lane = lane & 15u;
+  uint32_t t = __builtin_amdgcn_mov_dpp(v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  if (row_lane >= 1u) v = max(v, t);
+  t = __builtin_amdgcn_mov_dpp(v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  if (row_lane >= 2u) v = max(v, t);
+  t = __builtin_amdgcn_mov_dpp(v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  if (row_lane >= 4u) v = max(v, t);
+  t = __builtin_amdgcn_mov_dpp(v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  if (row_lane >= 8u) v = max(v, t);
+  t = __builtin_amdgcn_mov_dpp(v, 0x142, 0xf, 0xf, false);  // row_bcast:15
+  if ((lane & 31u) >= 16u) v = max(v, t);
+  t = __builtin_amdgcn_mov_dpp(v, 0x143, 0xf, 0xf, false);  // row_bcast:31
+  if (lane >= 32u) v = max(v, t);
+  return v;
+}
+
 // Exclusive scan of one value per thread over a 256-thread block; `scratch` holds 4 words.
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* scratch, uint32_t* total) {
   const uint32_t incl = wave_inclusive_scan(v);
