@@ -3,13 +3,13 @@
 //
 // The reference runs seven dispatches per round with indirect arguments (evaluate, scan, budget,
 // compact, scan, commit, scatter). HIP has no indirect dispatch, and a decoupled look-back turns
-// each scan into part of the kernel that produces its inputs, so a round here is three kernels:
+// each scan into part of the kernel that produces its inputs, so a round here is two kernels:
 //   k_lod_eval     per frontier cluster: drop / split / keep and the extra cost of a split, with the
 //                  frontier-order prefix of the extra cost (look-back);
-//   k_lod_budget   the largest prefix of refinements that fits the capacity (one thread, binary
-//                  search over the prefix ends, :116-132);
-//   k_lod_compact  denials, then the next frontier, leaf emission and packet offsets by look-back
-//                  scans, scattered in the same kernel, and the next round's state.
+//   k_lod_compact  the budget (the largest prefix of refinements that fits the capacity, a search
+//                  over the prefix ends, :116-132, in every block), denials, then the next frontier,
+//                  leaf emission and packet offsets by look-back scans, scattered in the same kernel,
+//                  and the next round's state.
 // k_lod_emit copies the leaf packets (> 32 leaves) cooperatively after the last round. Grids are
 // sized for the frontier capacity; blocks past the round's frontier exit at once. Per-round state
 // lives in its own record (state[r] read, state[r + 1] written), so no kernel overwrites what a
@@ -189,41 +189,53 @@ __global__ __launch_bounds__(256) void k_lod_eval(LodArgs a, uint32_t round) {
   }
 }
 
-// budgetSplatLOD (SplatLOD.metal:116-132).
-__global__ void k_lod_budget(LodArgs a, uint32_t round) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  LodState& s = a.state[round];
+// budgetSplatLOD (SplatLOD.metal:116-132), run by wave 0 of every compaction block (each gets the same
+// answer; a launch of its own would add one dependent kernel to each of the cut's rounds): the extra
+// cost the round accepts is all of it when it fits, else the inclusive prefix end of the longest run
+// of refinements, in frontier order, that fits. The prefix ends are non-decreasing, so the count of
+// those that fit comes from a 64-way search (one ballot of 64 probes per step).
+__device__ __forceinline__ uint32_t lod_budget(const LodArgs& a, const LodState& s) {
   const uint32_t available = a.capacity - s.count - s.active + s.drops;
-  if (s.total_extra <= available) {
-    s.accepted = s.total_extra;
-    return;
+  if (s.total_extra <= available) return s.total_extra;
+  const uint32_t lane = lane_id();
+  uint32_t lo = 0, hi = s.active;  // prefix ends below lo fit, from hi on they do not
+  while (hi > lo) {
+    const uint32_t step = (hi - lo + 63u) / 64u;
+    const uint32_t probe = lo + lane * step;
+    const uint64_t fits = __ballot(probe < hi && a.cost[2 * probe + 1] <= available);
+    const uint32_t m = static_cast<uint32_t>(__popcll(fits));  // a prefix of the lanes
+    if (m == 0u) break;
+    const uint32_t last = lo + (m - 1u) * step;
+    const uint32_t next = lo + m * step;
+    lo = last + 1u;
+    hi = min(hi, next);
   }
-  uint32_t low = 0, high = s.active;
-  while (low < high) {
-    const uint32_t t = low + (high - low) / 2;
-    if (a.cost[2 * t + 1] <= available) low = t + 1; else high = t;
-  }
-  s.accepted = low == 0 ? 0u : a.cost[2 * (low - 1) + 1];
+  return lo == 0u ? 0u : a.cost[2 * (lo - 1u) + 1];
 }
 
 // compactSplatLOD + commitSplatLOD + scatterSplatLOD (SplatLOD.metal:133-208).
 __global__ __launch_bounds__(256) void k_lod_compact(LodArgs a, uint32_t round) {
-  __shared__ uint32_t s_ticket, s_prefix[3], s_scan[4];
+  __shared__ uint32_t s_ticket, s_accepted, s_prefix[3], s_scan[4];
   const uint32_t tid = threadIdx.x;
   const LodState s = a.state[round];
   // At least one block runs, even for an empty frontier: it carries the state to the next round.
   const uint32_t blocks = max((s.active + 255u) / 256u, 1u);
   if (blockIdx.x >= blocks) return;
   if (tid == 0) s_ticket = atomicAdd(&a.ctrl->ticket[2 * round + 1], 1u);
+  if (tid < 64u) {
+    const uint32_t accepted = lod_budget(a, s);
+    if (tid == 0) s_accepted = accepted;
+  }
   __syncthreads();
   const uint32_t b = s_ticket;
+  const uint32_t accepted = s_accepted;
   const uint32_t t = b * 256u + tid;
   uint32_t kids = 0, splats = 0, packets = 0, denied = 0, flags = kDrop;
   LodCluster node{};
   if (t < s.active) {
     flags = a.cost[2 * t] & ~kCost;
     const uint32_t extra = a.cost[2 * t] & kCost;
-    if ((flags & kSplit) && extra > 0 && a.cost[2 * t + 1] > s.accepted) {
+    if ((flags & kSplit) && extra > 0 && a.cost[2 * t + 1] > accepted) {
       flags = 0;
       denied = 1;
     }
@@ -271,7 +283,7 @@ __global__ __launch_bounds__(256) void k_lod_compact(LodArgs a, uint32_t round) 
     n.count = s.count + s_prefix[1] + agg_splats;
     n.active = s_prefix[0] + agg_kids;
     n.packets = s.packets + s_prefix[2] + agg_packets;
-    n.accepted = s.accepted;
+    n.accepted = accepted;
     n.evaluated = s.evaluated + s.active;
     a.state[round + 1] = n;
   }
@@ -310,7 +322,6 @@ hipError_t launch_lod(const LodArgs& a, hipStream_t s) {
   const uint32_t blocks = (a.frontier_capacity + 255u) / 256u;
   for (uint32_t r = 0; r < a.rounds; ++r) {
     hipLaunchKernelGGL(k_lod_eval, dim3(blocks ? blocks : 1u), dim3(256), 0, s, a, r);
-    hipLaunchKernelGGL(k_lod_budget, dim3(1), dim3(64), 0, s, a, r);
     hipLaunchKernelGGL(k_lod_compact, dim3(blocks ? blocks : 1u), dim3(256), 0, s, a, r);
   }
   hipLaunchKernelGGL(k_lod_emit, dim3(1024), dim3(256), 0, s, a);
