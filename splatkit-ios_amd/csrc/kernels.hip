@@ -729,14 +729,26 @@ __global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict_
   uint32_t key[kSortItems];
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) key[j] = keys[min(base + j * 256u + tid, n - 1u)];
+  // Every row's validity first: the gapped layout's per-row scalar loads then share one wait, and the
+  // LDS adds below are not held behind it row by row (a scalar load and an LDS op share a counter).
+  uint32_t valid_rows = 0;
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) {
     const uint32_t row = base + j * 256u + wave * 64u;
-    const bool valid = sort_valid(row + lane, n, row_limit(seg, row, n));
+    valid_rows |= sort_valid(row + lane, n, row_limit(seg, row, n)) ? 1u << j : 0u;
+  }
+  // The previous lane's digit of every row, all the lane moves in flight together.
+  uint32_t prev[kSortItems];
+  if constexpr (RUNS) {
+#pragma unroll
+    for (uint32_t j = 0; j < kSortItems; ++j) prev[j] = __shfl_up((key[j] >> shift) & mask, 1u, 64);
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kSortItems; ++j) {
+    const bool valid = (valid_rows >> j) & 1u;
     const uint32_t d = (key[j] >> shift) & mask;
     if constexpr (RUNS) {
-      const uint32_t prev = __shfl_up(d, 1u, 64);
-      const bool start = valid && (lane == 0u || prev != d);
+      const bool start = valid && (lane == 0u || prev[j] != d);
       const uint64_t starts = __ballot(start);
       const uint32_t keys_in_row = static_cast<uint32_t>(__popcll(__ballot(valid)));
       const uint64_t later = starts & ~((2ull << lane) - 1ull);  // starts after this lane
