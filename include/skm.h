@@ -245,6 +245,11 @@ skm_status skm_read_frame_totals(skm_renderer* r, skm_frame_totals* out, int res
  * costs each raster block one device-wide atomic add, about 5% of a light frame's raster). No
  * counterpart in the reference: a profiling statistic of this backend. */
 void skm_count_raster_tests(skm_renderer* r, int enable);
+/* The tree cut's leading rounds whose frontier cannot exceed max_frontier clusters (the clusters at
+ * that depth of the tree) run in one single-block kernel instead of two grid-wide launches per round;
+ * the cut is the same either way. Default 4096; 0 runs every round grid-wide. No counterpart in the
+ * reference: a launch shape of this backend's cut. */
+void skm_set_lod_head(skm_renderer* r, uint32_t max_frontier);
 
 /* ---- the sort on its own (MetalRadixSort, MetalRadixSort.h:13-19) ---- */
 /* Stable ascending argsort of host keys below 2^key_bits, key_bits 1 to 32: ceil(key_bits / 8) LSD

@@ -139,6 +139,67 @@ __global__ void k_lod_prologue(LodArgs a) {
   for (uint32_t r = 1 + tid; r <= a.rounds; r += blockDim.x) a.state[r] = LodState{};
 }
 
+// One frontier cluster's decision (SplatLOD.metal:45-80): drop, split (with the extra cost of its
+// refinement) or keep.
+struct LodDecision {
+  uint32_t flags, extra, dropped;
+};
+__device__ __forceinline__ LodDecision lod_decide(const LodArgs& a, uint32_t round, uint32_t t) {
+  LodDecision d{0u, 0u, 0u};
+  const LodCluster node = a.clusters[a.frontier[round & 1u][t]];
+  const Camera& cam = *a.cam;
+  if (a.cull && lod_outside(cam, node)) {
+    d.flags = kDrop;
+    d.dropped = 1;
+  } else if (node.child_count + node.leaf_count > 1 &&
+             (a.pixel_limit == 0.0f || lod_error_pixels(cam, node, a.color_weight) > a.pixel_limit)) {
+    d.flags = kSplit;
+    d.extra = node.child_count + node.leaf_count - 1;
+  } else if (node.child_count == 1 && node.leaf_count == 0) {
+    d.flags = kSplit;
+  }
+  return d;
+}
+
+// One frontier cluster's compaction inputs (SplatLOD.metal:133-170): its flags after the budget's
+// denial, and the children, leaves and packet it contributes.
+struct LodEmit {
+  uint32_t flags, kids, splats, packets, denied;
+  LodCluster node;
+};
+__device__ __forceinline__ LodEmit lod_emit_counts(const LodArgs& a, uint32_t round, uint32_t t, uint32_t accepted) {
+  LodEmit e{};
+  e.flags = a.cost[2 * t] & ~kCost;
+  const uint32_t extra = a.cost[2 * t] & kCost;
+  if ((e.flags & kSplit) && extra > 0 && a.cost[2 * t + 1] > accepted) {
+    e.flags = 0;
+    e.denied = 1;
+  }
+  if (!(e.flags & kDrop)) {
+    e.node = a.clusters[a.frontier[round & 1u][t]];
+    e.kids = (e.flags & kSplit) ? e.node.child_count : 0u;
+    e.splats = (e.flags & kSplit) ? e.node.leaf_count : 1u;
+    e.packets = e.splats > 32u ? 1u : 0u;
+  }
+  return e;
+}
+
+// Writes one kept or split cluster's outputs (SplatLOD.metal:171-208): its children into the next
+// frontier from kid_base, its index or leaves from dest, or a packet for more than 32 leaves.
+__device__ __forceinline__ void lod_write(const LodArgs& a, uint32_t round, const LodEmit& e, uint32_t kid_base,
+                                          uint32_t dest, uint32_t packet) {
+  const bool split = (e.flags & kSplit) != 0;
+  uint32_t* next = a.frontier[(round + 1) & 1u];
+  for (uint32_t k = 0; k < e.kids; ++k) next[kid_base + k] = e.node.child_start + k;
+  if (e.splats > 32u) {
+    a.packets[packet] = make_uint4(e.node.leaf_start, e.splats, dest, 1u);
+  } else if (!split) {
+    a.indices[dest] = e.node.node;
+  } else {
+    for (uint32_t k = 0; k < e.splats; ++k) a.indices[dest + k] = a.leaves[e.node.leaf_start + k];
+  }
+}
+
 __global__ __launch_bounds__(256) void k_lod_eval(LodArgs a, uint32_t round) {
   __shared__ uint32_t s_ticket, s_prefix[2], s_scan[4];
   const uint32_t tid = threadIdx.x;
@@ -153,18 +214,10 @@ __global__ __launch_bounds__(256) void k_lod_eval(LodArgs a, uint32_t round) {
   const uint32_t t = b * 256u + tid;
   uint32_t extra = 0, flags = 0, dropped = 0;
   if (t < active) {
-    const LodCluster node = a.clusters[a.frontier[round & 1u][t]];
-    const Camera& cam = *a.cam;
-    if (a.cull && lod_outside(cam, node)) {
-      flags = kDrop;
-      dropped = 1;
-    } else if (node.child_count + node.leaf_count > 1 &&
-               (a.pixel_limit == 0.0f || lod_error_pixels(cam, node, a.color_weight) > a.pixel_limit)) {
-      flags = kSplit;
-      extra = node.child_count + node.leaf_count - 1;
-    } else if (node.child_count == 1 && node.leaf_count == 0) {
-      flags = kSplit;
-    }
+    const LodDecision d = lod_decide(a, round, t);
+    flags = d.flags;
+    extra = d.extra;
+    dropped = d.dropped;
   }
   uint32_t agg_extra, agg_drop;
   const uint32_t ex = block_exclusive_scan(extra, s_scan, &agg_extra);
@@ -230,22 +283,10 @@ __global__ __launch_bounds__(256) void k_lod_compact(LodArgs a, uint32_t round) 
   const uint32_t b = s_ticket;
   const uint32_t accepted = s_accepted;
   const uint32_t t = b * 256u + tid;
-  uint32_t kids = 0, splats = 0, packets = 0, denied = 0, flags = kDrop;
-  LodCluster node{};
-  if (t < s.active) {
-    flags = a.cost[2 * t] & ~kCost;
-    const uint32_t extra = a.cost[2 * t] & kCost;
-    if ((flags & kSplit) && extra > 0 && a.cost[2 * t + 1] > accepted) {
-      flags = 0;
-      denied = 1;
-    }
-    if (!(flags & kDrop)) {
-      node = a.clusters[a.frontier[round & 1u][t]];
-      kids = (flags & kSplit) ? node.child_count : 0u;
-      splats = (flags & kSplit) ? node.leaf_count : 1u;
-      packets = splats > 32u ? 1u : 0u;
-    }
-  }
+  LodEmit e{};
+  e.flags = kDrop;
+  if (t < s.active) e = lod_emit_counts(a, round, t, accepted);
+  const uint32_t kids = e.kids, splats = e.splats, packets = e.packets, denied = e.denied;
   uint32_t agg_kids, agg_splats, agg_packets, agg_denied;
   const uint32_t ek = block_exclusive_scan(kids, s_scan, &agg_kids);
   const uint32_t es = block_exclusive_scan(splats, s_scan, &agg_splats);
@@ -264,20 +305,8 @@ __global__ __launch_bounds__(256) void k_lod_compact(LodArgs a, uint32_t round) 
     }
   }
   __syncthreads();
-  if (t < s.active && !(flags & kDrop)) {
-    const bool split = (flags & kSplit) != 0;
-    uint32_t* next = a.frontier[(round + 1) & 1u];
-    const uint32_t kid_base = s_prefix[0] + ek;
-    for (uint32_t k = 0; k < kids; ++k) next[kid_base + k] = node.child_start + k;
-    const uint32_t dest = s.count + s_prefix[1] + es;
-    if (splats > 32u) {
-      a.packets[s.packets + s_prefix[2] + ep] = make_uint4(node.leaf_start, splats, dest, 1u);
-    } else if (!split) {
-      a.indices[dest] = node.node;
-    } else {
-      for (uint32_t k = 0; k < splats; ++k) a.indices[dest + k] = a.leaves[node.leaf_start + k];
-    }
-  }
+  if (t < s.active && !(e.flags & kDrop))
+    lod_write(a, round, e, s_prefix[0] + ek, s.count + s_prefix[1] + es, s.packets + s_prefix[2] + ep);
   if (b == blocks - 1u && tid == 0) {
     LodState n{};
     n.count = s.count + s_prefix[1] + agg_splats;
@@ -286,6 +315,88 @@ __global__ __launch_bounds__(256) void k_lod_compact(LodArgs a, uint32_t round) 
     n.accepted = accepted;
     n.evaluated = s.evaluated + s.active;
     a.state[round + 1] = n;
+  }
+}
+
+// The cut's leading rounds in one block (with the prologue's work): while the frontier can only be
+// small (round r evaluates clusters at depth r of the tree, and the host passes the number of leading
+// depths with at most kLodHeadFrontier clusters), a round is a few 1024-element chunks, and two
+// grid-wide launches per round (each a launch's latency plus look-back scans) would cost more than the
+// work. The block runs each round exactly as k_lod_eval and k_lod_compact do, chunk by chunk in
+// frontier order with carried prefixes in place of the look-back (the same saturating sums), and
+// writes the same state records, costs, frontier, indices and packets.
+__global__ __launch_bounds__(kLodHeadThreads) void k_lod_head(LodArgs a, uint32_t head_rounds) {
+  constexpr uint32_t kWaves = kLodHeadThreads / 64u;
+  __shared__ uint32_t s_scan[kWaves], s_accepted;
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) {
+    a.ctrl->epoch = a.ctrl->epoch + 1u;
+    a.ctrl->limited = 0;
+    a.frontier[0][0] = 0;
+  }
+  for (uint32_t k = tid; k < 2u * kLodMaxRounds; k += kLodHeadThreads) a.ctrl->ticket[k] = 0;
+  for (uint32_t r = 1 + tid; r <= a.rounds; r += kLodHeadThreads) a.state[r] = LodState{};
+  LodState s{};
+  s.active = a.cluster_count > 0 ? 1u : 0u;
+  __syncthreads();
+  for (uint32_t round = 0; round < head_rounds; ++round) {
+    // k_lod_eval over the frontier.
+    uint32_t carry_extra = 0, drops = 0;
+    for (uint32_t c0 = 0; c0 < s.active; c0 += kLodHeadThreads) {
+      const uint32_t t = c0 + tid;
+      LodDecision d{0u, 0u, 0u};
+      if (t < s.active) d = lod_decide(a, round, t);
+      uint32_t agg_extra, agg_drop;
+      const uint32_t ex = block_exclusive_scan_waves<kWaves>(d.extra, s_scan, &agg_extra);
+      block_exclusive_scan_waves<kWaves>(d.dropped, s_scan, &agg_drop);
+      if (t < s.active) {
+        a.cost[2 * t] = d.flags | d.extra;
+        a.cost[2 * t + 1] = sat_add(carry_extra, ex + d.extra);  // inclusive prefix end
+      }
+      carry_extra = sat_add(carry_extra, agg_extra);
+      drops += agg_drop;
+    }
+    s.total_extra = carry_extra;
+    s.drops = drops;
+    if (tid == 0) a.state[round] = s;
+    __syncthreads();  // the costs, written by every wave, are read by any
+    if (tid < 64u) {
+      const uint32_t accepted = lod_budget(a, s);
+      if (tid == 0) s_accepted = accepted;
+    }
+    __syncthreads();
+    const uint32_t accepted = s_accepted;
+    // k_lod_compact over the frontier.
+    uint32_t carry_kids = 0, carry_splats = 0, carry_packets = 0, denied = 0;
+    for (uint32_t c0 = 0; c0 < s.active; c0 += kLodHeadThreads) {
+      const uint32_t t = c0 + tid;
+      LodEmit e{};
+      e.flags = kDrop;
+      if (t < s.active) e = lod_emit_counts(a, round, t, accepted);
+      uint32_t agg_kids, agg_splats, agg_packets, agg_denied;
+      const uint32_t ek = block_exclusive_scan_waves<kWaves>(e.kids, s_scan, &agg_kids);
+      const uint32_t es = block_exclusive_scan_waves<kWaves>(e.splats, s_scan, &agg_splats);
+      const uint32_t ep = block_exclusive_scan_waves<kWaves>(e.packets, s_scan, &agg_packets);
+      block_exclusive_scan_waves<kWaves>(e.denied, s_scan, &agg_denied);
+      if (t < s.active && !(e.flags & kDrop))
+        lod_write(a, round, e, carry_kids + ek, s.count + carry_splats + es, s.packets + carry_packets + ep);
+      carry_kids += agg_kids;
+      carry_splats += agg_splats;
+      carry_packets += agg_packets;
+      denied += agg_denied;
+    }
+    LodState n{};
+    n.count = s.count + carry_splats;
+    n.active = carry_kids;
+    n.packets = s.packets + carry_packets;
+    n.accepted = accepted;
+    n.evaluated = s.evaluated + s.active;
+    if (tid == 0) {
+      if (denied) atomicAdd(&a.ctrl->limited, denied);
+      a.state[round + 1] = n;
+    }
+    s = n;
+    __syncthreads();  // the next frontier, written by every wave, is read by any
   }
 }
 
@@ -318,9 +429,13 @@ __global__ __launch_bounds__(256) void k_lod_emit(LodArgs a) {
 }  // namespace
 
 hipError_t launch_lod(const LodArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_lod_prologue, dim3(1), dim3(256), 0, s, a);
+  const uint32_t head = min(a.head_rounds, a.rounds);
+  if (head > 0)
+    hipLaunchKernelGGL(k_lod_head, dim3(1), dim3(kLodHeadThreads), 0, s, a, head);
+  else
+    hipLaunchKernelGGL(k_lod_prologue, dim3(1), dim3(256), 0, s, a);
   const uint32_t blocks = (a.frontier_capacity + 255u) / 256u;
-  for (uint32_t r = 0; r < a.rounds; ++r) {
+  for (uint32_t r = head; r < a.rounds; ++r) {
     hipLaunchKernelGGL(k_lod_eval, dim3(blocks ? blocks : 1u), dim3(256), 0, s, a, r);
     hipLaunchKernelGGL(k_lod_compact, dim3(blocks ? blocks : 1u), dim3(256), 0, s, a, r);
   }

@@ -145,10 +145,20 @@ struct LodThreshold {
 struct LodBuffers {
   DeviceBuffer clusters, leaves;
   uint32_t cluster_count = 0, rounds = 0, capacity = 0, frontier_capacity = 0;
+  std::vector<uint32_t> depth_clusters;  // clusters at each depth: the largest frontier of each round
 
   hipError_t upload(const skm_lod_tree_view* t, uint32_t cap, uint32_t n_rounds, hipStream_t stream) {
     cluster_count = static_cast<uint32_t>(t->cluster_count);
     rounds = n_rounds;
+    // Round r evaluates children of round r - 1's clusters, i.e. clusters at depth r (breadth-first:
+    // a child's index exceeds its parent's).
+    std::vector<uint32_t> depth(t->cluster_count, 0);
+    depth_clusters.assign(rounds + 1, 0);
+    for (uint64_t k = 0; k < t->cluster_count; ++k) {
+      const skm_lod_cluster& c = t->clusters[k];
+      if (depth[k] < depth_clusters.size()) ++depth_clusters[depth[k]];
+      for (uint32_t j = 0; j < c.child_count; ++j) depth[c.child_start + j] = depth[k] + 1;
+    }
     capacity = cap;
     frontier_capacity = std::max(1u, std::min(cap, cluster_count));
     hipError_t e = clusters.allocate(std::max<size_t>(t->cluster_count, 1) * 64);
@@ -180,14 +190,22 @@ struct LodBuffers {
     return e;
   }
 
+  // The leading rounds whose frontier cannot exceed head_frontier clusters (k_lod_head runs them).
+  uint32_t head_rounds(uint32_t head_frontier) const {
+    uint32_t r = 0;
+    while (r < rounds && r < depth_clusters.size() && depth_clusters[r] <= head_frontier) ++r;
+    return head_frontier == 0 ? 0u : r;
+  }
+
   skm::LodArgs args(const Camera* cam, const LodCut& c, uint32_t limit, float pixel_limit, float color_weight,
-                    int cull) const {
+                    int cull, uint32_t head_frontier) const {
     skm::LodArgs a{};
     a.cam = cam;
     a.clusters = clusters.as<skm::LodCluster>();
     a.leaves = leaves.as<uint32_t>();
     a.cluster_count = cluster_count;
     a.rounds = rounds;
+    a.head_rounds = head_rounds(head_frontier);
     a.capacity = std::max(1u, std::min(limit == 0 ? capacity : limit, capacity));
     a.frontier_capacity = frontier_capacity;
     a.pixel_limit = pixel_limit;
@@ -246,6 +264,7 @@ struct skm_renderer {
   float render_scale = 1.0f;
   bool linear = false;
   bool count_tests = false;  // skm_count_raster_tests
+  uint32_t lod_head_frontier = skm::kLodHeadFrontier;  // skm_set_lod_head
   bool vsync = true;
   uint32_t surface_w = 0, surface_h = 0;
   uint32_t generation = 0;
@@ -666,6 +685,9 @@ int skm_linear_blending(const skm_renderer* r) { return r && r->linear ? 1 : 0; 
 void skm_count_raster_tests(skm_renderer* r, int enable) {
   if (r) r->count_tests = enable != 0;
 }
+void skm_set_lod_head(skm_renderer* r, uint32_t max_frontier) {
+  if (r) r->lod_head_frontier = max_frontier;
+}
 void skm_set_vsync(skm_renderer* r, int vsync) {
   if (r) r->vsync = vsync != 0;
 }
@@ -952,7 +974,7 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   if (lod_frame) {
     // The slot's own cut state: the cut overlaps the other slots' frames.
     s.lod_limit = r->policy.lod_splat_limit == 0 ? r->lod.capacity : std::min(r->policy.lod_splat_limit, r->lod.capacity);
-    lod = r->lod.args(cam_dev, s.cut, s.lod_limit, r->threshold.pixels(), 4.0f, 1);
+    lod = r->lod.args(cam_dev, s.cut, s.lod_limit, r->threshold.pixels(), 4.0f, 1, r->lod_head_frontier);
   }
   skm::FrameArgs a{};
   a.cam = cam_dev;
@@ -1109,7 +1131,7 @@ skm_status skm_lod_select(skm_renderer* r, const skm_lod_tree_view* tree, const 
   Camera cam;
   fill_camera(cam, view16, proj16, width, height, false, 0.0f, nullptr, nullptr);
   if (auto st = r->check(cam_dev.allocate(sizeof(Camera)), "camera"); st != SKM_OK) return st;
-  const skm::LodArgs a = lod.args(cam_dev.as<Camera>(), cut, cap, pixel_limit, color_weight, cull);
+  const skm::LodArgs a = lod.args(cam_dev.as<Camera>(), cut, cap, pixel_limit, color_weight, cull, r->lod_head_frontier);
   skm::LodState final_state{};
   uint32_t limited = 0;
   hipError_t e = hipMemcpyAsync(cam_dev.ptr, &cam, sizeof(Camera), hipMemcpyHostToDevice, r->stream);

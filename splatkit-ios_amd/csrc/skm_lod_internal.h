@@ -30,6 +30,10 @@ struct LodState {
 };
 
 constexpr uint32_t kLodMaxRounds = 64;
+// The single-block head of the cut (k_lod_head) takes the leading rounds whose frontier cannot exceed
+// this many clusters.
+constexpr uint32_t kLodHeadFrontier = 4096;
+constexpr uint32_t kLodHeadThreads = 1024;
 
 struct LodCtrl {
   uint32_t epoch;
@@ -45,6 +49,7 @@ struct LodArgs {
   const uint32_t* leaves;
   uint32_t cluster_count;
   uint32_t rounds;
+  uint32_t head_rounds;        // leading rounds the single-block head runs (0: none)
   uint32_t capacity;           // most indices one cut emits
   uint32_t frontier_capacity;  // min(capacity, clusters)
   float pixel_limit;

@@ -53,15 +53,16 @@ __device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t v) {
   return v;
 }
 
-// Exclusive scan of one value per thread over a 256-thread block; `scratch` holds 4 words.
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* scratch, uint32_t* total) {
+// Exclusive scan of one value per thread over a block of WAVES waves; `scratch` holds WAVES words.
+template <uint32_t WAVES>
+__device__ __forceinline__ uint32_t block_exclusive_scan_waves(uint32_t v, uint32_t* scratch, uint32_t* total) {
   const uint32_t incl = wave_inclusive_scan(v);
   const uint32_t wave = threadIdx.x >> 6;
   if (lane_id() == 63) scratch[wave] = incl;
   __syncthreads();
   uint32_t before = 0, all = 0;
 #pragma unroll
-  for (uint32_t w = 0; w < 4; ++w) {
+  for (uint32_t w = 0; w < WAVES; ++w) {
     const uint32_t c = scratch[w];
     before += (w < wave) ? c : 0u;
     all += c;
@@ -69,6 +70,11 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
   __syncthreads();
   *total = all;
   return before + incl - v;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block; `scratch` holds 4 words.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* scratch, uint32_t* total) {
+  return block_exclusive_scan_waves<4>(v, scratch, total);
 }
 
 }  // namespace skm

@@ -175,6 +175,7 @@ API = {
     "skm_render_scale": (C.c_float, [_R]),
     "skm_set_linear_blending": (None, [_R, C.c_int]),
     "skm_count_raster_tests": (None, [_R, C.c_int]),
+    "skm_set_lod_head": (None, [_R, C.c_uint32]),
     "skm_linear_blending": (C.c_int, [_R]),
     "skm_set_vsync": (None, [_R, C.c_int]),
     "skm_set_draw_extent": (C.c_int, [_R, C.c_uint32, C.c_uint32]),
@@ -724,6 +725,10 @@ class Renderer:
     def count_raster_tests(self, enable=True):
         """Counts the raster's coverage tests into frame_totals()['raster_tests'] (off by default)."""
         self._lib.skm_count_raster_tests(self._h, 1 if enable else 0)
+
+    def set_lod_head(self, max_frontier):
+        """Leading cut rounds with frontiers of at most max_frontier clusters in one block (0: none)."""
+        self._lib.skm_set_lod_head(self._h, int(max_frontier))
 
     def lod_threshold(self):
         """The screen-space error the next tree-world frame's cut runs at (LodThreshold::pixels)."""

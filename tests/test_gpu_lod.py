@@ -60,6 +60,29 @@ def test_binary_tree_crosses_blocks(renderer, skm_mod, oracle, capacity):
     assert len(cut) == capacity and lc.covers_every_leaf_once(tree, cut)
 
 
+@pytest.mark.parametrize("head", [0, 64, 1 << 30])
+def test_cut_launch_shapes_give_the_same_cut(renderer, skm_mod, oracle, head):
+    """The single-block head of the cut (the leading rounds whose frontier can hold at most `head`
+    clusters) against the oracle: none (every round grid-wide), a few rounds, and the whole cut in one
+    block; a budget that denies refinements, so the budget search runs in both forms."""
+    say(f"case: cut with head frontier {head}")
+    renderer.set_lod_head(head)
+    try:
+        tree = lc.binary_tree(skm_mod)
+        view, proj, eye = lc.camera(0.0)
+        for capacity in (20000, 32768):
+            cut, _ = both(renderer, oracle, tree, view, proj, eye, lc.W, lc.H, capacity, 0.0, cull=False)
+            assert len(cut) == capacity and lc.covers_every_leaf_once(tree, cut)
+        cloud = skm_mod.make_cloud(200_000, 1, seed=56, layout=1, spread=20.0, area=60.0)
+        built = skm_mod.LodTree.build(cloud, 21, 32, threads=16)
+        proj = skm_mod.perspective(np.pi / 3, 16 / 9, 0.2, 1000.0)
+        for eye, pixel, budget in (((0, 3, 40), 1.5, 0), ((5, 2, 5), 0.5, 3_000)):
+            both(renderer, oracle, built, skm_mod.look_at(eye), proj, np.asarray(eye, np.float32), 1920, 1080, budget,
+                 pixel)
+    finally:
+        renderer.set_lod_head(4096)
+
+
 def test_built_trees_at_many_cameras(renderer, skm_mod, oracle):
     say("case: built trees, 200k splats, cameras x thresholds x budgets")
     cloud = skm_mod.make_cloud(200_000, 1, seed=55, layout=1, spread=20.0, area=60.0)
