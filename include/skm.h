@@ -338,6 +338,8 @@ typedef struct skm_batch_stats {
   double gpu_ms;      /* GPU time of the whole batch */
   uint64_t visible;   /* survivors over all views */
   uint64_t pairs;     /* tile pairs over all views */
+  uint32_t tile_passes;      /* radix passes of the tile sort over the pairs */
+  uint32_t view_sort_passes; /* radix passes on the view over the survivors (view-local tile keys), or 0 */
 } skm_batch_stats;
 
 /* Enqueues the batch (waits for the previous batch first). ranges NULL draws the whole world.

@@ -272,8 +272,21 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
   a.sort_counts = sort_counts_.as<uint32_t>();
   a.bin_starts = a.sort_counts + sort_words_;
   const DigitPlan depth_plan = make_digit_plan(policy.key16 ? 16u : 32u, 8u);
-  const DigitPlan tile_plan = make_digit_plan(key_bits_for(static_cast<uint32_t>(tiles - 1)), radix_bits);
+  DigitPlan tile_plan = make_digit_plan(key_bits_for(static_cast<uint32_t>(tiles - 1)), radix_bits);
+  // View-local tile keys (view << bits | tile index in the view) where the tile sort then takes fewer
+  // passes over the pairs than the sort on the view adds over the survivors (which are fewer).
+  uint32_t view_tiles = 0;
+  for (const ViewInfo& vi : info_) view_tiles = std::max(view_tiles, vi.tiles_x * vi.tiles_y);
+  const uint32_t local_bits = key_bits_for(view_tiles - 1), view_bits = key_bits_for(count - 1);
+  const DigitPlan local_plan = make_digit_plan(local_bits, radix_bits);
+  const int view_passes = make_digit_plan(view_bits, 8u).passes;
+  const bool local = count > 1 && local_bits + view_bits <= 32 && local_plan.passes < tile_plan.passes &&
+                     view_passes <= tile_plan.passes - local_plan.passes;
+  if (local) tile_plan = local_plan;
+  a.local_tile_bits = local ? local_bits : 0u;
+  local_tile_bits_ = a.local_tile_bits;
   depth_passes_ = depth_plan.passes;
+  survivor_passes_ = depth_plan.passes + (local ? view_passes : 0);
   tile_passes_ = tile_plan.passes;
   (void)world_count;
   (void)sh_degree;
@@ -284,7 +297,7 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
     // background and the world. Cameras, views and ranges are read from device memory.
     std::vector<uint64_t> key = {count, total, range_count, uint64_t(world.sh_degree), uint64_t(policy.key16),
                                  uint64_t(a.sort_variant), kTile, px, radix_bits,
-                                 tiles, uint64_t(depth_passes_), uint64_t(tile_passes_),
+                                 tiles, uint64_t(depth_passes_), uint64_t(tile_passes_), local_tile_bits_,
                                  reinterpret_cast<uint64_t>(world.px), world.capacity, pair_capacity_,
                                  lane_capacity_};
     for (int c = 0; c < 3; ++c) {
@@ -450,7 +463,7 @@ hipError_t Batch::read_order(uint32_t v, uint32_t* order, uint32_t capacity) con
   if (hipError_t e = survivor_tiles(); e != hipSuccess) return e;
   const uint32_t total = ctrl_host()->visible, n = view_visible(v);
   std::vector<uint32_t> all(total);
-  const uint32_t* src = order_[(depth_passes_ - 1) & 1].as<uint32_t>();
+  const uint32_t* src = order_[(survivor_passes_ - 1) & 1].as<uint32_t>();
   if (total) {
     if (hipError_t e = hipMemcpy(all.data(), src, size_t{total} * 4, hipMemcpyDeviceToHost); e != hipSuccess)
       return e;
@@ -469,30 +482,49 @@ hipError_t Batch::read_tiles(uint32_t v, uint64_t* pairs, uint32_t* pair_tiles, 
   hipError_t e = hipMemcpy(rg.data(), tile_ranges_.as<uint32_t>() + size_t{vi.tile_base} * 2, rg.size() * 4,
                            hipMemcpyDeviceToHost);
   if (e != hipSuccess) return e;
-  uint32_t p0 = 0, p1 = 0;
-  bool any = false;
+  // The view's tiles in order, each tile's pairs a run of the sorted pairs (contiguous for the view
+  // with global tile keys; interleaved with the other views' runs with view-local keys).
+  uint32_t lo = ~0u, hi = 0;
+  uint64_t total = 0;
   for (uint32_t t = 0; t < tiles; ++t) {
     if (rg[2 * t + 1] <= rg[2 * t]) continue;
-    if (!any) p0 = rg[2 * t];
-    p1 = rg[2 * t + 1];
-    any = true;
+    lo = std::min(lo, rg[2 * t]);
+    hi = std::max(hi, rg[2 * t + 1]);
+    total += rg[2 * t + 1] - rg[2 * t];
   }
-  *pairs = p1 - p0;
-  if (ranges != nullptr)
+  *pairs = total;
+  if (ranges != nullptr) {
+    uint32_t off = 0;
     for (uint32_t t = 0; t < std::min(tiles, tile_capacity); ++t) {
       const bool full = rg[2 * t + 1] > rg[2 * t];
-      ranges[2 * t] = full ? rg[2 * t] - p0 : 0;
-      ranges[2 * t + 1] = full ? rg[2 * t + 1] - p0 : 0;
+      ranges[2 * t] = full ? off : 0;
+      off += full ? rg[2 * t + 1] - rg[2 * t] : 0;
+      ranges[2 * t + 1] = full ? off : 0;
     }
-  const uint64_t n = std::min<uint64_t>(p1 - p0, capacity);
-  const int fin = tile_passes_ & 1;
-  if (pair_tiles && n) {
-    e = hipMemcpy(pair_tiles, pair_keys_[fin].as<uint32_t>() + p0, n * 4, hipMemcpyDeviceToHost);
-    for (uint64_t i = 0; e == hipSuccess && i < n; ++i) pair_tiles[i] -= vi.tile_base;
   }
-  if (e == hipSuccess && pair_values && n) {
+  const uint64_t n = std::min<uint64_t>(total, capacity);
+  if (n == 0 || (pair_tiles == nullptr && pair_values == nullptr)) return e;
+  const int fin = tile_passes_ & 1;
+  // Gather the view's runs from one copy of the span they lie in.
+  auto gather = [&](const DeviceBuffer& src, uint32_t* out) -> hipError_t {
+    std::vector<uint32_t> span(hi - lo);
+    if (hipError_t ce = hipMemcpy(span.data(), src.as<uint32_t>() + lo, span.size() * 4, hipMemcpyDeviceToHost);
+        ce != hipSuccess)
+      return ce;
+    uint64_t k = 0;
+    for (uint32_t t = 0; t < tiles && k < n; ++t)
+      for (uint32_t i = rg[2 * t]; i < rg[2 * t + 1] && k < n; ++i) out[k++] = span[i - lo];
+    return hipSuccess;
+  };
+  if (pair_tiles != nullptr) {
+    e = gather(pair_keys_[fin], pair_tiles);
+    const uint32_t mask = (1u << local_tile_bits_) - 1u;
+    for (uint64_t i = 0; e == hipSuccess && i < n; ++i)
+      pair_tiles[i] = local_tile_bits_ != 0 ? pair_tiles[i] & mask : pair_tiles[i] - vi.tile_base;
+  }
+  if (e == hipSuccess && pair_values != nullptr) {
     if (e = survivor_tiles(); e != hipSuccess) return e;
-    e = hipMemcpy(pair_values, pair_vals_[fin].as<uint32_t>() + p0, n * 4, hipMemcpyDeviceToHost);
+    e = gather(pair_vals_[fin], pair_values);
     for (uint64_t i = 0; e == hipSuccess && i < n; ++i) pair_values[i] = local_index(pair_values[i]);
   }
   return e;

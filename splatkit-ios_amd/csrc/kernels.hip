@@ -561,7 +561,8 @@ __global__ __launch_bounds__(256, SKM_SCATTER_WAVES) void k_onesweep(const uint3
                                                   const uint32_t* __restrict__ hist, uint32_t shift,
                                                   uint32_t mask, uint64_t* status, Ctrl* ctrl, uint32_t ticket,
                                                   uint32_t tag_pass, const uint32_t* __restrict__ offsets,
-                                                  uint32_t stride, const uint32_t* __restrict__ seg) {
+                                                  uint32_t stride, const uint32_t* __restrict__ seg,
+                                                  uint32_t view_keys) {
   __shared__ uint32_t s_ticket;
   __shared__ uint32_t s_wave_count[4][kRadix];
   __shared__ uint32_t s_base[kRadix];
@@ -685,6 +686,12 @@ __global__ __launch_bounds__(256, SKM_SCATTER_WAVES) void k_onesweep(const uint3
   }
 #pragma unroll
   for (uint32_t j = 0; j < kSortItems; ++j) rank[j] = s_base[(key[j] >> shift) & mask] + j * 256u + tid;
+  // view_keys (a batch's view count): the keys written are the values' views instead, the key of the
+  // sort on the view that follows (a survivor's position lies in a cull tile of its view).
+  if (view_keys != 0u) {
+#pragma unroll
+    for (uint32_t j = 0; j < kSortItems; ++j) key[j] = (val[j] / kCullTile) % view_keys;
+  }
   if (tile_total == kSortTile) {  // a full tile (every tile of a packed input but the last): no masks
     if (keys_out != nullptr) {
 #pragma unroll
@@ -983,8 +990,9 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
           r.ty1 = static_cast<int32_t>((rc >> 36) & 4095u);
           count = static_cast<uint32_t>((r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1));
           if (a.views != nullptr) {
-            const ViewInfo& vi = a.views[rc >> 48];
-            base = vi.tile_base;
+            const uint32_t view = static_cast<uint32_t>(rc >> 48);
+            const ViewInfo& vi = a.views[view];
+            base = a.local_tile_bits != 0u ? view << a.local_tile_bits : vi.tile_base;
             tiles_x = vi.tiles_x;
           }
         }
@@ -1062,9 +1070,13 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
 }
 
 // [start, end) of each tile's pairs by boundary detection: thread t takes keys [4t, 4t + 4) as one
-// 16-byte load (the key arrays are 256-byte aligned) and its two neighbours from the cache.
+// 16-byte load (the key arrays are 256-byte aligned) and its two neighbours from the cache. With
+// view-local keys (local_bits != 0) a key is view << local_bits | the tile's index in its view, and
+// its range goes to the view's tile_base + that index; equal keys are adjacent (the pairs are sorted
+// on the index, and each index's pairs come in view order).
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
-                                                uint32_t* ranges) {
+                                                uint32_t* ranges, const ViewInfo* __restrict__ views,
+                                                uint32_t local_bits) {
   const uint32_t n = *n_ptr;
   const uint32_t i0 = (blockIdx.x * 256u + threadIdx.x) * 4u;
   if (i0 >= n) return;
@@ -1086,8 +1098,11 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ key
     const uint32_t i = i0 + j;
     if (i >= n) break;
     const uint32_t t = k[1 + j];
-    if (i == 0u || k[j] != t) ranges[2 * t] = i;
-    if (i + 1u == n || k[2 + j] != t) ranges[2 * t + 1] = i + 1u;
+    const bool first = i == 0u || k[j] != t, last = i + 1u == n || k[2 + j] != t;
+    if (!first && !last) continue;
+    const uint32_t g = local_bits != 0u ? views[t >> local_bits].tile_base + (t & ((1u << local_bits) - 1u)) : t;
+    if (first) ranges[2 * g] = i;
+    if (last) ranges[2 * g + 1] = i + 1u;
   }
 }
 
@@ -1559,11 +1574,11 @@ template <bool LOOKBACK, int BITS>
 void sort_pass_bits(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
                     const uint32_t* n_ptr, uint32_t tiles, uint32_t* hist, uint32_t shift, uint32_t bits,
                     uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass, uint32_t* counts,
-                    const uint32_t* seg, bool runs, bool counted, hipStream_t s) {
+                    const uint32_t* seg, bool runs, bool counted, uint32_t view_keys, hipStream_t s) {
   const uint32_t mask = (1u << bits) - 1u;
   if constexpr (LOOKBACK) {
     hipLaunchKernelGGL((k_onesweep<true, BITS>), dim3(tiles), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist,
-                       shift, mask, status, ctrl, ticket, tag_pass, nullptr, 0u, seg);
+                       shift, mask, status, ctrl, ticket, tag_pass, nullptr, 0u, seg, view_keys);
   } else {
     if (counted) {
       // The producer of the keys has written this pass's tile counts.
@@ -1575,7 +1590,7 @@ void sort_pass_bits(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, ui
                          seg);
     hipLaunchKernelGGL(k_digit_scan, dim3(1u << bits), dim3(256), 0, s, counts, n_ptr, hist, tiles);
     hipLaunchKernelGGL((k_onesweep<false, BITS>), dim3(tiles), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist,
-                       shift, mask, status, ctrl, ticket, tag_pass, counts, tiles, seg);
+                       shift, mask, status, ctrl, ticket, tag_pass, counts, tiles, seg, view_keys);
   }
 }
 
@@ -1583,32 +1598,34 @@ template <bool LOOKBACK>
 void sort_pass_variant(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
                        const uint32_t* n_ptr, uint32_t tiles, uint32_t* hist, uint32_t shift, uint32_t bits,
                        uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass, uint32_t* counts,
-                       const uint32_t* seg, bool runs, bool counted, hipStream_t s) {
+                       const uint32_t* seg, bool runs, bool counted, uint32_t view_keys, hipStream_t s) {
   if (bits <= 6)
     sort_pass_bits<LOOKBACK, 6>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
-                                tag_pass, counts, seg, runs, counted, s);
+                                tag_pass, counts, seg, runs, counted, view_keys, s);
   else if (bits == 7)
     sort_pass_bits<LOOKBACK, 7>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
-                                tag_pass, counts, seg, runs, counted, s);
+                                tag_pass, counts, seg, runs, counted, view_keys, s);
   else
     sort_pass_bits<LOOKBACK, 8>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
-                                tag_pass, counts, seg, runs, counted, s);
+                                tag_pass, counts, seg, runs, counted, view_keys, s);
 }
 
 // Pass p of a sort under its digit plan; `seg` is the cull's gapped layout (first depth pass) or
-// null. The last pass, on the key's top digit, counts runs of equal digits; `counted`: the keys'
-// producer has written the pass's tile counts.
+// null. The last pass, on the key's top digit, counts runs of equal digits unless `top_runs` is
+// false; `counted`: the keys' producer has written the pass's tile counts; `view_keys`: the pass
+// writes the values' views as keys (k_onesweep).
 void sort_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, const uint32_t* n_ptr,
                uint32_t capacity, uint32_t* hist, const DigitPlan& plan, int p, uint64_t* status, Ctrl* ctrl,
                uint32_t ticket, uint32_t tag_pass, uint32_t* counts, int variant, const uint32_t* seg,
-               hipStream_t s, bool counted = false) {
+               hipStream_t s, bool counted = false, bool top_runs = true, uint32_t view_keys = 0u) {
   const uint32_t tiles = grid_for(capacity, kSortTile);
+  const bool runs = top_runs && p + 1 == plan.passes;
   if (variant == kSortOnesweep)
     sort_pass_variant<true>(kin, vin, kout, vout, n_ptr, tiles, hist, plan.shift[p], plan.bits[p], status, ctrl,
-                            ticket, tag_pass, counts, seg, p + 1 == plan.passes, counted, s);
+                            ticket, tag_pass, counts, seg, runs, counted, view_keys, s);
   else
     sort_pass_variant<false>(kin, vin, kout, vout, n_ptr, tiles, hist, plan.shift[p], plan.bits[p], status, ctrl,
-                             ticket, tag_pass, counts, seg, p + 1 == plan.passes, counted, s);
+                             ticket, tag_pass, counts, seg, runs, counted, view_keys, s);
 }
 
 }  // namespace
@@ -1729,15 +1746,31 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
   if (a.sort_variant == kSortOnesweep)
     hipLaunchKernelGGL(k_hist, dim3(hist_grid(a.capacity)), dim3(256), 0, s, a.cull_keys, cull_lanes, a.cull_counts,
                        a.hist, depth_plan);
+  // A batch with view-local tile keys sorts the survivors by view after depth (stable: depth order
+  // within each view); the last depth pass writes the views as the keys of those passes.
+  const bool view_sort = a.local_tile_bits != 0u;
   for (int p = 0; p < depth_passes; ++p) {
+    const bool last = p + 1 == depth_passes;
     const uint32_t* kin = p == 0 ? a.cull_keys : a.keys[(p - 1) & 1];
-    uint32_t* kout = (p + 1 < depth_passes) ? a.keys[p & 1] : nullptr;
+    uint32_t* kout = (!last || view_sort) ? a.keys[p & 1] : nullptr;
     const uint32_t* vin = p == 0 ? nullptr : a.order[(p - 1) & 1];
     sort_pass(kin, vin, kout, a.order[p & 1], p == 0 ? cull_lanes : visible, a.capacity, a.hist + p * 256, depth_plan, p,
               a.sort_status, a.ctrl, kTicketDepthSort + p, 2u + p, a.sort_counts, a.sort_variant,
-              p == 0 ? a.cull_counts : nullptr, s);
+              p == 0 ? a.cull_counts : nullptr, s, false, true, last && view_sort ? a.view_count : 0u);
   }
-  const uint32_t* order = a.order[(depth_passes - 1) & 1];
+  int survivor_passes = depth_passes;
+  if (view_sort) {
+    // Reduce then scan whatever the variant (the depth sort's histogram slots are free again). The
+    // views of neighbouring survivors differ, so the counts add key by key.
+    const DigitPlan view_plan = make_digit_plan(key_bits_for(a.view_count - 1u), 8u);
+    for (int v = 0; v < view_plan.passes; ++v, ++survivor_passes) {
+      const int q = survivor_passes;
+      sort_pass(a.keys[(q - 1) & 1], a.order[(q - 1) & 1], v + 1 < view_plan.passes ? a.keys[q & 1] : nullptr,
+                a.order[q & 1], visible, a.capacity, a.hist + v * 256, view_plan, v, a.sort_status, a.ctrl,
+                kTicketDepthSort, 2u, a.sort_counts, kSortReduceScan, nullptr, s, false, false);
+    }
+  }
+  const uint32_t* order = a.order[(survivor_passes - 1) & 1];
   if (marks) (void)hipEventRecord(marks[3], s);
   hipLaunchKernelGGL(k_bin_count, dim3(grid_for(a.capacity, 256u * kBinItems)), dim3(256), 0, s, a, order);
   hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, s, a);
@@ -1757,7 +1790,7 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
   }
   const int fin = tile_passes & 1;
   hipLaunchKernelGGL(k_ranges, dim3(grid_for(a.pair_capacity, 1024u)), dim3(256), 0, s,
-                     a.pair_keys[fin], sort_n, a.tile_ranges);
+                     a.pair_keys[fin], sort_n, a.tile_ranges, a.views, a.local_tile_bits);
   if (a.tile_order != nullptr) {
     uint32_t* counters = a.hist + kOrderCounters;
     hipLaunchKernelGGL(k_order_count, dim3(grid_for(tiles, 256)), dim3(256), 0, s, a.tile_ranges, tiles, counters);

@@ -32,7 +32,7 @@ using skm::fill_camera;
 using skm::PinnedBuffer;
 using skm::sh_stride;
 
-static_assert(sizeof(skm_view) == 152 && sizeof(skm_batch_tuning) == 24 && sizeof(skm_batch_stats) == 32 &&
+static_assert(sizeof(skm_view) == 152 && sizeof(skm_batch_tuning) == 24 && sizeof(skm_batch_stats) == 40 &&
                   sizeof(skm_config) == 24,
               "batch ABI layouts");
 
@@ -1457,6 +1457,8 @@ skm_status skm_read_batch_stats(skm_renderer* r, skm_batch_stats* out) {
   out->gpu_ms = r->batch->gpu_ms();
   out->visible = r->batch->visible();
   out->pairs = r->batch->pairs();
+  out->tile_passes = static_cast<uint32_t>(r->batch->tile_passes());
+  out->view_sort_passes = static_cast<uint32_t>(r->batch->view_sort_passes());
   return SKM_OK;
 }
 

@@ -52,6 +52,8 @@ class Batch {
   uint64_t visible() const { return ctrl_host()->visible; }
   uint64_t pairs() const { return ctrl_host()->pairs; }
   double gpu_ms() const { return gpu_ms_; }
+  int tile_passes() const { return tile_passes_; }
+  int view_sort_passes() const { return survivor_passes_ - depth_passes_; }
   const float* image_device() const { return image_.as<float>(); }
   uint64_t pixels() const { return info_.empty() ? 0 : info_.back().pixel_base + uint64_t{info_.back().width} * info_.back().height; }
   // Device bytes held now, and what a batch of this shape would hold (scratch grows, never shrinks).
@@ -98,6 +100,8 @@ class Batch {
   std::vector<ViewInfo> info_;
   std::vector<uint16_t> tile_view_host_;
   int depth_passes_ = 4, tile_passes_ = 2;
+  int survivor_passes_ = 4;        // depth passes, then the passes on the view (view-local tile keys)
+  uint32_t local_tile_bits_ = 0;   // nonzero: the tile keys are view << bits | the tile's index in the view
   hipStream_t stream_ = nullptr;
   hipEvent_t start_ = nullptr, end_ = nullptr;
   bool in_flight_ = false;

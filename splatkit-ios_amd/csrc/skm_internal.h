@@ -199,6 +199,11 @@ struct FrameArgs {
   uint32_t* view_visible;      // survivors per view
   const uint16_t* tile_view;   // the view of each global tile
   uint32_t total_tiles;        // tiles of all views
+  // Views in one batch (view_count > 1): nonzero, the tile keys of the binning are
+  // view << local_tile_bits | the tile's index within its view, and the depth sort ends with a pass
+  // on the view, so that the pairs come out grouped by view and the tile sort needs only the local
+  // bits. Zero: tile keys are global tile ids.
+  uint32_t local_tile_bits;
   // Radix sort variant and its tile-count scratch (256 words per kSortTile-key tile of the larger of the
   // survivor and pair capacities).
   int sort_variant;

@@ -108,7 +108,7 @@ class TuningResult(C.Structure):
 
 class BatchStats(C.Structure):
     _fields_ = [("views", C.c_uint32), ("graph", C.c_uint32), ("gpu_ms", C.c_double), ("visible", C.c_uint64),
-                ("pairs", C.c_uint64)]
+                ("pairs", C.c_uint64), ("tile_passes", C.c_uint32), ("view_sort_passes", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):
@@ -683,7 +683,8 @@ class Renderer:
     def batch_stats(self):
         s = BatchStats()
         self._check(self._lib.skm_read_batch_stats(self._h, C.byref(s)))
-        return {"views": s.views, "graph": bool(s.graph), "gpu_ms": s.gpu_ms, "visible": s.visible, "pairs": s.pairs}
+        return {"views": s.views, "graph": bool(s.graph), "gpu_ms": s.gpu_ms, "visible": s.visible, "pairs": s.pairs,
+                "tile_passes": s.tile_passes, "view_sort_passes": s.view_sort_passes}
 
     def read_view_frame(self, v, fmt=PIXELS_RGBA32F):
         w, h = self._views[v].width, self._views[v].height

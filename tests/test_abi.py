@@ -61,7 +61,7 @@ def test_struct_layouts_match_the_reference(skm_mod):
     # The batch extension's structs (static_assert-ed to the same sizes in renderer.cpp).
     assert C.sizeof(skm_mod.View) == 152 and C.sizeof(skm_mod.BatchTuning) == 24
     assert C.sizeof(skm_mod.Config) == 24
-    assert C.sizeof(skm_mod.BatchStats) == 32
+    assert C.sizeof(skm_mod.BatchStats) == 40
 
 
 def test_renderer_creation_fails_cleanly_without_a_device(skm_mod):

@@ -121,6 +121,33 @@ def test_batch_ranges_and_reordered_views(renderer, skm_mod):
     assert count == first[2]["count"] and np.array_equal(proj, first[2]["projected"])
 
 
+@pytest.mark.parametrize("count,res,radix,view_passes", [
+    (7, None, 8, 0),              # global tile ids: the view-local ids would take as many passes
+    (24, ((1280, 720),), 8, 1),   # 86,400 tiles (17 bits: 3 passes) vs 3,600 per view (12 bits: 2 passes)
+    (24, ((1280, 720),), 6, 1),   # 3 passes (6, 6, 5 bits) vs 2 of 6
+    (300, ((256, 256),), 8, 2),   # 76,800 tiles (3 passes) vs 256 per view (1 pass); 300 views: 2 view passes
+])
+def test_view_local_tile_keys(renderer, skm_mod, count, res, radix, view_passes):
+    say(f"case: {count} views, resolutions {res}, radix {radix}: {view_passes} passes on the view expected")
+    cloud = skm_mod.make_cloud(80_000, 1, seed=47, layout=1, spread=3.0, area=6.0)
+    renderer.upload_world(cloud)
+    renderer.apply_policy(sort_depth=32)
+    views = small_views(skm_mod, count, seed=17, **({"resolutions": res} if res else {}))
+    renderer.render_views(views, radix_bits=radix)
+    st = renderer.batch_stats()
+    say(f"batch: {st}")
+    assert st["view_sort_passes"] == view_passes
+    got = {k: batched(renderer, k) for k in sorted({0, 1, count // 2, count - 1})}
+    renderer.set_tuning(radix_bits=radix)
+    try:
+        for k, b in got.items():
+            assert_same(b, single(renderer, views[k], False), f"view {k}")
+    finally:
+        renderer.set_tuning()
+    counts = [renderer.read_view_visible(k, capacity=1)[0] for k in range(count)]
+    assert sum(counts) == st["visible"]
+
+
 def test_64_view_batch_of_the_3m_scene(renderer, skm_mod):
     say("case: 64 views (720p/1080p/1440p) of the 3M SH3 scene, one graph launch")
     cloud = skm_mod.scene_c1()
