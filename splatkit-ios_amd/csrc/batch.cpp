@@ -107,8 +107,9 @@ hipError_t Batch::ensure(uint32_t views, uint64_t lanes, uint64_t pairs, uint32_
   const size_t sort = static_cast<size_t>(std::max(lane_capacity_, pair_capacity_) / kSortTile + 2) * kRadix;
   if (e == hipSuccess && sort > sort_words_) {
     need(sort_status_, sort * 8);
-    // The tile counts, then the binning's block starts (3 per sort tile of survivors, and 2 spare).
-    need(sort_counts_, (sort + sort / kRadix * 3 + 2) * 4);
+    // The tile counts, then the binning's block totals and starts (kBinBlocksPerSortTile per sort tile
+    // of survivors, and 2 spare).
+    need(sort_counts_, (sort + sort / kRadix * kBinBlocksPerSortTile + 2) * 4);
     if (e == hipSuccess) e = hipMemsetAsync(sort_status_.ptr, 0, sort_status_.bytes, stream);
     if (e == hipSuccess) sort_words_ = sort;
   }

@@ -434,8 +434,9 @@ struct skm_renderer {
     if (blocks <= x.sort_status_blocks) return SKM_OK;
     DeviceBuffer b, c;
     if (auto st = check(b.allocate(blocks * skm::kRadix * 8u), "sort status"); st != SKM_OK) return st;
-    // The tile counts, then the binning's block starts (3 per sort tile of survivors, and 2 spare).
-    if (auto st = check(c.allocate((blocks * (skm::kRadix + 3u) + 2u) * 4u), "sort counts"); st != SKM_OK) return st;
+    // The tile counts, then the binning's block totals and starts (kBinBlocksPerSortTile per sort tile
+    // of survivors, and 2 spare).
+    if (auto st = check(c.allocate((blocks * (skm::kRadix + skm::kBinBlocksPerSortTile) + 2u) * 4u), "sort counts"); st != SKM_OK) return st;
     if (auto st = check(hipMemsetAsync(b.ptr, 0, b.bytes, st_), "clear"); st != SKM_OK) return st;
     x.sort_status = std::move(b);
     x.sort_counts = std::move(c);

@@ -124,13 +124,18 @@ constexpr uint32_t kThreads = 256;
 #endif
 constexpr uint32_t kCullItems = SKM_CULL_ITEMS;
 constexpr uint32_t kCullTile = kThreads * kCullItems;
-constexpr uint32_t kBinItems = 4;   // survivors per thread of the binning (1024 per block)
+#ifndef SKM_BIN_ITEMS
+#define SKM_BIN_ITEMS 4
+#endif
+constexpr uint32_t kBinItems = SKM_BIN_ITEMS;  // survivors per thread of the binning (1024 per block)
 // Keys per thread of a radix pass (a block sorts 256 x kSortItems keys); a build may set it.
 #ifndef SKM_SORT_ITEMS
 #define SKM_SORT_ITEMS 12
 #endif
 constexpr uint32_t kSortItems = SKM_SORT_ITEMS;
 constexpr uint32_t kSortTile = kThreads * kSortItems;  // keys per radix-pass block (3072 at 12)
+// Binning blocks per radix-pass block of survivors (the binning's block totals take one word each).
+constexpr uint32_t kBinBlocksPerSortTile = (kSortTile + kThreads * kBinItems - 1u) / (kThreads * kBinItems);
 constexpr uint32_t kRadix = 256;
 constexpr uint32_t kMaxRanges = 65536;  // MetalVisibilityPass.h:72
 constexpr uint32_t kOrderCounters = 2u * 4u * 256u;  // offset of the tile-order counters in hist
