@@ -438,65 +438,72 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
   if (blockIdx.x >= blocks) return;
   s_byte255[tid] = static_cast<float>(tid) / 255.0f;
   __syncthreads();
-  const uint32_t b = xcd_tile(blockIdx.x, blocks);
-  const uint32_t view = a.view_count > 1u ? b % a.view_count : 0u;
-  const uint32_t lt = a.view_count > 1u ? b / a.view_count : b;  // the lane tile
-  const Camera& cam = a.cam[view];
-  uint32_t key[kItems], rec[kItems][8];
-  uint64_t ballot[kItems];
+  // A tree cut's frame (INDEXED) is launched with a grid estimated from earlier frames; when the cut
+  // has more lanes, the blocks walk the logical blocks in strides of the grid (a multiple of the XCD
+  // count, so each block stays in its XCD's range).
+  for (uint32_t i = blockIdx.x; i < blocks; i += gridDim.x) {
+    if (i != blockIdx.x) __syncthreads();  // every read of the previous logical block's s_seg is done
+    const uint32_t b = xcd_tile(i, blocks);
+    const uint32_t view = a.view_count > 1u ? b % a.view_count : 0u;
+    const uint32_t lt = a.view_count > 1u ? b / a.view_count : b;  // the lane tile
+    const Camera& cam = a.cam[view];
+    uint32_t key[kItems], rec[kItems][8];
+    uint64_t ballot[kItems];
 #pragma unroll
-  for (uint32_t k = 0; k < kItems; ++k) {
-    const uint32_t t = lt * kTile + k * 256u + tid;
-    bool visible = false;
-    key[k] = 0;
+    for (uint32_t k = 0; k < kItems; ++k) {
+      const uint32_t t = lt * kTile + k * 256u + tid;
+      bool visible = false;
+      key[k] = 0;
 #pragma unroll
-    for (int w = 0; w < 8; ++w) rec[k][w] = 0;
-    if (t < total) {
-      uint32_t index;
-      if constexpr (INDEXED) {
-        index = a.active[t];
-      } else {
-        const uint32_t r = find_range(a.range_starts, a.range_count, t);
-        index = a.ranges[2 * r] + (t - a.range_starts[r]);
+      for (int w = 0; w < 8; ++w) rec[k][w] = 0;
+      if (t < total) {
+        uint32_t index;
+        if constexpr (INDEXED) {
+          index = a.active[t];
+        } else {
+          const uint32_t r = find_range(a.range_starts, a.range_count, t);
+          index = a.ranges[2 * r] + (t - a.range_starts[r]);
+        }
+        visible = cull_lane<DEG, KEY16>(cam, a.world, index, key[k], rec[k], s_byte255);
       }
-      visible = cull_lane<DEG, KEY16>(cam, a.world, index, key[k], rec[k], s_byte255);
+      ballot[k] = __ballot(visible);
+      if (lane_id() == 0) s_seg[k * 4 + wave] = static_cast<uint32_t>(__popcll(ballot[k]));
     }
-    ballot[k] = __ballot(visible);
-    if (lane_id() == 0) s_seg[k * 4 + wave] = static_cast<uint32_t>(__popcll(ballot[k]));
-  }
-  __syncthreads();
-  uint32_t aggregate = 0;
+    __syncthreads();
+    uint32_t aggregate = 0;
 #pragma unroll
-  for (uint32_t g = 0; g < kItems * 4; ++g) aggregate += s_seg[g];
-  if (tid == 0) {
-    a.cull_counts[b] = aggregate;
-    if (b == 0) a.ctrl->cull_lanes = blocks * kTile;
-    if (aggregate != 0u) {
-      atomicAdd(&a.ctrl->visible, aggregate);
-      if (a.view_visible != nullptr) atomicAdd(&a.view_visible[view], aggregate);
+    for (uint32_t g = 0; g < kItems * 4; ++g) aggregate += s_seg[g];
+    if (tid == 0) {
+      a.cull_counts[b] = aggregate;
+      if (b == 0) a.ctrl->cull_lanes = blocks * kTile;
+      if (aggregate != 0u) {
+        atomicAdd(&a.ctrl->visible, aggregate);
+        if (a.view_visible != nullptr) atomicAdd(&a.view_visible[view], aggregate);
+      }
     }
-  }
-  uint32_t before = b * kTile;
+    uint32_t before = b * kTile;
 #pragma unroll
-  for (uint32_t k = 0; k < kItems; ++k) {
-    uint32_t seg = before;
+    for (uint32_t k = 0; k < kItems; ++k) {
+      uint32_t seg = before;
 #pragma unroll
-    for (uint32_t w = 0; w < 4; ++w) seg += (w < wave) ? s_seg[k * 4 + w] : 0u;
-    if ((ballot[k] >> lane_id()) & 1ull) {
-      const uint32_t off = seg + rank_below(ballot[k]);
-      a.cull_keys[off] = key[k];
-      uint4* dst = reinterpret_cast<uint4*>(a.projected + static_cast<size_t>(off) * 8u);
-      dst[0] = make_uint4(rec[k][0], rec[k][1], rec[k][2], rec[k][3]);
-      dst[1] = make_uint4(rec[k][4], rec[k][5], rec[k][6], rec[k][7]);
-      // The binning's tile rectangle, from the record just written (read back in depth order, a
-      // gather of 8 bytes instead of the 32-byte record).
-      Rect r;
-      const uint32_t tiles = tile_rect(rec[k], static_cast<uint32_t>(cam.screen_size[0]),
-                                       static_cast<uint32_t>(cam.screen_size[1]), a.tile, r);
-      a.rects[off] = pack_rect(r, tiles, view);
+      for (uint32_t w = 0; w < 4; ++w) seg += (w < wave) ? s_seg[k * 4 + w] : 0u;
+      if ((ballot[k] >> lane_id()) & 1ull) {
+        const uint32_t off = seg + rank_below(ballot[k]);
+        a.cull_keys[off] = key[k];
+        uint4* dst = reinterpret_cast<uint4*>(a.projected + static_cast<size_t>(off) * 8u);
+        dst[0] = make_uint4(rec[k][0], rec[k][1], rec[k][2], rec[k][3]);
+        dst[1] = make_uint4(rec[k][4], rec[k][5], rec[k][6], rec[k][7]);
+        // The binning's tile rectangle, from the record just written (read back in depth order, a
+        // gather of 8 bytes instead of the 32-byte record).
+        Rect r;
+        const uint32_t tiles = tile_rect(rec[k], static_cast<uint32_t>(cam.screen_size[0]),
+                                         static_cast<uint32_t>(cam.screen_size[1]), a.tile, r);
+        a.rects[off] = pack_rect(r, tiles, view);
+      }
+#pragma unroll
+      for (uint32_t w = 0; w < 4; ++w) before += s_seg[k * 4 + w];
     }
-#pragma unroll
-    for (uint32_t w = 0; w < 4; ++w) before += s_seg[k * 4 + w];
+    if constexpr (!INDEXED) break;  // a range-drawn frame's grid is exactly `blocks`
   }
 }
 
@@ -569,145 +576,153 @@ __global__ __launch_bounds__(256, SKM_SCATTER_WAVES) void k_onesweep(const uint3
   __shared__ uint32_t s_scan[4];
   __shared__ uint32_t s_keys[kSortTile];
   __shared__ uint32_t s_vals[kSortTile];
-  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
   const uint32_t n = *n_ptr;
   const uint32_t blocks = (n + kSortTile - 1u) / kSortTile;
-  if (blockIdx.x >= blocks) return;  // before the ticket: exactly `blocks` tickets are taken
-  if (LOOKBACK && tid == 0) s_ticket = atomicAdd(&ctrl->ticket[ticket], 1u);
+  // Tiles i = blockIdx.x, blockIdx.x + gridDim.x, ... < blocks: a grid sized for the capacity takes one
+  // tile per block; a smaller grid (a multiple of the XCD count) walks the tiles. The look-back
+  // variant is launched with a block per tile of the capacity (its tickets order the tiles).
+  for (uint32_t i = blockIdx.x; i < blocks; i += gridDim.x) {
+    // The thread's index passes an empty asm each tile, so lane-dependent offsets are computed per
+    // tile rather than hoisted out of the loop, where they would hold registers across the tile.
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const uint32_t wave = tid >> 6, lane = tid & 63u;
+    if (LOOKBACK && tid == 0) s_ticket = atomicAdd(&ctrl->ticket[ticket], 1u);
 #pragma unroll
-  for (uint32_t w = 0; w < 4; ++w) s_wave_count[w][tid] = 0;
-  __syncthreads();
-  const uint32_t b = LOOKBACK ? s_ticket : xcd_tile(blockIdx.x, blocks);
-  const uint32_t base = b * kSortTile;
+    for (uint32_t w = 0; w < 4; ++w) s_wave_count[w][tid] = 0;
+    __syncthreads();
+    const uint32_t b = LOOKBACK ? s_ticket : xcd_tile(i, blocks);
+    const uint32_t base = b * kSortTile;
 
-  uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
-  // Every load first (positions clamped to n - 1), so a thread waits for one round trip. The inputs
-  // are read once per pass, so they are loaded non-temporally: they then displace fewer of the
-  // scatter's partially written lines from the XCD's L2 before those lines are merged.
+    uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+    // Every load first (positions clamped to n - 1), so a thread waits for one round trip. The inputs
+    // are read once per pass, so they are loaded non-temporally: they then displace fewer of the
+    // scatter's partially written lines from the XCD's L2 before those lines are merged.
 #pragma unroll
-  for (uint32_t j = 0; j < kSortItems; ++j) {
-    const uint32_t pos = min(base + wave * (kSortItems * 64u) + j * 64u + lane, n - 1u);
-    key[j] = __builtin_nontemporal_load(keys_in + pos);
-    val[j] = vals_in != nullptr ? __builtin_nontemporal_load(vals_in + pos) : pos;
-  }
-  uint32_t valid_rows = 0;  // bit j: row j's position holds a key
+    for (uint32_t j = 0; j < kSortItems; ++j) {
+      const uint32_t pos = min(base + wave * (kSortItems * 64u) + j * 64u + lane, n - 1u);
+      key[j] = __builtin_nontemporal_load(keys_in + pos);
+      val[j] = vals_in != nullptr ? __builtin_nontemporal_load(vals_in + pos) : pos;
+    }
+    uint32_t valid_rows = 0;  // bit j: row j's position holds a key
 #pragma unroll
-  for (uint32_t j = 0; j < kSortItems; ++j) {
-    const uint32_t row = base + wave * (kSortItems * 64u) + j * 64u;
-    const bool valid = sort_valid(row + lane, n, row_limit(seg, row, n));
-    valid_rows |= valid ? 1u << j : 0u;
-    key[j] = valid ? key[j] : 0u;
-    val[j] = valid ? val[j] : 0u;
-  }
+    for (uint32_t j = 0; j < kSortItems; ++j) {
+      const uint32_t row = base + wave * (kSortItems * 64u) + j * 64u;
+      const bool valid = sort_valid(row + lane, n, row_limit(seg, row, n));
+      valid_rows |= valid ? 1u << j : 0u;
+      key[j] = valid ? key[j] : 0u;
+      val[j] = valid ? val[j] : 0u;
+    }
 #pragma unroll
-  for (uint32_t j = 0; j < kSortItems; ++j) {
-    const bool valid = (valid_rows >> j) & 1u;
-    const uint32_t d = (key[j] >> shift) & mask;  // mask < 2^BITS
-    const uint64_t peers = digit_peers<BITS>(d, __ballot(valid));
-    const uint32_t below = rank_below(peers);
-    const uint32_t before = s_wave_count[wave][d];
-    rank[j] = before + below;
-    // The lowest lane of each digit (no peer below it) stores the digit's new count.
-    if (valid && below == 0u) s_wave_count[wave][d] = before + static_cast<uint32_t>(__popcll(peers));
-  }
-  __syncthreads();
-  // Per digit (thread = digit): offsets of each wave, the tile's total, and its start in the tile.
-  const uint32_t c0 = s_wave_count[0][tid], c1 = s_wave_count[1][tid];
-  const uint32_t c2 = s_wave_count[2][tid], c3 = s_wave_count[3][tid];
-  const uint32_t count = c0 + c1 + c2 + c3;
-  uint32_t tile_total;
-  const uint32_t start = block_exclusive_scan(count, s_scan, &tile_total);
-  // Where wave w's keys of digit `tid` start in the staged tile.
-  s_wave_count[0][tid] = start;
-  s_wave_count[1][tid] = start + c0;
-  s_wave_count[2][tid] = start + c0 + c1;
-  s_wave_count[3][tid] = start + c0 + c1 + c2;
-  if constexpr (LOOKBACK) {
-    // Global start of each digit: exclusive scan of the pass histogram.
-    uint32_t all;
-    const uint32_t gstart = block_exclusive_scan(hist[tid], s_scan, &all);
-    const uint32_t tag = make_tag(ctrl->epoch, tag_pass);
-    // Per-digit look-back across tiles.
-    uint64_t* st = status + static_cast<size_t>(b) * kRadix;
-    uint32_t excl = 0;
-    if (b == 0) {
-      status_store(st + tid, status_word(tag, kFlagPrefix, count));
-    } else {
-      status_store(st + tid, status_word(tag, kFlagAggregate, count));
-      // kLookback predecessors per round trip (loads in flight together), nearest first, until one
-      // holds an inclusive prefix.
-      int32_t k = static_cast<int32_t>(b) - 1;
-      while (k >= 0) {
-        uint64_t sw[kLookback];
+    for (uint32_t j = 0; j < kSortItems; ++j) {
+      const bool valid = (valid_rows >> j) & 1u;
+      const uint32_t d = (key[j] >> shift) & mask;  // mask < 2^BITS
+      const uint64_t peers = digit_peers<BITS>(d, __ballot(valid));
+      const uint32_t below = rank_below(peers);
+      const uint32_t before = s_wave_count[wave][d];
+      rank[j] = before + below;
+      // The lowest lane of each digit (no peer below it) stores the digit's new count.
+      if (valid && below == 0u) s_wave_count[wave][d] = before + static_cast<uint32_t>(__popcll(peers));
+    }
+    __syncthreads();
+    // Per digit (thread = digit): offsets of each wave, the tile's total, and its start in the tile.
+    const uint32_t c0 = s_wave_count[0][tid], c1 = s_wave_count[1][tid];
+    const uint32_t c2 = s_wave_count[2][tid], c3 = s_wave_count[3][tid];
+    const uint32_t count = c0 + c1 + c2 + c3;
+    uint32_t tile_total;
+    const uint32_t start = block_exclusive_scan(count, s_scan, &tile_total);
+    // Where wave w's keys of digit `tid` start in the staged tile.
+    s_wave_count[0][tid] = start;
+    s_wave_count[1][tid] = start + c0;
+    s_wave_count[2][tid] = start + c0 + c1;
+    s_wave_count[3][tid] = start + c0 + c1 + c2;
+    if constexpr (LOOKBACK) {
+      // Global start of each digit: exclusive scan of the pass histogram.
+      uint32_t all;
+      const uint32_t gstart = block_exclusive_scan(hist[tid], s_scan, &all);
+      const uint32_t tag = make_tag(ctrl->epoch, tag_pass);
+      // Per-digit look-back across tiles.
+      uint64_t* st = status + static_cast<size_t>(b) * kRadix;
+      uint32_t excl = 0;
+      if (b == 0) {
+        status_store(st + tid, status_word(tag, kFlagPrefix, count));
+      } else {
+        status_store(st + tid, status_word(tag, kFlagAggregate, count));
+        // kLookback predecessors per round trip (loads in flight together), nearest first, until one
+        // holds an inclusive prefix.
+        int32_t k = static_cast<int32_t>(b) - 1;
+        while (k >= 0) {
+          uint64_t sw[kLookback];
 #pragma unroll
-        for (int32_t j = 0; j < static_cast<int32_t>(kLookback); ++j)
-          sw[j] = (k - j >= 0) ? status_load(status + static_cast<size_t>(k - j) * kRadix + tid) : 0ull;
-        int32_t j = 0;
-        bool done = false;
+          for (int32_t j = 0; j < static_cast<int32_t>(kLookback); ++j)
+            sw[j] = (k - j >= 0) ? status_load(status + static_cast<size_t>(k - j) * kRadix + tid) : 0ull;
+          int32_t j = 0;
+          bool done = false;
 #pragma unroll
-        for (int32_t i = 0; i < static_cast<int32_t>(kLookback); ++i) {
-          if (done || j != i || k - i < 0) continue;
-          const uint32_t flag = static_cast<uint32_t>(sw[i] >> 32) & 3u;
-          if (static_cast<uint32_t>(sw[i] >> 34) != tag || flag == 0u) continue;  // not published yet
-          excl += static_cast<uint32_t>(sw[i]);
-          j = i + 1;
-          if (flag == kFlagPrefix) done = true;
+          for (int32_t i = 0; i < static_cast<int32_t>(kLookback); ++i) {
+            if (done || j != i || k - i < 0) continue;
+            const uint32_t flag = static_cast<uint32_t>(sw[i] >> 32) & 3u;
+            if (static_cast<uint32_t>(sw[i] >> 34) != tag || flag == 0u) continue;  // not published yet
+            excl += static_cast<uint32_t>(sw[i]);
+            j = i + 1;
+            if (flag == kFlagPrefix) done = true;
+          }
+          if (done) break;
+          k -= j;
         }
-        if (done) break;
-        k -= j;
+        status_store(st + tid, status_word(tag, kFlagPrefix, excl + count));
       }
-      status_store(st + tid, status_word(tag, kFlagPrefix, excl + count));
+      s_base[tid] = gstart + excl - start;
+    } else {
+      // Global start of each digit from the digit totals (zero for digits above the mask).
+      uint32_t all;
+      const uint32_t gstart = block_exclusive_scan(tid <= mask ? hist[tid] : 0u, s_scan, &all);
+      s_base[tid] = gstart + offsets[static_cast<size_t>(tid) * stride + b] - start;
     }
-    s_base[tid] = gstart + excl - start;
-  } else {
-    // Global start of each digit from the digit totals (zero for digits above the mask).
-    uint32_t all;
-    const uint32_t gstart = block_exclusive_scan(tid <= mask ? hist[tid] : 0u, s_scan, &all);
-    s_base[tid] = gstart + offsets[static_cast<size_t>(tid) * stride + b] - start;
-  }
-  __syncthreads();
-  // Stage the tile in digit order in LDS.
+    __syncthreads();
+    // Stage the tile in digit order in LDS.
 #pragma unroll
-  for (uint32_t j = 0; j < kSortItems; ++j) {
-    if ((valid_rows >> j) & 1u) {
-      const uint32_t d = (key[j] >> shift) & mask;
-      const uint32_t local = s_wave_count[wave][d] + rank[j];
-      s_keys[local] = key[j];
-      s_vals[local] = val[j];
+    for (uint32_t j = 0; j < kSortItems; ++j) {
+      if ((valid_rows >> j) & 1u) {
+        const uint32_t d = (key[j] >> shift) & mask;
+        const uint32_t local = s_wave_count[wave][d] + rank[j];
+        s_keys[local] = key[j];
+        s_vals[local] = val[j];
+      }
     }
-  }
-  __syncthreads();
-  // Write the staged tile: every LDS read of the thread's keys first, then their bases, then the
-  // stores (one round trip of each instead of two per key).
+    __syncthreads();
+    // Write the staged tile: every LDS read of the thread's keys first, then their bases, then the
+    // stores (one round trip of each instead of two per key).
 #pragma unroll
-  for (uint32_t j = 0; j < kSortItems; ++j) {
-    key[j] = s_keys[j * 256u + tid];
-    val[j] = s_vals[j * 256u + tid];
-  }
-#pragma unroll
-  for (uint32_t j = 0; j < kSortItems; ++j) rank[j] = s_base[(key[j] >> shift) & mask] + j * 256u + tid;
-  // view_keys (a batch's view count): the keys written are the values' views instead, the key of the
-  // sort on the view that follows (a survivor's position lies in a cull tile of its view).
-  if (view_keys != 0u) {
-#pragma unroll
-    for (uint32_t j = 0; j < kSortItems; ++j) key[j] = (val[j] / kCullTile) % view_keys;
-  }
-  if (tile_total == kSortTile) {  // a full tile (every tile of a packed input but the last): no masks
-    if (keys_out != nullptr) {
-#pragma unroll
-      for (uint32_t j = 0; j < kSortItems; ++j) keys_out[rank[j]] = key[j];
+    for (uint32_t j = 0; j < kSortItems; ++j) {
+      key[j] = s_keys[j * 256u + tid];
+      val[j] = s_vals[j * 256u + tid];
     }
 #pragma unroll
-    for (uint32_t j = 0; j < kSortItems; ++j) vals_out[rank[j]] = val[j];
-  } else {
-    if (keys_out != nullptr) {
+    for (uint32_t j = 0; j < kSortItems; ++j) rank[j] = s_base[(key[j] >> shift) & mask] + j * 256u + tid;
+    // view_keys (a batch's view count): the keys written are the values' views instead, the key of the
+    // sort on the view that follows (a survivor's position lies in a cull tile of its view).
+    if (view_keys != 0u) {
+#pragma unroll
+      for (uint32_t j = 0; j < kSortItems; ++j) key[j] = (val[j] / kCullTile) % view_keys;
+    }
+    if (tile_total == kSortTile) {  // a full tile (every tile of a packed input but the last): no masks
+      if (keys_out != nullptr) {
+#pragma unroll
+        for (uint32_t j = 0; j < kSortItems; ++j) keys_out[rank[j]] = key[j];
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < kSortItems; ++j) vals_out[rank[j]] = val[j];
+    } else {
+      if (keys_out != nullptr) {
+#pragma unroll
+        for (uint32_t j = 0; j < kSortItems; ++j)
+          if (j * 256u + tid < tile_total) keys_out[rank[j]] = key[j];
+      }
 #pragma unroll
       for (uint32_t j = 0; j < kSortItems; ++j)
-        if (j * 256u + tid < tile_total) keys_out[rank[j]] = key[j];
+        if (j * 256u + tid < tile_total) vals_out[rank[j]] = val[j];
     }
-#pragma unroll
-    for (uint32_t j = 0; j < kSortItems; ++j)
-      if (j * 256u + tid < tile_total) vals_out[rank[j]] = val[j];
   }
 }
 
@@ -724,49 +739,54 @@ __global__ __launch_bounds__(256) void k_tile_counts(const uint32_t* __restrict_
                                                      uint32_t shift, uint32_t mask, uint32_t* __restrict__ counts,
                                                      uint32_t stride, const uint32_t* __restrict__ seg) {
   __shared__ uint32_t s_h[4][kRadix];
-  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
   const uint32_t n = *n_ptr;
   const uint32_t blocks = (n + kSortTile - 1u) / kSortTile;
-  if (blockIdx.x >= blocks) return;
+  // Tiles i = blockIdx.x, blockIdx.x + gridDim.x, ... < blocks (a grid smaller than the capacity's
+  // tiles is a multiple of the XCD count).
+  for (uint32_t i = blockIdx.x; i < blocks; i += gridDim.x) {
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));  // lane-dependent offsets per tile, not held across the loop
+    const uint32_t wave = tid >> 6, lane = tid & 63u;
 #pragma unroll
-  for (uint32_t w = 0; w < 4; ++w) s_h[w][tid] = 0;
-  __syncthreads();
-  const uint32_t b = xcd_tile(blockIdx.x, blocks);
-  const uint32_t base = b * kSortTile;
-  uint32_t key[kSortItems];
+    for (uint32_t w = 0; w < 4; ++w) s_h[w][tid] = 0;  // column tid: read by this thread only below
+    __syncthreads();
+    const uint32_t b = xcd_tile(i, blocks);
+    const uint32_t base = b * kSortTile;
+    uint32_t key[kSortItems];
 #pragma unroll
-  for (uint32_t j = 0; j < kSortItems; ++j) key[j] = keys[min(base + j * 256u + tid, n - 1u)];
-  // Every row's validity first: the gapped layout's per-row scalar loads then share one wait, and the
-  // LDS adds below are not held behind it row by row (a scalar load and an LDS op share a counter).
-  uint32_t valid_rows = 0;
+    for (uint32_t j = 0; j < kSortItems; ++j) key[j] = keys[min(base + j * 256u + tid, n - 1u)];
+    // Every row's validity first: the gapped layout's per-row scalar loads then share one wait, and the
+    // LDS adds below are not held behind it row by row (a scalar load and an LDS op share a counter).
+    uint32_t valid_rows = 0;
 #pragma unroll
-  for (uint32_t j = 0; j < kSortItems; ++j) {
-    const uint32_t row = base + j * 256u + wave * 64u;
-    valid_rows |= sort_valid(row + lane, n, row_limit(seg, row, n)) ? 1u << j : 0u;
-  }
-  // The previous lane's digit of every row, all the lane moves in flight together.
-  uint32_t prev[kSortItems];
-  if constexpr (RUNS) {
-#pragma unroll
-    for (uint32_t j = 0; j < kSortItems; ++j) prev[j] = __shfl_up((key[j] >> shift) & mask, 1u, 64);
-  }
-#pragma unroll
-  for (uint32_t j = 0; j < kSortItems; ++j) {
-    const bool valid = (valid_rows >> j) & 1u;
-    const uint32_t d = (key[j] >> shift) & mask;
-    if constexpr (RUNS) {
-      const bool start = valid && (lane == 0u || prev[j] != d);
-      const uint64_t starts = __ballot(start);
-      const uint32_t keys_in_row = static_cast<uint32_t>(__popcll(__ballot(valid)));
-      const uint64_t later = starts & ~((2ull << lane) - 1ull);  // starts after this lane
-      const uint32_t end = later != 0ull ? static_cast<uint32_t>(__builtin_ctzll(later)) : keys_in_row;
-      if (start) atomicAdd(&s_h[wave][d], end - lane);
-    } else {
-      if (valid) atomicAdd(&s_h[wave][d], 1u);
+    for (uint32_t j = 0; j < kSortItems; ++j) {
+      const uint32_t row = base + j * 256u + wave * 64u;
+      valid_rows |= sort_valid(row + lane, n, row_limit(seg, row, n)) ? 1u << j : 0u;
     }
+    // The previous lane's digit of every row, all the lane moves in flight together.
+    uint32_t prev[kSortItems];
+    if constexpr (RUNS) {
+#pragma unroll
+      for (uint32_t j = 0; j < kSortItems; ++j) prev[j] = __shfl_up((key[j] >> shift) & mask, 1u, 64);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kSortItems; ++j) {
+      const bool valid = (valid_rows >> j) & 1u;
+      const uint32_t d = (key[j] >> shift) & mask;
+      if constexpr (RUNS) {
+        const bool start = valid && (lane == 0u || prev[j] != d);
+        const uint64_t starts = __ballot(start);
+        const uint32_t keys_in_row = static_cast<uint32_t>(__popcll(__ballot(valid)));
+        const uint64_t later = starts & ~((2ull << lane) - 1ull);  // starts after this lane
+        const uint32_t end = later != 0ull ? static_cast<uint32_t>(__builtin_ctzll(later)) : keys_in_row;
+        if (start) atomicAdd(&s_h[wave][d], end - lane);
+      } else {
+        if (valid) atomicAdd(&s_h[wave][d], 1u);
+      }
+    }
+    __syncthreads();
+    counts[static_cast<size_t>(tid) * stride + b] = s_h[0][tid] + s_h[1][tid] + s_h[2][tid] + s_h[3][tid];
   }
-  __syncthreads();
-  counts[static_cast<size_t>(tid) * stride + b] = s_h[0][tid] + s_h[1][tid] + s_h[2][tid] + s_h[3][tid];
 }
 
 // Reduce then scan, step 2: block d scans digit d's row of tile counts in place (exclusive, from 0)
@@ -843,31 +863,32 @@ __device__ __forceinline__ uint64_t wave_inclusive_scan64(uint64_t v) {
 __global__ __launch_bounds__(256) void k_bin_count(FrameArgs a, const uint32_t* __restrict__ order) {
   constexpr uint32_t kTile = 256u * kBinItems;
   __shared__ uint32_t s_scan[4];
-  const uint32_t tid = threadIdx.x, b = blockIdx.x;
+  const uint32_t tid = threadIdx.x;
   const uint32_t n = a.ctrl->visible;
   const uint32_t blocks = (n + kTile - 1u) / kTile;
-  if (b >= blocks) return;
-  // All loads first (indices clamped to the last survivor), so a thread waits for two round trips
-  // rather than two per item.
-  uint32_t value[kBinItems];
-  uint64_t rc[kBinItems];
+  for (uint32_t b = blockIdx.x; b < blocks; b += gridDim.x) {
+    // All loads first (indices clamped to the last survivor), so a thread waits for two round trips
+    // rather than two per item.
+    uint32_t value[kBinItems];
+    uint64_t rc[kBinItems];
 #pragma unroll
-  for (uint32_t k = 0; k < kBinItems; ++k) value[k] = order[min(b * kTile + k * 256u + tid, n - 1u)];
+    for (uint32_t k = 0; k < kBinItems; ++k) value[k] = order[min(b * kTile + k * 256u + tid, n - 1u)];
 #pragma unroll
-  for (uint32_t k = 0; k < kBinItems; ++k) rc[k] = a.rects[value[k]];
-  uint32_t sum = 0;
+    for (uint32_t k = 0; k < kBinItems; ++k) rc[k] = a.rects[value[k]];
+    uint32_t sum = 0;
 #pragma unroll
-  for (uint32_t k = 0; k < kBinItems; ++k) {
-    const uint32_t i = b * kTile + k * 256u + tid;
-    if (i < n) {
-      a.keys[0][i] = static_cast<uint32_t>(rc[k]);
-      a.keys[1][i] = static_cast<uint32_t>(rc[k] >> 32);
-      sum += rect_pairs(rc[k]);
+    for (uint32_t k = 0; k < kBinItems; ++k) {
+      const uint32_t i = b * kTile + k * 256u + tid;
+      if (i < n) {
+        a.keys[0][i] = static_cast<uint32_t>(rc[k]);
+        a.keys[1][i] = static_cast<uint32_t>(rc[k] >> 32);
+        sum += rect_pairs(rc[k]);
+      }
     }
+    uint32_t total;
+    (void)block_exclusive_scan(sum, s_scan, &total);
+    if (tid == 0) a.bin_starts[b] = total;
   }
-  uint32_t total;
-  (void)block_exclusive_scan(sum, s_scan, &total);
-  if (tid == 0) a.bin_starts[b] = total;
 }
 
 // Binning, the scan: one block turns the block totals into block starts in place (exclusive,
@@ -934,7 +955,8 @@ __global__ __launch_bounds__(256) void k_bin_scan(FrameArgs a) {
 // reduce-then-scan passes (`digit_mask` != 0) it also counts that pass's digits (the tile id's low
 // bits) in LDS and writes the tile's counts, which the pass would otherwise read every key again for.
 constexpr uint32_t kEmitChunk = kSortTile;
-__global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __restrict__ order, uint32_t digit_mask) {
+__global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __restrict__ order, uint32_t digit_mask,
+                                             uint32_t stride) {
   constexpr uint32_t kItems = kBinItems;
   constexpr uint32_t kTile = 256u * kItems;
   __shared__ uint32_t s_scan[4];
@@ -943,129 +965,134 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
   __shared__ uint32_t s_h[4][kRadix];
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
   const uint32_t total = a.ctrl->sort_pairs;
-  const uint32_t e0 = blockIdx.x * kEmitChunk;
-  if (e0 >= total) return;
-  const uint32_t e1 = min(e0 + kEmitChunk, total);
+  const uint32_t chunks = (total + kEmitChunk - 1u) / kEmitChunk;
   const uint32_t n = a.ctrl->visible;
   const uint32_t blocks = (n + kTile - 1u) / kTile;
   const uint32_t* starts = a.bin_starts;  // exclusive pair start of each survivor block
+  // Chunks blockIdx.x, blockIdx.x + gridDim.x, ... (a grid smaller than the capacity's chunks walks
+  // them); the tile counts go to counts[d * stride + chunk], as k_tile_counts lays them out (stride:
+  // the pair capacity's tiles).
+  for (uint32_t chunk = blockIdx.x; chunk < chunks; chunk += gridDim.x) {
+    const uint32_t e0 = chunk * kEmitChunk;
+    const uint32_t e1 = min(e0 + kEmitChunk, total);
 #pragma unroll
-  for (uint32_t w = 0; w < 4; ++w) s_h[w][tid] = 0u;  // published by the first __syncthreads below
-  // The survivor block of pair e0: the last block whose start is <= e0.
-  uint32_t lo = 0, hi = blocks;
-  while (hi - lo > 1u) {
-    const uint32_t step = (hi - lo + 63u) / 64u;
-    const uint32_t probe = lo + lane * step;
-    const uint64_t ok = __ballot(probe < hi && starts[probe] <= e0);
-    lo += (63u - static_cast<uint32_t>(__builtin_clzll(ok))) * step;
-    hi = min(hi, lo + step);
-  }
-  s_mark[wave][lane] = 0u;
-  for (uint32_t b = lo; b < blocks; ++b) {
-    const uint32_t block_start = starts[b];
-    if (block_start >= e1) break;
-    uint32_t values[kItems], lo_words[kItems], hi_words[kItems];
-#pragma unroll
-    for (uint32_t k = 0; k < kItems; ++k) {
-      const uint32_t i = min(b * kTile + k * 256u + tid, n - 1u);
-      values[k] = order[i];
-      lo_words[k] = a.keys[0][i];
-      hi_words[k] = a.keys[1][i];
+    for (uint32_t w = 0; w < 4; ++w) s_h[w][tid] = 0u;  // published by the first __syncthreads below
+    // The survivor block of pair e0: the last block whose start is <= e0.
+    uint32_t lo = 0, hi = blocks;
+    while (hi - lo > 1u) {
+      const uint32_t step = (hi - lo + 63u) / 64u;
+      const uint32_t probe = lo + lane * step;
+      const uint64_t ok = __ballot(probe < hi && starts[probe] <= e0);
+      lo += (63u - static_cast<uint32_t>(__builtin_clzll(ok))) * step;
+      hi = min(hi, lo + step);
     }
-    __syncthreads();  // the previous block's table is no longer read
-    uint32_t aggregate = 0;
+    s_mark[wave][lane] = 0u;
+    for (uint32_t b = lo; b < blocks; ++b) {
+      const uint32_t block_start = starts[b];
+      if (block_start >= e1) break;
+      uint32_t values[kItems], lo_words[kItems], hi_words[kItems];
 #pragma unroll
-    for (uint32_t k = 0; k < kItems; ++k) {
-      const uint32_t slot = k * 256u + tid;
-      const uint32_t i = b * kTile + slot;
-      uint32_t count = 0, value = 0, base = 0, tiles_x = a.tiles_x;
-      Rect r{0, 0, -1, -1};
-      if (i < n) {
-        value = values[k];
-        const uint64_t rc = static_cast<uint64_t>(lo_words[k]) | (static_cast<uint64_t>(hi_words[k]) << 32);
-        if (rc != kNoRect) {
-          r.tx0 = static_cast<int32_t>(rc & 4095u);
-          r.ty0 = static_cast<int32_t>((rc >> 12) & 4095u);
-          r.tx1 = static_cast<int32_t>((rc >> 24) & 4095u);
-          r.ty1 = static_cast<int32_t>((rc >> 36) & 4095u);
-          count = static_cast<uint32_t>((r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1));
-          if (a.views != nullptr) {
-            const uint32_t view = static_cast<uint32_t>(rc >> 48);
-            const ViewInfo& vi = a.views[view];
-            base = a.local_tile_bits != 0u ? view << a.local_tile_bits : vi.tile_base;
-            tiles_x = vi.tiles_x;
+      for (uint32_t k = 0; k < kItems; ++k) {
+        const uint32_t i = min(b * kTile + k * 256u + tid, n - 1u);
+        values[k] = order[i];
+        lo_words[k] = a.keys[0][i];
+        hi_words[k] = a.keys[1][i];
+      }
+      __syncthreads();  // the previous block's table is no longer read
+      uint32_t aggregate = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < kItems; ++k) {
+        const uint32_t slot = k * 256u + tid;
+        const uint32_t i = b * kTile + slot;
+        uint32_t count = 0, value = 0, base = 0, tiles_x = a.tiles_x;
+        Rect r{0, 0, -1, -1};
+        if (i < n) {
+          value = values[k];
+          const uint64_t rc = static_cast<uint64_t>(lo_words[k]) | (static_cast<uint64_t>(hi_words[k]) << 32);
+          if (rc != kNoRect) {
+            r.tx0 = static_cast<int32_t>(rc & 4095u);
+            r.ty0 = static_cast<int32_t>((rc >> 12) & 4095u);
+            r.tx1 = static_cast<int32_t>((rc >> 24) & 4095u);
+            r.ty1 = static_cast<int32_t>((rc >> 36) & 4095u);
+            count = static_cast<uint32_t>((r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1));
+            if (a.views != nullptr) {
+              const uint32_t view = static_cast<uint32_t>(rc >> 48);
+              const ViewInfo& vi = a.views[view];
+              base = a.local_tile_bits != 0u ? view << a.local_tile_bits : vi.tile_base;
+              tiles_x = vi.tiles_x;
+            }
           }
         }
+        uint32_t row_total;
+        const uint32_t excl = block_exclusive_scan(count, s_scan, &row_total);
+        s_excl[slot] = aggregate + excl;
+        s_val[slot] = value;
+        s_origin[slot] = static_cast<uint32_t>(r.tx0) | (static_cast<uint32_t>(r.ty0) << 16);
+        s_width[slot] = static_cast<uint32_t>(r.tx1 - r.tx0 + 1) | (tiles_x << 16);
+        s_base[slot] = base;
+        aggregate += row_total;
       }
-      uint32_t row_total;
-      const uint32_t excl = block_exclusive_scan(count, s_scan, &row_total);
-      s_excl[slot] = aggregate + excl;
-      s_val[slot] = value;
-      s_origin[slot] = static_cast<uint32_t>(r.tx0) | (static_cast<uint32_t>(r.ty0) << 16);
-      s_width[slot] = static_cast<uint32_t>(r.tx1 - r.tx0 + 1) | (tiles_x << 16);
-      s_base[slot] = base;
-      aggregate += row_total;
-    }
-    if (tid == 0) s_excl[kTile] = aggregate;
-    __syncthreads();
-    auto find = [&](uint32_t e) {  // last slot with s_excl[slot] <= e
-      uint32_t l = 0, h = kTile - 1u;
-      while (l < h) {
-        const uint32_t mid = (l + h + 1u) / 2u;
-        if (s_excl[mid] <= e) l = mid; else h = mid - 1u;
-      }
-      return l;
-    };
-    // This block's pairs in [e0, e1), as block-local indices [first_pair, end_pair).
-    const uint32_t first_pair = e0 > block_start ? e0 - block_start : 0u;
-    const uint32_t end_pair = min(e1 - block_start, aggregate);
-    const uint32_t c_first = first_pair / 64u, c_end = (end_pair + 63u) / 64u;
-    const uint32_t chunks = c_end > c_first ? c_end - c_first : 0u;
-    const uint32_t c0 = c_first + chunks * wave / 4u, c1 = c_first + chunks * (wave + 1u) / 4u;
-    uint32_t s0 = c0 < c1 ? find(c0 * 64u) : 0u;
-    for (uint32_t c = c0; c < c1; ++c) {
-      const uint32_t first = c * 64u;
-      const uint32_t s = s0 + lane;
-      if (s < kTile) {
-        const uint32_t ex = s_excl[s], cnt = s_excl[s + 1u] - ex;
-        if (cnt != 0u && ex < first + 64u && ex + cnt > first)
-          s_mark[wave][ex > first ? ex - first : 0u] = s + 1u;
-      }
-      const uint32_t m = wave_inclusive_max(s_mark[wave][lane]);
-      s_mark[wave][lane] = 0u;
-      const uint32_t e = first + lane;
-      uint32_t owner = m != 0u ? m - 1u : 0u;
-      if (e >= first_pair && e < end_pair) {
-        if (m == 0u || e < s_excl[owner] || e >= s_excl[owner + 1u]) owner = find(e);
-        const uint32_t k = e - s_excl[owner];
-        const uint32_t wdt = s_width[owner] & 0xffffu, stride = s_width[owner] >> 16;
-        // k / wdt from a reciprocal estimate, corrected to the exact quotient.
-        uint32_t q = static_cast<uint32_t>(static_cast<float>(k) * __builtin_amdgcn_rcpf(static_cast<float>(wdt)));
-        int32_t rem = static_cast<int32_t>(k - q * wdt);
-        while (rem < 0) {
-          --q;
-          rem += static_cast<int32_t>(wdt);
+      if (tid == 0) s_excl[kTile] = aggregate;
+      __syncthreads();
+      auto find = [&](uint32_t e) {  // last slot with s_excl[slot] <= e
+        uint32_t l = 0, h = kTile - 1u;
+        while (l < h) {
+          const uint32_t mid = (l + h + 1u) / 2u;
+          if (s_excl[mid] <= e) l = mid; else h = mid - 1u;
         }
-        while (rem >= static_cast<int32_t>(wdt)) {
-          ++q;
-          rem -= static_cast<int32_t>(wdt);
+        return l;
+      };
+      // This block's pairs in [e0, e1), as block-local indices [first_pair, end_pair).
+      const uint32_t first_pair = e0 > block_start ? e0 - block_start : 0u;
+      const uint32_t end_pair = min(e1 - block_start, aggregate);
+      const uint32_t c_first = first_pair / 64u, c_end = (end_pair + 63u) / 64u;
+      const uint32_t chunks = c_end > c_first ? c_end - c_first : 0u;
+      const uint32_t c0 = c_first + chunks * wave / 4u, c1 = c_first + chunks * (wave + 1u) / 4u;
+      uint32_t s0 = c0 < c1 ? find(c0 * 64u) : 0u;
+      for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t first = c * 64u;
+        const uint32_t s = s0 + lane;
+        if (s < kTile) {
+          const uint32_t ex = s_excl[s], cnt = s_excl[s + 1u] - ex;
+          if (cnt != 0u && ex < first + 64u && ex + cnt > first)
+            s_mark[wave][ex > first ? ex - first : 0u] = s + 1u;
         }
-        const uint32_t tx = (s_origin[owner] & 0xffffu) + static_cast<uint32_t>(rem);
-        const uint32_t ty = (s_origin[owner] >> 16) + q;
-        const uint32_t idx = block_start + e;  // < total <= pair capacity
-        const uint32_t tile = s_base[owner] + ty * stride + tx;
-        a.pair_keys[0][idx] = tile;
-        a.pair_vals[0][idx] = s_val[owner];
-        if (digit_mask != 0u) atomicAdd(&s_h[wave][tile & digit_mask], 1u);
+        const uint32_t m = wave_inclusive_max(s_mark[wave][lane]);
+        s_mark[wave][lane] = 0u;
+        const uint32_t e = first + lane;
+        uint32_t owner = m != 0u ? m - 1u : 0u;
+        if (e >= first_pair && e < end_pair) {
+          if (m == 0u || e < s_excl[owner] || e >= s_excl[owner + 1u]) owner = find(e);
+          const uint32_t k = e - s_excl[owner];
+          const uint32_t wdt = s_width[owner] & 0xffffu, stride = s_width[owner] >> 16;
+          // k / wdt from a reciprocal estimate, corrected to the exact quotient.
+          uint32_t q = static_cast<uint32_t>(static_cast<float>(k) * __builtin_amdgcn_rcpf(static_cast<float>(wdt)));
+          int32_t rem = static_cast<int32_t>(k - q * wdt);
+          while (rem < 0) {
+            --q;
+            rem += static_cast<int32_t>(wdt);
+          }
+          while (rem >= static_cast<int32_t>(wdt)) {
+            ++q;
+            rem -= static_cast<int32_t>(wdt);
+          }
+          const uint32_t tx = (s_origin[owner] & 0xffffu) + static_cast<uint32_t>(rem);
+          const uint32_t ty = (s_origin[owner] >> 16) + q;
+          const uint32_t idx = block_start + e;  // < total <= pair capacity
+          const uint32_t tile = s_base[owner] + ty * stride + tx;
+          a.pair_keys[0][idx] = tile;
+          a.pair_vals[0][idx] = s_val[owner];
+          if (digit_mask != 0u) atomicAdd(&s_h[wave][tile & digit_mask], 1u);
+        }
+        s0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(owner), 63));
       }
-      s0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(owner), 63));
     }
-  }
-  if (digit_mask != 0u) {
-    __syncthreads();
-    // counts[d * stride + tile] as k_tile_counts lays them out (stride: the pair capacity's tiles).
-    if (tid <= digit_mask)
-      a.sort_counts[static_cast<size_t>(tid) * gridDim.x + blockIdx.x] = s_h[0][tid] + s_h[1][tid] + s_h[2][tid] + s_h[3][tid];
+    if (digit_mask != 0u) {
+      __syncthreads();
+      if (tid <= digit_mask)
+        a.sort_counts[static_cast<size_t>(tid) * stride + chunk] =
+            s_h[0][tid] + s_h[1][tid] + s_h[2][tid] + s_h[3][tid];
+    }
   }
 }
 
@@ -1078,31 +1105,31 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ key
                                                 uint32_t* ranges, const ViewInfo* __restrict__ views,
                                                 uint32_t local_bits) {
   const uint32_t n = *n_ptr;
-  const uint32_t i0 = (blockIdx.x * 256u + threadIdx.x) * 4u;
-  if (i0 >= n) return;
-  uint32_t k[6];
-  if (i0 + 4u <= n) {
-    const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
-    k[1] = v.x;
-    k[2] = v.y;
-    k[3] = v.z;
-    k[4] = v.w;
-  } else {
+  for (uint32_t i0 = (blockIdx.x * 256u + threadIdx.x) * 4u; i0 < n; i0 += gridDim.x * 1024u) {
+    uint32_t k[6];
+    if (i0 + 4u <= n) {
+      const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
+      k[1] = v.x;
+      k[2] = v.y;
+      k[3] = v.z;
+      k[4] = v.w;
+    } else {
 #pragma unroll
-    for (uint32_t j = 0; j < 4; ++j) k[1 + j] = i0 + j < n ? keys[i0 + j] : ~0u;
-  }
-  k[0] = i0 > 0u ? keys[i0 - 1u] : ~0u;
-  k[5] = i0 + 4u < n ? keys[i0 + 4u] : ~0u;
+      for (uint32_t j = 0; j < 4; ++j) k[1 + j] = i0 + j < n ? keys[i0 + j] : ~0u;
+    }
+    k[0] = i0 > 0u ? keys[i0 - 1u] : ~0u;
+    k[5] = i0 + 4u < n ? keys[i0 + 4u] : ~0u;
 #pragma unroll
-  for (uint32_t j = 0; j < 4; ++j) {
-    const uint32_t i = i0 + j;
-    if (i >= n) break;
-    const uint32_t t = k[1 + j];
-    const bool first = i == 0u || k[j] != t, last = i + 1u == n || k[2 + j] != t;
-    if (!first && !last) continue;
-    const uint32_t g = local_bits != 0u ? views[t >> local_bits].tile_base + (t & ((1u << local_bits) - 1u)) : t;
-    if (first) ranges[2 * g] = i;
-    if (last) ranges[2 * g + 1] = i + 1u;
+    for (uint32_t j = 0; j < 4; ++j) {
+      const uint32_t i = i0 + j;
+      if (i >= n) break;
+      const uint32_t t = k[1 + j];
+      const bool first = i == 0u || k[j] != t, last = i + 1u == n || k[2 + j] != t;
+      if (!first && !last) continue;
+      const uint32_t g = local_bits != 0u ? views[t >> local_bits].tile_base + (t & ((1u << local_bits) - 1u)) : t;
+      if (first) ranges[2 * g] = i;
+      if (last) ranges[2 * g + 1] = i + 1u;
+    }
   }
 }
 
@@ -1563,6 +1590,19 @@ uint32_t grid_for(uint32_t items, uint32_t per_block) {
   return b == 0u ? 1u : b;
 }
 
+// The grid of a kernel whose blocks walk their work in strides of the grid: `estimate` items (0: the
+// capacity) in blocks of `per`, rounded up to a multiple of the XCD count, at most the capacity's
+// blocks. Scratch is sized for the largest frame (the whole world, or a tree cut's capacity), and a
+// block per unit of it spends microseconds dispatching blocks that find no work; a grid from the
+// counts of earlier frames keeps the hardware's in-order dispatch of the blocks that do work, and a
+// frame with more than the estimate loops.
+uint32_t work_grid(uint32_t capacity, uint32_t estimate, uint32_t per) {
+  const uint32_t full = grid_for(capacity, per);
+  if (estimate == 0u || estimate >= capacity) return full;
+  const uint32_t b = (grid_for(estimate, per) + kXcds - 1u) / kXcds * kXcds;
+  return std::min(b, full);
+}
+
 // Blocks of the histogram kernel: enough to stream the keys, few enough that the per-block global
 // adds into each of the histogram's words stay short (each word takes one add per block).
 uint32_t hist_grid(uint32_t capacity) { return std::min(std::max(grid_for(capacity, 16384u), 16u), 2048u); }
@@ -1572,9 +1612,10 @@ uint32_t hist_grid(uint32_t capacity) { return std::min(std::max(grid_for(capaci
 // holds 256 words per kSortTile-key tile of `capacity` (reduce then scan only).
 template <bool LOOKBACK, int BITS>
 void sort_pass_bits(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
-                    const uint32_t* n_ptr, uint32_t tiles, uint32_t* hist, uint32_t shift, uint32_t bits,
-                    uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass, uint32_t* counts,
-                    const uint32_t* seg, bool runs, bool counted, uint32_t view_keys, hipStream_t s) {
+                    const uint32_t* n_ptr, uint32_t tiles, uint32_t grid, uint32_t* hist, uint32_t shift,
+                    uint32_t bits, uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass,
+                    uint32_t* counts, const uint32_t* seg, bool runs, bool counted, uint32_t view_keys,
+                    hipStream_t s) {
   const uint32_t mask = (1u << bits) - 1u;
   if constexpr (LOOKBACK) {
     hipLaunchKernelGGL((k_onesweep<true, BITS>), dim3(tiles), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist,
@@ -1583,49 +1624,53 @@ void sort_pass_bits(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, ui
     if (counted) {
       // The producer of the keys has written this pass's tile counts.
     } else if (runs)
-      hipLaunchKernelGGL(k_tile_counts<true>, dim3(tiles), dim3(256), 0, s, kin, n_ptr, shift, mask, counts, tiles,
+      hipLaunchKernelGGL(k_tile_counts<true>, dim3(grid), dim3(256), 0, s, kin, n_ptr, shift, mask, counts, tiles,
                          seg);
     else
-      hipLaunchKernelGGL(k_tile_counts<false>, dim3(tiles), dim3(256), 0, s, kin, n_ptr, shift, mask, counts, tiles,
+      hipLaunchKernelGGL(k_tile_counts<false>, dim3(grid), dim3(256), 0, s, kin, n_ptr, shift, mask, counts, tiles,
                          seg);
     hipLaunchKernelGGL(k_digit_scan, dim3(1u << bits), dim3(256), 0, s, counts, n_ptr, hist, tiles);
-    hipLaunchKernelGGL((k_onesweep<false, BITS>), dim3(tiles), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist,
+    hipLaunchKernelGGL((k_onesweep<false, BITS>), dim3(grid), dim3(256), 0, s, kin, vin, kout, vout, n_ptr, hist,
                        shift, mask, status, ctrl, ticket, tag_pass, counts, tiles, seg, view_keys);
   }
 }
 
 template <bool LOOKBACK>
 void sort_pass_variant(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
-                       const uint32_t* n_ptr, uint32_t tiles, uint32_t* hist, uint32_t shift, uint32_t bits,
-                       uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass, uint32_t* counts,
-                       const uint32_t* seg, bool runs, bool counted, uint32_t view_keys, hipStream_t s) {
+                       const uint32_t* n_ptr, uint32_t tiles, uint32_t grid, uint32_t* hist, uint32_t shift,
+                       uint32_t bits, uint64_t* status, Ctrl* ctrl, uint32_t ticket, uint32_t tag_pass,
+                       uint32_t* counts, const uint32_t* seg, bool runs, bool counted, uint32_t view_keys,
+                       hipStream_t s) {
   if (bits <= 6)
-    sort_pass_bits<LOOKBACK, 6>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
+    sort_pass_bits<LOOKBACK, 6>(kin, vin, kout, vout, n_ptr, tiles, grid, hist, shift, bits, status, ctrl, ticket,
                                 tag_pass, counts, seg, runs, counted, view_keys, s);
   else if (bits == 7)
-    sort_pass_bits<LOOKBACK, 7>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
+    sort_pass_bits<LOOKBACK, 7>(kin, vin, kout, vout, n_ptr, tiles, grid, hist, shift, bits, status, ctrl, ticket,
                                 tag_pass, counts, seg, runs, counted, view_keys, s);
   else
-    sort_pass_bits<LOOKBACK, 8>(kin, vin, kout, vout, n_ptr, tiles, hist, shift, bits, status, ctrl, ticket,
+    sort_pass_bits<LOOKBACK, 8>(kin, vin, kout, vout, n_ptr, tiles, grid, hist, shift, bits, status, ctrl, ticket,
                                 tag_pass, counts, seg, runs, counted, view_keys, s);
 }
 
 // Pass p of a sort under its digit plan; `seg` is the cull's gapped layout (first depth pass) or
 // null. The last pass, on the key's top digit, counts runs of equal digits unless `top_runs` is
 // false; `counted`: the keys' producer has written the pass's tile counts; `view_keys`: the pass
-// writes the values' views as keys (k_onesweep).
+// writes the values' views as keys (k_onesweep); `estimate`: the keys expected (0: the capacity),
+// which sizes the reduce-then-scan kernels' grids (the look-back variant takes a block per tile).
 void sort_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, const uint32_t* n_ptr,
                uint32_t capacity, uint32_t* hist, const DigitPlan& plan, int p, uint64_t* status, Ctrl* ctrl,
                uint32_t ticket, uint32_t tag_pass, uint32_t* counts, int variant, const uint32_t* seg,
-               hipStream_t s, bool counted = false, bool top_runs = true, uint32_t view_keys = 0u) {
+               hipStream_t s, bool counted = false, bool top_runs = true, uint32_t view_keys = 0u,
+               uint32_t estimate = 0u) {
   const uint32_t tiles = grid_for(capacity, kSortTile);
   const bool runs = top_runs && p + 1 == plan.passes;
   if (variant == kSortOnesweep)
-    sort_pass_variant<true>(kin, vin, kout, vout, n_ptr, tiles, hist, plan.shift[p], plan.bits[p], status, ctrl,
-                            ticket, tag_pass, counts, seg, runs, counted, view_keys, s);
+    sort_pass_variant<true>(kin, vin, kout, vout, n_ptr, tiles, tiles, hist, plan.shift[p], plan.bits[p], status,
+                            ctrl, ticket, tag_pass, counts, seg, runs, counted, view_keys, s);
   else
-    sort_pass_variant<false>(kin, vin, kout, vout, n_ptr, tiles, hist, plan.shift[p], plan.bits[p], status, ctrl,
-                             ticket, tag_pass, counts, seg, runs, counted, view_keys, s);
+    sort_pass_variant<false>(kin, vin, kout, vout, n_ptr, tiles, work_grid(capacity, estimate, kSortTile), hist,
+                             plan.shift[p], plan.bits[p], status, ctrl, ticket, tag_pass, counts, seg, runs, counted,
+                             view_keys, s);
 }
 
 }  // namespace
@@ -1730,7 +1775,8 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
                      lod != nullptr ? &lod->ctrl->final_limited : nullptr);
   const uint32_t view_lanes = grid_for(a.total, 256u * kCullItems) * 256u * kCullItems;
   const uint32_t lanes = a.active != nullptr ? a.capacity : view_lanes * a.view_count;
-  const uint32_t cull_blocks = grid_for(lanes, 256u * kCullItems);
+  const uint32_t cull_blocks = a.active != nullptr ? work_grid(lanes, a.lane_estimate, 256u * kCullItems)
+                                                   : grid_for(lanes, 256u * kCullItems);
   switch (a.world.sh_degree) {
     case 0: launch_cull_deg<0>(a, cull_blocks, s); break;
     case 1: launch_cull_deg<1>(a, cull_blocks, s); break;
@@ -1756,7 +1802,8 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
     const uint32_t* vin = p == 0 ? nullptr : a.order[(p - 1) & 1];
     sort_pass(kin, vin, kout, a.order[p & 1], p == 0 ? cull_lanes : visible, a.capacity, a.hist + p * 256, depth_plan, p,
               a.sort_status, a.ctrl, kTicketDepthSort + p, 2u + p, a.sort_counts, a.sort_variant,
-              p == 0 ? a.cull_counts : nullptr, s, false, true, last && view_sort ? a.view_count : 0u);
+              p == 0 ? a.cull_counts : nullptr, s, false, true, last && view_sort ? a.view_count : 0u,
+              p == 0 ? a.lane_estimate : a.survivor_estimate);
   }
   int survivor_passes = depth_passes;
   if (view_sort) {
@@ -1767,17 +1814,19 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
       const int q = survivor_passes;
       sort_pass(a.keys[(q - 1) & 1], a.order[(q - 1) & 1], v + 1 < view_plan.passes ? a.keys[q & 1] : nullptr,
                 a.order[q & 1], visible, a.capacity, a.hist + v * 256, view_plan, v, a.sort_status, a.ctrl,
-                kTicketDepthSort, 2u, a.sort_counts, kSortReduceScan, nullptr, s, false, false);
+                kTicketDepthSort, 2u, a.sort_counts, kSortReduceScan, nullptr, s, false, false, 0u,
+                a.survivor_estimate);
     }
   }
   const uint32_t* order = a.order[(survivor_passes - 1) & 1];
   if (marks) (void)hipEventRecord(marks[3], s);
-  hipLaunchKernelGGL(k_bin_count, dim3(grid_for(a.capacity, 256u * kBinItems)), dim3(256), 0, s, a, order);
+  hipLaunchKernelGGL(k_bin_count, dim3(work_grid(a.capacity, a.survivor_estimate, 256u * kBinItems)), dim3(256), 0, s,
+                     a, order);
   hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, s, a);
   // With reduce-then-scan passes the emission counts the tile sort's first digits itself.
   const bool bin_counts = a.sort_variant != kSortOnesweep && tile_passes > 0;
-  hipLaunchKernelGGL(k_bin, dim3(grid_for(a.pair_capacity, kEmitChunk)), dim3(256), 0, s, a, order,
-                     bin_counts ? (1u << tile_plan.bits[0]) - 1u : 0u);
+  hipLaunchKernelGGL(k_bin, dim3(work_grid(a.pair_capacity, a.pair_estimate, kEmitChunk)), dim3(256), 0, s, a, order,
+                     bin_counts ? (1u << tile_plan.bits[0]) - 1u : 0u, grid_for(a.pair_capacity, kSortTile));
   const uint32_t* sort_n = &a.ctrl->sort_pairs;
   uint32_t* tile_hist = a.hist + 4 * 256;
   if (a.sort_variant == kSortOnesweep)
@@ -1786,10 +1835,10 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
   for (int p = 0; p < tile_passes; ++p) {
     sort_pass(a.pair_keys[p & 1], a.pair_vals[p & 1], a.pair_keys[(p + 1) & 1], a.pair_vals[(p + 1) & 1],
               sort_n, a.pair_capacity, tile_hist + p * 256, tile_plan, p, a.sort_status, a.ctrl, kTicketTileSort + p,
-              6u + p, a.sort_counts, a.sort_variant, nullptr, s, p == 0 && bin_counts);
+              6u + p, a.sort_counts, a.sort_variant, nullptr, s, p == 0 && bin_counts, true, 0u, a.pair_estimate);
   }
   const int fin = tile_passes & 1;
-  hipLaunchKernelGGL(k_ranges, dim3(grid_for(a.pair_capacity, 1024u)), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_ranges, dim3(work_grid(a.pair_capacity, a.pair_estimate, 1024u)), dim3(256), 0, s,
                      a.pair_keys[fin], sort_n, a.tile_ranges, a.views, a.local_tile_bits);
   if (a.tile_order != nullptr) {
     uint32_t* counters = a.hist + kOrderCounters;

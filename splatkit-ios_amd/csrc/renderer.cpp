@@ -313,6 +313,10 @@ struct skm_renderer {
   uint64_t drops = 0;
   double stage_ms[6] = {};
   uint32_t last_draw_count = 0;
+  // The latest finished frame's cull lanes, survivors and tile pairs (valid once counts_known), from
+  // which the next frames' grids are estimated.
+  uint32_t last_lanes = 0, last_pairs = 0;
+  bool counts_known = false;
   skm_frame_totals totals{};
 
   // View batches (skm_draw_views).
@@ -372,6 +376,9 @@ struct skm_renderer {
       threshold.update(st->count, limited, s.lod_limit);
     }
     last_draw_count = c->visible;
+    last_lanes = c->cull_lanes;
+    last_pairs = c->pairs;
+    counts_known = true;
     last_dropped = c->overflow != 0;
     if (last_dropped) {
       // Counted apart: a dropped frame's work stopped after the binning, so its timings and pairs
@@ -813,6 +820,7 @@ skm_status skm_upload_world(skm_renderer* r, const skm_cloud_view* cloud, int ma
   r->lod = LodBuffers{};
   for (Slot& sl : r->active()) sl.cut = LodCut{};  // a flat world keeps no cut state
   r->completed_world_frame = false;
+  r->counts_known = false;
   return r->check(hipStreamSynchronize(r->stream), "upload");
 }
 
@@ -853,6 +861,7 @@ skm_status skm_upload_lod_world(skm_renderer* r, const skm_lod_tree_view* tree, 
   r->lod_world = true;
   r->threshold.reset(r->policy.lod_error_pixels);
   r->completed_world_frame = false;
+  r->counts_known = false;
   return r->check(hipStreamSynchronize(r->stream), "upload");
 }
 
@@ -874,6 +883,7 @@ skm_status skm_create_slab(skm_renderer* r, uint32_t capacity, int sh_degree) {
   r->lod = LodBuffers{};
   for (Slot& sl : r->active()) sl.cut = LodCut{};  // a flat world keeps no cut state
   r->completed_world_frame = false;
+  r->counts_known = false;
   return r->check(hipStreamSynchronize(r->stream), "slab");
 }
 
@@ -1017,6 +1027,16 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
   a.view_count = 1;
   a.total_tiles = tiles_x * tiles_y;
   a.sort_variant = r->sort_variant;
+  if (r->counts_known) {
+    // Half again the latest finished frame's counts, and a few blocks more: frames of one camera path
+    // change gradually (a frame beyond the estimate is still drawn whole, its blocks loop).
+    auto ahead = [](uint64_t v, uint32_t slack) {
+      return static_cast<uint32_t>(std::min<uint64_t>(v + v / 2 + slack, 0xffffffffull));
+    };
+    a.lane_estimate = ahead(r->last_lanes, 8u * skm::kCullTile);
+    a.survivor_estimate = ahead(r->last_draw_count, 8u * skm::kSortTile);
+    a.pair_estimate = ahead(r->last_pairs, 8u * skm::kSortTile);
+  }
   a.sort_counts = x.sort_counts.as<uint32_t>();
   a.bin_starts = a.sort_counts + x.sort_status_blocks * skm::kRadix;
   a.raster_px = r->raster_px;
