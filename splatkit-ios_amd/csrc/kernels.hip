@@ -438,72 +438,65 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a) {
   if (blockIdx.x >= blocks) return;
   s_byte255[tid] = static_cast<float>(tid) / 255.0f;
   __syncthreads();
-  // A tree cut's frame (INDEXED) is launched with a grid estimated from earlier frames; when the cut
-  // has more lanes, the blocks walk the logical blocks in strides of the grid (a multiple of the XCD
-  // count, so each block stays in its XCD's range).
-  for (uint32_t i = blockIdx.x; i < blocks; i += gridDim.x) {
-    if (i != blockIdx.x) __syncthreads();  // every read of the previous logical block's s_seg is done
-    const uint32_t b = xcd_tile(i, blocks);
-    const uint32_t view = a.view_count > 1u ? b % a.view_count : 0u;
-    const uint32_t lt = a.view_count > 1u ? b / a.view_count : b;  // the lane tile
-    const Camera& cam = a.cam[view];
-    uint32_t key[kItems], rec[kItems][8];
-    uint64_t ballot[kItems];
+  const uint32_t b = xcd_tile(blockIdx.x, blocks);
+  const uint32_t view = a.view_count > 1u ? b % a.view_count : 0u;
+  const uint32_t lt = a.view_count > 1u ? b / a.view_count : b;  // the lane tile
+  const Camera& cam = a.cam[view];
+  uint32_t key[kItems], rec[kItems][8];
+  uint64_t ballot[kItems];
 #pragma unroll
-    for (uint32_t k = 0; k < kItems; ++k) {
-      const uint32_t t = lt * kTile + k * 256u + tid;
-      bool visible = false;
-      key[k] = 0;
+  for (uint32_t k = 0; k < kItems; ++k) {
+    const uint32_t t = lt * kTile + k * 256u + tid;
+    bool visible = false;
+    key[k] = 0;
 #pragma unroll
-      for (int w = 0; w < 8; ++w) rec[k][w] = 0;
-      if (t < total) {
-        uint32_t index;
-        if constexpr (INDEXED) {
-          index = a.active[t];
-        } else {
-          const uint32_t r = find_range(a.range_starts, a.range_count, t);
-          index = a.ranges[2 * r] + (t - a.range_starts[r]);
-        }
-        visible = cull_lane<DEG, KEY16>(cam, a.world, index, key[k], rec[k], s_byte255);
+    for (int w = 0; w < 8; ++w) rec[k][w] = 0;
+    if (t < total) {
+      uint32_t index;
+      if constexpr (INDEXED) {
+        index = a.active[t];
+      } else {
+        const uint32_t r = find_range(a.range_starts, a.range_count, t);
+        index = a.ranges[2 * r] + (t - a.range_starts[r]);
       }
-      ballot[k] = __ballot(visible);
-      if (lane_id() == 0) s_seg[k * 4 + wave] = static_cast<uint32_t>(__popcll(ballot[k]));
+      visible = cull_lane<DEG, KEY16>(cam, a.world, index, key[k], rec[k], s_byte255);
     }
-    __syncthreads();
-    uint32_t aggregate = 0;
+    ballot[k] = __ballot(visible);
+    if (lane_id() == 0) s_seg[k * 4 + wave] = static_cast<uint32_t>(__popcll(ballot[k]));
+  }
+  __syncthreads();
+  uint32_t aggregate = 0;
 #pragma unroll
-    for (uint32_t g = 0; g < kItems * 4; ++g) aggregate += s_seg[g];
-    if (tid == 0) {
-      a.cull_counts[b] = aggregate;
-      if (b == 0) a.ctrl->cull_lanes = blocks * kTile;
-      if (aggregate != 0u) {
-        atomicAdd(&a.ctrl->visible, aggregate);
-        if (a.view_visible != nullptr) atomicAdd(&a.view_visible[view], aggregate);
-      }
+  for (uint32_t g = 0; g < kItems * 4; ++g) aggregate += s_seg[g];
+  if (tid == 0) {
+    a.cull_counts[b] = aggregate;
+    if (b == 0) a.ctrl->cull_lanes = blocks * kTile;
+    if (aggregate != 0u) {
+      atomicAdd(&a.ctrl->visible, aggregate);
+      if (a.view_visible != nullptr) atomicAdd(&a.view_visible[view], aggregate);
     }
-    uint32_t before = b * kTile;
+  }
+  uint32_t before = b * kTile;
 #pragma unroll
-    for (uint32_t k = 0; k < kItems; ++k) {
-      uint32_t seg = before;
+  for (uint32_t k = 0; k < kItems; ++k) {
+    uint32_t seg = before;
 #pragma unroll
-      for (uint32_t w = 0; w < 4; ++w) seg += (w < wave) ? s_seg[k * 4 + w] : 0u;
-      if ((ballot[k] >> lane_id()) & 1ull) {
-        const uint32_t off = seg + rank_below(ballot[k]);
-        a.cull_keys[off] = key[k];
-        uint4* dst = reinterpret_cast<uint4*>(a.projected + static_cast<size_t>(off) * 8u);
-        dst[0] = make_uint4(rec[k][0], rec[k][1], rec[k][2], rec[k][3]);
-        dst[1] = make_uint4(rec[k][4], rec[k][5], rec[k][6], rec[k][7]);
-        // The binning's tile rectangle, from the record just written (read back in depth order, a
-        // gather of 8 bytes instead of the 32-byte record).
-        Rect r;
-        const uint32_t tiles = tile_rect(rec[k], static_cast<uint32_t>(cam.screen_size[0]),
-                                         static_cast<uint32_t>(cam.screen_size[1]), a.tile, r);
-        a.rects[off] = pack_rect(r, tiles, view);
-      }
-#pragma unroll
-      for (uint32_t w = 0; w < 4; ++w) before += s_seg[k * 4 + w];
+    for (uint32_t w = 0; w < 4; ++w) seg += (w < wave) ? s_seg[k * 4 + w] : 0u;
+    if ((ballot[k] >> lane_id()) & 1ull) {
+      const uint32_t off = seg + rank_below(ballot[k]);
+      a.cull_keys[off] = key[k];
+      uint4* dst = reinterpret_cast<uint4*>(a.projected + static_cast<size_t>(off) * 8u);
+      dst[0] = make_uint4(rec[k][0], rec[k][1], rec[k][2], rec[k][3]);
+      dst[1] = make_uint4(rec[k][4], rec[k][5], rec[k][6], rec[k][7]);
+      // The binning's tile rectangle, from the record just written (read back in depth order, a
+      // gather of 8 bytes instead of the 32-byte record).
+      Rect r;
+      const uint32_t tiles = tile_rect(rec[k], static_cast<uint32_t>(cam.screen_size[0]),
+                                       static_cast<uint32_t>(cam.screen_size[1]), a.tile, r);
+      a.rects[off] = pack_rect(r, tiles, view);
     }
-    if constexpr (!INDEXED) break;  // a range-drawn frame's grid is exactly `blocks`
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) before += s_seg[k * 4 + w];
   }
 }
 
@@ -1775,8 +1768,10 @@ hipError_t launch_frame(const FrameArgs& a, const LodArgs* lod, const DigitPlan&
                      lod != nullptr ? &lod->ctrl->final_limited : nullptr);
   const uint32_t view_lanes = grid_for(a.total, 256u * kCullItems) * 256u * kCullItems;
   const uint32_t lanes = a.active != nullptr ? a.capacity : view_lanes * a.view_count;
-  const uint32_t cull_blocks = a.active != nullptr ? work_grid(lanes, a.lane_estimate, 256u * kCullItems)
-                                                   : grid_for(lanes, 256u * kCullItems);
+  // A tree cut's cull takes a block per cull tile of the cut's capacity (the blocks past the cut's
+  // lanes exit at once): a loop over tiles past an estimated grid made the kernel spill scalar
+  // registers and cost more than the empty blocks.
+  const uint32_t cull_blocks = grid_for(lanes, 256u * kCullItems);
   switch (a.world.sh_degree) {
     case 0: launch_cull_deg<0>(a, cull_blocks, s); break;
     case 1: launch_cull_deg<1>(a, cull_blocks, s); break;

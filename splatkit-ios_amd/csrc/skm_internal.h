@@ -209,8 +209,8 @@ struct FrameArgs {
   // on the view, so that the pairs come out grouped by view and the tile sort needs only the local
   // bits. Zero: tile keys are global tile ids.
   uint32_t local_tile_bits;
-  // Expected cull lanes (tree cuts), survivors and tile pairs, from earlier frames (0: the capacity):
-  // they size the grids of the kernels whose blocks walk that work, which loop past an estimate.
+  // Expected cull lanes, survivors and tile pairs, from earlier frames (0: the capacity): they size the
+  // grids of the radix passes, binning and ranges, whose blocks loop past an estimate.
   uint32_t lane_estimate, survivor_estimate, pair_estimate;
   // Radix sort variant and its tile-count scratch (256 words per kSortTile-key tile of the larger of the
   // survivor and pair capacities).

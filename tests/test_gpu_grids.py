@@ -1,7 +1,7 @@
 """Grids sized from earlier frames. After the first frame of a world, the renderer sizes the grids of
-the kernels that walk survivors and tile pairs (and, for tree cuts, the indexed cull's lanes) from
-the latest finished frame's counts; a frame with much more work than that loops its blocks over the
-rest. Such a frame, drawn right after a light one, must still match the oracle bit for bit."""
+the radix passes, binning and ranges from the latest finished frame's counts (cull lanes, survivors,
+tile pairs); a frame with much more work than that loops their blocks over the rest. Such a frame,
+drawn right after a light one, must still match the oracle bit for bit."""
 import sys
 
 import numpy as np
@@ -53,15 +53,17 @@ def test_large_cut_after_a_small_one(renderer, skm_mod, oracle):
     away_eye = np.array([0.0, 2.0, 18.0], np.float32)  # facing away from the scene: a small cut
     renderer.render(skm_mod.look_at(away_eye, (0.0, 2.0, 60.0)), proj, away_eye)
     small = len(renderer.read_lod_cut()[0])
-    small_count = renderer.read_visible(capacity=1)[0]
+    small_pairs = renderer.read_tiles()[0]
     eye = np.array([0.0, 2.0, 18.0], np.float32)
     view = skm_mod.look_at(eye)
     g = gpu_frame(renderer, skm_mod, None, view, proj, eye, w, h, upload=False, lod_error=1.5)
     cut, _ = renderer.read_lod_cut()
-    say(f"small cut: {small} indices, {small_count} survivors; large cut: {len(cut)} indices, {g['count']} survivors")
-    # The large cut's lanes exceed the estimate from the small one: the indexed cull loops (the
-    # survivor and pair loops are the flat test's).
+    say(f"small cut: {small} indices, {small_pairs} pairs; large cut: {len(cut)} indices, {g['count']} survivors, "
+        f"{g['pairs']} pairs")
+    # The large cut's first depth pass (over the cut's lanes) and its tile pairs exceed the estimates
+    # made from the small cut: those kernels loop.
     assert len(cut) > 1.5 * (small + 512) + 2 * LANE_SLACK
+    assert g["pairs"] > 1.5 * small_pairs + 2 * SURVIVOR_SLACK
     cam = oracle.make_camera(view, proj, w, h, eye)
     o_cut, _ = oracle.lod_select(cam, tree.clusters, tree.leaves, tree.rounds, budget, 1.5, 4.0, True)
     assert np.array_equal(cut, o_cut)
