@@ -286,6 +286,19 @@ hipError_t Batch::launch(const World& world, uint32_t world_count, int sh_degree
   if (local) tile_plan = local_plan;
   a.local_tile_bits = local ? local_bits : 0u;
   local_tile_bits_ = a.local_tile_bits;
+  // The cull's layout spans exactly `lanes`. An eager launch (a replayed graph keeps the grids it was
+  // captured with) estimates survivors from the latest finished launch per lane and pairs per pixel.
+  a.lane_estimate = static_cast<uint32_t>(lanes);
+  if (tuning.use_graph == 0 && counts_known_ && last_lanes_ != 0 && last_pixels_ != 0) {
+    auto ahead = [](uint64_t last, uint64_t now, uint64_t then, uint64_t slack) {
+      const double v = static_cast<double>(last) * static_cast<double>(now) / static_cast<double>(then);
+      return static_cast<uint32_t>(std::min(v * 1.5 + static_cast<double>(slack), 4294967295.0));
+    };
+    a.survivor_estimate = ahead(last_visible_, lanes, last_lanes_, 8u * kSortTile);
+    a.pair_estimate = ahead(last_pairs_, pixels, last_pixels_, 8u * kSortTile);
+  }
+  launch_lanes_ = lanes;
+  launch_pixels_ = pixels;
   depth_passes_ = depth_plan.passes;
   survivor_passes_ = depth_plan.passes + (local ? view_passes : 0);
   tile_passes_ = tile_plan.passes;
@@ -354,6 +367,11 @@ hipError_t Batch::wait(bool* dropped) {
   float ms = 0.0f;
   (void)hipEventElapsedTime(&ms, start_, end_);
   gpu_ms_ = ms;
+  last_lanes_ = launch_lanes_;
+  last_pixels_ = launch_pixels_;
+  last_visible_ = ctrl_host()->visible;
+  last_pairs_ = ctrl_host()->pairs;
+  counts_known_ = true;
   if (ctrl_host()->overflow != 0) {
     *dropped = true;
     pair_want_ = std::min<uint64_t>(uint64_t{ctrl_host()->pairs} + ctrl_host()->pairs / 4 + 4096, kLaneLimit);

@@ -111,6 +111,11 @@ class Batch {
   hipGraphExec_t exec_ = nullptr;
   std::vector<uint64_t> graph_key_;
   bool graph_used_ = false;
+  // The lanes and pixels of the launch in flight, and the counts of the latest finished launch with
+  // its lanes and pixels: an eager launch's grids are estimated from them.
+  uint64_t launch_lanes_ = 0, launch_pixels_ = 0;
+  uint64_t last_lanes_ = 0, last_pixels_ = 0, last_visible_ = 0, last_pairs_ = 0;
+  bool counts_known_ = false;
 };
 
 // The renderer's batch and stream, for the server (renderer.cpp).
