@@ -212,7 +212,32 @@ SERVING_RESOLUTIONS = ((1280, 720), (1920, 1080), (2560, 1440), (3840, 2160))
 SERVING_MODES = (("continuous_batching", 4, 3), ("one_request_per_launch", 1, 1))
 
 
-def serving(skm, rates, per_rate, modes=SERVING_MODES):
+def pixels_to_host(block_bytes, copies=96, streams=3):
+    """Device-to-host copy rate into pinned host memory for blocks of one served request's BGRA8 pixels
+    (the last step of every request), with copies on `streams` streams, after a warm-up round."""
+    import torch
+
+    n = int(block_bytes)
+    src = torch.empty(8 * n, dtype=torch.uint8, device="cuda")
+    dst = [torch.empty(n, dtype=torch.uint8, pin_memory=True) for _ in range(8)]
+    ss = [torch.cuda.Stream() for _ in range(streams)]
+    rate = 0.0
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(copies):
+            with torch.cuda.stream(ss[i % streams]):
+                dst[i % 8].copy_(src[(i % 8) * n:(i % 8 + 1) * n], non_blocking=True)
+        torch.cuda.synchronize()
+        rate = copies * n / (time.perf_counter() - t0)
+    del src, dst
+    return rate
+
+
+SERVING_MEAN_BYTES = 4.0 * sum(w * h for w, h in SERVING_RESOLUTIONS) / len(SERVING_RESOLUTIONS)
+
+
+def serving(skm, rates, per_rate, modes=SERVING_MODES, d2h=None):
     """8 resident scenes (0.5M-10M splats, SH0-3 cyclic, drawn like the 3M scene), requests with a
     random scene, a random camera on a sphere of radius 4-12 m and a resolution from 720p to 2160p,
     arriving as a Poisson stream at each rate. Continuous batching against one request per launch."""
@@ -275,6 +300,13 @@ def serving(skm, rates, per_rate, modes=SERVING_MODES):
     out["max_frames_per_s"] = best
     if "continuous_batching" in best and "one_request_per_launch" in best:
         out["speedup_at_saturation"] = round(best["continuous_batching"] / best["one_request_per_launch"], 3)
+    # Every request's pixels cross PCIe to host memory: the copy rate bounds the requests per second.
+    if d2h:
+        out["pixels_to_host"] = {"mean_request_bytes": int(SERVING_MEAN_BYTES), "d2h_GB_s": round(d2h / 1e9, 1),
+                                 "ceiling_requests_per_s": round(d2h / SERVING_MEAN_BYTES, 1),
+                                 "note": "device-to-host copies of mean-request-sized BGRA8 blocks into pinned host "
+                                         "memory on 3 streams, measured in a child process before this one uses "
+                                         "the GPU; requests/s cannot exceed d2h / mean_request_bytes"}
     return out
 
 
@@ -494,6 +526,8 @@ def main():
                     help="requests per offered rate in the serving leg (0 skips it)")
     ap.add_argument("--serving-rates", default="150,300,600,1200,2400")
     ap.add_argument("--lod-leaves", type=int, default=20_000_000, help="LOD leg's leaves (0 skips it)")
+    ap.add_argument("--d2h-probe", action="store_true",
+                    help="print the device-to-host copy rate of served-request-sized blocks (bytes/s) and exit")
     ap.add_argument("--lod-frames", type=int, default=120)
     ap.add_argument("--sort-sizes", default="7000000,16000000,154000000",
                     help="radix-sort leg: key counts (empty skips it)")
@@ -518,6 +552,18 @@ def main():
     if args.counter_leg:
         print(json.dumps(counter_leg(skm, args.counter_leg)))
         return
+    if args.d2h_probe:
+        print(pixels_to_host(SERVING_MEAN_BYTES))
+        return
+    # The copy rate is measured in a child process (torch's own runtime), before this process uses the GPU.
+    d2h = None
+    if args.serving_requests > 0:
+        import subprocess
+
+        probe = subprocess.run([sys.executable, os.path.abspath(__file__), "--d2h-probe"], capture_output=True,
+                               text=True, timeout=300)
+        if probe.returncode == 0 and probe.stdout.strip():
+            d2h = float(probe.stdout.strip().splitlines()[-1])
     if args.copy_calibration_mib > 0:
         import torch
 
@@ -703,7 +749,7 @@ def main():
            else None)
     served = None
     if args.serving_requests > 0:
-        served = serving(skm, [float(x) for x in args.serving_rates.split(",")], args.serving_requests)
+        served = serving(skm, [float(x) for x in args.serving_rates.split(",")], args.serving_requests, d2h=d2h)
 
     cpu = None
     pixel_err = None
