@@ -121,19 +121,21 @@ def test_batch_ranges_and_reordered_views(renderer, skm_mod):
     assert count == first[2]["count"] and np.array_equal(proj, first[2]["projected"])
 
 
-@pytest.mark.parametrize("count,res,radix,view_passes", [
-    (7, None, 8, 0),              # global tile ids: the view-local ids would take as many passes
-    (24, ((1280, 720),), 8, 1),   # 86,400 tiles (17 bits: 3 passes) vs 3,600 per view (12 bits: 2 passes)
-    (24, ((1280, 720),), 6, 1),   # 3 passes (6, 6, 5 bits) vs 2 of 6
-    (300, ((256, 256),), 8, 2),   # 76,800 tiles (3 passes) vs 256 per view (1 pass); 300 views: 2 view passes
+@pytest.mark.parametrize("count,res,radix,view_passes,sort_kernel", [
+    (7, None, 8, 0, 0),              # global tile ids: the view-local ids would take as many passes
+    (24, ((1280, 720),), 8, 1, 0),   # 86,400 tiles (17 bits: 3 passes) vs 3,600 per view (12 bits: 2 passes)
+    (24, ((1280, 720),), 8, 1, 1),   # the same with the look-back passes (the pass on the view is reduce-then-scan)
+    (24, ((1280, 720),), 6, 1, 0),   # 3 passes (6, 6, 5 bits) vs 2 of 6
+    (300, ((256, 256),), 8, 2, 0),   # 76,800 tiles (3 passes) vs 256 per view (1 pass); 300 views: 2 view passes
 ])
-def test_view_local_tile_keys(renderer, skm_mod, count, res, radix, view_passes):
-    say(f"case: {count} views, resolutions {res}, radix {radix}: {view_passes} passes on the view expected")
+def test_view_local_tile_keys(renderer, skm_mod, count, res, radix, view_passes, sort_kernel):
+    say(f"case: {count} views, resolutions {res}, radix {radix}, sort kernel {sort_kernel}: {view_passes} passes "
+        "on the view expected")
     cloud = skm_mod.make_cloud(80_000, 1, seed=47, layout=1, spread=3.0, area=6.0)
     renderer.upload_world(cloud)
     renderer.apply_policy(sort_depth=32)
     views = small_views(skm_mod, count, seed=17, **({"resolutions": res} if res else {}))
-    renderer.render_views(views, radix_bits=radix)
+    renderer.render_views(views, radix_bits=radix, sort_kernel=sort_kernel)
     st = renderer.batch_stats()
     say(f"batch: {st}")
     assert st["view_sort_passes"] == view_passes
