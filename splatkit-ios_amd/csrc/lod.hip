@@ -402,13 +402,17 @@ __global__ __launch_bounds__(kLodHeadThreads) void k_lod_head(LodArgs a, uint32_
 
 // emitSplatLOD (SplatLOD.metal:210-220): one packet per block iteration.
 // One wave per packet (a packet holds 33 to a cluster's worth of leaves, 256 at most in the trees the
-// builder makes), four lanes' worth of leaf references loaded before any index is stored, so each wave
-// waits for about one round trip per packet and thousands of packets are in flight at once.
+// builder makes), four lanes' worth of leaf references loaded before any index is stored, and the
+// wave's next packet descriptor loaded while it copies the current one, so each wave waits for about
+// one round trip per packet and thousands of packets are in flight at once.
 __global__ __launch_bounds__(256) void k_lod_emit(LodArgs a) {
   const LodState s = a.state[a.rounds];
   const uint32_t waves = gridDim.x * (blockDim.x >> 6), lane = lane_id();
-  for (uint32_t p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); p < s.packets; p += waves) {
-    const uint4 packet = a.packets[p];
+  uint32_t p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  uint4 next = p < s.packets ? a.packets[p] : make_uint4(0u, 0u, 0u, 0u);
+  for (; p < s.packets; p += waves) {
+    const uint4 packet = next;
+    if (p + waves < s.packets) next = a.packets[p + waves];
     for (uint32_t k0 = 0; k0 < packet.y; k0 += 256u) {
       uint32_t v[4];
 #pragma unroll
