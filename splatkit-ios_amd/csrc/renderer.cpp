@@ -316,6 +316,7 @@ struct skm_renderer {
   // The latest finished frame's cull lanes, survivors and tile pairs (valid once counts_known), from
   // which the next frames' grids are estimated.
   uint32_t last_lanes = 0, last_pairs = 0;
+  uint64_t last_pixels = 0;  // that frame's extent
   bool counts_known = false;
   skm_frame_totals totals{};
 
@@ -378,6 +379,7 @@ struct skm_renderer {
     last_draw_count = c->visible;
     last_lanes = c->cull_lanes;
     last_pairs = c->pairs;
+    last_pixels = uint64_t{s.width} * s.height;
     counts_known = true;
     last_dropped = c->overflow != 0;
     if (last_dropped) {
@@ -1035,7 +1037,10 @@ skm_status skm_draw(skm_renderer* r, const skm_frame* f) {
     };
     a.lane_estimate = ahead(r->last_lanes, 8u * skm::kCullTile);
     a.survivor_estimate = ahead(r->last_draw_count, 8u * skm::kSortTile);
-    a.pair_estimate = ahead(r->last_pairs, 8u * skm::kSortTile);
+    // Pairs per pixel: a frame at another extent scales with its pixels.
+    const uint64_t px = uint64_t{W} * H;
+    a.pair_estimate = ahead(r->last_pixels != 0 ? r->last_pairs * px / r->last_pixels : r->last_pairs,
+                            8u * skm::kSortTile);
   }
   a.sort_counts = x.sort_counts.as<uint32_t>();
   a.bin_starts = a.sort_counts + x.sort_status_blocks * skm::kRadix;
