@@ -247,7 +247,7 @@ skm_status skm_read_frame_totals(skm_renderer* r, skm_frame_totals* out, int res
 void skm_count_raster_tests(skm_renderer* r, int enable);
 /* The tree cut's leading rounds whose frontier cannot exceed max_frontier clusters (the clusters at
  * that depth of the tree) run in one single-block kernel instead of two grid-wide launches per round;
- * the cut is the same either way. Default 16384; 0 runs every round grid-wide. No counterpart in the
+ * the cut is the same either way. Default 4096; 0 runs every round grid-wide. No counterpart in the
  * reference: a launch shape of this backend's cut. */
 void skm_set_lod_head(skm_renderer* r, uint32_t max_frontier);
 

@@ -32,7 +32,7 @@ struct LodState {
 constexpr uint32_t kLodMaxRounds = 64;
 // The single-block head of the cut (k_lod_head) takes the leading rounds whose frontier cannot exceed
 // this many clusters.
-constexpr uint32_t kLodHeadFrontier = 16384;
+constexpr uint32_t kLodHeadFrontier = 4096;
 constexpr uint32_t kLodHeadThreads = 1024;
 
 struct LodCtrl {

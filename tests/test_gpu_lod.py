@@ -80,7 +80,7 @@ def test_cut_launch_shapes_give_the_same_cut(renderer, skm_mod, oracle, head):
             both(renderer, oracle, built, skm_mod.look_at(eye), proj, np.asarray(eye, np.float32), 1920, 1080, budget,
                  pixel)
     finally:
-        renderer.set_lod_head(16384)  # the default
+        renderer.set_lod_head(4096)  # the default
 
 
 def test_built_trees_at_many_cameras(renderer, skm_mod, oracle):
