@@ -851,8 +851,9 @@ __device__ __forceinline__ uint64_t wave_inclusive_scan64(uint64_t v) {
 }
 
 // Binning, first pass (reduce then scan) over the survivors in depth order, kBinItems * 256 per
-// block: a block gathers its survivors' tile rectangles into depth order (low words in keys[0], high
-// words in keys[1]: the depth sort is done with both) and stores its pair total in bin_starts[b].
+// survivor block (blocks b = blockIdx.x, blockIdx.x + gridDim.x, ...): a block gathers its survivors'
+// tile rectangles into depth order (low words in keys[0], high words in keys[1]: the depth sort is
+// done with both) and stores its pair total in bin_starts[b].
 __global__ __launch_bounds__(256) void k_bin_count(FrameArgs a, const uint32_t* __restrict__ order) {
   constexpr uint32_t kTile = 256u * kBinItems;
   __shared__ uint32_t s_scan[4];
@@ -1090,7 +1091,8 @@ __global__ __launch_bounds__(256) void k_bin(FrameArgs a, const uint32_t* __rest
 }
 
 // [start, end) of each tile's pairs by boundary detection: thread t takes keys [4t, 4t + 4) as one
-// 16-byte load (the key arrays are 256-byte aligned) and its two neighbours from the cache. With
+// 16-byte load (the key arrays are 256-byte aligned) and its two neighbours from the cache, and the
+// next group of 4 a grid's threads further while any are left (the grid comes from earlier frames). With
 // view-local keys (local_bits != 0) a key is view << local_bits | the tile's index in its view, and
 // its range goes to the view's tile_base + that index; equal keys are adjacent (the pairs are sorted
 // on the index, and each index's pairs come in view order).
