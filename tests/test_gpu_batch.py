@@ -150,6 +150,29 @@ def test_view_local_tile_keys(renderer, skm_mod, count, res, radix, view_passes,
     assert sum(counts) == st["visible"]
 
 
+def test_eager_batch_after_a_light_one(renderer, skm_mod):
+    """An eager batch (no graph: the serving loop's launches) sizes its grids from the latest launch,
+    scaled by lanes and pixels; a batch with far more survivors and pairs per pixel than that loops its
+    blocks past the estimate and must still equal single draws."""
+    say("case: eager batch of 2 small far views, then 6 close views at 1280x720")
+    cloud = skm_mod.make_cloud(250_000, 1, seed=48, layout=1, spread=3.0, area=6.0)
+    renderer.upload_world(cloud)
+    renderer.apply_policy(sort_depth=32)
+    proj_small = skm_mod.perspective(np.pi / 3, 1.0, 0.2, 100.0)
+    far = [(skm_mod.look_at(np.array([0.0, 2.0, z], np.float32), (0.0, 2.0, z + 40.0)), proj_small,
+            np.array([0.0, 2.0, z], np.float32), 256, 256) for z in (20.0, 25.0)]
+    renderer.render_views(far, graph=False)
+    light = renderer.batch_stats()
+    views = small_views(skm_mod, 6, seed=19, resolutions=((1280, 720),))
+    renderer.render_views(views, graph=False)
+    st = renderer.batch_stats()
+    say(f"light batch: {light}; heavy batch: {st}")
+    assert not st["graph"] and st["visible"] > 10 * max(1, light["visible"]) and st["pairs"] > 10 * max(1, light["pairs"])
+    got = {k: batched(renderer, k) for k in (0, 3, 5)}
+    for k, b in got.items():
+        assert_same(b, single(renderer, views[k], False), f"view {k}")
+
+
 def test_64_view_batch_of_the_3m_scene(renderer, skm_mod):
     say("case: 64 views (720p/1080p/1440p) of the 3M SH3 scene, one graph launch")
     cloud = skm_mod.scene_c1()
